@@ -1,0 +1,208 @@
+/*
+ * qiw.h -- C ABI of libqiw, the MI355X (gfx950) quasi Monte Carlo diagram-evaluation
+ * engine behind the inchworm solver's `inchworm!` and `correlator_2p`.
+ *
+ * Every entry point takes plain pointers and sizes.  Complex numbers are interleaved
+ * (re, im) float64 pairs, i.e. C99 `double _Complex` / Julia `ComplexF64`.
+ * Sector indices are 0-based.  Matrices are column-major.  The library copies every
+ * input during the call; outputs are written into caller-owned arrays before the call
+ * returns (all entry points are synchronous).  No entry point throws; each returns
+ * QIW_OK (0) or a negative QIW_E* code and leaves a message for qiw_last_error().
+ * One context per host thread; a context is not re-entrant.
+ *
+ * Block-table layout ("BT layout") used for every block-diagonal quantity
+ * (P(tau), step results, per-order contributions): sector s of dimension m_s occupies
+ * the m_s*m_s complex slots starting at off_s = sum_{s'<s} m_{s'}^2, element (r, c) at
+ * off_s + r + c*m_s.  A table over the imaginary-time grid stores n_tau such blocks of
+ * sum_s m_s^2 complex numbers, tau index slowest.
+ */
+#ifndef QIW_H
+#define QIW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QIW_ABI_VERSION 1
+
+/* ---- return codes (every reference @assert on this path maps to one) ---------------- */
+enum {
+    QIW_OK = 0,
+    QIW_E_ARG = -1,          /* malformed argument (null pointer, bad size)              */
+    QIW_E_NOT_POW2 = -2,     /* N_samples is neither 0 nor a power of 2
+                                (src/inchworm.jl:343, :930)                              */
+    QIW_E_OFFDIAG = -3,      /* a diagram leaf ends in a sector other than the one it
+                                started in (src/topology_eval.jl:462-464)                */
+    QIW_E_NOT_SBM = -4,      /* operator has more than one block per column
+                                (src/sector_block_matrix.jl:57-61)                       */
+    QIW_E_DIM = -5,          /* Sobol dimension beyond the supplied table
+                                (src/scrambled_sobol.jl:67)                              */
+    QIW_E_STATE = -6,        /* call out of order (e.g. no model set)                   */
+    QIW_E_DEVICE = -7,       /* HIP runtime failure                                      */
+    QIW_E_LIMIT = -8,        /* size beyond a compiled-in limit of the GPU kernels      */
+    QIW_E_NO_GPU = -9        /* no usable gfx950 device                                  */
+};
+
+/* ---- interpolation kinds of time-dependent quantities ------------------------------- */
+enum {
+    QIW_GF_GRID_LINEAR = 0,  /* Keldysh ImaginaryTimeGF stored on the grid, linear in the
+                                time difference                                          */
+    QIW_GF_CUBIC_SPLINE = 1, /* SplineInterpolatedGF: natural cubic spline through the
+                                grid values (src/spline_gf.jl:194-242)                   */
+    QIW_GF_INC_SPLINE = 2    /* IncSplineImaginaryTimeGF: incremental quadratic spline
+                                (src/spline_gf.jl:273-425, src/utility.jl:106-157)      */
+};
+
+/* ---- arithmetic of the chain products ------------------------------------------------ */
+enum {
+    QIW_PREC_FP64 = 0,       /* all arithmetic in float64                                */
+    QIW_PREC_FP32 = 1        /* block chain products in float32; Sobol-to-simplex map,
+                                accumulation (compensated) and P bookkeeping in float64  */
+};
+
+/* One block-sparse operator, the reference's SectorBlockMatrix
+ * (src/sector_block_matrix.jl:44): for every block, column sector s_i maps to row
+ * sector s_f through an m_f x m_i matrix.  At most one block per column sector. */
+typedef struct {
+    int32_t n_blocks;
+    const int32_t *s_i;      /* [n_blocks] column (initial) sector                       */
+    const int32_t *s_f;      /* [n_blocks] row (final) sector                            */
+    const double *data;      /* complex128, blocks concatenated in order, each
+                                column-major m_f x m_i                                   */
+} qiw_sbm;
+
+/* The expansion problem, the reference's `Expansion` (src/expansion.jl:107-197) after
+ * the caller has solved the local problem.  The imaginary-time grid is uniform,
+ * tau_j = j * beta / (n_tau - 1), j = 0..n_tau-1 (kd.ImaginaryTimeGrid). */
+typedef struct {
+    double beta;
+    int32_t n_tau;
+    int32_t n_sectors;
+    const int32_t *sector_dim;     /* [n_sectors] block sizes m_s                         */
+    const double *energies;        /* [sum m_s] eigenvalues of H_loc per sector (float64),
+                                      sector-major; defines the atomic propagators        */
+    int32_t ppgf_kind;             /* QIW_GF_GRID_LINEAR (interpolate_ppgf=false) or
+                                      QIW_GF_INC_SPLINE (interpolate_ppgf=true)           */
+    int32_t n_pairs;               /* InteractionPairs (src/expansion.jl:79-90)           */
+    const qiw_sbm *pair_op_i;      /* [n_pairs] operator_i (earlier, "head" end)          */
+    const qiw_sbm *pair_op_f;      /* [n_pairs] operator_f (later, "tail" end)            */
+    const double *pair_prop;       /* [n_pairs][n_tau] complex128: propagator values on
+                                      the grid, g_p(tau_j)                                */
+    const int32_t *pair_prop_kind; /* [n_pairs] QIW_GF_GRID_LINEAR or QIW_GF_CUBIC_SPLINE */
+    const int32_t *pair_prop_xi;   /* [n_pairs] statistics sign (-1 fermionic, +1 bosonic),
+                                      used when the time difference is negative           */
+    int32_t n_corr;                /* correlator operator pairs (A, B), add_corr_operators! */
+    const qiw_sbm *corr_A;         /* [n_corr] A, placed at tau                            */
+    const qiw_sbm *corr_B;         /* [n_corr] B, placed at 0                              */
+} qiw_model;
+
+/* Parameters of one inchworm!/correlator_2p run (keyword arguments of
+ * src/inchworm.jl:332-339 and :1075-1081). */
+typedef struct {
+    int32_t n_orders;
+    const int32_t *orders;         /* orders of a regular (bold) step                     */
+    int32_t n_orders_bare;
+    const int32_t *orders_bare;    /* orders of the initial (bare) step                   */
+    int64_t N_samples;             /* qMC points per (order, n_pts_after) group, 0 or 2^n */
+    int32_t n_pts_after_max;       /* <= 0 means unrestricted                             */
+    int32_t precision;             /* QIW_PREC_*                                          */
+} qiw_run;
+
+typedef struct qiw_ctx qiw_ctx;
+
+/* ---- context --------------------------------------------------------------------------
+ * Replaces the per-rank state of the MPI setup (src/mpi.jl:39-54): one context drives
+ * one GPU. */
+int qiw_create(qiw_ctx **out, int device);
+void qiw_destroy(qiw_ctx *ctx);
+const char *qiw_last_error(const qiw_ctx *ctx);
+int qiw_abi_version(void);
+
+/* ---- inputs ----------------------------------------------------------------------------
+ * qiw_set_model: stages an Expansion (src/expansion.jl:147-197).  Resets the bold
+ * propagator to the atomic one (ppgf.atomic_ppgf, src/ppgf.jl:144-207; for
+ * QIW_GF_INC_SPLINE also initial_ppgf_derivative, src/ppgf.jl:710-720). */
+int qiw_set_model(qiw_ctx *ctx, const qiw_model *model);
+
+/* qiw_set_sobol: direction numbers of the (optionally scrambled) Sobol sequence, the
+ * state of ScrambledSobolSeq (src/scrambled_sobol.jl:50-143) after its constructor:
+ * m[d*32 + j] is m[d+1, j+1] of the reference (already multiplied by 2^(31-j) and, when
+ * scrambled, already LMS-scrambled), shift[d] is the initial point x (NULL = zeros,
+ * the unscrambled sequence).  Points of an order-k group use dimensions 0..2k-1. */
+int qiw_set_sobol(qiw_ctx *ctx, int32_t dim, const uint32_t *m, const uint32_t *shift);
+
+/* qiw_set_ppgf: overwrite the bold propagator state (expansion.P) with a table in BT
+ * layout over the grid.  n_known grid points are valid.  der0 (BT layout, one block) is
+ * the tau-derivative at 0 of the incremental spline (QIW_GF_INC_SPLINE only, else NULL). */
+int qiw_set_ppgf(qiw_ctx *ctx, int32_t n_known, const double *P, const double *der0);
+/* qiw_get_ppgf: read expansion.P back (same layout); n_known_out may be NULL. */
+int qiw_get_ppgf(qiw_ctx *ctx, int32_t *n_known_out, double *P, double *der0);
+
+/* ---- drivers ---------------------------------------------------------------------------
+ * qiw_inchworm: inchworm!(expansion, grid, orders, orders_bare, N_samples;
+ * n_pts_after_max) (src/inchworm.jl:332-498).  The bare step and all n_tau-2 bold steps
+ * run on the device.  Outputs (caller-owned, may be NULL):
+ *   P_out            [n_tau][sum m^2]   expansion.P after the run
+ *   P_orders_out     [n_orders_all][n_tau][sum m^2]  per-order contributions, orders_all
+ *                    sorted ascending (union of orders and orders_bare)
+ *   P_orders_std_out same shape; NaN for sampled orders (one sequence), 0 for order 0,
+ *                    as mean_std_from_randomization with N_seqs = 1 returns. */
+int qiw_inchworm(qiw_ctx *ctx, const qiw_run *run, double *P_out, double *P_orders_out,
+                 double *P_orders_std_out);
+
+/* qiw_inchworm_step: one regular step inchworm_step(expansion, c, tau_i, tau_w, tau_f,
+ * top_data) (src/inchworm.jl:123-203) with tau_i = grid[0], tau_w = grid[n_w],
+ * tau_f = grid[n_w+1], on the current expansion.P; does not modify it.  Outputs
+ * value [sum m^2] and per-order [n_orders][sum m^2] (orders as in run->orders). */
+int qiw_inchworm_step(qiw_ctx *ctx, const qiw_run *run, int32_t n_w, double *value_out,
+                      double *orders_out);
+
+/* qiw_inchworm_step_bare: the initial step inchworm_step_bare (src/inchworm.jl:228-304)
+ * between grid[0] and grid[1], orders from run->orders_bare. */
+int qiw_inchworm_step_bare(qiw_ctx *ctx, const qiw_run *run, double *value_out,
+                           double *orders_out);
+
+/* qiw_correlator_2p: correlator_2p(expansion, grid, orders, N_samples)
+ * (src/inchworm.jl:918-1052, :1075-1084) for every (A, B) pair of the model, using the
+ * current expansion.P.  corr_out [n_corr][n_tau] complex128: <A(tau_j) B(0)> / Z. */
+int qiw_correlator_2p(qiw_ctx *ctx, const qiw_run *run, double *corr_out);
+
+/* ---- observables (src/ppgf.jl:611-636, :729-733) --------------------------------------
+ * partition_function(P) = sum_s tr iP_s(beta); density_matrix after normalize!(P, beta):
+ * rho_out [sum m^2] complex (BT layout).  Neither modifies the context. */
+int qiw_partition_function(qiw_ctx *ctx, double *Z_out /* complex128 */);
+int qiw_density_matrix(qiw_ctx *ctx, double *rho_out);
+
+/* ---- component entry points, exposed for parity testing ------------------------------
+ * qiw_sobol_points: points first..first+n-1 (0-based) of the staged sequence in
+ * dimensions 0..dim-1, computed on the device; out [n][dim] float64
+ * (ScrambledSobolSeq next!, src/scrambled_sobol.jl:158-173). */
+int qiw_sobol_points(qiw_ctx *ctx, int32_t dim, int64_t first, int64_t n, double *out);
+
+/* qiw_simplex_points: device Sobol points pushed through DoubleSimplexRootTransform
+ * (src/qmc_integrate.jl:398-449) for d_before/d_after on [t_i, t_w, t_f]; with
+ * d_after = 0 the RootTransform (src/qmc_integrate.jl:220-245) on [t_i, t_f].
+ * out [n][d_before + d_after] float64 times, descending within a point. */
+int qiw_simplex_points(qiw_ctx *ctx, int32_t d_before, int32_t d_after, double t_i,
+                       double t_w, double t_f, int64_t n, double *out);
+
+/* ---- measurement ---------------------------------------------------------------------
+ * Statistics of the last qiw_inchworm / qiw_correlator_2p call on this context. */
+typedef struct {
+    double wall_s;              /* host wall time of the call                             */
+    double kernel_ms;           /* summed device time of the evaluation kernels           */
+    int64_t n_kernel_launches;  /* evaluation kernel launches                             */
+    int64_t diagram_evals;      /* (topology, sample) pairs evaluated                     */
+    int64_t leaves;             /* (topology, sample, initial sector, pair assignment)
+                                   leaves evaluated                                       */
+    double flops;               /* algorithmic real FLOPs of the chain products           */
+} qiw_stats;
+int qiw_get_stats(qiw_ctx *ctx, qiw_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QIW_H */

@@ -1,0 +1,341 @@
+"""Caller-side construction of an expansion problem (the part the reference delegates to
+KeldyshED.jl and Keldysh.jl): Fock-space operators, exact diagonalization in symmetry
+sectors, operator blocks in the eigenbasis, hybridization tables from a bath density of
+states, and the `Expansion` container (src/expansion.jl:107-310).
+
+Only setup work happens here; none of it is on the sampled hot path.
+"""
+import numpy as np
+
+from . import abi
+
+
+# ------------------------------------------------------------------------------------
+# Fock space with Jordan-Wigner fermions
+# ------------------------------------------------------------------------------------
+class FockSpace:
+    """Fermionic Fock space over an ordered list of mode labels.  Basis state b has bit i
+    set when mode i is occupied; c_i carries the sign (-1)^(number of occupied modes
+    below i)."""
+
+    def __init__(self, labels):
+        self.labels = list(labels)
+        self.n_modes = len(self.labels)
+        self.dim = 1 << self.n_modes
+
+    def index(self, label):
+        return self.labels.index(label)
+
+    def c(self, label):
+        i = self.index(label)
+        M = np.zeros((self.dim, self.dim))
+        for b in range(self.dim):
+            if b >> i & 1:
+                sign = -1.0 if bin(b & ((1 << i) - 1)).count("1") % 2 else 1.0
+                M[b ^ (1 << i), b] = sign
+        return M
+
+    def c_dag(self, label):
+        return self.c(label).T.copy()
+
+    def n(self, label):
+        return self.c_dag(label) @ self.c(label)
+
+    def occupation(self, b, label):
+        return b >> self.index(label) & 1
+
+
+# ------------------------------------------------------------------------------------
+# Exact diagonalization in sectors
+# ------------------------------------------------------------------------------------
+class EDCore:
+    """Diagonalizes H inside the sectors given by `sector_key(basis_state)`; sectors are
+    ordered by sorted key.  Operators are represented by their blocks between eigenbases
+    (the role of KeldyshED's EDCore / operator_blocks)."""
+
+    def __init__(self, H, fock, sector_key):
+        self.fock = fock
+        keys = {}
+        for b in range(fock.dim):
+            keys.setdefault(sector_key(b), []).append(b)
+        self.sector_keys = sorted(keys)
+        self.subspaces = [keys[k] for k in self.sector_keys]
+        sec_of = np.empty(fock.dim, dtype=int)
+        for s, states in enumerate(self.subspaces):
+            sec_of[states] = s
+        off = H.copy()
+        for s, states in enumerate(self.subspaces):
+            off[np.ix_(states, states)] = 0.0
+        if np.abs(off).max(initial=0.0) > 1e-12:
+            raise ValueError("Hamiltonian is not block-diagonal in the requested sectors")
+        self.energies, self.U = [], []
+        for states in self.subspaces:
+            E, U = np.linalg.eigh(H[np.ix_(states, states)])
+            self.energies.append(E)
+            self.U.append(U)
+        self.dims = [len(s) for s in self.subspaces]
+
+    def operator_blocks(self, O, tol=1e-13):
+        """SectorBlockMatrix of O: dict s_i -> (s_f, m_f x m_i block in the eigenbases)
+        (operator_to_sector_block_matrix, src/sector_block_matrix.jl:52-66)."""
+        sbm = {}
+        for si, st_i in enumerate(self.subspaces):
+            for sf, st_f in enumerate(self.subspaces):
+                blk = O[np.ix_(st_f, st_i)]
+                if np.abs(blk).max(initial=0.0) <= tol:
+                    continue
+                if si in sbm:
+                    raise ValueError("Operator is not representable by a SectorBlockMatrix "
+                                     "(more than one non-zero blocks per column)")
+                sbm[si] = (sf, self.U[sf].T.conj() @ blk @ self.U[si])
+        return sbm
+
+    def partition_function(self, beta):
+        emin = min(e.min() for e in self.energies)
+        return sum(np.exp(-beta * (e - emin)).sum() for e in self.energies) * np.exp(-beta * emin)
+
+    def density_matrix(self, beta):
+        emin = min(e.min() for e in self.energies)
+        w = [np.exp(-beta * (e - emin)) for e in self.energies]
+        z = sum(x.sum() for x in w)
+        return [np.diag(x / z) for x in w]
+
+
+# ------------------------------------------------------------------------------------
+# Scalar imaginary-time Green's functions on the uniform grid
+# ------------------------------------------------------------------------------------
+class ImaginaryTimeGrid:
+    def __init__(self, beta, n_tau):
+        self.beta = float(beta)
+        self.n_tau = int(n_tau)
+        self.tau = np.arange(self.n_tau) * (self.beta / (self.n_tau - 1))
+
+    def __len__(self):
+        return self.n_tau
+
+
+class ScalarGF:
+    """g(tau_j) on the grid plus its interpolation kind and statistics sign."""
+
+    def __init__(self, values, kind=abi.QIW_GF_GRID_LINEAR, xi=-1):
+        self.values = np.asarray(values, dtype=np.complex128)
+        self.kind = kind
+        self.xi = xi
+
+    def __mul__(self, a):
+        return ScalarGF(self.values * a, self.kind, self.xi)
+
+    __rmul__ = __mul__
+
+
+def spline_interpolated(g):
+    """SplineInterpolatedGF(g) (src/spline_gf.jl:65-72): same values, cubic-spline lookups."""
+    return ScalarGF(g.values, abi.QIW_GF_CUBIC_SPLINE, g.xi)
+
+
+def ph_conj(g):
+    """g(beta - tau) (src/utility.jl:210-217, src/spline_gf.jl:250-255)."""
+    return ScalarGF(g.values[::-1].copy(), g.kind, g.xi)
+
+
+def _fermi_weighted_exp(eps, beta, tau):
+    """(1 - n_F(eps)) exp(-eps tau) for 0 <= tau <= beta without overflow."""
+    eps = np.asarray(eps, float)[..., None]
+    tau = np.asarray(tau, float)[None, ...]
+    pos = eps >= 0
+    a = np.where(pos, np.exp(-eps * tau) / (1.0 + np.exp(-beta * np.abs(eps))),
+                 np.exp(eps * (beta - tau)) / (1.0 + np.exp(-beta * np.abs(eps))))
+    return a
+
+
+def delta_dos_gf(eps, beta, grid, weight=1.0):
+    """Free fermion of level eps, G(tau) = -i (1 - n_F(eps)) e^{-eps tau} (kd.DeltaDOS)."""
+    return ScalarGF(-1j * weight * _fermi_weighted_exp([eps], beta, grid.tau)[0])
+
+
+def bethe_gf(t, beta, grid, n_quad=400):
+    """Semi-elliptic density of states of half-bandwidth 2t (kd.bethe_dos(t=t)),
+    G(tau) = -i int d eps rho(eps) (1 - n_F(eps)) e^{-eps tau}, by Gauss-Legendre on
+    eps = 2t cos(theta), where rho d eps = (2/pi) sin^2(theta) d theta."""
+    x, w = np.polynomial.legendre.leggauss(n_quad)
+    theta = 0.5 * np.pi * (x + 1.0)
+    wt = 0.5 * np.pi * w * (2.0 / np.pi) * np.sin(theta) ** 2
+    eps = 2.0 * t * np.cos(theta)
+    vals = (wt[:, None] * _fermi_weighted_exp(eps, beta, grid.tau)).sum(axis=0)
+    return ScalarGF(-1j * vals)
+
+
+# ------------------------------------------------------------------------------------
+# Expansion
+# ------------------------------------------------------------------------------------
+class InteractionPair:
+    """InteractionPair(operator_f, operator_i, propagator) (src/expansion.jl:79-90):
+    operator_i sits at the earlier time, operator_f at the later one."""
+
+    def __init__(self, operator_f, operator_i, propagator):
+        self.operator_f = operator_f
+        self.operator_i = operator_i
+        self.propagator = propagator
+
+
+class Expansion:
+    """Strong-coupling expansion problem (src/expansion.jl:107-197).  `P` holds the bold
+    propagator table in BT layout (n_tau, sum m^2) complex, updated by inchworm()."""
+
+    def __init__(self, ed, grid, interaction_pairs, corr_operators=(), interpolate_ppgf=False):
+        self.ed = ed
+        self.grid = grid
+        self.pairs = list(interaction_pairs)
+        self.interpolate_ppgf = bool(interpolate_ppgf)
+        self.identity_mat = {s: (s, np.eye(d)) for s, d in enumerate(ed.dims)}
+        self.pair_operator_mat = [(ed.operator_blocks(p.operator_i), ed.operator_blocks(p.operator_f))
+                                  for p in self.pairs]
+        self.corr_operators = []
+        self.corr_operators_mat = []
+        for A, B in corr_operators:
+            self.add_corr_operators((A, B))
+        self.subspace_attachable_pairs = [
+            [p for p, ops in enumerate(self.pair_operator_mat) if s in ops[0]]
+            for s in range(len(ed.dims))]
+        self.bt_offsets = np.cumsum([0] + [d * d for d in ed.dims])[:-1]
+        self.bt_size = int(sum(d * d for d in ed.dims))
+        self.P = self.atomic_ppgf()
+        self.P_der0 = self.initial_ppgf_derivative()
+        self.P_n_known = 1 if self.interpolate_ppgf else grid.n_tau
+
+    @property
+    def ppgf_kind(self):
+        return abi.QIW_GF_INC_SPLINE if self.interpolate_ppgf else abi.QIW_GF_GRID_LINEAR
+
+    def add_corr_operators(self, ops):
+        """add_corr_operators! (src/expansion.jl:303-310)."""
+        A, B = ops
+        self.corr_operators.append((A, B))
+        self.corr_operators_mat.append((self.ed.operator_blocks(A), self.ed.operator_blocks(B)))
+        self._struct = None
+
+    def _lambda(self):
+        beta = self.grid.beta
+        return np.log(self.ed.partition_function(beta)) / beta
+
+    def atomic_ppgf(self):
+        """P_s(tau) = -i exp(-tau (E_s + lambda)), Tr iP(beta) = 1 (src/ppgf.jl:144-207)."""
+        lam = self._lambda()
+        P = np.zeros((self.grid.n_tau, self.bt_size), dtype=np.complex128)
+        for s, E in enumerate(self.ed.energies):
+            d = len(E)
+            o = self.bt_offsets[s]
+            for i in range(d):
+                P[:, o + i + i * d] = -1j * np.exp(-self.grid.tau * (E[i] + lam))
+        return P
+
+    def initial_ppgf_derivative(self):
+        """dP/dtau at 0: -i diag(-E - lambda) (src/ppgf.jl:710-720)."""
+        lam = self._lambda()
+        der = np.zeros(self.bt_size, dtype=np.complex128)
+        for s, E in enumerate(self.ed.energies):
+            d = len(E)
+            o = self.bt_offsets[s]
+            for i in range(d):
+                der[o + i + i * d] = -1j * (-E[i] - lam)
+        return der
+
+    def block(self, table_row, s):
+        d = self.ed.dims[s]
+        o = self.bt_offsets[s]
+        return table_row[o:o + d * d].reshape(d, d, order="F")
+
+    def to_struct(self):
+        """The qiw_model this expansion is handed over as."""
+        props = np.array([p.propagator.values for p in self.pairs]).reshape(len(self.pairs), self.grid.n_tau)
+        return abi.ModelStruct(
+            self.grid.beta, self.grid.n_tau, self.ed.dims, self.ed.energies, self.ppgf_kind,
+            self.pair_operator_mat, props, [p.propagator.kind for p in self.pairs],
+            [p.propagator.xi for p in self.pairs], self.corr_operators_mat)
+
+
+# ------------------------------------------------------------------------------------
+# Model families of the benchmark configurations
+# ------------------------------------------------------------------------------------
+def anderson_single_orbital(beta=10.0, n_tau=64, eps=0.1, U=1.0, V=0.5, D=2.0,
+                            interpolate=False, with_correlators=False):
+    """Single-orbital Anderson impurity H = eps (n_up + n_dn) + U n_up n_dn, 4 states in
+    1x1 blocks by (N_up, N_dn), semi-elliptic bath of half-bandwidth D, Delta = V^2 G_bethe,
+    pairs c^dag_s/c_s and their particle-hole conjugates (README.md:55-137)."""
+    fock = FockSpace(["up", "dn"])
+    H = eps * (fock.n("up") + fock.n("dn")) + U * fock.n("up") @ fock.n("dn")
+    ed = EDCore(H, fock, lambda b: (fock.occupation(b, "up"), fock.occupation(b, "dn")))
+    grid = ImaginaryTimeGrid(beta, n_tau)
+    delta = V ** 2 * bethe_gf(D / 2.0, beta, grid)
+    if interpolate:
+        delta = spline_interpolated(delta)
+    pairs = []
+    for s in ["up", "dn"]:
+        pairs.append(InteractionPair(fock.c_dag(s), fock.c(s), delta))
+        pairs.append(InteractionPair(fock.c(s), fock.c_dag(s), ph_conj(delta)))
+    corr = []
+    if with_correlators:
+        corr = [(-fock.c("up"), fock.c_dag("up")),
+                (fock.n("up"), fock.n("dn")),
+                (fock.c_dag("up") @ fock.c("dn"), fock.c_dag("dn") @ fock.c("up"))]
+    return Expansion(ed, grid, pairs, corr, interpolate_ppgf=interpolate)
+
+
+def kanamori(n_orb, U, J, eps_shift, beta, n_tau, V, D=2.0, interpolate=False):
+    """n_orb-orbital Hubbard-Kanamori impurity (spin-rotation invariant):
+    H = mu sum n + U sum_a n_a,up n_a,dn + (U-2J) sum_{a<b,s} n_a,s n_b,-s
+        + (U-3J) sum_{a<b,s} n_a,s n_b,s
+        - J sum_{a!=b} c+_a,up c_a,dn c+_b,dn c_b,up + J sum_{a!=b} c+_a,up c+_a,dn c_b,dn c_b,up,
+    with mu = eps_shift - [U/2 + (n_orb - 1)(2U - 5J)/2] (eps_shift = 0 is half filling).
+    Sectors by total particle number; each spin-orbital couples to its own semi-elliptic
+    bath (half-bandwidth D, hopping V): pairs c^dag_i/c_i and particle-hole conjugates."""
+    labels = [(a, s) for a in range(n_orb) for s in ("up", "dn")]
+    fock = FockSpace(labels)
+    n = {l: fock.n(l) for l in labels}
+    mu = eps_shift - (U / 2.0 + (n_orb - 1) * (2 * U - 5 * J) / 2.0)
+    H = mu * sum(n.values())
+    for a in range(n_orb):
+        H = H + U * n[(a, "up")] @ n[(a, "dn")]
+    for a in range(n_orb):
+        for b in range(a + 1, n_orb):
+            for s, sb in (("up", "dn"), ("dn", "up")):
+                H = H + (U - 2 * J) * n[(a, s)] @ n[(b, sb)]
+                H = H + (U - 3 * J) * n[(a, s)] @ n[(b, s)]
+    cd = {l: fock.c_dag(l) for l in labels}
+    c = {l: fock.c(l) for l in labels}
+    for a in range(n_orb):
+        for b in range(n_orb):
+            if a == b:
+                continue
+            H = H - J * cd[(a, "up")] @ c[(a, "dn")] @ cd[(b, "dn")] @ c[(b, "up")]
+            H = H + J * cd[(a, "up")] @ cd[(a, "dn")] @ c[(b, "dn")] @ c[(b, "up")]
+    ed = EDCore(H, fock, lambda b: bin(b).count("1"))
+    grid = ImaginaryTimeGrid(beta, n_tau)
+    delta = V ** 2 * bethe_gf(D / 2.0, beta, grid)
+    if interpolate:
+        delta = spline_interpolated(delta)
+    pairs = []
+    for l in labels:
+        pairs.append(InteractionPair(cd[l], c[l], delta))
+        pairs.append(InteractionPair(c[l], cd[l], ph_conj(delta)))
+    return Expansion(ed, grid, pairs, interpolate_ppgf=interpolate)
+
+
+def single_state_golden_model(sector_order=("occupied", "empty")):
+    """The model of test/inchworm.jl:53-67, :180-212: H = -mu n (mu = 0.1), one bath level
+    eps = 0.1 (kd.DeltaDOS) with V = -0.1, beta = 10, n_tau = 20, cubic-spline Delta and
+    incremental-spline P."""
+    beta, mu, eps, V, n_tau = 10.0, 0.1, 0.1, -0.1, 20
+    fock = FockSpace(["0"])
+    H = -mu * fock.n("0")
+    order = {"empty": 0, "occupied": 1}
+    rank = {order[k]: i for i, k in enumerate(sector_order)}
+    ed = EDCore(H, fock, lambda b: rank[fock.occupation(b, "0")])
+    grid = ImaginaryTimeGrid(beta, n_tau)
+    delta = V ** 2 * delta_dos_gf(eps, beta, grid)
+    delta_rev = ph_conj(delta)  # -Delta[t2, t1, false] = Delta(beta - tau) for fermions
+    pairs = [InteractionPair(fock.c_dag("0"), fock.c("0"), spline_interpolated(delta)),
+             InteractionPair(fock.c("0"), fock.c_dag("0"), spline_interpolated(delta_rev))]
+    exp = Expansion(ed, grid, pairs, interpolate_ppgf=True)
+    exp.fock = fock
+    return exp

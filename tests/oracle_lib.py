@@ -1,0 +1,131 @@
+"""ctypes binding of the CPU float64 oracle (oracle/build/liborc.so).
+
+Test infrastructure: the oracle is the checker the GPU path is compared against and is
+never part of the product path."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from qiw import abi
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORC_PATH = os.path.join(REPO, "oracle", "build", "liborc.so")
+_orc = None
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORC_PATH):
+            subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True,
+                           capture_output=True)
+        L = C.CDLL(ORC_PATH)
+        f64p, u32p, i32p = C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_sobol_direction_numbers.argtypes = [C.c_int, u32p, u32p, C.c_int, C.POINTER(C.c_uint8),
+                                                  C.POINTER(C.c_uint8), u32p, u32p]
+        L.orc_sobol_points.argtypes = [C.c_int, u32p, u32p, C.c_int64, C.c_int, C.c_int64, f64p]
+        L.orc_simplex_points.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, f64p,
+                                         C.c_int64, f64p, f64p]
+        L.orc_topologies.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p]
+        L.orc_inchworm.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p, C.c_int,
+                                   f64p, f64p, f64p]
+        L.orc_inchworm_step.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p,
+                                        C.c_int, f64p, f64p, C.c_int, f64p, f64p]
+        L.orc_inchworm_step_bare.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p,
+                                             f64p, f64p]
+        L.orc_correlator_2p.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p,
+                                        C.c_int, f64p, f64p, C.c_int, C.c_int, f64p]
+        L.orc_eval_topologies.argtypes = [C.POINTER(abi.qiw_model), C.c_int, C.c_int, C.c_int, C.c_double,
+                                          C.c_double, C.c_double, C.c_int, i32p, i32p, f64p, f64p]
+        L.orc_gf_eval.argtypes = [C.c_int, C.c_int, C.c_double, f64p, C.c_int, f64p, C.c_int, C.c_int,
+                                  f64p, f64p, f64p]
+        L.orc_get_counters.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        _orc = L
+    return _orc
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def check(rc):
+    if rc != 0:
+        raise abi.QiwError(rc, lib().orc_last_error().decode())
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def counters():
+    a, b, c = C.c_int64(), C.c_int64(), C.c_double()
+    lib().orc_get_counters(C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def reset_counters():
+    lib().orc_reset_counters()
+
+
+def sobol_tables(max_dim):
+    from qiw.sobol import direction_numbers
+    m = np.ascontiguousarray(direction_numbers(max_dim))
+    return m, np.zeros(max_dim, dtype=np.uint32)
+
+
+def inchworm(exp, orders, orders_bare, N, n_pts_after_max=0, n_steps_max=-1, sobol=None):
+    ms = exp.to_struct()
+    run = abi.RunStruct(orders, orders_bare, N, n_pts_after_max)
+    max_order = max(list(orders) + list(orders_bare) + [1])
+    m, x0 = sobol if sobol is not None else sobol_tables(2 * max_order)
+    n_tau, bt = exp.grid.n_tau, exp.bt_size
+    oall = sorted(set(orders) | set(orders_bare))
+    P = np.zeros((n_tau, bt), np.complex128)
+    Po = np.zeros((len(oall), n_tau, bt), np.complex128)
+    Ps = np.zeros((len(oall), n_tau, bt), np.complex128)
+    check(lib().orc_inchworm(ms.ref(), run.ref(), _p(m, C.c_uint32), _p(x0, C.c_uint32), int(n_steps_max),
+                             _p(P, C.c_double), _p(Po, C.c_double), _p(Ps, C.c_double)))
+    return P, {o: Po[i] for i, o in enumerate(oall)}, {o: Ps[i] for i, o in enumerate(oall)}
+
+
+def inchworm_step(exp, orders, N, jw, P=None, n_known=None, der0=None, sobol=None, n_pts_after_max=0):
+    ms = exp.to_struct()
+    run = abi.RunStruct(orders, [], N, n_pts_after_max)
+    m, x0 = sobol if sobol is not None else sobol_tables(2 * max(list(orders) + [1]))
+    P = np.ascontiguousarray(exp.P if P is None else P, dtype=np.complex128)
+    der0 = np.ascontiguousarray(exp.P_der0 if der0 is None else der0, dtype=np.complex128)
+    n_known = exp.P_n_known if n_known is None else n_known
+    val = np.zeros(exp.bt_size, np.complex128)
+    per = np.zeros((len(orders), exp.bt_size), np.complex128)
+    check(lib().orc_inchworm_step(ms.ref(), run.ref(), _p(m, C.c_uint32), _p(x0, C.c_uint32), int(n_known),
+                                  _p(P, C.c_double), _p(der0, C.c_double), int(jw), _p(val, C.c_double),
+                                  _p(per, C.c_double)))
+    return val, per
+
+
+def inchworm_step_bare(exp, orders_bare, N, sobol=None):
+    ms = exp.to_struct()
+    run = abi.RunStruct([], orders_bare, N)
+    m, x0 = sobol if sobol is not None else sobol_tables(2 * max(list(orders_bare) + [1]))
+    val = np.zeros(exp.bt_size, np.complex128)
+    per = np.zeros((len(orders_bare), exp.bt_size), np.complex128)
+    check(lib().orc_inchworm_step_bare(ms.ref(), run.ref(), _p(m, C.c_uint32), _p(x0, C.c_uint32),
+                                       _p(val, C.c_double), _p(per, C.c_double)))
+    return val, per
+
+
+def correlator_2p(exp, orders, N, P=None, n_known=None, der0=None, tau_first=0, tau_count=-1, sobol=None):
+    ms = exp.to_struct()
+    run = abi.RunStruct(orders, [], N)
+    m, x0 = sobol if sobol is not None else sobol_tables(2 * max(list(orders) + [1]))
+    P = np.ascontiguousarray(exp.P if P is None else P, dtype=np.complex128)
+    der0 = np.ascontiguousarray(exp.P_der0 if der0 is None else der0, dtype=np.complex128)
+    n_known = exp.P_n_known if n_known is None else n_known
+    out = np.zeros((len(exp.corr_operators), exp.grid.n_tau), np.complex128)
+    check(lib().orc_correlator_2p(ms.ref(), run.ref(), _p(m, C.c_uint32), _p(x0, C.c_uint32), int(n_known),
+                                  _p(P, C.c_double), _p(der0, C.c_double), int(tau_first), int(tau_count),
+                                  _p(out, C.c_double)))
+    return out
