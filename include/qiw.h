@@ -192,8 +192,9 @@ int qiw_simplex_points(qiw_ctx *ctx, int32_t d_before, int32_t d_after, double t
  * Statistics of the last qiw_inchworm / qiw_correlator_2p call on this context. */
 typedef struct {
     double wall_s;              /* host wall time of the call                             */
-    double kernel_ms;           /* summed device time of the evaluation kernels           */
-    int64_t n_kernel_launches;  /* evaluation kernel launches                             */
+    double kernel_ms;           /* summed device time of the evaluator launches (HIP events
+                                   on the launch stream around each launch)               */
+    int64_t n_kernel_launches;  /* evaluator launches                                     */
     int64_t diagram_evals;      /* (topology, sample) pairs evaluated                     */
     int64_t leaves;             /* (topology, sample, initial sector, pair assignment)
                                    leaves evaluated                                       */

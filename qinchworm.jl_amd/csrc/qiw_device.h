@@ -1,0 +1,131 @@
+// qiw_device.h -- data structures shared by the host driver (qiw_host.cpp) and the gfx950
+// kernels (qiw_kernels.hip) of libqiw.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#define QIW_KMAX 6                      // largest expansion order of a compiled program
+#define QIW_NMAX (2 * QIW_KMAX + 3)     // largest number of configuration positions
+#define QIW_WAVE 64
+
+// ---- program words (one 64-bit word per trie edge / control op) -------------------------
+// bits  0- 3 : level L (configuration position, 1-based)           [EDGE]
+// bit   4    : RESTORE  parent value from the stack slot of level L-1
+// bit   5    : SAVE     push the new value to the stack slot of level L
+// bit   6    : LEAF     acc += coef * value
+// bit   7    : TAIL     multiply the arc (hybridization) slot
+// bits  8-15 : P slot   ((L-2) * n_sectors + s_in)                  [EDGE]
+//              sector s0                                              [FLUSH]
+// bits 16-23 : W slot   (arc * n_tables + table)                     [EDGE]
+// bits 24-47 : coefficient index (LEAF) / topology index (TOPO)
+// bits 48-51 : op       0 EDGE, 1 TOPO, 2 FLUSH, 3 END
+enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3 };
+#define QIW_F_RESTORE (1ull << 4)
+#define QIW_F_SAVE (1ull << 5)
+#define QIW_F_LEAF (1ull << 6)
+#define QIW_F_TAIL (1ull << 7)
+
+// Kinds of integration group
+enum { QIW_G_BOLD = 0, QIW_G_BARE = 1, QIW_G_CORR = 2 };
+
+// One (order, n_pts_after) group of diagrams, compiled.
+struct QiwGroup {
+    int32_t kind;          // QIW_G_*
+    int32_t order;         // k
+    int32_t n_after;       // points after tau_w (bold/corr); 2k for bare
+    int32_t n_nodes;       // configuration positions
+    int32_t d_before, d_after;
+    int32_t pos_w;         // position of the inch / A operator node (0 for bare)
+    int32_t n_pslots, n_wslots;
+    int32_t prog_off;      // offset into the program word array
+    int32_t topo_off;      // offset into the arc table: [n_top][k] x (head, tail)
+    int32_t coef_off;      // offset into the coefficient array
+    int64_t ut_off;        // offset (in doubles) of this group's simplex coordinates
+    int32_t n_samples;     // qMC points (1 for order 0: exact evaluation)
+    int32_t sampled;       // 0 for order 0
+};
+
+// A unit of work of one launch: 64 samples of one group at one time configuration.
+struct QiwTask {
+    int32_t group;
+    int32_t first_sample;
+    int32_t tau_index;     // bold: n_w (tau_w = grid[n_w]); corr: index of tau_A; bare: 0
+    int32_t out_index;     // row of the per-task partial-sum output
+};
+
+// Interpolation tables of the hybridization functions, all stored as i*Delta (real).
+struct QiwDeltaTables {
+    int32_t n_tables;
+    int32_t n_tau;
+    double h, beta;
+    const int32_t *kind;   // [n_tables] QIW_GF_GRID_LINEAR / QIW_GF_CUBIC_SPLINE
+    const double *coef;    // [n_tables][n_tau-1][4] local cubic (linear: c2 = c3 = 0)
+};
+
+// Arguments of the diagram evaluator (k_eval_scalar)
+struct EvalArgs {
+    const QiwGroup *groups;
+    const QiwTask *tasks;
+    const uint64_t *prog;
+    const uint8_t *arcs;
+    const double *coefs;
+    const double *ut;          // simplex coordinates of all groups
+    const double *ptab;        // [n_tau][ns] i*P on the grid (bold groups)
+    const double *e_shift;     // [ns] E_s + lambda (bare groups, exact atomic P0)
+    QiwDeltaTables dtab;
+    int ns;
+    int n_tau;
+    double h, beta;
+    int lds_slots;             // slots per lane column
+    int step_w;                // tau_w index of a bold step
+    double *partial;           // [n_out][ns]
+};
+
+// Arguments of the step bookkeeping kernel (k_finalize_step)
+struct FinalizeArgs {
+    const QiwGroup *groups;
+    const int32_t *group_task_first;   // [n_groups] first out row
+    const int32_t *group_task_count;   // [n_groups]
+    const int32_t *group_order_slot;   // [n_groups] slot in the order-resolved output
+    int n_groups;
+    int n_order_slots;
+    const double *partial;             // [rows][bt]
+    int bt;                            // outputs per row (sum m^2)
+    const int32_t *diag_idx;           // [n_diag] BT indices of diagonal elements
+    int n_diag;
+    double *ptab;                      // [n_tau][bt] i*P
+    double *porders;                   // [n_order_slots][n_tau][bt] raw i*P contributions
+    double *lambda_out;                // [n_tau] per-step normalization constants
+    int n_tau;
+    int n_known;                       // rows rescaled by normalize! (n_tau for grid P)
+    double h;
+    int jf;                            // row receiving the step result
+    double tau_w, tau_f;               // step times (tau_i = 0)
+    int bare;                          // RootTransform Jacobian (initial step)
+    int normalize;                     // apply normalize!(P, tau_f)
+    double *scratch;                   // [n_order_slots][bt]
+};
+
+struct CorrFinalizeArgs {
+    const QiwGroup *groups;
+    const int32_t *row_task_first;   // [n_rows * n_groups]
+    const int32_t *row_task_count;
+    int n_groups;
+    int n_rows;                      // n_corr * n_tau
+    int n_tau;
+    const double *partial;           // [tasks][ns] trace contributions per initial sector
+    int ns;
+    double h, beta;
+    double *out;                     // [n_rows]
+};
+
+// Launch wrappers (qiw_kernels.hip)
+hipError_t qiw_launch_sobol_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int64_t first,
+                                   int64_t n, double *out);
+hipError_t qiw_launch_simplex_coords(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
+                                     int d_after, int64_t n, double *out);
+hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
+                                     int d_after, double t_i, double t_w, double t_f, int64_t n, double *out);
+hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks);
+hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
+hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);

@@ -1,0 +1,1181 @@
+// qiw_host.cpp -- the C ABI of libqiw (include/qiw.h): context, model staging, compilation
+// of configuration tries into device programs, and the inchworm!/correlator_2p drivers.
+//
+// The drivers keep all per-step state on the device: every inchworm step is one launch of
+// the evaluator over all (order, n_pts_after, topology, sample) work of the step followed by
+// one launch of the bookkeeping kernel (set_ppgf! + normalize!), so the n_tau-2 serial steps
+// run without a host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/qiw.h"
+#include "qiw_device.h"
+
+typedef std::complex<double> cd;
+
+// ---------------------------------------------------------------------------------------
+// Topologies (src/diagrammatics.jl:288-337)
+// ---------------------------------------------------------------------------------------
+namespace {
+
+struct Topology {
+    std::vector<std::pair<int, int>> pairs;
+    int parity;
+};
+
+void gen_topologies(std::vector<std::pair<int, int>> &partial, int parity, std::vector<int> unpaired,
+                    std::vector<Topology> &out) {
+    if (unpaired.empty()) {
+        out.push_back({partial, parity});
+        return;
+    }
+    for (size_t i = 2; i <= unpaired.size(); ++i) {
+        // pairing the first open point with the i-th adds a transposition per skipped point
+        std::vector<int> rest;
+        for (size_t q = 1; q < unpaired.size(); ++q)
+            if (q != i - 1) rest.push_back(unpaired[q]);
+        partial.push_back({unpaired[0], unpaired[i - 1]});
+        gen_topologies(partial, (i % 2 == 0) ? parity : -parity, rest, out);
+        partial.pop_back();
+    }
+}
+
+bool arcs_cross(std::pair<int, int> a, std::pair<int, int> b) {
+    if (a.first > a.second) std::swap(a.first, a.second);
+    if (b.first > b.second) std::swap(b.first, b.second);
+    return (a.first < b.first && b.first < a.second && a.second < b.second) ||
+           (b.first < a.first && a.first < b.second && b.second < a.second);
+}
+
+bool doubly_k_connected(const Topology &t, int k) {
+    auto straddles = [k](std::pair<int, int> p) {
+        return (p.first <= k && p.second > k) || (p.second <= k && p.first > k);
+    };
+    std::vector<std::pair<int, int>> stack, rest;
+    for (auto &p : t.pairs) (straddles(p) ? stack : rest).push_back(p);
+    while (!stack.empty()) {
+        auto c = stack.back();
+        stack.pop_back();
+        for (int i = (int)rest.size() - 1; i >= 0; --i)
+            if (arcs_cross(c, rest[i])) {
+                stack.push_back(rest[i]);
+                rest.erase(rest.begin() + i);
+            }
+    }
+    return rest.empty();
+}
+
+std::vector<Topology> topologies_at_order(int order, int k, bool external_arc) {
+    std::vector<Topology> all;
+    std::vector<std::pair<int, int>> partial;
+    std::vector<int> pts;
+    for (int i = 1; i <= 2 * order; ++i) pts.push_back(i);
+    gen_topologies(partial, 1, pts, all);
+    if (k < 0) return all;
+    std::vector<Topology> out;
+    for (auto &t : all)
+        if (doubly_k_connected(t, k)) out.push_back(t);
+    if (external_arc)
+        for (auto &t : out) t.parity *= (k % 2 == 0) ? 1 : -1;
+    return out;
+}
+
+// ---------------------------------------------------------------------------------------
+// Model in the representation the kernels use: i*P, i*Delta and the operator elements are
+// real for every model the kernels accept.
+// ---------------------------------------------------------------------------------------
+struct ScalarOp {  // one-dimensional-sector SectorBlockMatrix: s_i -> (s_f, value)
+    std::vector<int> sf;
+    std::vector<double> val;
+};
+
+struct HostModel {
+    double beta = 0, h = 0;
+    int ntau = 0, ns = 0;
+    std::vector<int> dims;
+    std::vector<std::vector<double>> E;
+    double lambda0 = 0;
+    int ppgf_kind = QIW_GF_GRID_LINEAR;
+    int np = 0, ncorr = 0;
+    std::vector<ScalarOp> op_i, op_f, corr_A, corr_B;
+    std::vector<int> pair_table;  // pair -> distinct hybridization table
+    int ntab = 0;
+    std::vector<double> dcoef;    // [ntab][ntau-1][4]
+    std::vector<int32_t> dkind;
+    std::vector<std::vector<int>> attachable;
+    int bt = 0;
+    std::vector<int> bt_off;
+    bool scalar = true;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct qiw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool has_model = false;
+    HostModel m;
+    // Sobol state
+    int sob_dim = 0;
+    std::vector<uint32_t> sob_m, sob_x;
+    // propagator state (host mirror of the device table), i*P in BT layout
+    std::vector<double> ip;  // [ntau][bt]
+    int n_known = 1;
+    std::vector<cd> der0;      // incremental-spline derivative dP/dtau at 0 (bookkeeping of normalize!)
+    std::vector<double> lambdas;
+    // device buffers
+    std::map<std::string, DevBuf> bufs;
+    qiw_stats stats{};
+};
+
+namespace {
+
+int fail(qiw_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(ctx, QIW_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void *dev(qiw_ctx *c, const std::string &name, size_t bytes, int *rc) {
+    DevBuf &b = c->bufs[name];
+    if (b.n < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        if (hipMalloc(&b.p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+            *rc = fail(c, QIW_E_DEVICE, "hipMalloc failed for " + name);
+            return nullptr;
+        }
+        b.n = std::max<size_t>(bytes, 256);
+    }
+    return b.p;
+}
+
+// natural cubic spline of uniformly spaced real values -> per-interval local cubic
+// coefficients (src/spline_gf.jl:194-202: BSpline(Cubic(Line(OnGrid()))) on the grid)
+void natural_spline_coef(const std::vector<double> &y, double h, double *c) {
+    int n = (int)y.size();
+    std::vector<double> M(n, 0.0);
+    if (n >= 3) {
+        int m = n - 2;
+        std::vector<double> cp(m), dp(m);
+        for (int i = 0; i < m; ++i) {
+            double rhs = 6.0 * (y[i + 2] - 2.0 * y[i + 1] + y[i]) / (h * h);
+            double den = 4.0 - (i > 0 ? cp[i - 1] : 0.0);
+            cp[i] = (i < m - 1 ? 1.0 : 0.0) / den;
+            dp[i] = (rhs - (i > 0 ? dp[i - 1] : 0.0)) / den;
+        }
+        for (int i = m - 1; i >= 0; --i) M[i + 1] = dp[i] - (i < m - 1 ? cp[i] * M[i + 2] : 0.0);
+    }
+    const double h26 = h * h / 6.0;
+    for (int j = 0; j < n - 1; ++j) {
+        c[4 * j + 0] = y[j];
+        c[4 * j + 1] = (y[j + 1] - y[j]) - h26 * (2.0 * M[j] + M[j + 1]);
+        c[4 * j + 2] = 3.0 * h26 * M[j];
+        c[4 * j + 3] = h26 * (M[j + 1] - M[j]);
+    }
+}
+
+int load_scalar_op(qiw_ctx *c, const qiw_sbm &in, const std::vector<int> &dims, ScalarOp &out, bool *scalar) {
+    int ns = (int)dims.size();
+    out.sf.assign(ns, -1);
+    out.val.assign(ns, 0.0);
+    size_t off = 0;
+    for (int b = 0; b < in.n_blocks; ++b) {
+        int si = in.s_i[b], sf = in.s_f[b];
+        if (si < 0 || si >= ns || sf < 0 || sf >= ns)
+            return fail(c, QIW_E_ARG, "operator block sector index out of range");
+        if (out.sf[si] != -1)
+            return fail(c, QIW_E_NOT_SBM,
+                        "Operator is not representable by a SectorBlockMatrix (more than one non-zero blocks per column)");
+        size_t sz = (size_t)dims[sf] * dims[si];
+        if (sz != 1) *scalar = false;
+        for (size_t e = 0; e < sz; ++e)
+            if (std::fabs(in.data[2 * (off + e) + 1]) > 1e-13 * (1.0 + std::fabs(in.data[2 * (off + e)])))
+                return fail(c, QIW_E_ARG, "complex operator matrix elements are not supported by the GPU kernels");
+        out.sf[si] = sf;
+        out.val[si] = in.data[2 * off];
+        off += sz;
+    }
+    return QIW_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Group compilation: the depth-first traversal of src/topology_eval.jl:454-556 for every
+// (topology, initial sector), recorded as a trie and emitted in preorder.
+// ---------------------------------------------------------------------------------------
+enum NodeKind { N_PAIR_HEAD, N_PAIR_TAIL, N_IDENTITY, N_OPERATOR };
+
+struct Compiled {
+    QiwGroup g{};
+    std::vector<uint64_t> prog;
+    std::vector<uint8_t> arcs;
+    std::vector<double> coefs;
+    int64_t leaves_per_sample = 0;
+    int64_t edges_per_sample = 0;
+    int n_top = 0;
+};
+
+struct TrieNode {
+    int level, pslot, wslot, parent;
+    bool tail, leaf;
+    double coef;
+    std::vector<int> children;
+};
+
+struct Compiler {
+    const HostModel &m;
+    int kind, order, n_after, corr;
+    int n = 0, pos_w = 0;
+    std::vector<int> conf_kind, conf_arc;
+    std::vector<int> selected;
+    std::vector<TrieNode> trie;
+    int s0 = 0, parity = 1;
+    std::string err;
+
+    Compiler(const HostModel &m_, int kind_, int order_, int n_after_, int corr_)
+        : m(m_), kind(kind_), order(order_), n_after(n_after_), corr(corr_) {}
+
+    int add(int parent, int level, int s_in, int wslot, bool tail) {
+        TrieNode t{level, (level - 2) * m.ns + s_in, wslot, parent, tail, false, 0.0, {}};
+        trie.push_back(t);
+        int id = (int)trie.size() - 1;
+        if (parent >= 0) trie[parent].children.push_back(id);
+        return id;
+    }
+
+    bool dfs(int node, int pos, int s, double c) {
+        if (pos > n) {
+            if (s != s0) {
+                err = "A block off-diagonal contribution to a pseudo-particle propagator detected, try passing "
+                      "more 'symmetry_breakers' when constructing the local exact diagonalization object";
+                return false;
+            }
+            trie[node].leaf = true;
+            trie[node].coef = c * parity * ((order % 2 == 0) ? 1.0 : -1.0);
+            return true;
+        }
+        int k = conf_kind[pos];
+        if (k == N_PAIR_HEAD) {
+            int a = conf_arc[pos];
+            for (int p : m.attachable[s]) {
+                selected[a] = p;
+                int id = add(node, pos, s, 0, false);
+                if (!dfs(id, pos + 1, m.op_i[p].sf[s], c * m.op_i[p].val[s])) return false;
+            }
+        } else if (k == N_PAIR_TAIL) {
+            int a = conf_arc[pos];
+            int p = selected[a];
+            if (m.op_f[p].sf[s] >= 0) {
+                int id = add(node, pos, s, a * m.ntab + m.pair_table[p], true);
+                if (!dfs(id, pos + 1, m.op_f[p].sf[s], c * m.op_f[p].val[s])) return false;
+            }
+        } else if (k == N_OPERATOR) {
+            const ScalarOp &A = m.corr_A[corr];
+            if (A.sf[s] >= 0) {
+                int id = add(node, pos, s, 0, false);
+                if (!dfs(id, pos + 1, A.sf[s], c * A.val[s])) return false;
+            }
+        } else {
+            int id = add(node, pos, s, 0, false);
+            if (!dfs(id, pos + 1, s, c)) return false;
+        }
+        return true;
+    }
+
+    void emit(int id, bool first_child, Compiled &out, std::map<double, int> &coef_idx) {
+        const TrieNode &t = trie[id];
+        uint64_t w = (uint64_t)t.level | ((uint64_t)t.pslot << 8) | ((uint64_t)t.wslot << 16);
+        if (!first_child) w |= QIW_F_RESTORE;
+        if (t.children.size() >= 2) w |= QIW_F_SAVE;
+        if (t.tail) w |= QIW_F_TAIL;
+        if (t.leaf) {
+            auto it = coef_idx.find(t.coef);
+            int ci;
+            if (it == coef_idx.end()) {
+                ci = (int)out.coefs.size();
+                out.coefs.push_back(t.coef);
+                coef_idx[t.coef] = ci;
+            } else {
+                ci = it->second;
+            }
+            w |= QIW_F_LEAF | ((uint64_t)ci << 24);
+            out.leaves_per_sample++;
+        }
+        out.edges_per_sample++;
+        out.prog.push_back(w);
+        for (size_t i = 0; i < t.children.size(); ++i) emit(t.children[i], i == 0, out, coef_idx);
+    }
+
+    bool compile(Compiled &out) {
+        n = (kind == QIW_G_BARE) ? 2 * order + 2 : 2 * order + 3;
+        int d_after = (kind == QIW_G_BARE) ? 2 * order : n_after;
+        int d_before = 2 * order - d_after;
+        pos_w = (kind == QIW_G_BARE) ? 0 : d_before + 2;
+        conf_kind.assign(n + 2, N_IDENTITY);
+        conf_arc.assign(n + 2, -1);
+        if (kind == QIW_G_CORR) {
+            conf_kind[1] = N_OPERATOR;  // B at tau = 0; handled at the root
+            conf_kind[pos_w] = N_OPERATOR;
+        }
+        std::vector<int> top_to_conf;  // topology point -> configuration position
+        for (int pos = n; pos >= 1; --pos) {
+            bool fixed = (pos == 1 || pos == n || pos == pos_w);
+            if (!fixed) top_to_conf.push_back(pos);
+        }
+        std::vector<Topology> tops = (kind == QIW_G_BARE)
+                                         ? topologies_at_order(order, -1, false)
+                                         : topologies_at_order(order, n_after, kind == QIW_G_CORR);
+        selected.assign(std::max(order, 1), -1);
+        std::map<double, int> coef_idx;
+        out.n_top = (int)tops.size();
+        for (int ti = 0; ti < (int)tops.size(); ++ti) {
+            const Topology &top = tops[ti];
+            parity = top.parity;
+            for (int a = 0; a < order; ++a) {
+                int head = top_to_conf[top.pairs[a].second - 1];
+                int tail = top_to_conf[top.pairs[a].first - 1];
+                conf_kind[head] = N_PAIR_HEAD;
+                conf_kind[tail] = N_PAIR_TAIL;
+                conf_arc[head] = conf_arc[tail] = a;
+                out.arcs.push_back((uint8_t)head);
+                out.arcs.push_back((uint8_t)tail);
+            }
+            if (order > 0) out.prog.push_back(((uint64_t)QIW_OP_TOPO << 48) | ((uint64_t)ti << 24));
+            for (s0 = 0; s0 < m.ns; ++s0) {
+                trie.clear();
+                // root: configuration position 1 (identity, or the B operator)
+                double c = 1.0;
+                int s = s0;
+                if (kind == QIW_G_CORR) {
+                    const ScalarOp &B = m.corr_B[corr];
+                    if (B.sf[s0] < 0) continue;
+                    c = B.val[s0];
+                    s = B.sf[s0];
+                }
+                TrieNode root{1, 0, 0, -1, false, false, 0.0, {}};
+                trie.push_back(root);
+                if (!dfs(0, 2, s, c)) return false;
+                if (trie[0].children.empty()) continue;
+                for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], false, out, coef_idx);
+                out.prog.push_back(((uint64_t)QIW_OP_FLUSH << 48) | ((uint64_t)s0 << 8));
+            }
+        }
+        out.prog.push_back((uint64_t)QIW_OP_END << 48);
+        QiwGroup &g = out.g;
+        g.kind = kind;
+        g.order = order;
+        g.n_after = d_after;
+        g.n_nodes = n;
+        g.d_before = d_before;
+        g.d_after = d_after;
+        g.pos_w = pos_w;
+        g.n_pslots = (n - 1) * m.ns;
+        g.n_wslots = order * m.ntab;
+        return true;
+    }
+};
+
+int lds_slots(const QiwGroup &g, int ns) { return g.n_pslots + g.n_wslots + g.n_nodes + 1 + ns; }
+
+// ---------------------------------------------------------------------------------------
+// A batch of compiled groups resident on the device
+// ---------------------------------------------------------------------------------------
+struct Batch {
+    std::vector<Compiled> groups;
+    std::vector<QiwTask> tasks;
+    std::vector<int32_t> g_first, g_count, g_slot;  // per group: task rows, order slot
+    int lds = 0;
+    double flops_per_launch = 0;  // algorithmic FLOPs of one launch
+    int64_t leaves_per_launch = 0, diagram_evals_per_launch = 0;
+};
+
+struct DevBatch {
+    QiwGroup *groups = nullptr;
+    QiwTask *tasks = nullptr;
+    uint64_t *prog = nullptr;
+    uint8_t *arcs = nullptr;
+    double *coefs = nullptr;
+    double *ut = nullptr;
+    int32_t *g_first = nullptr, *g_count = nullptr, *g_slot = nullptr;
+};
+
+int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d) {
+    int rc = 0;
+    std::vector<uint64_t> prog;
+    std::vector<uint8_t> arcs;
+    std::vector<double> coefs;
+    std::vector<QiwGroup> gs;
+    int64_t ut_total = 0;
+    for (auto &cg : b.groups) {
+        QiwGroup g = cg.g;
+        g.prog_off = (int32_t)prog.size();
+        g.topo_off = (int32_t)arcs.size();
+        g.coef_off = (int32_t)coefs.size();
+        g.sampled = g.order > 0;
+        g.n_samples = g.sampled ? (int32_t)N : 1;
+        g.ut_off = ut_total;
+        if (g.sampled) ut_total += (int64_t)(g.d_before + g.d_after) * N;
+        prog.insert(prog.end(), cg.prog.begin(), cg.prog.end());
+        arcs.insert(arcs.end(), cg.arcs.begin(), cg.arcs.end());
+        coefs.insert(coefs.end(), cg.coefs.begin(), cg.coefs.end());
+        cg.g = g;
+        gs.push_back(g);
+    }
+    if (arcs.empty()) arcs.push_back(0);
+    if (coefs.empty()) coefs.push_back(0.0);
+    d.groups = (QiwGroup *)dev(c, tag + ".groups", gs.size() * sizeof(QiwGroup), &rc);
+    d.prog = (uint64_t *)dev(c, tag + ".prog", prog.size() * 8, &rc);
+    d.arcs = (uint8_t *)dev(c, tag + ".arcs", arcs.size(), &rc);
+    d.coefs = (double *)dev(c, tag + ".coefs", coefs.size() * 8, &rc);
+    d.ut = (double *)dev(c, tag + ".ut", std::max<int64_t>(ut_total, 1) * 8, &rc);
+    d.tasks = (QiwTask *)dev(c, tag + ".tasks", b.tasks.size() * sizeof(QiwTask), &rc);
+    d.g_first = (int32_t *)dev(c, tag + ".gfirst", b.g_first.size() * 4 + 4, &rc);
+    d.g_count = (int32_t *)dev(c, tag + ".gcount", b.g_count.size() * 4 + 4, &rc);
+    d.g_slot = (int32_t *)dev(c, tag + ".gslot", b.g_slot.size() * 4 + 4, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(d.groups, gs.data(), gs.size() * sizeof(QiwGroup), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.prog, prog.data(), prog.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.arcs, arcs.data(), arcs.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.coefs, coefs.data(), coefs.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.tasks, b.tasks.data(), b.tasks.size() * sizeof(QiwTask), hipMemcpyHostToDevice));
+    if (!b.g_first.empty()) {
+        HIPCHK(c, hipMemcpy(d.g_first, b.g_first.data(), b.g_first.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.g_count, b.g_count.data(), b.g_count.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.g_slot, b.g_slot.data(), b.g_slot.size() * 4, hipMemcpyHostToDevice));
+    }
+    // simplex coordinates of every sampled group (step independent)
+    uint32_t *sm = (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc);
+    uint32_t *sx = (uint32_t *)dev(c, "sobol.x", c->sob_x.size() * 4, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(sm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(sx, c->sob_x.data(), c->sob_x.size() * 4, hipMemcpyHostToDevice));
+    for (auto &cg : b.groups) {
+        if (!cg.g.sampled) continue;
+        int dim = cg.g.d_before + cg.g.d_after;
+        if (dim > c->sob_dim) return fail(c, QIW_E_DIM, "Sobol dimension beyond the staged direction numbers");
+        HIPCHK(c, qiw_launch_simplex_coords(c->stream, sm, sx, cg.g.d_before, cg.g.d_after, N, d.ut + cg.g.ut_off));
+    }
+    return 0;
+}
+
+int check_run(qiw_ctx *c, const qiw_run *run) {
+    if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged (qiw_set_model)");
+    if (!run) return fail(c, QIW_E_ARG, "null run");
+    if (!(run->N_samples == 0 || (run->N_samples > 0 && (run->N_samples & (run->N_samples - 1)) == 0)))
+        return fail(c, QIW_E_NOT_POW2, "N_samples must be 0 or a power of 2");
+    if (run->N_samples > (int64_t)1 << 30) return fail(c, QIW_E_LIMIT, "N_samples beyond 2^30");
+    for (int i = 0; i < run->n_orders; ++i)
+        if (run->orders[i] < 0 || run->orders[i] > QIW_KMAX) return fail(c, QIW_E_LIMIT, "order beyond QIW_KMAX");
+    for (int i = 0; i < run->n_orders_bare; ++i)
+        if (run->orders_bare[i] < 0 || run->orders_bare[i] > QIW_KMAX)
+            return fail(c, QIW_E_LIMIT, "order beyond QIW_KMAX");
+    if (!c->m.scalar)
+        return fail(c, QIW_E_LIMIT, "sectors of dimension > 1 need the block kernels");
+    if (c->sob_dim == 0) return fail(c, QIW_E_STATE, "no Sobol direction numbers staged (qiw_set_sobol)");
+    return 0;
+}
+
+std::vector<int> orders_all(const qiw_run *run) {
+    std::vector<int> v(run->orders, run->orders + run->n_orders);
+    v.insert(v.end(), run->orders_bare, run->orders_bare + run->n_orders_bare);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
+
+// Build the groups of a bold step (src/inchworm.jl:422-452) or the bare step (:373-392).
+int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vector<int> &oall, Batch &b) {
+    const HostModel &m = c->m;
+    int n_o = bare ? run->n_orders_bare : run->n_orders;
+    const int32_t *ord = bare ? run->orders_bare : run->orders;
+    for (int oi = 0; oi < n_o; ++oi) {
+        int order = ord[oi];
+        if (order > 0 && run->N_samples <= 0) continue;
+        std::vector<int> nas;
+        if (bare)
+            nas.push_back(2 * order);
+        else if (order == 0)
+            nas.push_back(0);
+        else
+            for (int na = 1; na <= 2 * order - 1; ++na)
+                if (run->n_pts_after_max <= 0 || na <= run->n_pts_after_max) nas.push_back(na);
+        for (int na : nas) {
+            Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1);
+            Compiled cg;
+            if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
+            if (cg.n_top == 0) continue;
+            if (cg.g.n_pslots > 255 || cg.g.n_wslots > 255) return fail(c, QIW_E_LIMIT, "too many slots");
+            b.lds = std::max(b.lds, lds_slots(cg.g, m.ns));
+            int slot = (int)(std::find(oall.begin(), oall.end(), order) - oall.begin());
+            int64_t N = order > 0 ? run->N_samples : 1;
+            int gi = (int)b.groups.size();
+            b.g_first.push_back((int32_t)b.tasks.size());
+            int nt = (int)((N + QIW_WAVE - 1) / QIW_WAVE);
+            for (int t = 0; t < nt; ++t)
+                b.tasks.push_back({gi, t * QIW_WAVE, 0, (int32_t)b.tasks.size()});
+            b.g_count.push_back(nt);
+            b.g_slot.push_back(slot);
+            b.flops_per_launch += (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
+            b.leaves_per_launch += N * cg.leaves_per_sample;
+            b.diagram_evals_per_launch += N * cg.n_top;
+            b.groups.push_back(std::move(cg));
+        }
+    }
+    return 0;
+}
+
+int ensure_model_buffers(qiw_ctx *c, double **dcoef, int32_t **dkind, double **eshift) {
+    int rc = 0;
+    const HostModel &m = c->m;
+    *dcoef = (double *)dev(c, "delta.coef", std::max<size_t>(m.dcoef.size(), 1) * 8, &rc);
+    *dkind = (int32_t *)dev(c, "delta.kind", std::max<size_t>(m.dkind.size(), 1) * 4, &rc);
+    *eshift = (double *)dev(c, "eshift", m.ns * 8, &rc);
+    if (rc) return rc;
+    if (!m.dcoef.empty()) HIPCHK(c, hipMemcpy(*dcoef, m.dcoef.data(), m.dcoef.size() * 8, hipMemcpyHostToDevice));
+    if (!m.dkind.empty()) HIPCHK(c, hipMemcpy(*dkind, m.dkind.data(), m.dkind.size() * 4, hipMemcpyHostToDevice));
+    std::vector<double> es(m.ns);
+    for (int s = 0; s < m.ns; ++s) es[s] = m.E[s][0] + m.lambda0;
+    HIPCHK(c, hipMemcpy(*eshift, es.data(), m.ns * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+EvalArgs make_eval_args(qiw_ctx *c, const DevBatch &d, int lds, double *ptab, double *eshift, double *dcoef,
+                        int32_t *dkind, double *partial) {
+    const HostModel &m = c->m;
+    EvalArgs a{};
+    a.groups = d.groups;
+    a.tasks = d.tasks;
+    a.prog = d.prog;
+    a.arcs = d.arcs;
+    a.coefs = d.coefs;
+    a.ut = d.ut;
+    a.ptab = ptab;
+    a.e_shift = eshift;
+    a.dtab.n_tables = m.ntab;
+    a.dtab.n_tau = m.ntau;
+    a.dtab.h = m.h;
+    a.dtab.beta = m.beta;
+    a.dtab.kind = dkind;
+    a.dtab.coef = dcoef;
+    a.ns = m.ns;
+    a.n_tau = m.ntau;
+    a.h = m.h;
+    a.beta = m.beta;
+    a.lds_slots = lds;
+    a.step_w = 0;
+    a.partial = partial;
+    return a;
+}
+
+void atomic_ip(const HostModel &m, std::vector<double> &ip) {  // src/ppgf.jl:144-207
+    ip.assign((size_t)m.ntau * m.bt, 0.0);
+    for (int j = 0; j < m.ntau; ++j)
+        for (int s = 0; s < m.ns; ++s)
+            for (int i = 0; i < m.dims[s]; ++i)
+                ip[(size_t)j * m.bt + m.bt_off[s] + i + i * m.dims[s]] = std::exp(-(j * m.h) * (m.E[s][i] + m.lambda0));
+}
+
+void store_complex_from_ip(double ipv, double *out) {  // P = -i * (i P)
+    out[0] = 0.0;
+    out[1] = -ipv;
+}
+
+}  // namespace
+
+// =======================================================================================
+// C ABI
+// =======================================================================================
+extern "C" int qiw_abi_version(void) { return QIW_ABI_VERSION; }
+
+extern "C" int qiw_create(qiw_ctx **out, int device) {
+    if (!out) return QIW_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return QIW_E_NO_GPU;
+    qiw_ctx *c = new qiw_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return QIW_E_DEVICE;
+    }
+    *out = c;
+    return QIW_OK;
+}
+
+extern "C" void qiw_destroy(qiw_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char *qiw_last_error(const qiw_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
+    if (!c || !mi) return QIW_E_ARG;
+    if (mi->n_tau < 3 || mi->n_sectors < 1 || !(mi->beta > 0)) return fail(c, QIW_E_ARG, "bad grid or sectors");
+    HostModel m;
+    m.beta = mi->beta;
+    m.ntau = mi->n_tau;
+    m.h = m.beta / (m.ntau - 1);
+    m.ns = mi->n_sectors;
+    m.dims.assign(mi->sector_dim, mi->sector_dim + m.ns);
+    size_t off = 0;
+    double emin = 1e300;
+    for (int s = 0; s < m.ns; ++s) {
+        if (m.dims[s] < 1) return fail(c, QIW_E_ARG, "empty sector");
+        m.E.emplace_back(mi->energies + off, mi->energies + off + m.dims[s]);
+        off += m.dims[s];
+        for (double e : m.E.back()) emin = std::min(emin, e);
+        if (m.dims[s] != 1) m.scalar = false;
+    }
+    double zs = 0;
+    for (auto &v : m.E)
+        for (double e : v) zs += std::exp(-m.beta * (e - emin));
+    m.lambda0 = (std::log(zs) - m.beta * emin) / m.beta;  // log(Z)/beta, src/exact_atomic_ppgf.jl:130-135
+    m.ppgf_kind = mi->ppgf_kind;
+    if (m.ppgf_kind != QIW_GF_GRID_LINEAR && m.ppgf_kind != QIW_GF_INC_SPLINE)
+        return fail(c, QIW_E_ARG, "ppgf_kind must be QIW_GF_GRID_LINEAR or QIW_GF_INC_SPLINE");
+    m.np = mi->n_pairs;
+    m.op_i.resize(m.np);
+    m.op_f.resize(m.np);
+    m.pair_table.resize(m.np);
+    std::vector<std::vector<double>> tabs;
+    std::vector<int> tab_kind;
+    for (int p = 0; p < m.np; ++p) {
+        int rc = load_scalar_op(c, mi->pair_op_i[p], m.dims, m.op_i[p], &m.scalar);
+        if (rc) return rc;
+        if ((rc = load_scalar_op(c, mi->pair_op_f[p], m.dims, m.op_f[p], &m.scalar))) return rc;
+        // i * Delta_p on the grid must be real
+        std::vector<double> y(m.ntau);
+        for (int j = 0; j < m.ntau; ++j) {
+            double re = mi->pair_prop[2 * ((size_t)p * m.ntau + j)], im = mi->pair_prop[2 * ((size_t)p * m.ntau + j) + 1];
+            if (std::fabs(re) > 1e-13 * (1.0 + std::fabs(im)))
+                return fail(c, QIW_E_ARG, "i*Delta must be real for the GPU kernels");
+            y[j] = -im;
+        }
+        int kind = mi->pair_prop_kind[p];
+        if (kind != QIW_GF_GRID_LINEAR && kind != QIW_GF_CUBIC_SPLINE)
+            return fail(c, QIW_E_ARG, "pair propagator kind must be GRID_LINEAR or CUBIC_SPLINE");
+        int found = -1;
+        for (size_t t = 0; t < tabs.size(); ++t)
+            if (tab_kind[t] == kind && tabs[t] == y) found = (int)t;
+        if (found < 0) {
+            tabs.push_back(y);
+            tab_kind.push_back(kind);
+            found = (int)tabs.size() - 1;
+        }
+        m.pair_table[p] = found;
+    }
+    m.ntab = (int)tabs.size();
+    m.dcoef.assign((size_t)std::max(m.ntab, 1) * (m.ntau - 1) * 4, 0.0);
+    m.dkind = tab_kind;
+    for (int t = 0; t < m.ntab; ++t) {
+        double *cc = m.dcoef.data() + (size_t)t * (m.ntau - 1) * 4;
+        if (tab_kind[t] == QIW_GF_CUBIC_SPLINE) {
+            natural_spline_coef(tabs[t], m.h, cc);
+        } else {
+            for (int j = 0; j < m.ntau - 1; ++j) {
+                cc[4 * j] = tabs[t][j];
+                cc[4 * j + 1] = tabs[t][j + 1] - tabs[t][j];
+            }
+        }
+    }
+    m.ncorr = mi->n_corr;
+    m.corr_A.resize(m.ncorr);
+    m.corr_B.resize(m.ncorr);
+    for (int i = 0; i < m.ncorr; ++i) {
+        int rc = load_scalar_op(c, mi->corr_A[i], m.dims, m.corr_A[i], &m.scalar);
+        if (rc) return rc;
+        if ((rc = load_scalar_op(c, mi->corr_B[i], m.dims, m.corr_B[i], &m.scalar))) return rc;
+    }
+    m.attachable.assign(m.ns, {});  // src/expansion.jl:180-183
+    for (int s = 0; s < m.ns; ++s)
+        for (int p = 0; p < m.np; ++p)
+            if (m.op_i[p].sf[s] >= 0) m.attachable[s].push_back(p);
+    m.bt_off.resize(m.ns);
+    m.bt = 0;
+    for (int s = 0; s < m.ns; ++s) {
+        m.bt_off[s] = m.bt;
+        m.bt += m.dims[s] * m.dims[s];
+    }
+    c->m = m;
+    c->has_model = true;
+    atomic_ip(c->m, c->ip);
+    c->n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? 1 : m.ntau;
+    c->der0.assign(m.bt, cd(0.0));  // initial_ppgf_derivative: -i diag(-E - lambda), src/ppgf.jl:710-720
+    for (int s = 0; s < m.ns; ++s)
+        for (int i = 0; i < m.dims[s]; ++i) c->der0[m.bt_off[s] + i + i * m.dims[s]] = cd(0.0, m.E[s][i] + m.lambda0);
+    c->lambdas.assign(m.ntau, 0.0);
+    return QIW_OK;
+}
+
+extern "C" int qiw_set_sobol(qiw_ctx *c, int32_t dim, const uint32_t *m, const uint32_t *shift) {
+    if (!c || !m || dim < 1 || dim > 2 * QIW_KMAX) return c ? fail(c, QIW_E_DIM, "Sobol dimension out of range") : QIW_E_ARG;
+    c->sob_dim = dim;
+    c->sob_m.assign(m, m + (size_t)dim * 32);
+    c->sob_x.assign(dim, 0u);
+    if (shift) c->sob_x.assign(shift, shift + dim);
+    return QIW_OK;
+}
+
+extern "C" int qiw_set_ppgf(qiw_ctx *c, int32_t n_known, const double *P, const double *der0) {
+    if (!c || !P) return QIW_E_ARG;
+    if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
+    const HostModel &m = c->m;
+    if (n_known < 1 || n_known > m.ntau) return fail(c, QIW_E_ARG, "n_known out of range");
+    for (size_t i = 0; i < (size_t)m.ntau * m.bt; ++i) {
+        double re = P[2 * i], im = P[2 * i + 1];
+        if (std::fabs(re) > 1e-12 * (1.0 + std::fabs(im))) return fail(c, QIW_E_ARG, "i*P must be real for the GPU kernels");
+        c->ip[i] = -im;
+    }
+    c->n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? n_known : m.ntau;
+    if (der0)
+        for (int e = 0; e < m.bt; ++e) c->der0[e] = cd(der0[2 * e], der0[2 * e + 1]);
+    return QIW_OK;
+}
+
+extern "C" int qiw_get_ppgf(qiw_ctx *c, int32_t *n_known_out, double *P, double *der0) {
+    if (!c) return QIW_E_ARG;
+    if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
+    const HostModel &m = c->m;
+    if (n_known_out) *n_known_out = c->n_known;
+    if (P)
+        for (size_t i = 0; i < (size_t)m.ntau * m.bt; ++i) store_complex_from_ip(c->ip[i], P + 2 * i);
+    if (der0)
+        for (int e = 0; e < m.bt; ++e) {
+            der0[2 * e] = c->der0[e].real();
+            der0[2 * e + 1] = c->der0[e].imag();
+        }
+    return QIW_OK;
+}
+
+namespace {
+
+// Runs the initial step and then bold steps jw = first_w .. last_w on the device; the
+// device tables are initialized from the host mirror and copied back at the end.
+int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int last_w, bool normalize,
+                 bool store_host_P, double *P_orders_out, double *P_orders_std_out, std::vector<double> *step_value,
+                 std::vector<double> *step_orders) {
+    const HostModel &m = c->m;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> oall = orders_all(run);
+    if (step_orders) {  // single step: order slots follow run->orders
+        oall.assign(run->orders, run->orders + run->n_orders);
+    }
+    int n_slots = std::max<int>((int)oall.size(), 1);
+    int rc = 0;
+    Batch bare_b, bold_b;
+    DevBatch bare_d, bold_d;
+    if (do_bare && (rc = build_step_batch(c, run, true, oall, bare_b))) return rc;
+    if (last_w >= first_w && (rc = build_step_batch(c, run, false, oall, bold_b))) return rc;
+    if (!bare_b.groups.empty() && (rc = upload(c, "bare", bare_b, run->N_samples, bare_d))) return rc;
+    if (!bold_b.groups.empty() && (rc = upload(c, "bold", bold_b, run->N_samples, bold_d))) return rc;
+    double *dcoef, *eshift;
+    int32_t *dkind;
+    if ((rc = ensure_model_buffers(c, &dcoef, &dkind, &eshift))) return rc;
+    size_t rows = std::max(bare_b.tasks.size(), bold_b.tasks.size());
+    double *partial = (double *)dev(c, "partial", std::max<size_t>(rows, 1) * m.bt * 8, &rc);
+    double *ptab = (double *)dev(c, "ptab", (size_t)m.ntau * m.bt * 8, &rc);
+    double *porders = (double *)dev(c, "porders", (size_t)n_slots * m.ntau * m.bt * 8, &rc);
+    double *lam = (double *)dev(c, "lambda", (size_t)m.ntau * 8, &rc);
+    double *scratch = (double *)dev(c, "fin.scratch", (size_t)n_slots * m.bt * 8, &rc);
+    int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
+    if (rc) return rc;
+    std::vector<int32_t> dg;
+    for (int s = 0; s < m.ns; ++s)
+        for (int i = 0; i < m.dims[s]; ++i) dg.push_back(m.bt_off[s] + i + i * m.dims[s]);
+    HIPCHK(c, hipMemcpy(diag, dg.data(), dg.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemsetAsync(porders, 0, (size_t)n_slots * m.ntau * m.bt * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(lam, 0, (size_t)m.ntau * 8, c->stream));
+
+    FinalizeArgs fa{};
+    fa.partial = partial;
+    fa.bt = m.bt;
+    fa.diag_idx = diag;
+    fa.n_diag = (int)dg.size();
+    fa.ptab = ptab;
+    fa.porders = porders;
+    fa.lambda_out = lam;
+    fa.n_tau = m.ntau;
+    fa.h = m.h;
+    fa.n_order_slots = n_slots;
+    fa.scratch = scratch;
+
+    // every evaluator launch is bracketed by an event pair on the launch stream
+    std::vector<hipEvent_t> ev;
+    auto mark = [&](void) -> hipError_t {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        ev.push_back(e);
+        return hipEventRecord(e, c->stream);
+    };
+    int64_t launches = 0;
+    int n_known = c->n_known;
+    if (!bare_b.groups.empty()) {
+        EvalArgs a = make_eval_args(c, bare_d, bare_b.lds, ptab, eshift, dcoef, dkind, partial);
+        HIPCHK(c, mark());
+        HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, (int)bare_b.tasks.size()));
+        HIPCHK(c, mark());
+        fa.groups = bare_d.groups;
+        fa.group_task_first = bare_d.g_first;
+        fa.group_task_count = bare_d.g_count;
+        fa.group_order_slot = bare_d.g_slot;
+        fa.n_groups = (int)bare_b.groups.size();
+        fa.jf = 1;
+        fa.tau_w = 0.0;
+        fa.tau_f = m.h;
+        fa.bare = 1;
+        fa.normalize = 0;
+        fa.n_known = 0;
+        HIPCHK(c, qiw_launch_finalize_step(c->stream, fa));
+        ++launches;
+        if (m.ppgf_kind == QIW_GF_INC_SPLINE) n_known = 2;
+        c->stats.flops += bare_b.flops_per_launch;
+        c->stats.leaves += bare_b.leaves_per_launch;
+        c->stats.diagram_evals += bare_b.diagram_evals_per_launch;
+    }
+    if (!bold_b.groups.empty()) {
+        EvalArgs a = make_eval_args(c, bold_d, bold_b.lds, ptab, eshift, dcoef, dkind, partial);
+        fa.groups = bold_d.groups;
+        fa.group_task_first = bold_d.g_first;
+        fa.group_task_count = bold_d.g_count;
+        fa.group_order_slot = bold_d.g_slot;
+        fa.n_groups = (int)bold_b.groups.size();
+        fa.bare = 0;
+        for (int jw = first_w; jw <= last_w; ++jw) {
+            a.step_w = jw;
+            HIPCHK(c, mark());
+            HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, (int)bold_b.tasks.size()));
+            HIPCHK(c, mark());
+            fa.jf = jw + 1;
+            fa.tau_w = jw * m.h;
+            fa.tau_f = (jw + 1) * m.h;
+            if (m.ppgf_kind == QIW_GF_INC_SPLINE && normalize) n_known = std::max(n_known, jw + 2);
+            fa.n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? n_known : m.ntau;
+            fa.normalize = normalize ? 1 : 0;
+            HIPCHK(c, qiw_launch_finalize_step(c->stream, fa));
+            ++launches;
+            c->stats.flops += bold_b.flops_per_launch;
+            c->stats.leaves += bold_b.leaves_per_launch;
+            c->stats.diagram_evals += bold_b.diagram_evals_per_launch;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        c->stats.kernel_ms += ms;
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    c->stats.n_kernel_launches += launches;
+
+    std::vector<double> ptab_h((size_t)m.ntau * m.bt), por((size_t)n_slots * m.ntau * m.bt), lam_h(m.ntau);
+    HIPCHK(c, hipMemcpy(ptab_h.data(), ptab, ptab_h.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(por.data(), porders, por.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(lam_h.data(), lam, lam_h.size() * 8, hipMemcpyDeviceToHost));
+    if (store_host_P) {
+        c->ip = ptab_h;
+        c->n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? n_known : m.ntau;
+        c->lambdas = lam_h;
+    }
+    if (P_orders_out || P_orders_std_out) {
+        const double nan = std::nan("");
+        for (size_t o = 0; o < oall.size(); ++o) {
+            bool in_bare = std::find(run->orders_bare, run->orders_bare + run->n_orders_bare, oall[o]) !=
+                           run->orders_bare + run->n_orders_bare;
+            bool in_bold = std::find(run->orders, run->orders + run->n_orders, oall[o]) != run->orders + run->n_orders;
+            for (int j = 0; j < m.ntau; ++j)
+                for (int e = 0; e < m.bt; ++e) {
+                    size_t i = ((size_t)o * m.ntau + j) * m.bt + e;
+                    if (P_orders_out) store_complex_from_ip(por[i], P_orders_out + 2 * i);
+                    if (P_orders_std_out) {
+                        bool sampled_here = oall[o] > 0 && run->N_samples > 0 &&
+                                            ((j == 1 && in_bare && do_bare) || (j >= first_w + 1 && j <= last_w + 1 && in_bold));
+                        P_orders_std_out[2 * i] = sampled_here ? nan : 0.0;
+                        P_orders_std_out[2 * i + 1] = sampled_here ? nan : 0.0;
+                    }
+                }
+        }
+    }
+    if (step_value) {
+        int jf = do_bare ? 1 : first_w + 1;
+        step_value->assign(ptab_h.begin() + (size_t)jf * m.bt, ptab_h.begin() + (size_t)(jf + 1) * m.bt);
+        step_orders->clear();
+        for (int o = 0; o < n_slots; ++o)
+            step_orders->insert(step_orders->end(), por.begin() + ((size_t)o * m.ntau + jf) * m.bt,
+                                por.begin() + ((size_t)o * m.ntau + jf + 1) * m.bt);
+    }
+    c->stats.wall_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int qiw_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, double *P_orders_out,
+                            double *P_orders_std_out) {
+    if (!c) return QIW_E_ARG;
+    int rc = check_run(c, run);
+    if (rc) return rc;
+    (void)hipSetDevice(c->device);
+    c->stats = qiw_stats{};
+    atomic_ip(c->m, c->ip);  // inchworm! starts from the staged atomic propagator
+    c->n_known = (c->m.ppgf_kind == QIW_GF_INC_SPLINE) ? 1 : c->m.ntau;
+    for (int s = 0; s < c->m.ns; ++s)
+        for (int i = 0; i < c->m.dims[s]; ++i)
+            c->der0[c->m.bt_off[s] + i + i * c->m.dims[s]] = cd(0.0, c->m.E[s][i] + c->m.lambda0);
+    rc = run_inchworm(c, run, true, 1, c->m.ntau - 2, true, true, P_orders_out, P_orders_std_out, nullptr, nullptr);
+    if (rc) return rc;
+    if (c->m.ppgf_kind == QIW_GF_INC_SPLINE) {
+        // derivative at 0 of the incremental splines after every normalize!
+        // (src/ppgf.jl:687-688: der - i*lambda*P(0) with P(0) = -i on the diagonal)
+        for (int j = 2; j < c->m.ntau; ++j)
+            for (int s = 0; s < c->m.ns; ++s)
+                for (int i = 0; i < c->m.dims[s]; ++i) c->der0[c->m.bt_off[s] + i + i * c->m.dims[s]] -= c->lambdas[j];
+    }
+    if (P_out) qiw_get_ppgf(c, nullptr, P_out, nullptr);
+    return QIW_OK;
+}
+
+extern "C" int qiw_inchworm_step(qiw_ctx *c, const qiw_run *run, int32_t n_w, double *value_out, double *orders_out) {
+    if (!c) return QIW_E_ARG;
+    int rc = check_run(c, run);
+    if (rc) return rc;
+    if (n_w < 1 || n_w > c->m.ntau - 2) return fail(c, QIW_E_ARG, "n_w out of range");
+    (void)hipSetDevice(c->device);
+    c->stats = qiw_stats{};
+    std::vector<double> val, per;
+    rc = run_inchworm(c, run, false, n_w, n_w, false, false, nullptr, nullptr, &val, &per);
+    if (rc) return rc;
+    for (int e = 0; e < c->m.bt; ++e)
+        if (value_out) store_complex_from_ip(val[e], value_out + 2 * e);
+    if (orders_out)
+        for (size_t i = 0; i < per.size() && i < (size_t)run->n_orders * c->m.bt; ++i)
+            store_complex_from_ip(per[i], orders_out + 2 * i);
+    return QIW_OK;
+}
+
+extern "C" int qiw_inchworm_step_bare(qiw_ctx *c, const qiw_run *run, double *value_out, double *orders_out) {
+    if (!c) return QIW_E_ARG;
+    int rc = check_run(c, run);
+    if (rc) return rc;
+    (void)hipSetDevice(c->device);
+    c->stats = qiw_stats{};
+    qiw_run r2 = *run;
+    r2.orders = run->orders_bare;  // order slots follow orders_bare
+    r2.n_orders = run->n_orders_bare;
+    std::vector<double> val, per;
+    rc = run_inchworm(c, &r2, true, 1, 0, false, false, nullptr, nullptr, &val, &per);
+    if (rc) return rc;
+    for (int e = 0; e < c->m.bt; ++e)
+        if (value_out) store_complex_from_ip(val[e], value_out + 2 * e);
+    if (orders_out)
+        for (size_t i = 0; i < per.size() && i < (size_t)run->n_orders_bare * c->m.bt; ++i)
+            store_complex_from_ip(per[i], orders_out + 2 * i);
+    return QIW_OK;
+}
+
+extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_out) {
+    if (!c || !corr_out) return QIW_E_ARG;
+    int rc = check_run(c, run);
+    if (rc) return rc;
+    (void)hipSetDevice(c->device);
+    c->stats = qiw_stats{};
+    auto t0 = std::chrono::steady_clock::now();
+    const HostModel &m = c->m;
+    std::memset(corr_out, 0, sizeof(double) * 2 * (size_t)m.ncorr * m.ntau);
+    if (m.ncorr == 0 || run->n_orders == 0) return QIW_OK;
+    double Z = 0;  // partition_function(P) = sum_s tr iP_s(beta), src/ppgf.jl:611-617
+    for (int s = 0; s < m.ns; ++s)
+        for (int i = 0; i < m.dims[s]; ++i) Z += c->ip[(size_t)(m.ntau - 1) * m.bt + m.bt_off[s] + i + i * m.dims[s]];
+    double *dcoef, *eshift;
+    int32_t *dkind;
+    if ((rc = ensure_model_buffers(c, &dcoef, &dkind, &eshift))) return rc;
+    double *ptab = (double *)dev(c, "ptab", (size_t)m.ntau * m.bt * 8, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
+    for (int ci = 0; ci < m.ncorr; ++ci) {
+        // groups (src/inchworm.jl:957-981): every n_pts_after of every order
+        Batch b;
+        bool first_is_order0 = false;
+        for (int oi = 0; oi < run->n_orders; ++oi) {
+            int order = run->orders[oi];
+            if (order > 0 && run->N_samples <= 0) continue;
+            int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
+            for (int na = lo; na <= hi; ++na) {
+                Compiler comp(m, QIW_G_CORR, order, na, ci);
+                Compiled cg;
+                if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
+                if (cg.n_top == 0) continue;
+                if (b.groups.empty() && order == 0) first_is_order0 = true;
+                b.lds = std::max(b.lds, lds_slots(cg.g, m.ns));
+                b.groups.push_back(std::move(cg));
+            }
+        }
+        if (b.groups.empty()) continue;
+        int ng = (int)b.groups.size();
+        std::vector<int32_t> rf((size_t)m.ntau * ng, 0), rcnt((size_t)m.ntau * ng, 0);
+        for (int j = 0; j < m.ntau; ++j)
+            for (int gi = 0; gi < ng; ++gi) {
+                // tau_A = tau_B: only the leading order-0 group (src/inchworm.jl:1014-1024)
+                if (j == 0 && !(gi == 0 && first_is_order0)) continue;
+                int64_t N = b.groups[gi].g.order > 0 ? run->N_samples : 1;
+                int nt = (int)((N + QIW_WAVE - 1) / QIW_WAVE);
+                rf[(size_t)j * ng + gi] = (int32_t)b.tasks.size();
+                rcnt[(size_t)j * ng + gi] = nt;
+                for (int t = 0; t < nt; ++t) b.tasks.push_back({gi, t * QIW_WAVE, j, (int32_t)b.tasks.size()});
+                b.flops_per_launch += (double)N * (2.0 * b.groups[gi].edges_per_sample + 2.0 * b.groups[gi].leaves_per_sample);
+                b.leaves_per_launch += N * b.groups[gi].leaves_per_sample;
+                b.diagram_evals_per_launch += N * b.groups[gi].n_top;
+            }
+        DevBatch d;
+        if ((rc = upload(c, "corr", b, run->N_samples, d))) return rc;
+        double *partial = (double *)dev(c, "corr.partial", b.tasks.size() * m.ns * 8, &rc);
+        int32_t *drf = (int32_t *)dev(c, "corr.rf", rf.size() * 4, &rc);
+        int32_t *drc = (int32_t *)dev(c, "corr.rc", rcnt.size() * 4, &rc);
+        double *dout = (double *)dev(c, "corr.out", (size_t)m.ntau * 8, &rc);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpy(drf, rf.data(), rf.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(drc, rcnt.data(), rcnt.size() * 4, hipMemcpyHostToDevice));
+        hipEvent_t e0, e1;
+        HIPCHK(c, hipEventCreate(&e0));
+        HIPCHK(c, hipEventCreate(&e1));
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        EvalArgs a = make_eval_args(c, d, b.lds, ptab, eshift, dcoef, dkind, partial);
+        HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, (int)b.tasks.size()));
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        CorrFinalizeArgs fa{};
+        fa.groups = d.groups;
+        fa.row_task_first = drf;
+        fa.row_task_count = drc;
+        fa.n_groups = ng;
+        fa.n_rows = m.ntau;
+        fa.n_tau = m.ntau;
+        fa.partial = partial;
+        fa.ns = m.ns;
+        fa.h = m.h;
+        fa.beta = m.beta;
+        fa.out = dout;
+        HIPCHK(c, qiw_launch_finalize_corr(c->stream, fa));
+        std::vector<double> out(m.ntau);
+        HIPCHK(c, hipMemcpyAsync(out.data(), dout, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->stats.kernel_ms += ms;
+        c->stats.n_kernel_launches += 1;
+        c->stats.flops += b.flops_per_launch;
+        c->stats.leaves += b.leaves_per_launch;
+        c->stats.diagram_evals += b.diagram_evals_per_launch;
+        for (int j = 0; j < m.ntau; ++j) {  // value = -i * out / Z
+            corr_out[2 * ((size_t)ci * m.ntau + j)] = 0.0;
+            corr_out[2 * ((size_t)ci * m.ntau + j) + 1] = -out[j] / Z;
+        }
+    }
+    c->stats.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return QIW_OK;
+}
+
+extern "C" int qiw_partition_function(qiw_ctx *c, double *Z_out) {
+    if (!c || !Z_out) return QIW_E_ARG;
+    if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
+    const HostModel &m = c->m;
+    double Z = 0;
+    for (int s = 0; s < m.ns; ++s)
+        for (int i = 0; i < m.dims[s]; ++i) Z += c->ip[(size_t)(m.ntau - 1) * m.bt + m.bt_off[s] + i + i * m.dims[s]];
+    Z_out[0] = Z;
+    Z_out[1] = 0.0;
+    return QIW_OK;
+}
+
+extern "C" int qiw_density_matrix(qiw_ctx *c, double *rho_out) {
+    if (!c || !rho_out) return QIW_E_ARG;
+    double Z[2];
+    int rc = qiw_partition_function(c, Z);
+    if (rc) return rc;
+    const HostModel &m = c->m;
+    // normalize!(P, beta) then rho = i P(beta)  (src/ppgf.jl:629-636, :729-733)
+    double lam = std::log(Z[0]) / m.beta;
+    double f = std::exp(-m.beta * lam);
+    for (int e = 0; e < m.bt; ++e) {
+        rho_out[2 * e] = c->ip[(size_t)(m.ntau - 1) * m.bt + e] * f;
+        rho_out[2 * e + 1] = 0.0;
+    }
+    return QIW_OK;
+}
+
+extern "C" int qiw_sobol_points(qiw_ctx *c, int32_t dim, int64_t first, int64_t n, double *out) {
+    if (!c || !out || n < 0 || first < 0) return QIW_E_ARG;
+    if (dim < 1 || dim > c->sob_dim) return fail(c, QIW_E_DIM, "Sobol dimension beyond the staged direction numbers");
+    (void)hipSetDevice(c->device);
+    int rc = 0;
+    uint32_t *sm = (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc);
+    uint32_t *sx = (uint32_t *)dev(c, "sobol.x", c->sob_x.size() * 4, &rc);
+    double *o = (double *)dev(c, "sobol.out", (size_t)std::max<int64_t>(n, 1) * dim * 8, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(sm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(sx, c->sob_x.data(), c->sob_x.size() * 4, hipMemcpyHostToDevice));
+    if (n == 0) return QIW_OK;
+    HIPCHK(c, qiw_launch_sobol_points(c->stream, sm, sx, dim, first, n, o));
+    HIPCHK(c, hipMemcpyAsync(out, o, (size_t)n * dim * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QIW_OK;
+}
+
+extern "C" int qiw_simplex_points(qiw_ctx *c, int32_t d_before, int32_t d_after, double t_i, double t_w, double t_f,
+                                  int64_t n, double *out) {
+    if (!c || !out || n < 0 || d_before < 0 || d_after < 0) return QIW_E_ARG;
+    int d = d_before + d_after;
+    if (d < 1 || d > c->sob_dim) return fail(c, QIW_E_DIM, "Sobol dimension beyond the staged direction numbers");
+    (void)hipSetDevice(c->device);
+    int rc = 0;
+    uint32_t *sm = (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc);
+    uint32_t *sx = (uint32_t *)dev(c, "sobol.x", c->sob_x.size() * 4, &rc);
+    double *o = (double *)dev(c, "simplex.out", (size_t)std::max<int64_t>(n, 1) * d * 8, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(sm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(sx, c->sob_x.data(), c->sob_x.size() * 4, hipMemcpyHostToDevice));
+    if (n == 0) return QIW_OK;
+    HIPCHK(c, qiw_launch_simplex_points(c->stream, sm, sx, d_before, d_after, t_i, t_w, t_f, n, o));
+    HIPCHK(c, hipMemcpyAsync(out, o, (size_t)n * d * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QIW_OK;
+}
+
+extern "C" int qiw_get_stats(qiw_ctx *c, qiw_stats *out) {
+    if (!c || !out) return QIW_E_ARG;
+    *out = c->stats;
+    return QIW_OK;
+}

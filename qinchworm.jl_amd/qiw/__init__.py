@@ -1,0 +1,9 @@
+"""qiw -- MI355X (gfx950) quasi Monte Carlo diagram evaluation for the inchworm solver.
+
+Host-side Python mirror of QInchworm.jl's inchworm!/correlator_2p API over the C ABI of
+libqiw.so (include/qiw.h).  Setup (model construction, topologies, Sobol direction numbers)
+happens here; all sampling and diagram evaluation runs in the native library on the GPU.
+"""
+from . import abi, diagrammatics, model, sobol  # noqa: F401
+from .inchworm import (context, correlator_2p, density_matrix, inchworm, inchworm_step,  # noqa: F401
+                       inchworm_step_bare, normalize, partition_function, stats)

@@ -1,0 +1,143 @@
+"""Host-side mirror of the reference's drivers (src/inchworm.jl) over the libqiw C ABI.
+
+    inchworm(expansion, grid, orders, orders_bare, N_samples; n_pts_after_max)
+        -> (P_orders, P_orders_std), updates expansion.P          (inchworm!, :332-498)
+    inchworm_step(expansion, n_w, orders, N_samples)            (inchworm_step, :123-203)
+    inchworm_step_bare(expansion, orders_bare, N_samples)       (inchworm_step_bare, :228-304)
+    correlator_2p(expansion, grid, orders, N_samples)           (correlator_2p, :1075-1084)
+    partition_function(expansion), density_matrix(expansion)    (src/ppgf.jl:611-617, :729-733)
+
+All evaluation runs in libqiw on the GPU; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .sobol import ScrambledSobolSeq
+
+_ctx = None
+
+
+def context(device=0):
+    """The process-wide libqiw context (one GPU)."""
+    global _ctx
+    if _ctx is None:
+        _ctx = abi.Context(device)
+    return _ctx
+
+
+def _stage(ctx, expansion, max_order, seq=None):
+    ms = expansion.to_struct()
+    ctx.check(ctx.lib.qiw_set_model(ctx.h, ms.ref()))
+    D = max(2 * max_order, 1)
+    seq = seq or ScrambledSobolSeq(D)
+    if seq.D < D:
+        raise ValueError("Sobol sequence has too few dimensions")
+    m = np.ascontiguousarray(seq.m[:D])
+    x0 = np.ascontiguousarray(seq.x0[:D])
+    ctx.check(ctx.lib.qiw_set_sobol(ctx.h, D, abi.ptr(m, C.c_uint32), abi.ptr(x0, C.c_uint32)))
+    return ms
+
+
+def _set_P(ctx, expansion):
+    P = np.ascontiguousarray(expansion.P, dtype=np.complex128)
+    d0 = np.ascontiguousarray(expansion.P_der0, dtype=np.complex128)
+    ctx.check(ctx.lib.qiw_set_ppgf(ctx.h, int(expansion.P_n_known), P.ctypes.data_as(abi._f64p),
+                                   d0.ctypes.data_as(abi._f64p)))
+
+
+def _get_P(ctx, expansion):
+    P = np.zeros_like(expansion.P)
+    d0 = np.zeros(expansion.bt_size, np.complex128)
+    nk = C.c_int32()
+    ctx.check(ctx.lib.qiw_get_ppgf(ctx.h, C.byref(nk), P.ctypes.data_as(abi._f64p), d0.ctypes.data_as(abi._f64p)))
+    expansion.P, expansion.P_der0, expansion.P_n_known = P, d0, nk.value
+
+
+def stats(ctx=None):
+    ctx = ctx or context()
+    st = abi.qiw_stats()
+    ctx.check(ctx.lib.qiw_get_stats(ctx.h, C.byref(st)))
+    return {f: getattr(st, f) for f, _ in abi.qiw_stats._fields_}
+
+
+def inchworm(expansion, grid, orders, orders_bare, N_samples, n_pts_after_max=None,
+             precision=abi.QIW_PREC_FP64, seq=None, ctx=None):
+    """inchworm! (src/inchworm.jl:332-498): writes the bold propagator into expansion.P and
+    returns the order-resolved contributions and their standard deviations as
+    {order: array (n_tau, sum m^2)}."""
+    ctx = ctx or context()
+    orders, orders_bare = list(orders), list(orders_bare)
+    _stage(ctx, expansion, max(orders + orders_bare + [0]), seq)
+    run = abi.RunStruct(orders, orders_bare, N_samples, n_pts_after_max or 0, precision)
+    oall = sorted(set(orders) | set(orders_bare))
+    nt, bt = grid.n_tau, expansion.bt_size
+    Po = np.zeros((len(oall), nt, bt), np.complex128)
+    Ps = np.zeros((len(oall), nt, bt), np.complex128)
+    P = np.zeros((nt, bt), np.complex128)
+    ctx.check(ctx.lib.qiw_inchworm(ctx.h, run.ref(), P.ctypes.data_as(abi._f64p), Po.ctypes.data_as(abi._f64p),
+                                   Ps.ctypes.data_as(abi._f64p)))
+    _get_P(ctx, expansion)
+    return {o: Po[i] for i, o in enumerate(oall)}, {o: Ps[i] for i, o in enumerate(oall)}
+
+
+def inchworm_step(expansion, n_w, orders, N_samples, n_pts_after_max=None, seq=None, ctx=None):
+    """One regular step (src/inchworm.jl:123-203) with tau_i = grid[0], tau_w = grid[n_w],
+    tau_f = grid[n_w + 1] on the current expansion.P.  Returns (value, per-order)."""
+    ctx = ctx or context()
+    orders = list(orders)
+    _stage(ctx, expansion, max(orders + [0]), seq)
+    _set_P(ctx, expansion)
+    run = abi.RunStruct(orders, [], N_samples, n_pts_after_max or 0)
+    val = np.zeros(expansion.bt_size, np.complex128)
+    per = np.zeros((len(orders), expansion.bt_size), np.complex128)
+    ctx.check(ctx.lib.qiw_inchworm_step(ctx.h, run.ref(), int(n_w), val.ctypes.data_as(abi._f64p),
+                                        per.ctypes.data_as(abi._f64p)))
+    return val, per
+
+
+def inchworm_step_bare(expansion, orders_bare, N_samples, seq=None, ctx=None):
+    """The initial step (src/inchworm.jl:228-304) on [grid[0], grid[1]]."""
+    ctx = ctx or context()
+    orders_bare = list(orders_bare)
+    _stage(ctx, expansion, max(orders_bare + [0]), seq)
+    run = abi.RunStruct([], orders_bare, N_samples)
+    val = np.zeros(expansion.bt_size, np.complex128)
+    per = np.zeros((len(orders_bare), expansion.bt_size), np.complex128)
+    ctx.check(ctx.lib.qiw_inchworm_step_bare(ctx.h, run.ref(), val.ctypes.data_as(abi._f64p),
+                                             per.ctypes.data_as(abi._f64p)))
+    return val, per
+
+
+def correlator_2p(expansion, grid, orders, N_samples, seq=None, ctx=None):
+    """correlator_2p (src/inchworm.jl:1075-1084): one array of n_tau complex values
+    <A(tau) B(0)> per (A, B) pair of expansion.corr_operators."""
+    ctx = ctx or context()
+    orders = list(orders)
+    _stage(ctx, expansion, max(orders + [0]), seq)
+    _set_P(ctx, expansion)
+    run = abi.RunStruct(orders, [], N_samples)
+    out = np.zeros((len(expansion.corr_operators), grid.n_tau), np.complex128)
+    ctx.check(ctx.lib.qiw_correlator_2p(ctx.h, run.ref(), out.ctypes.data_as(abi._f64p)))
+    return [out[i] for i in range(out.shape[0])]
+
+
+def partition_function(expansion):
+    """Z = sum_s tr i P_s(beta) (src/ppgf.jl:611-617)."""
+    row = expansion.P[-1]
+    return sum(1j * np.trace(expansion.block(row, s)) for s in range(len(expansion.ed.dims)))
+
+
+def normalize(expansion, beta):
+    """normalize!(P, beta) (src/ppgf.jl:629-636): rescale so that Z = 1; returns lambda."""
+    Z = partition_function(expansion)
+    lam = np.log(Z) / beta
+    expansion.P = expansion.P * np.exp(-expansion.grid.tau * lam)[:, None]
+    return lam
+
+
+def density_matrix(expansion):
+    """rho = i P(beta) of a normalized propagator (src/ppgf.jl:729-733)."""
+    row = expansion.P[-1]
+    return [1j * expansion.block(row, s) for s in range(len(expansion.ed.dims))]

@@ -1,0 +1,40 @@
+"""The C ABI library loads and exports every entry point include/qiw.h declares (CPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from qiw import abi
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    hdr = open(os.path.join(REPO, "include", "qiw.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?(qiw_\w+)\s*\(", hdr, re.M)))
+
+
+def test_header_declarations_are_exported():
+    lib = abi.load_library()
+    decl = declared_symbols()
+    assert len(decl) >= 17
+    assert sorted(abi.EXPORTS) == decl
+    for name in decl:
+        assert hasattr(lib, name), name
+
+
+def test_abi_version_and_no_gpu_error():
+    lib = abi.load_library()
+    assert lib.qiw_abi_version() == 1
+    if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
+        h = C.c_void_p()
+        assert lib.qiw_create(C.byref(h), 0) == abi.QIW_E_NO_GPU
+
+
+def test_struct_layout_matches_header():
+    # offsets a C caller (or Julia's ccall with an isbits struct) relies on
+    assert C.sizeof(abi.qiw_sbm) == 32
+    assert abi.qiw_model.n_pairs.offset == 36 and abi.qiw_model.pair_op_i.offset == 40
+    assert abi.qiw_run.N_samples.offset == 32 and C.sizeof(abi.qiw_run) == 48
+    assert C.sizeof(abi.qiw_stats) == 48
