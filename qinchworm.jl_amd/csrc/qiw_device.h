@@ -8,20 +8,18 @@
 #define QIW_NMAX (2 * QIW_KMAX + 3)     // largest number of configuration positions
 #define QIW_WAVE 64
 
-// ---- program words (one 64-bit word per trie edge / control op) -------------------------
-// bits  0- 3 : level L (configuration position, 1-based)           [EDGE]
-// bit   4    : RESTORE  parent value from the stack slot of level L-1
-// bit   5    : SAVE     push the new value to the stack slot of level L
-// bit   6    : LEAF     acc += coef * value
-// bit   7    : TAIL     multiply the arc (hybridization) slot
-// bits  8-15 : P slot   ((L-2) * n_sectors + s_in)                  [EDGE]
-//              sector s0                                              [FLUSH]
-// bits 16-23 : W slot   (arc * n_tables + table)                     [EDGE]
-// bits 24-47 : coefficient index (LEAF) / topology index (TOPO)
-// bits 48-51 : op       0 EDGE, 1 TOPO, 2 FLUSH, 3 END
-enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3 };
-#define QIW_F_RESTORE (1ull << 4)
-#define QIW_F_SAVE (1ull << 5)
+// ---- program words ----------------------------------------------------------------------
+// Scalar programs (k_eval_scalar_t, all sectors 1x1), one word per trie node in preorder:
+// ROOT   bits  8-15 : sector after position 1, bits 24-31 : number of level-2 children;
+//        the next word is the constant of position 1 (float64 bits)
+// EDGE   bit   6    : LEAF (last position): the next word is the leaf coefficient (float64)
+//        bit   7    : TAIL  multiply the arc value in W slot bits 16-23 (arc * n_tables + table)
+//        bits  8-15 : sector after this node's operator (propagator sector of its children)
+//        bits 24-31 : number of children
+// TOPO   bits 8a..8a+3 / 8a+4..8a+7 : head / tail position of arc a (a < order)
+// FLUSH  bits  8-15 : initial sector s0 (out[s0] += acc; acc = 0)
+// all    bits 48-51 : op 0 EDGE, 1 TOPO, 2 FLUSH, 3 END, 4 NOP, 5 ROOT
+enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3, QIW_OP_NOP = 4, QIW_OP_ROOT = 5 };
 #define QIW_F_LEAF (1ull << 6)
 #define QIW_F_TAIL (1ull << 7)
 
@@ -76,8 +74,9 @@ struct EvalArgs {
     int ns;
     int n_tau;
     double h, beta;
-    int lds_slots;             // slots per lane column
+    int lds_slots;             // LDS doubles per wavefront
     int step_w;                // tau_w index of a bold step
+    int max_order;             // largest order of the launch's groups
     double *partial;           // [n_out][ns]
 };
 
@@ -104,6 +103,7 @@ struct FinalizeArgs {
     int bare;                          // RootTransform Jacobian (initial step)
     int normalize;                     // apply normalize!(P, tau_f)
     double *scratch;                   // [n_order_slots][bt]
+    double *gsum;                      // [n_groups][bt] per-group sums of the partial rows
 };
 
 struct CorrFinalizeArgs {
@@ -113,10 +113,22 @@ struct CorrFinalizeArgs {
     int n_groups;
     int n_rows;                      // n_corr * n_tau
     int n_tau;
-    const double *partial;           // [tasks][ns] trace contributions per initial sector
-    int ns;
+    const double *partial;           // [tasks][bt] per-task block sums
+    int bt;
+    const int32_t *diag_idx;         // BT indices of the diagonal elements (the trace)
+    int n_diag;
     double h, beta;
     double *out;                     // [n_rows]
+};
+
+// Arguments of the block-sector evaluator (k_eval_block)
+struct BlockArgs {
+    EvalArgs e;
+    const double *op_data;     // operator blocks, column-major
+    const int32_t *op_desc;    // [n_blocks][3] offset, m_f, m_i
+    const int32_t *dims;       // [ns]
+    const int32_t *bt_off;     // [ns]
+    int bt, mmax;
 };
 
 // Launch wrappers (qiw_kernels.hip)
@@ -126,6 +138,7 @@ hipError_t qiw_launch_simplex_coords(hipStream_t st, const uint32_t *m, const ui
                                      int d_after, int64_t n, double *out);
 hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
                                      int d_after, double t_i, double t_w, double t_f, int64_t n, double *out);
-hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks);
+hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
+hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes);
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);

@@ -93,9 +93,10 @@ std::vector<Topology> topologies_at_order(int order, int k, bool external_arc) {
 // Model in the representation the kernels use: i*P, i*Delta and the operator elements are
 // real for every model the kernels accept.
 // ---------------------------------------------------------------------------------------
-struct ScalarOp {  // one-dimensional-sector SectorBlockMatrix: s_i -> (s_f, value)
+struct ScalarOp {  // SectorBlockMatrix: s_i -> (s_f, value of a 1x1 block, block id)
     std::vector<int> sf;
     std::vector<double> val;
+    std::vector<int> blk;   // index into HostModel::op_blocks
 };
 
 struct HostModel {
@@ -114,7 +115,11 @@ struct HostModel {
     std::vector<std::vector<int>> attachable;
     int bt = 0;
     std::vector<int> bt_off;
-    bool scalar = true;
+    bool scalar = true;          // every sector is one-dimensional
+    bool scalar_kernel = true;   // programs for k_eval_scalar_t (scalar model within its register budget)
+    int mmax = 1;
+    std::vector<double> op_data;    // all operator blocks, column-major, concatenated
+    std::vector<int32_t> op_desc;   // [n_blocks][3]: offset, m_f, m_i
 };
 
 }  // namespace
@@ -200,10 +205,12 @@ void natural_spline_coef(const std::vector<double> &y, double h, double *c) {
     }
 }
 
-int load_scalar_op(qiw_ctx *c, const qiw_sbm &in, const std::vector<int> &dims, ScalarOp &out, bool *scalar) {
+int load_scalar_op(qiw_ctx *c, const qiw_sbm &in, const std::vector<int> &dims, ScalarOp &out, bool *scalar,
+                   std::vector<double> &op_data, std::vector<int32_t> &op_desc) {
     int ns = (int)dims.size();
     out.sf.assign(ns, -1);
     out.val.assign(ns, 0.0);
+    out.blk.assign(ns, -1);
     size_t off = 0;
     for (int b = 0; b < in.n_blocks; ++b) {
         int si = in.s_i[b], sf = in.s_f[b];
@@ -219,6 +226,11 @@ int load_scalar_op(qiw_ctx *c, const qiw_sbm &in, const std::vector<int> &dims, 
                 return fail(c, QIW_E_ARG, "complex operator matrix elements are not supported by the GPU kernels");
         out.sf[si] = sf;
         out.val[si] = in.data[2 * off];
+        out.blk[si] = (int)op_desc.size() / 3;
+        op_desc.push_back((int32_t)op_data.size());
+        op_desc.push_back(dims[sf]);
+        op_desc.push_back(dims[si]);
+        for (size_t e = 0; e < sz; ++e) op_data.push_back(in.data[2 * (off + e)]);
         off += sz;
     }
     return QIW_OK;
@@ -245,6 +257,7 @@ struct TrieNode {
     bool tail, leaf;
     double coef;
     std::vector<int> children;
+    int s_in = 0, s_out = 0, blk = 0xFFFF;  // block programs: sectors and operator block
 };
 
 struct Compiler {
@@ -260,8 +273,8 @@ struct Compiler {
     Compiler(const HostModel &m_, int kind_, int order_, int n_after_, int corr_)
         : m(m_), kind(kind_), order(order_), n_after(n_after_), corr(corr_) {}
 
-    int add(int parent, int level, int s_in, int wslot, bool tail) {
-        TrieNode t{level, (level - 2) * m.ns + s_in, wslot, parent, tail, false, 0.0, {}};
+    int add(int parent, int level, int s_in, int wslot, bool tail, int s_out = 0, int blk = 0xFFFF) {
+        TrieNode t{level, (level - 2) * m.ns + s_in, wslot, parent, tail, false, 0.0, {}, s_in, s_out, blk};
         trie.push_back(t);
         int id = (int)trie.size() - 1;
         if (parent >= 0) trie[parent].children.push_back(id);
@@ -280,55 +293,55 @@ struct Compiler {
             return true;
         }
         int k = conf_kind[pos];
+        // block programs keep the operator blocks as matrices: no constant folding
+        const bool fold = m.scalar_kernel;
         if (k == N_PAIR_HEAD) {
             int a = conf_arc[pos];
             for (int p : m.attachable[s]) {
                 selected[a] = p;
-                int id = add(node, pos, s, 0, false);
-                if (!dfs(id, pos + 1, m.op_i[p].sf[s], c * m.op_i[p].val[s])) return false;
+                int id = add(node, pos, s, 0, false, m.op_i[p].sf[s], m.op_i[p].blk[s]);
+                if (!dfs(id, pos + 1, m.op_i[p].sf[s], fold ? c * m.op_i[p].val[s] : c)) return false;
             }
         } else if (k == N_PAIR_TAIL) {
             int a = conf_arc[pos];
             int p = selected[a];
             if (m.op_f[p].sf[s] >= 0) {
-                int id = add(node, pos, s, a * m.ntab + m.pair_table[p], true);
-                if (!dfs(id, pos + 1, m.op_f[p].sf[s], c * m.op_f[p].val[s])) return false;
+                int id = add(node, pos, s, a * m.ntab + m.pair_table[p], true, m.op_f[p].sf[s], m.op_f[p].blk[s]);
+                if (!dfs(id, pos + 1, m.op_f[p].sf[s], fold ? c * m.op_f[p].val[s] : c)) return false;
             }
         } else if (k == N_OPERATOR) {
             const ScalarOp &A = m.corr_A[corr];
             if (A.sf[s] >= 0) {
-                int id = add(node, pos, s, 0, false);
-                if (!dfs(id, pos + 1, A.sf[s], c * A.val[s])) return false;
+                int id = add(node, pos, s, 0, false, A.sf[s], A.blk[s]);
+                if (!dfs(id, pos + 1, A.sf[s], fold ? c * A.val[s] : c)) return false;
             }
         } else {
-            int id = add(node, pos, s, 0, false);
+            int id = add(node, pos, s, 0, false, s, 0xFFFF);
             if (!dfs(id, pos + 1, s, c)) return false;
         }
         return true;
     }
 
-    void emit(int id, bool first_child, Compiled &out, std::map<double, int> &coef_idx) {
+    void emit(int id, Compiled &out) {
         const TrieNode &t = trie[id];
-        uint64_t w = (uint64_t)t.level | ((uint64_t)t.pslot << 8) | ((uint64_t)t.wslot << 16);
-        if (!first_child) w |= QIW_F_RESTORE;
-        if (t.children.size() >= 2) w |= QIW_F_SAVE;
-        if (t.tail) w |= QIW_F_TAIL;
-        if (t.leaf) {
-            auto it = coef_idx.find(t.coef);
-            int ci;
-            if (it == coef_idx.end()) {
-                ci = (int)out.coefs.size();
-                out.coefs.push_back(t.coef);
-                coef_idx[t.coef] = ci;
-            } else {
-                ci = it->second;
-            }
-            w |= QIW_F_LEAF | ((uint64_t)ci << 24);
-            out.leaves_per_sample++;
+        uint64_t w;
+        if (m.scalar_kernel) {  // structured walk: sector after the node, child count
+            w = ((uint64_t)t.s_out << 8) | ((uint64_t)t.wslot << 16) | ((uint64_t)t.children.size() << 24);
+        } else {
+            w = (uint64_t)t.level | ((uint64_t)t.s_in << 8) | ((uint64_t)t.wslot << 16) |
+                ((uint64_t)(t.blk & 0xFFFF) << 24) | ((uint64_t)t.s_out << 40);
         }
+        if (t.tail) w |= QIW_F_TAIL;
+        if (t.leaf) w |= QIW_F_LEAF;
         out.edges_per_sample++;
         out.prog.push_back(w);
-        for (size_t i = 0; i < t.children.size(); ++i) emit(t.children[i], i == 0, out, coef_idx);
+        if (t.leaf) {  // the leaf coefficient follows as a raw float64 word
+            uint64_t cw;
+            std::memcpy(&cw, &t.coef, 8);
+            out.prog.push_back(cw);
+            out.leaves_per_sample++;
+        }
+        for (size_t i = 0; i < t.children.size(); ++i) emit(t.children[i], out);
     }
 
     bool compile(Compiled &out) {
@@ -351,11 +364,11 @@ struct Compiler {
                                          ? topologies_at_order(order, -1, false)
                                          : topologies_at_order(order, n_after, kind == QIW_G_CORR);
         selected.assign(std::max(order, 1), -1);
-        std::map<double, int> coef_idx;
         out.n_top = (int)tops.size();
         for (int ti = 0; ti < (int)tops.size(); ++ti) {
             const Topology &top = tops[ti];
             parity = top.parity;
+            uint64_t topo_word = (uint64_t)QIW_OP_TOPO << 48;
             for (int a = 0; a < order; ++a) {
                 int head = top_to_conf[top.pairs[a].second - 1];
                 int tail = top_to_conf[top.pairs[a].first - 1];
@@ -364,24 +377,41 @@ struct Compiler {
                 conf_arc[head] = conf_arc[tail] = a;
                 out.arcs.push_back((uint8_t)head);
                 out.arcs.push_back((uint8_t)tail);
+                topo_word |= ((uint64_t)head << (8 * a)) | ((uint64_t)tail << (8 * a + 4));
             }
-            if (order > 0) out.prog.push_back(((uint64_t)QIW_OP_TOPO << 48) | ((uint64_t)ti << 24));
+            if (order > 0) out.prog.push_back(topo_word);
             for (s0 = 0; s0 < m.ns; ++s0) {
                 trie.clear();
                 // root: configuration position 1 (identity, or the B operator)
                 double c = 1.0;
-                int s = s0;
+                int s = s0, root_blk = 0xFFFF;
                 if (kind == QIW_G_CORR) {
                     const ScalarOp &B = m.corr_B[corr];
                     if (B.sf[s0] < 0) continue;
-                    c = B.val[s0];
+                    c = m.scalar ? B.val[s0] : 1.0;
                     s = B.sf[s0];
+                    root_blk = B.blk[s0];
                 }
                 TrieNode root{1, 0, 0, -1, false, false, 0.0, {}};
                 trie.push_back(root);
                 if (!dfs(0, 2, s, c)) return false;
                 if (trie[0].children.empty()) continue;
-                for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], false, out, coef_idx);
+                if (trie[0].children.size() > 255) {
+                    err = "too many pair interactions attachable to one sector";
+                    return false;
+                }
+                if (m.scalar_kernel) {
+                    out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | ((uint64_t)s << 8) |
+                                       ((uint64_t)trie[0].children.size() << 24));
+                    const double one = 1.0;  // constants of position 1 are folded into the leaves
+                    uint64_t cw;
+                    std::memcpy(&cw, &one, 8);
+                    out.prog.push_back(cw);
+                } else {
+                    out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | ((uint64_t)s0 << 8) |
+                                       ((uint64_t)(root_blk & 0xFFFF) << 24) | ((uint64_t)s << 40));
+                }
+                for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], out);
                 out.prog.push_back(((uint64_t)QIW_OP_FLUSH << 48) | ((uint64_t)s0 << 8));
             }
         }
@@ -400,7 +430,14 @@ struct Compiler {
     }
 };
 
-int lds_slots(const QiwGroup &g, int ns) { return g.n_pslots + g.n_wslots + g.n_nodes + 1 + ns; }
+// LDS doubles of one evaluator wavefront.  Scalar programs keep their state in registers;
+// block programs hold per-sample state shared by the lanes (iP slots, W slots, stack, X,
+// acc, out).
+int lds_slots(const QiwGroup &g, const HostModel &m) {
+    if (m.scalar_kernel) return 0;  // register-resident walk
+    int mm = m.mmax * m.mmax;
+    return (g.n_nodes - 1) * m.bt + g.n_wslots + 1 + (g.n_nodes + 1) * mm + 2 * mm + m.bt;
+}
 
 // ---------------------------------------------------------------------------------------
 // A batch of compiled groups resident on the device
@@ -433,6 +470,8 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d)
     int64_t ut_total = 0;
     for (auto &cg : b.groups) {
         QiwGroup g = cg.g;
+        // programs start on a 64-word block boundary (the NOP placement assumes it)
+        while (prog.size() % QIW_WAVE) prog.push_back((uint64_t)QIW_OP_END << 48);
         g.prog_off = (int32_t)prog.size();
         g.topo_off = (int32_t)arcs.size();
         g.coef_off = (int32_t)coefs.size();
@@ -446,6 +485,8 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d)
         cg.g = g;
         gs.push_back(g);
     }
+    // two blocks of END words after the last program: the evaluator prefetches one block ahead
+    prog.insert(prog.end(), 2 * QIW_WAVE, (uint64_t)QIW_OP_END << 48);
     if (arcs.empty()) arcs.push_back(0);
     if (coefs.empty()) coefs.push_back(0.0);
     d.groups = (QiwGroup *)dev(c, tag + ".groups", gs.size() * sizeof(QiwGroup), &rc);
@@ -494,8 +535,6 @@ int check_run(qiw_ctx *c, const qiw_run *run) {
     for (int i = 0; i < run->n_orders_bare; ++i)
         if (run->orders_bare[i] < 0 || run->orders_bare[i] > QIW_KMAX)
             return fail(c, QIW_E_LIMIT, "order beyond QIW_KMAX");
-    if (!c->m.scalar)
-        return fail(c, QIW_E_LIMIT, "sectors of dimension > 1 need the block kernels");
     if (c->sob_dim == 0) return fail(c, QIW_E_STATE, "no Sobol direction numbers staged (qiw_set_sobol)");
     return 0;
 }
@@ -529,8 +568,9 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             Compiled cg;
             if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
             if (cg.n_top == 0) continue;
-            if (cg.g.n_pslots > 255 || cg.g.n_wslots > 255) return fail(c, QIW_E_LIMIT, "too many slots");
-            b.lds = std::max(b.lds, lds_slots(cg.g, m.ns));
+            if (cg.g.n_wslots > 255 || m.ns > 255)
+                return fail(c, QIW_E_LIMIT, "too many slots");
+            b.lds = std::max(b.lds, lds_slots(cg.g, m));
             int slot = (int)(std::find(oall.begin(), oall.end(), order) - oall.begin());
             int64_t N = order > 0 ? run->N_samples : 1;
             int gi = (int)b.groups.size();
@@ -554,20 +594,59 @@ int ensure_model_buffers(qiw_ctx *c, double **dcoef, int32_t **dkind, double **e
     const HostModel &m = c->m;
     *dcoef = (double *)dev(c, "delta.coef", std::max<size_t>(m.dcoef.size(), 1) * 8, &rc);
     *dkind = (int32_t *)dev(c, "delta.kind", std::max<size_t>(m.dkind.size(), 1) * 4, &rc);
-    *eshift = (double *)dev(c, "eshift", m.ns * 8, &rc);
+    size_t n_eig = 0;
+    for (int s = 0; s < m.ns; ++s) n_eig += m.dims[s];
+    *eshift = (double *)dev(c, "eshift", n_eig * 8, &rc);
     if (rc) return rc;
     if (!m.dcoef.empty()) HIPCHK(c, hipMemcpy(*dcoef, m.dcoef.data(), m.dcoef.size() * 8, hipMemcpyHostToDevice));
     if (!m.dkind.empty()) HIPCHK(c, hipMemcpy(*dkind, m.dkind.data(), m.dkind.size() * 4, hipMemcpyHostToDevice));
-    std::vector<double> es(m.ns);
-    for (int s = 0; s < m.ns; ++s) es[s] = m.E[s][0] + m.lambda0;
-    HIPCHK(c, hipMemcpy(*eshift, es.data(), m.ns * 8, hipMemcpyHostToDevice));
+    std::vector<double> es;  // E + lambda per eigenvalue, sector-major
+    for (int s = 0; s < m.ns; ++s)
+        for (double e : m.E[s]) es.push_back(e + m.lambda0);
+    HIPCHK(c, hipMemcpy(*eshift, es.data(), es.size() * 8, hipMemcpyHostToDevice));
     return 0;
 }
 
-EvalArgs make_eval_args(qiw_ctx *c, const DevBatch &d, int lds, double *ptab, double *eshift, double *dcoef,
+// Launches the evaluator matching the model's sector structure.
+int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks) {
+    const HostModel &m = c->m;
+    const size_t lds_bytes = (size_t)a.lds_slots * sizeof(double);
+    if (lds_bytes > 160 * 1024) return fail(c, QIW_E_LIMIT, "evaluator state exceeds the 160 KiB of LDS of a CU");
+    if (m.scalar_kernel) {
+        HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab));
+        return 0;
+    }
+    int rc = 0;
+    double *opd = (double *)dev(c, "op.data", m.op_data.size() * 8, &rc);
+    int32_t *opdesc = (int32_t *)dev(c, "op.desc", std::max<size_t>(m.op_desc.size(), 1) * 4, &rc);
+    int32_t *dims = (int32_t *)dev(c, "dims", m.ns * 4, &rc);
+    int32_t *btoff = (int32_t *)dev(c, "btoff", m.ns * 4, &rc);
+    if (rc) return rc;
+    std::vector<int32_t> d32(m.dims.begin(), m.dims.end()), o32(m.bt_off.begin(), m.bt_off.end());
+    HIPCHK(c, hipMemcpyAsync(opd, m.op_data.data(), m.op_data.size() * 8, hipMemcpyHostToDevice, c->stream));
+    if (!m.op_desc.empty())
+        HIPCHK(c, hipMemcpyAsync(opdesc, m.op_desc.data(), m.op_desc.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dims, d32.data(), m.ns * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(btoff, o32.data(), m.ns * 4, hipMemcpyHostToDevice, c->stream));
+    BlockArgs b{};
+    b.e = a;
+    b.op_data = opd;
+    b.op_desc = opdesc;
+    b.dims = dims;
+    b.bt_off = btoff;
+    b.bt = m.bt;
+    b.mmax = m.mmax;
+    HIPCHK(c, qiw_launch_eval_block(c->stream, b, n_tasks, lds_bytes));
+    return 0;
+}
+
+EvalArgs make_eval_args(qiw_ctx *c, const Batch &bt, const DevBatch &d, double *ptab, double *eshift, double *dcoef,
                         int32_t *dkind, double *partial) {
     const HostModel &m = c->m;
     EvalArgs a{};
+    int lds = bt.lds;
+    a.max_order = 0;
+    for (auto &cg : bt.groups) a.max_order = std::max(a.max_order, cg.g.order);
     a.groups = d.groups;
     a.tasks = d.tasks;
     a.prog = d.prog;
@@ -671,9 +750,9 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
     std::vector<std::vector<double>> tabs;
     std::vector<int> tab_kind;
     for (int p = 0; p < m.np; ++p) {
-        int rc = load_scalar_op(c, mi->pair_op_i[p], m.dims, m.op_i[p], &m.scalar);
+        int rc = load_scalar_op(c, mi->pair_op_i[p], m.dims, m.op_i[p], &m.scalar, m.op_data, m.op_desc);
         if (rc) return rc;
-        if ((rc = load_scalar_op(c, mi->pair_op_f[p], m.dims, m.op_f[p], &m.scalar))) return rc;
+        if ((rc = load_scalar_op(c, mi->pair_op_f[p], m.dims, m.op_f[p], &m.scalar, m.op_data, m.op_desc))) return rc;
         // i * Delta_p on the grid must be real
         std::vector<double> y(m.ntau);
         for (int j = 0; j < m.ntau; ++j) {
@@ -713,9 +792,9 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
     m.corr_A.resize(m.ncorr);
     m.corr_B.resize(m.ncorr);
     for (int i = 0; i < m.ncorr; ++i) {
-        int rc = load_scalar_op(c, mi->corr_A[i], m.dims, m.corr_A[i], &m.scalar);
+        int rc = load_scalar_op(c, mi->corr_A[i], m.dims, m.corr_A[i], &m.scalar, m.op_data, m.op_desc);
         if (rc) return rc;
-        if ((rc = load_scalar_op(c, mi->corr_B[i], m.dims, m.corr_B[i], &m.scalar))) return rc;
+        if ((rc = load_scalar_op(c, mi->corr_B[i], m.dims, m.corr_B[i], &m.scalar, m.op_data, m.op_desc))) return rc;
     }
     m.attachable.assign(m.ns, {});  // src/expansion.jl:180-183
     for (int s = 0; s < m.ns; ++s)
@@ -726,7 +805,11 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
     for (int s = 0; s < m.ns; ++s) {
         m.bt_off[s] = m.bt;
         m.bt += m.dims[s] * m.dims[s];
+        m.mmax = std::max(m.mmax, m.dims[s]);
     }
+    if (m.mmax > 32) return fail(c, QIW_E_LIMIT, "sector blocks larger than 32 are not supported");
+    m.scalar_kernel = m.scalar && m.ns <= 8 && m.ntab <= 4;
+    if (m.op_data.empty()) m.op_data.push_back(0.0);
     c->m = m;
     c->has_model = true;
     atomic_ip(c->m, c->ip);
@@ -808,6 +891,8 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     double *porders = (double *)dev(c, "porders", (size_t)n_slots * m.ntau * m.bt * 8, &rc);
     double *lam = (double *)dev(c, "lambda", (size_t)m.ntau * 8, &rc);
     double *scratch = (double *)dev(c, "fin.scratch", (size_t)n_slots * m.bt * 8, &rc);
+    double *gsum = (double *)dev(c, "fin.gsum",
+                                 (size_t)std::max(bare_b.groups.size(), bold_b.groups.size()) * m.bt * 8 + 8, &rc);
     int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
     if (rc) return rc;
     std::vector<int32_t> dg;
@@ -830,6 +915,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     fa.h = m.h;
     fa.n_order_slots = n_slots;
     fa.scratch = scratch;
+    fa.gsum = gsum;
 
     // every evaluator launch is bracketed by an event pair on the launch stream
     std::vector<hipEvent_t> ev;
@@ -843,9 +929,9 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     int64_t launches = 0;
     int n_known = c->n_known;
     if (!bare_b.groups.empty()) {
-        EvalArgs a = make_eval_args(c, bare_d, bare_b.lds, ptab, eshift, dcoef, dkind, partial);
+        EvalArgs a = make_eval_args(c, bare_b, bare_d, ptab, eshift, dcoef, dkind, partial);
         HIPCHK(c, mark());
-        HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, (int)bare_b.tasks.size()));
+        if ((rc = launch_eval(c, a, (int)bare_b.tasks.size()))) return rc;
         HIPCHK(c, mark());
         fa.groups = bare_d.groups;
         fa.group_task_first = bare_d.g_first;
@@ -866,7 +952,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         c->stats.diagram_evals += bare_b.diagram_evals_per_launch;
     }
     if (!bold_b.groups.empty()) {
-        EvalArgs a = make_eval_args(c, bold_d, bold_b.lds, ptab, eshift, dcoef, dkind, partial);
+        EvalArgs a = make_eval_args(c, bold_b, bold_d, ptab, eshift, dcoef, dkind, partial);
         fa.groups = bold_d.groups;
         fa.group_task_first = bold_d.g_first;
         fa.group_task_count = bold_d.g_count;
@@ -876,7 +962,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         for (int jw = first_w; jw <= last_w; ++jw) {
             a.step_w = jw;
             HIPCHK(c, mark());
-            HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, (int)bold_b.tasks.size()));
+            if ((rc = launch_eval(c, a, (int)bold_b.tasks.size()))) return rc;
             HIPCHK(c, mark());
             fa.jf = jw + 1;
             fa.tau_w = jw * m.h;
@@ -1038,7 +1124,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
                 if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
                 if (cg.n_top == 0) continue;
                 if (b.groups.empty() && order == 0) first_is_order0 = true;
-                b.lds = std::max(b.lds, lds_slots(cg.g, m.ns));
+                b.lds = std::max(b.lds, lds_slots(cg.g, m));
                 b.groups.push_back(std::move(cg));
             }
         }
@@ -1060,7 +1146,8 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
             }
         DevBatch d;
         if ((rc = upload(c, "corr", b, run->N_samples, d))) return rc;
-        double *partial = (double *)dev(c, "corr.partial", b.tasks.size() * m.ns * 8, &rc);
+        double *partial = (double *)dev(c, "corr.partial", b.tasks.size() * m.bt * 8, &rc);
+        int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
         int32_t *drf = (int32_t *)dev(c, "corr.rf", rf.size() * 4, &rc);
         int32_t *drc = (int32_t *)dev(c, "corr.rc", rcnt.size() * 4, &rc);
         double *dout = (double *)dev(c, "corr.out", (size_t)m.ntau * 8, &rc);
@@ -1071,8 +1158,8 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         HIPCHK(c, hipEventCreate(&e0));
         HIPCHK(c, hipEventCreate(&e1));
         HIPCHK(c, hipEventRecord(e0, c->stream));
-        EvalArgs a = make_eval_args(c, d, b.lds, ptab, eshift, dcoef, dkind, partial);
-        HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, (int)b.tasks.size()));
+        EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
+        if ((rc = launch_eval(c, a, (int)b.tasks.size()))) return rc;
         HIPCHK(c, hipEventRecord(e1, c->stream));
         CorrFinalizeArgs fa{};
         fa.groups = d.groups;
@@ -1081,8 +1168,14 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         fa.n_groups = ng;
         fa.n_rows = m.ntau;
         fa.n_tau = m.ntau;
+        std::vector<int32_t> dg;
+        for (int s = 0; s < m.ns; ++s)
+            for (int i = 0; i < m.dims[s]; ++i) dg.push_back(m.bt_off[s] + i + i * m.dims[s]);
+        HIPCHK(c, hipMemcpy(diag, dg.data(), dg.size() * 4, hipMemcpyHostToDevice));
         fa.partial = partial;
-        fa.ns = m.ns;
+        fa.bt = m.bt;
+        fa.diag_idx = diag;
+        fa.n_diag = (int)dg.size();
         fa.h = m.h;
         fa.beta = m.beta;
         fa.out = dout;

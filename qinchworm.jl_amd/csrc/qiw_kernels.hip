@@ -113,10 +113,75 @@ __device__ __forceinline__ double delta_lookup(const QiwDeltaTables &dt, int tab
 // The diagram evaluator for models whose sectors are all one-dimensional.
 // ---------------------------------------------------------------------------------------
 
-#define LDS(i) lds[(i) * QIW_WAVE + lane]
+// Program words reach the wavefront as 64-word blocks: lane l loads word base+l with one
+// coalesced vector load and each word is broadcast with v_readlane.  A block is loaded
+// when the previous one is used up, so the memory latency is paid once per 64 words.
+struct ProgStream {
+    const uint64_t *__restrict__ base;
+    uint64_t blk;
+    int idx, lane;
+    __device__ __forceinline__ void init(const uint64_t *p, int l) {
+        base = p;
+        lane = l;
+        blk = p[l];
+        idx = 0;
+    }
+    __device__ __forceinline__ uint64_t next() {
+        if (idx == QIW_WAVE) {
+            base += QIW_WAVE;
+            blk = base[lane];
+            idx = 0;
+        }
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)blk, idx);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(blk >> 32), idx);
+        ++idx;
+        return ((uint64_t)hi << 32) | lo;
+    }
+};
 
-__global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar(EvalArgs a) {
-    extern __shared__ double lds[];
+// Register-resident state of one lane (one qMC sample).  All array indices are
+// compile-time constants inside the walk; values chosen by a uniform program field are
+// picked with select chains.
+template <int KM, int SM, int TM>
+struct ScalarState {
+    static constexpr int NM = 2 * KM + 3;
+    double P[NM - 1][SM];   // i*P_s(t_L - t_{L-1}) for L = 2..n
+    double W[KM * TM];      // i*Delta_tab(t_tail - t_head) per arc and table
+    double out[SM];         // per initial sector sums
+    double acc;
+    ProgStream ps;
+};
+
+template <int N>
+__device__ __forceinline__ double pick(const double (&v)[N], int i) {
+    double r = v[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) r = (i == q) ? v[q] : r;
+    return r;
+}
+
+// Scalar trie, format of the structured walk (see qiw_device.h): every node word carries
+// the sector after its operator and the number of its children, so a node's children
+// share one propagator factor: base = parent * i*P_L[s_par].
+template <int L, int KM, int SM, int TM>
+__device__ __forceinline__ void walk(ScalarState<KM, SM, TM> &st, int n, double vpar, int s_par, int nchild) {
+    if constexpr (L <= 2 * KM + 3) {
+        const double base = vpar * pick(st.P[L - 2], s_par);
+        for (int c = 0; c < nchild; ++c) {
+            const uint64_t w = st.ps.next();
+            const double v = (w & QIW_F_TAIL) ? base * pick(st.W, (int)((w >> 16) & 0xFF)) : base;
+            if (w & QIW_F_LEAF) {
+                st.acc += __longlong_as_double((long long)st.ps.next()) * v;
+            } else if (L < n) {
+                walk<L + 1, KM, SM, TM>(st, n, v, (int)((w >> 8) & 0xFF), (int)((w >> 24) & 0xFF));
+            }
+        }
+    }
+}
+
+template <int KM, int SM, int TM>
+__global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
+    constexpr int NM = 2 * KM + 3;
     const int lane = threadIdx.x;
     const QiwTask task = a.tasks[blockIdx.x];
     const QiwGroup g = a.groups[task.group];
@@ -124,110 +189,310 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar(EvalArgs a) {
     const int ntab = a.dtab.n_tables;
     const double inv_h = 1.0 / a.h;
 
-    // slot layout: P slots | W slots | value stack [0..n] | per-s0 outputs
-    const int p_base = 0, w_base = g.n_pslots, v_base = w_base + g.n_wslots, o_base = v_base + n + 1;
-
-    // ---- contour times of the configuration positions (1-based) ----
-    double tau_i, tau_w, tau_f;
+    double tau_i = 0.0, tau_w, tau_f;
     if (g.kind == QIW_G_BARE) {
-        tau_i = 0.0;
         tau_w = 0.0;
         tau_f = a.h;
     } else if (g.kind == QIW_G_BOLD) {
-        tau_i = 0.0;
         tau_w = a.step_w * a.h;
         tau_f = (a.step_w + 1) * a.h;
     } else {
-        tau_i = 0.0;
         tau_w = task.tau_index * a.h;
         tau_f = a.beta;
     }
     const int64_t sample = (int64_t)task.first_sample + lane;
     const bool active = sample < g.n_samples;
-    double t[QIW_NMAX + 1];
+
+    // ---- contour times of the configuration positions (1-based) ----
+    double t[NM + 1];
     const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
 #pragma unroll
-    for (int p = 0; p <= QIW_NMAX; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
+    for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
     if (g.sampled) {
         const int64_t N = g.n_samples;
         const double *u = a.ut + g.ut_off + (active ? sample : 0);
         const double dfw = tau_f - tau_w, dwi = tau_w - tau_i;
 #pragma unroll
-        for (int j = 0; j < 2 * QIW_KMAX; ++j) {
+        for (int j = 0; j < 2 * KM; ++j) {
             if (j < g.d_after) {
-                double v = u[j * N] * dfw + tau_w;
+                const double v = u[j * N] * dfw + tau_w;
 #pragma unroll
-                for (int p = 1; p <= QIW_NMAX; ++p)
+                for (int p = 1; p <= NM; ++p)
                     if (p == n - 1 - j) t[p] = v;
             } else if (j < g.d_after + g.d_before) {
-                double v = u[j * N] * dwi + tau_i;
+                const double v = u[j * N] * dwi + tau_i;
                 const int pos = g.pos_w - 1 - (j - g.d_after);
 #pragma unroll
-                for (int p = 1; p <= QIW_NMAX; ++p)
+                for (int p = 1; p <= NM; ++p)
                     if (p == pos) t[p] = v;
             }
         }
     }
 
-    // ---- propagator slots: i*P_s(t_L - t_{L-1}) for L = 2..n (ordering clamp) ----
+    ScalarState<KM, SM, TM> st;
+    // ---- propagator values i*P_s(t_L - t_{L-1}), L = 2..n (ordering clamp) ----
 #pragma unroll
-    for (int L = 2; L <= QIW_NMAX; ++L) {
-        if (L <= n) {
-            double dt = t[L] - t[L - 1];
-            dt = dt > 0.0 ? dt : 0.0;
-            for (int s = 0; s < ns; ++s) {
-                double v = (g.kind == QIW_G_BARE) ? exp(-dt * a.e_shift[s])
-                                                  : ptab_lookup(a.ptab, ns, s, a.n_tau, inv_h, dt);
-                LDS(p_base + (L - 2) * ns + s) = v;
-            }
+    for (int L = 2; L <= NM; ++L) {
+        double dt = t[L] - t[L - 1];
+        dt = dt > 0.0 ? dt : 0.0;
+        if (g.kind == QIW_G_BARE) {
+#pragma unroll
+            for (int s = 0; s < SM; ++s) st.P[L - 2][s] = (s < ns && L <= n) ? exp(-dt * a.e_shift[s < ns ? s : 0]) : 0.0;
+        } else {
+            const double x = dt * inv_h;
+            int j = (int)floor(x);
+            j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+            const double f = x - (double)j;
+            const double *r0 = a.ptab + (int64_t)j * ns, *r1 = r0 + ns;
+#pragma unroll
+            for (int s = 0; s < SM; ++s) st.P[L - 2][s] = (s < ns && L <= n) ? (1.0 - f) * r0[s] + f * r1[s] : 0.0;
         }
     }
-    LDS(v_base + 1) = 1.0;
-    for (int s = 0; s < ns; ++s) LDS(o_base + s) = 0.0;
+#pragma unroll
+    for (int s = 0; s < SM; ++s) st.out[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < KM * TM; ++q) st.W[q] = 0.0;
+    st.acc = 0.0;
 
     // ---- walk the program ----
-    double cur = 1.0, acc = 0.0;
-    const uint64_t *__restrict__ prog = a.prog + g.prog_off;
-    for (int pc = 0;; ++pc) {
-        const uint64_t w = prog[pc];
+    st.ps.init(a.prog + g.prog_off, lane);
+    for (;;) {
+        const uint64_t w = st.ps.next();
         const int op = (int)((w >> 48) & 0xF);
-        if (op == QIW_OP_EDGE) {
-            const int L = (int)(w & 0xF);
-            double v = (w & QIW_F_RESTORE) ? LDS(v_base + L - 1) : cur;
-            v *= LDS(p_base + (int)((w >> 8) & 0xFF));
-            if (w & QIW_F_TAIL) v *= LDS(w_base + (int)((w >> 16) & 0xFF));
-            cur = v;
-            if (w & QIW_F_SAVE) LDS(v_base + L) = v;
-            if (w & QIW_F_LEAF) acc += a.coefs[g.coef_off + (int)((w >> 24) & 0xFFFFFF)] * v;
+        if (op == QIW_OP_ROOT) {
+            walk<2, KM, SM, TM>(st, n, __longlong_as_double((long long)st.ps.next()), (int)((w >> 8) & 0xFF),
+                                (int)((w >> 24) & 0xFF));
+        } else if (op == QIW_OP_FLUSH) {
+            const int s0 = (int)((w >> 8) & 0xFF);
+#pragma unroll
+            for (int s = 0; s < SM; ++s) st.out[s] += (s == s0) ? st.acc : 0.0;
+            st.acc = 0.0;
         } else if (op == QIW_OP_TOPO) {
-            const uint8_t *arc = a.arcs + g.topo_off + (int)((w >> 24) & 0xFFFFFF) * 2 * k;
-            for (int ai = 0; ai < k; ++ai) {
-                const int head = arc[2 * ai], tail = arc[2 * ai + 1];
+            // arc values of one topology: arc ai has head at bits 8ai..8ai+3, tail at 8ai+4..
+#pragma unroll
+            for (int ai = 0; ai < KM; ++ai) {
+                const int head = (int)((w >> (8 * ai)) & 0xF), tail = (int)((w >> (8 * ai + 4)) & 0xF);
                 double th = 0.0, tt = 0.0;
 #pragma unroll
-                for (int p = 1; p <= QIW_NMAX; ++p) {
+                for (int p = 1; p <= NM; ++p) {
                     th = (p == head) ? t[p] : th;
                     tt = (p == tail) ? t[p] : tt;
                 }
                 double z = tt - th;
                 z = z > 0.0 ? z : 0.0;
-                for (int tb = 0; tb < ntab; ++tb) LDS(w_base + ai * ntab + tb) = delta_lookup(a.dtab, tb, z);
+                const double x = z * inv_h;
+                int j = (int)floor(x);
+                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+                const double u = x - (double)j;
+#pragma unroll
+                for (int tb = 0; tb < TM; ++tb) {
+                    if (ai < k && tb < ntab) {
+                        const double *c = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
+                        st.W[ai * TM + tb] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                    }
+                }
             }
-        } else if (op == QIW_OP_FLUSH) {
-            LDS(o_base + (int)((w >> 8) & 0xFF)) += acc;
-            acc = 0.0;
-        } else {
+        } else if (op == QIW_OP_END) {
             break;
         }
     }
 
     // ---- wavefront reduction of the per-s0 outputs ----
-    for (int s = 0; s < ns; ++s) {
-        double v = active ? LDS(o_base + s) : 0.0;
 #pragma unroll
-        for (int off = QIW_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, QIW_WAVE);
-        if (lane == 0) a.partial[(int64_t)task.out_index * ns + s] = v;
+    for (int s = 0; s < SM; ++s) {
+        if (s < ns) {
+            double v = active ? st.out[s] : 0.0;
+#pragma unroll
+            for (int off = QIW_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, QIW_WAVE);
+            if (lane == 0) a.partial[(int64_t)task.out_index * ns + s] = v;
+        }
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// The diagram evaluator for models with sector blocks larger than 1x1.
+//
+// One wavefront walks the trie for the 64 samples of its task one after the other; for
+// each sample the lanes share the matrix work of every chain product: the node matrix of
+// an edge is O_block * iP_{s_in}(t_L - t_{L-1}) (the reference's node_mats,
+// src/topology_eval.jl:376-388) applied to the parent partial product, and lane e computes
+// element e of the m_out x m_s0 result.  Per-sample state lives in LDS:
+//   iP slots  [(L-2)*bt + BT offset]   interpolated propagator blocks of every segment
+//   W slots   [arc*ntab + tab]         arc values i*Delta
+//   stack     [L][mmax*mmax]           partial products (column-major, leading dim mmax)
+//   X         [mmax*mmax]              i*P * parent, before the operator block
+//   acc, out  accumulators of the current initial sector and of all sectors (BT layout)
+// ---------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
+    extern __shared__ double lds[];
+    const EvalArgs &a = b.e;
+    const int lane = threadIdx.x;
+    const QiwTask task = a.tasks[blockIdx.x];
+    const QiwGroup g = a.groups[task.group];
+    const int n = g.n_nodes, k = g.order, bt = b.bt, mm = b.mmax * b.mmax, ld = b.mmax;
+    const int ntab = a.dtab.n_tables;
+    const double inv_h = 1.0 / a.h;
+    double *ip = lds;
+    double *wsl = ip + (n - 1) * bt;
+    double *stk = wsl + k * ntab + 1;
+    double *X = stk + (n + 1) * mm;
+    double *acc = X + mm;
+    double *out = acc + mm;
+    for (int e = lane; e < bt; e += QIW_WAVE) out[e] = 0.0;
+
+    double tau_i = 0.0, tau_w, tau_f;
+    if (g.kind == QIW_G_BARE) {
+        tau_w = 0.0;
+        tau_f = a.h;
+    } else if (g.kind == QIW_G_BOLD) {
+        tau_w = a.step_w * a.h;
+        tau_f = (a.step_w + 1) * a.h;
+    } else {
+        tau_w = task.tau_index * a.h;
+        tau_f = a.beta;
+    }
+    const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
+    const int n_here = g.sampled ? min(QIW_WAVE, g.n_samples - task.first_sample) : 1;
+    for (int r = 0; r < n_here; ++r) {
+        const int64_t sample = (int64_t)task.first_sample + r;
+        // times of the configuration positions (identical in every lane)
+        double t[QIW_NMAX + 1];
+#pragma unroll
+        for (int p = 0; p <= QIW_NMAX; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
+        if (g.sampled) {
+            const int64_t N = g.n_samples;
+            const double *u = a.ut + g.ut_off + sample;
+#pragma unroll
+            for (int j = 0; j < 2 * QIW_KMAX; ++j) {
+                if (j < g.d_after) {
+                    const double v = u[j * N] * (tau_f - tau_w) + tau_w;
+#pragma unroll
+                    for (int p = 1; p <= QIW_NMAX; ++p)
+                        if (p == n - 1 - j) t[p] = v;
+                } else if (j < g.d_after + g.d_before) {
+                    const double v = u[j * N] * (tau_w - tau_i) + tau_i;
+                    const int pos = g.pos_w - 1 - (j - g.d_after);
+#pragma unroll
+                    for (int p = 1; p <= QIW_NMAX; ++p)
+                        if (p == pos) t[p] = v;
+                }
+            }
+        }
+        // iP slots
+#pragma unroll
+        for (int L = 2; L <= QIW_NMAX; ++L) {
+            if (L <= n) {
+                double dt = t[L] - t[L - 1];
+                dt = dt > 0.0 ? dt : 0.0;
+                double *dst = ip + (L - 2) * bt;
+                if (g.kind == QIW_G_BARE) {
+                    // exact atomic propagator: diagonal exp(-dt (E + lambda))
+                    for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = 0.0;
+                    for (int s = 0, o = 0; s < a.ns; ++s) {
+                        const int m = b.dims[s];
+                        for (int i = lane; i < m; i += QIW_WAVE) dst[b.bt_off[s] + i + i * m] = exp(-dt * a.e_shift[o + i]);
+                        o += m;
+                    }
+                } else {
+                    const double x = dt * inv_h;
+                    int j = (int)floor(x);
+                    j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+                    const double f = x - (double)j;
+                    const double *r0 = a.ptab + (int64_t)j * bt, *r1 = r0 + bt;
+                    for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = (1.0 - f) * r0[e] + f * r1[e];
+                }
+            }
+        }
+        __syncthreads();
+        // walk
+        int s0 = 0, m0 = 1;
+        const uint64_t *__restrict__ prog = a.prog + g.prog_off;
+        for (int pc = 0;; ++pc) {
+            const uint64_t w = prog[pc];
+            const int op = (int)((w >> 48) & 0xF);
+            if (op == QIW_OP_EDGE) {
+                const int L = (int)(w & 0xF);
+                const int s_in = (int)((w >> 8) & 0xFF), s_out = (int)((w >> 40) & 0xFF);
+                const int blk = (int)((w >> 24) & 0xFFFF);
+                const int mi = b.dims[s_in], mo = b.dims[s_out];
+                const double *par = stk + (L - 1) * mm;
+                const double *P = ip + (L - 2) * bt + b.bt_off[s_in];
+                const double wv = (w & QIW_F_TAIL) ? wsl[(int)((w >> 16) & 0xFF)] : 1.0;
+                double *dst = stk + L * mm;
+                if (blk == 0xFFFF) {  // identity / inch node: iP * parent
+                    for (int e = lane; e < mi * m0; e += QIW_WAVE) {
+                        const int i = e % mi, j = e / mi;
+                        double v = 0.0;
+                        for (int q = 0; q < mi; ++q) v += P[i + q * mi] * par[q + j * ld];
+                        dst[i + j * ld] = v * wv;
+                    }
+                } else {
+                    for (int e = lane; e < mi * m0; e += QIW_WAVE) {
+                        const int i = e % mi, j = e / mi;
+                        double v = 0.0;
+                        for (int q = 0; q < mi; ++q) v += P[i + q * mi] * par[q + j * ld];
+                        X[i + j * ld] = v;
+                    }
+                    __syncthreads();
+                    const double *O = b.op_data + b.op_desc[3 * blk];
+                    for (int e = lane; e < mo * m0; e += QIW_WAVE) {
+                        const int i = e % mo, j = e / mo;
+                        double v = 0.0;
+                        for (int q = 0; q < mi; ++q) v += O[i + q * mo] * X[q + j * ld];
+                        dst[i + j * ld] = v * wv;
+                    }
+                }
+                __syncthreads();
+                if (w & QIW_F_LEAF) {
+                    const double c = __longlong_as_double((long long)prog[++pc]);
+                    for (int e = lane; e < m0 * m0; e += QIW_WAVE) {
+                        const int i = e % m0, j = e / m0;
+                        acc[i + j * m0] += c * dst[i + j * ld];
+                    }
+                }
+            } else if (op == QIW_OP_ROOT) {
+                s0 = (int)((w >> 8) & 0xFF);
+                const int s1 = (int)((w >> 40) & 0xFF);
+                const int blk = (int)((w >> 24) & 0xFFFF);
+                m0 = b.dims[s0];
+                const int m1 = b.dims[s1];
+                for (int e = lane; e < m1 * m0; e += QIW_WAVE) {
+                    const int i = e % m1, j = e / m1;
+                    stk[1 * mm + i + j * ld] =
+                        (blk == 0xFFFF) ? (i == j ? 1.0 : 0.0) : b.op_data[b.op_desc[3 * blk] + i + j * m1];
+                }
+                for (int e = lane; e < m0 * m0; e += QIW_WAVE) acc[e] = 0.0;
+                __syncthreads();
+            } else if (op == QIW_OP_TOPO) {
+                if (lane < k * ntab) {
+                    const int ai = lane / ntab, tb = lane % ntab;
+                    const int head = (int)((w >> (8 * ai)) & 0xF), tail = (int)((w >> (8 * ai + 4)) & 0xF);
+                    double th = 0.0, tt = 0.0;
+#pragma unroll
+                    for (int p = 1; p <= QIW_NMAX; ++p) {
+                        th = (p == head) ? t[p] : th;
+                        tt = (p == tail) ? t[p] : tt;
+                    }
+                    double z = tt - th;
+                    z = z > 0.0 ? z : 0.0;
+                    const double x = z * inv_h;
+                    int j = (int)floor(x);
+                    j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+                    const double u = x - (double)j;
+                    const double *c = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
+                    wsl[lane] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                }
+                __syncthreads();
+            } else if (op == QIW_OP_FLUSH) {
+                for (int e = lane; e < m0 * m0; e += QIW_WAVE) out[b.bt_off[s0] + e] += acc[e];
+                __syncthreads();
+            } else if (op == QIW_OP_END) {
+                break;
+            }
+        }
+    }
+    for (int e = lane; e < bt; e += QIW_WAVE) a.partial[(int64_t)task.out_index * bt + e] = out[e];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -243,6 +508,22 @@ __device__ double simplex_volume(int d, double edge) {  // _simplex_volume, src/
     return p;
 }
 
+// Per-group sums of the partial rows: block (group, 64-element chunk); the 4 wavefronts
+// take every 4th row and are combined in wave order, so the sum order is fixed.
+__global__ __launch_bounds__(256) void k_reduce_rows(const int32_t *__restrict__ first, const int32_t *__restrict__ count,
+                                                     const double *__restrict__ partial, int bt,
+                                                     double *__restrict__ gsum) {
+    __shared__ double red[4][64];
+    const int g = blockIdx.x, e = blockIdx.y * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+    const int f0 = first[g], nr = count[g];
+    double s = 0.0;
+    if (e < bt)
+        for (int r = r0; r < nr; r += 4) s += partial[(int64_t)(f0 + r) * bt + e];
+    red[r0][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (r0 == 0 && e < bt) gsum[(int64_t)g * bt + e] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
 __global__ __launch_bounds__(256) void k_finalize_step(FinalizeArgs a) {
     const int tid = threadIdx.x;
     const int jf = a.jf;
@@ -253,9 +534,7 @@ __global__ __launch_bounds__(256) void k_finalize_step(FinalizeArgs a) {
     for (int e = tid; e < a.bt; e += blockDim.x) {
         for (int gi = 0; gi < a.n_groups; ++gi) {
             const QiwGroup g = a.groups[gi];
-            double s = 0.0;
-            const int f0 = a.group_task_first[gi], nt = a.group_task_count[gi];
-            for (int r = 0; r < nt; ++r) s += a.partial[(int64_t)(f0 + r) * a.bt + e];
+            const double s = a.gsum[(int64_t)gi * a.bt + e];
             double fac = 1.0;
             if (g.sampled) {
                 double J = a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
@@ -308,7 +587,7 @@ __global__ __launch_bounds__(256) void k_finalize_corr(CorrFinalizeArgs a) {
         const QiwGroup g = a.groups[gi];
         double s = 0.0;
         for (int r = 0; r < nt; ++r)
-            for (int e = 0; e < a.ns; ++e) s += a.partial[(int64_t)(f0 + r) * a.ns + e];
+            for (int e = 0; e < a.n_diag; ++e) s += a.partial[(int64_t)(f0 + r) * a.bt + a.diag_idx[e]];
         double fac = 1.0;
         if (g.sampled) {
             double J = simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
@@ -344,13 +623,27 @@ hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const ui
     return hipGetLastError();
 }
 
-hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks) {
-    size_t lds = (size_t)a.lds_slots * QIW_WAVE * sizeof(double);
-    hipLaunchKernelGGL(k_eval_scalar, dim3(n_tasks), dim3(QIW_WAVE), lds, st, a);
+// Register budget of the scalar walk: KM = largest order, SM = sectors, TM = tables.
+hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm) {
+    if (km <= 4 && sm <= 4 && tm <= 2)
+        hipLaunchKernelGGL((k_eval_scalar_t<4, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    else if (km <= 4 && sm <= 8 && tm <= 4)
+        hipLaunchKernelGGL((k_eval_scalar_t<4, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    else if (km <= QIW_KMAX && sm <= 8 && tm <= 4)
+        hipLaunchKernelGGL((k_eval_scalar_t<QIW_KMAX, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes) {
+    hipLaunchKernelGGL(k_eval_block, dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, b);
     return hipGetLastError();
 }
 
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a) {
+    hipLaunchKernelGGL(k_reduce_rows, dim3(a.n_groups, (a.bt + 63) / 64), dim3(256), 0, st, a.group_task_first,
+                       a.group_task_count, a.partial, a.bt, a.gsum);
     hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(256), 0, st, a);
     return hipGetLastError();
 }

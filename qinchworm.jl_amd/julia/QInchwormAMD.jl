@@ -1,0 +1,215 @@
+# QInchwormAMD.jl -- ccall binding of libqiw (include/qiw.h) for QInchworm.jl.
+#
+# Drop-in bodies for `inchworm!` and `correlator_2p` (src/inchworm.jl:332-498, :1075-1084):
+# the caller keeps building `Expansion` (KeldyshED/Keldysh) and calls these functions with
+# the reference's signatures; sampling and diagram evaluation run on one MI355X.
+#
+# Conventions: sector indices are passed 0-based; complex arrays are ComplexF64 (interleaved,
+# identical to C99 double _Complex); block matrices column-major; P tables in "BT layout"
+# (per grid point, all sector blocks concatenated).
+module QInchwormAMD
+
+using Keldysh; kd = Keldysh
+using KeldyshED
+using QInchworm.expansion: Expansion
+using QInchworm.spline_gf: SplineInterpolatedGF, IncSplineImaginaryTimeGF
+using QInchworm.scrambled_sobol: ScrambledSobolSeq
+using QInchworm.randomization: RandomizationParams
+
+const libqiw = get(ENV, "QIW_LIBRARY", joinpath(@__DIR__, "..", "lib", "libqiw.so"))
+
+const QIW_GF_GRID_LINEAR = Int32(0)
+const QIW_GF_CUBIC_SPLINE = Int32(1)
+const QIW_GF_INC_SPLINE = Int32(2)
+
+struct QiwSbm
+    n_blocks::Int32
+    s_i::Ptr{Int32}
+    s_f::Ptr{Int32}
+    data::Ptr{ComplexF64}
+end
+
+struct QiwModel
+    beta::Float64
+    n_tau::Int32
+    n_sectors::Int32
+    sector_dim::Ptr{Int32}
+    energies::Ptr{Float64}
+    ppgf_kind::Int32
+    n_pairs::Int32
+    pair_op_i::Ptr{QiwSbm}
+    pair_op_f::Ptr{QiwSbm}
+    pair_prop::Ptr{ComplexF64}
+    pair_prop_kind::Ptr{Int32}
+    pair_prop_xi::Ptr{Int32}
+    n_corr::Int32
+    corr_A::Ptr{QiwSbm}
+    corr_B::Ptr{QiwSbm}
+end
+
+struct QiwRun
+    n_orders::Int32
+    orders::Ptr{Int32}
+    n_orders_bare::Int32
+    orders_bare::Ptr{Int32}
+    N_samples::Int64
+    n_pts_after_max::Int32
+    precision::Int32
+end
+
+mutable struct Context
+    h::Ptr{Cvoid}
+end
+
+function check(ctx::Context, rc)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:qiw_last_error, libqiw), Cstring, (Ptr{Cvoid},), ctx.h))
+    error("libqiw error $(rc): $(msg)")
+end
+
+function Context(device::Integer = 0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:qiw_create, libqiw), Cint, (Ref{Ptr{Cvoid}}, Cint), h, device)
+    rc == 0 || error("qiw_create failed with code $(rc)")
+    ctx = Context(h[])
+    finalizer(c -> ccall((:qiw_destroy, libqiw), Cvoid, (Ptr{Cvoid},), c.h), ctx)
+    return ctx
+end
+
+const default_ctx = Ref{Union{Context, Nothing}}(nothing)
+context() = something(default_ctx[], (default_ctx[] = Context()))
+
+# SectorBlockMatrix -> (s_i, s_f, data) arrays kept alive by `keep`
+function sbm_arrays(sbm, keep)
+    keys_sorted = sort(collect(keys(sbm)))
+    s_i = Int32[k - 1 for k in keys_sorted]
+    s_f = Int32[sbm[k][1] - 1 for k in keys_sorted]
+    data = isempty(keys_sorted) ? ComplexF64[0] : reduce(vcat, [vec(sbm[k][2]) for k in keys_sorted])
+    push!(keep, s_i, s_f, data)
+    return QiwSbm(length(s_i), pointer(s_i), pointer(s_f), pointer(data))
+end
+
+# Grid values and interpolation kind of an InteractionPair propagator
+function prop_table(g, n_tau)
+    if g isa SplineInterpolatedGF
+        return vec(g.GF.mat.data)[1:n_tau], QIW_GF_CUBIC_SPLINE, Int32(Int(g.GF.ξ))
+    end
+    return vec(g.mat.data)[1:n_tau], QIW_GF_GRID_LINEAR, Int32(Int(g.ξ))
+end
+
+function stage!(ctx::Context, exp::Expansion, grid::kd.ImaginaryTimeGrid, max_order::Int, keep)
+    n_tau = length(grid)
+    dims = Int32[length(s) for s in exp.ed.subspaces]
+    energies = reduce(vcat, [Float64.(e) for e in KeldyshED.energies(exp.ed)])
+    ppgf_kind = first(exp.P) isa IncSplineImaginaryTimeGF ? QIW_GF_INC_SPLINE : QIW_GF_GRID_LINEAR
+    op_i = [sbm_arrays(p[1], keep) for p in exp.pair_operator_mat]
+    op_f = [sbm_arrays(p[2], keep) for p in exp.pair_operator_mat]
+    tabs = [prop_table(p.propagator, n_tau) for p in exp.pairs]
+    props = reduce(vcat, [t[1] for t in tabs])
+    kinds = Int32[t[2] for t in tabs]
+    xis = Int32[t[3] for t in tabs]
+    cA = [sbm_arrays(p[1], keep) for p in exp.corr_operators_mat]
+    cB = [sbm_arrays(p[2], keep) for p in exp.corr_operators_mat]
+    push!(keep, dims, energies, op_i, op_f, props, kinds, xis, cA, cB)
+    model = Ref(QiwModel(grid.contour.β, n_tau, length(dims), pointer(dims), pointer(energies), ppgf_kind,
+                         length(exp.pairs), pointer(op_i), pointer(op_f), pointer(props), pointer(kinds),
+                         pointer(xis), length(cA), isempty(cA) ? C_NULL : pointer(cA),
+                         isempty(cB) ? C_NULL : pointer(cB)))
+    check(ctx, ccall((:qiw_set_model, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwModel}), ctx.h, model))
+    # Sobol state of the reference's own generator (unscrambled by default)
+    D = max(2 * max_order, 1)
+    seq = ScrambledSobolSeq(D)
+    m = Matrix{UInt32}(transpose(seq.m))          # m[d*32 + j] = seq.m[d+1, j+1]
+    x = copy(seq.x)
+    push!(keep, m, x)
+    check(ctx, ccall((:qiw_set_sobol, libqiw), Cint, (Ptr{Cvoid}, Int32, Ptr{UInt32}, Ptr{UInt32}),
+                     ctx.h, D, m, x))
+end
+
+# BT-layout table of expansion.P on the grid
+function ppgf_table(exp::Expansion, n_tau)
+    bt = sum(length(s)^2 for s in exp.ed.subspaces)
+    T = zeros(ComplexF64, bt, n_tau)
+    for j in 1:n_tau
+        off = 0
+        for P_s in exp.P
+            GF = P_s isa IncSplineImaginaryTimeGF ? P_s.GF : P_s
+            blk = GF.mat.data[:, :, j]
+            T[off+1:off+length(blk), j] = vec(blk)
+            off += length(blk)
+        end
+    end
+    return T
+end
+
+function store_ppgf!(exp::Expansion, T, grid)
+    for (j, τ) in enumerate(grid)
+        off = 0
+        for P_s in exp.P
+            GF = P_s isa IncSplineImaginaryTimeGF ? P_s.GF : P_s
+            n = size(GF.mat.data, 1)
+            GF[τ, grid[1]] = reshape(T[off+1:off+n*n, j], n, n)
+            off += n * n
+        end
+    end
+end
+
+"""
+    inchworm!(expansion, grid, orders, orders_bare, N_samples; n_pts_after_max)
+
+Drop-in for QInchworm.inchworm! on one MI355X.  Returns (P_orders, P_orders_std) as
+Dict{Int, Matrix{ComplexF64}} in BT layout (sum m^2 x n_tau) and writes expansion.P.
+"""
+function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_bare, N_samples::Int64;
+                   n_pts_after_max::Int64 = typemax(Int64), rand_params::RandomizationParams = RandomizationParams(),
+                   ctx::Context = context())
+    @assert N_samples == 0 || ispow2(N_samples)
+    @assert rand_params.N_seqs == 1 && rand_params.rng === nothing "randomized qMC is not offloaded"
+    keep = Any[]
+    orders_v = Int32.(collect(orders)); bare_v = Int32.(collect(orders_bare))
+    stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, bare_v, Int32[0]))), keep)
+    n_tau = length(grid)
+    bt = sum(length(s)^2 for s in exp.ed.subspaces)
+    oall = sort(unique(vcat(orders_v, bare_v)))
+    P = zeros(ComplexF64, bt, n_tau)
+    Po = zeros(ComplexF64, bt, n_tau, length(oall))
+    Ps = zeros(ComplexF64, bt, n_tau, length(oall))
+    run = Ref(QiwRun(length(orders_v), pointer(orders_v), length(bare_v), pointer(bare_v), N_samples,
+                     n_pts_after_max == typemax(Int64) ? 0 : Int32(n_pts_after_max), 0))
+    GC.@preserve keep orders_v bare_v begin
+        check(ctx, ccall((:qiw_inchworm, libqiw), Cint,
+                         (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                         ctx.h, run, P, Po, Ps))
+    end
+    store_ppgf!(exp, P, grid)
+    return (Dict(Int(o) => Po[:, :, i] for (i, o) in enumerate(oall)),
+            Dict(Int(o) => Ps[:, :, i] for (i, o) in enumerate(oall)))
+end
+
+"""
+    correlator_2p(expansion, grid, orders, N_samples)
+
+Drop-in for QInchworm.correlator_2p on one MI355X: one Vector{ComplexF64} of n_tau values
+<A(tau) B(0)> per pair in expansion.corr_operators.
+"""
+function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64;
+                       ctx::Context = context())
+    @assert N_samples == 0 || ispow2(N_samples)
+    keep = Any[]
+    orders_v = Int32.(collect(orders))
+    stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, Int32[0]))), keep)
+    n_tau = length(grid)
+    T = ppgf_table(exp, n_tau)
+    der0 = zeros(ComplexF64, size(T, 1))
+    check(ctx, ccall((:qiw_set_ppgf, libqiw), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                     ctx.h, n_tau, T, der0))
+    out = zeros(ComplexF64, n_tau, length(exp.corr_operators))
+    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, 0))
+    GC.@preserve keep orders_v begin
+        check(ctx, ccall((:qiw_correlator_2p, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}),
+                         ctx.h, run, out))
+    end
+    return [out[:, c] for c in 1:size(out, 2)]
+end
+
+end # module QInchwormAMD

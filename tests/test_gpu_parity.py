@@ -145,5 +145,5 @@ def test_error_codes(ctx):
     assert e.value.code == abi.QIW_E_NOT_POW2
     k = model.kanamori(2, 2.0, 0.2, -1.0, 10.0, 8, 0.5)
     with pytest.raises(abi.QiwError) as e:
-        qiw.inchworm(k, k.grid, [0, 1], [0, 1], 2 ** 4)
+        qiw.inchworm(k, k.grid, [0, 7], [0, 1], 2 ** 4)   # order beyond the compiled limit
     assert e.value.code == abi.QIW_E_LIMIT
