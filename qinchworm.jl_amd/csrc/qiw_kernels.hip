@@ -152,11 +152,18 @@ struct ScalarState {
     ProgStream ps;
 };
 
+// v[i] for a uniform i as a v_cndmask chain over registers.  The empty asm makes each
+// element an opaque register value, so the select chain is not rewritten into an indexed
+// load from a private (scratch) copy of the array.
 template <int N>
 __device__ __forceinline__ double pick(const double (&v)[N], int i) {
     double r = v[0];
 #pragma unroll
-    for (int q = 1; q < N; ++q) r = (i == q) ? v[q] : r;
+    for (int q = 1; q < N; ++q) {
+        double x = v[q];
+        asm volatile("" : "+v"(x));
+        r = (i == q) ? x : r;
+    }
     return r;
 }
 
