@@ -55,6 +55,17 @@ def cpu_baseline(N, steps=2):
                       f"{evals} diagram evals in {dt:.2f} s, OMP threads={threads}"}
 
 
+def copy_probe(mib):
+    """A plain copy kernel of known size: 2 x mib MiB of HBM traffic per launch."""
+    import torch
+    a = torch.ones(mib * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,7 +73,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log2n", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE hardware counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
               "designed but not built", file=sys.stderr)

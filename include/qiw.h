@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define QIW_ABI_VERSION 1
+#define QIW_ABI_VERSION 2
 
 /* ---- return codes (every reference @assert on this path maps to one) ---------------- */
 enum {
@@ -201,6 +201,58 @@ typedef struct {
     double flops;               /* algorithmic real FLOPs of the chain products           */
 } qiw_stats;
 int qiw_get_stats(qiw_ctx *ctx, qiw_stats *out);
+
+/* ---- learned warp of the unit cube ----------------------------------------------------
+ * No counterpart in the reference, which integrates with plain qMC
+ * (src/qmc_integrate.jl:497-507).  A normalising flow of coupling layers with monotone
+ * rational-quadratic splines on [0,1] (Durkan et al., NeurIPS 2019), conditioned on the
+ * order, the (order, n_pts_after) group and the step fraction tau_w / beta, maps each
+ * Sobol point x of a bold-step group to z = G(x); the group's integral becomes
+ * mean_i f(z_i) |dG/dx|(x_i), which is unbiased for any parameters.  Training fits the
+ * density of the warped points to |f| on samples of the plain method.
+ *
+ * Coordinates: double [n][8], a point of order k uses the first 2k.  Group index of the
+ * bold group (k, n_after): (k-1)^2 + n_after - 1 (k <= 4).  Parameters: float32, per
+ * layer W1 [25][H], b1 [H], W2 [H][H], b2 [H], W3 [H][4(3K+1)], b3 [4(3K+1)] row-major,
+ * then the group embedding table [16][16]. */
+typedef struct {
+    int32_t n_layers;          /* coupling layers (4)                                     */
+    int32_t n_bins;            /* spline bins K (8), <= 16                                */
+    int32_t hidden;            /* conditioner width H (64), <= 128                        */
+    int32_t iters;             /* Adam iterations of qiw_warp_train                       */
+    int32_t batch;             /* samples per iteration                                   */
+    int32_t n_train_steps;     /* bold steps whose integrands form the training set       */
+    int64_t n_train;           /* plain qMC points per (training step, group), 2^n        */
+    double lr;                 /* Adam step size                                          */
+    uint64_t seed;             /* parameter initialisation and minibatch order            */
+} qiw_warp_config;
+
+/* qiw_warp_init: (re)initialise the flow: Glorot-uniform hidden layers, zero output layer,
+ * so the untrained warp is the identity map. */
+int qiw_warp_init(qiw_ctx *ctx, const qiw_warp_config *cfg);
+
+/* qiw_warp_params: number of parameters (n_out, may be NULL), then copy them out (get,
+ * may be NULL) and/or replace them (set, may be NULL; applied after the copy). */
+int qiw_warp_params(qiw_ctx *ctx, int64_t *n_out, float *get, const float *set);
+
+/* qiw_warp_apply: inverse = 0: the training direction z -> x with log q(z)
+ * (logdet = log |dx/dz|); inverse = 1: sampling x -> z = G(x) with log |dz/dx|. */
+int qiw_warp_apply(qiw_ctx *ctx, int32_t inverse, int64_t n, const double *pts, const int32_t *order,
+                   const int32_t *group, const double *step, double *out, double *logdet);
+
+/* qiw_warp_loss_grad: loss = -mean_i coef_i log q(z_i) and its gradient [n_params]. */
+int qiw_warp_loss_grad(qiw_ctx *ctx, int64_t n, const double *z, const int32_t *order,
+                       const int32_t *group, const double *step, const double *coef, double *loss,
+                       float *grad);
+
+/* qiw_warp_train: collect |integrand| of every bold group of run at n_train_steps steps
+ * spread over the grid (plain qMC, the current expansion.P, e.g. after qiw_inchworm),
+ * then run cfg->iters Adam iterations.  loss_history [iters] may be NULL. */
+int qiw_warp_train(qiw_ctx *ctx, const qiw_run *run, double *loss_history);
+
+/* qiw_warp_enable: when on, qiw_inchworm and qiw_inchworm_step use warped points for
+ * bold groups of order <= 4 (bare step and correlators stay plain). */
+int qiw_warp_enable(qiw_ctx *ctx, int32_t on);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,123 @@
+// qiw_ctx.h -- internal state of a libqiw context, shared by the host translation units
+// (qiw_host.cpp: model, programs and drivers; qiw_warp.cpp: the learned warp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/qiw.h"
+#include "qiw_device.h"
+
+namespace qiwi {
+
+typedef std::complex<double> cd;
+
+// ---------------------------------------------------------------------------------------
+// Model in the representation the kernels use: i*P, i*Delta and the operator elements are
+// real for every model the kernels accept.
+// ---------------------------------------------------------------------------------------
+struct ScalarOp {  // SectorBlockMatrix: s_i -> (s_f, value of a 1x1 block, block id)
+    std::vector<int> sf;
+    std::vector<double> val;
+    std::vector<int> blk;   // index into HostModel::op_blocks
+};
+
+struct HostModel {
+    double beta = 0, h = 0;
+    int ntau = 0, ns = 0;
+    std::vector<int> dims;
+    std::vector<std::vector<double>> E;
+    double lambda0 = 0;
+    int ppgf_kind = QIW_GF_GRID_LINEAR;
+    int np = 0, ncorr = 0;
+    std::vector<ScalarOp> op_i, op_f, corr_A, corr_B;
+    std::vector<int> pair_table;  // pair -> distinct hybridization table
+    int ntab = 0;
+    std::vector<double> dcoef;    // [ntab][ntau-1][4]
+    std::vector<int32_t> dkind;
+    std::vector<std::vector<int>> attachable;
+    int bt = 0;
+    std::vector<int> bt_off;
+    bool scalar = true;          // every sector is one-dimensional
+    bool scalar_kernel = true;   // programs for k_eval_scalar_t (scalar model within its register budget)
+    bool vec_ok = false;         // programs for k_eval_vec_t up to order 4 (blocks <= 6, <= 4 tables)
+    int mmax = 1;
+    std::vector<double> op_data;    // all operator blocks, column-major, concatenated
+    std::vector<int32_t> op_desc;   // [n_blocks][3]: offset, m_f, m_i
+};
+
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct WarpState;  // learned warp of the unit cube (qiw_warp.cpp)
+void warp_destroy(WarpState *w);
+bool warp_enabled(const WarpState *w);
+
+// A bold-step group whose sample points the warp moves.
+struct WarpGroup {
+    int order, n_after, d_before, d_after, gid;
+    int64_t off;  // ut offset (evaluation) or per-sample offset (training data)
+};
+
+}  // namespace qiwi
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct qiw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool has_model = false;
+    qiwi::HostModel m;
+    bool vec_staged = false;  // operator blocks of the vector walk are on the device
+    // Sobol state
+    int sob_dim = 0;
+    std::vector<uint32_t> sob_m, sob_x;
+    // propagator state (host mirror of the device table), i*P in BT layout
+    std::vector<double> ip;  // [ntau][bt]
+    int n_known = 1;
+    std::vector<qiwi::cd> der0;      // incremental-spline derivative dP/dtau at 0 (bookkeeping of normalize!)
+    std::vector<double> lambdas;
+    // device buffers
+    std::map<std::string, qiwi::DevBuf> bufs;
+    qiw_stats stats{};
+    qiwi::WarpState *warp = nullptr;  // learned warp, or null
+};
+
+namespace qiwi {
+
+// Records msg as the context's last error and returns code.
+int fail(qiw_ctx *c, int code, const std::string &msg);
+
+// Device buffer `name` of at least `bytes` bytes, kept by the context between calls.
+void *dev(qiw_ctx *c, const std::string &name, size_t bytes, int *rc);
+
+// Learned warp in the drivers (qiw_warp.cpp): unit-cube points and conditions of the
+// weighted groups of a run (n samples each), then per bold step their warped simplex
+// coordinates and weights, written into the groups' rows of ut.
+int warp_prepare(qiw_ctx *c, const std::vector<WarpGroup> &gs, int64_t n);
+int warp_points(qiw_ctx *c, double step_frac, double *ut);
+
+// Training data (qiw_host.cpp): sum of |integrand| per sample of every bold group of run
+// (order <= 4) at each step in `steps`, plain qMC with n points, current expansion.P.
+// mag [steps][groups][n]; gs[i].off is unused.
+int warp_collect(qiw_ctx *c, const qiw_run *run, const std::vector<int> &steps, int64_t n,
+                 std::vector<WarpGroup> &gs, std::vector<double> &mag);
+
+}  // namespace qiwi
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(ctx, QIW_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)

@@ -18,8 +18,22 @@
 //        bits 24-31 : number of children
 // TOPO   bits 8a..8a+3 / 8a+4..8a+7 : head / tail position of arc a (a < order)
 // FLUSH  bits  8-15 : initial sector s0 (out[s0] += acc; acc = 0)
-// all    bits 48-51 : op 0 EDGE, 1 TOPO, 2 FLUSH, 3 END, 4 NOP, 5 ROOT
-enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3, QIW_OP_NOP = 4, QIW_OP_ROOT = 5 };
+// all    bits 48-51 : op 0 EDGE, 1 TOPO, 2 FLUSH, 3 END, 4 NOP, 5 ROOT, 6 SEG
+//
+// Vector programs (k_eval_vec_t, blocks m_s <= MB): one segment per initial sector s0,
+// SEG, then TOPO ROOT EDGE... per topology, then FLUSH (replayed for every column of s0):
+// SEG    bits  0-5  : m_s0, bits 8-15 : s0, bits 52-63 : BT offset of the s0 block
+// ROOT   bits  0-5  : dimension of the sector after position 1, bits 24-31 : children,
+//        bits 32-47 : operator block B (offset into the operator data; 0xFFFF identity),
+//        bits 52-63 : BT offset of that sector's block
+// EDGE   bits  0-5  : dimension m_o of the sector after the node's operator,
+//        bit 6 LEAF (coefficient word follows), bit 7 TAIL (W slot bits 16-23),
+//        bits  8-15 : that sector, bits 24-31 : children,
+//        bits 32-47 : operator block (offset into the operator data; 0xFFFF identity),
+//        bits 52-63 : BT offset of the sector's block
+// FLUSH  the column of s0 is complete
+enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3, QIW_OP_NOP = 4, QIW_OP_ROOT = 5,
+       QIW_OP_SEG = 6 };
 #define QIW_F_LEAF (1ull << 6)
 #define QIW_F_TAIL (1ull << 7)
 
@@ -41,6 +55,9 @@ struct QiwGroup {
     int64_t ut_off;        // offset (in doubles) of this group's simplex coordinates
     int32_t n_samples;     // qMC points (1 for order 0: exact evaluation)
     int32_t sampled;       // 0 for order 0
+    int32_t weighted;      // row d_before + d_after of the coordinates holds per-sample weights
+    int32_t pad_;
+    int64_t ps_off;        // offset of the group's samples in EvalArgs::per_sample
 };
 
 // A unit of work of one launch: 64 samples of one group at one time configuration.
@@ -78,6 +95,7 @@ struct EvalArgs {
     int step_w;                // tau_w index of a bold step
     int max_order;             // largest order of the launch's groups
     double *partial;           // [n_out][ns]
+    double *per_sample;        // optional: sum of |integrand| over the outputs, per sample
 };
 
 // Arguments of the step bookkeeping kernel (k_finalize_step)
@@ -131,14 +149,29 @@ struct BlockArgs {
     int bt, mmax;
 };
 
+// Arguments of the vector evaluator (k_eval_vec_t)
+struct VecArgs {
+    EvalArgs e;
+    const double2 *pt;         // [n_tau-1][bt] (value, slope) of the i*P table (bold groups)
+    const double *ediag;       // [bt] E + lambda on the diagonal positions (bare groups)
+    const double *ops;         // operator blocks, column-major, padded by 64 entries
+    int bt;
+};
+
 // Launch wrappers (qiw_kernels.hip)
 hipError_t qiw_launch_sobol_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int64_t first,
                                    int64_t n, double *out);
 hipError_t qiw_launch_simplex_coords(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
                                      int d_after, int64_t n, double *out);
+hipError_t qiw_launch_sobol_cube(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int stride,
+                                 int64_t n, double *out);
+hipError_t qiw_launch_warp_coords(hipStream_t st, const double *z, int stride, const double *logj, int d_before,
+                                  int d_after, int64_t n, double *out);
 hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
                                      int d_after, double t_i, double t_w, double t_f, int64_t n, double *out);
 hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
 hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes);
+hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm);
+hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out);
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);

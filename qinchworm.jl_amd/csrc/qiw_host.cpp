@@ -18,9 +18,10 @@
 #include <vector>
 
 #include "../../include/qiw.h"
+#include "qiw_ctx.h"
 #include "qiw_device.h"
 
-typedef std::complex<double> cd;
+using namespace qiwi;
 
 // ---------------------------------------------------------------------------------------
 // Topologies (src/diagrammatics.jl:288-337)
@@ -89,81 +90,14 @@ std::vector<Topology> topologies_at_order(int order, int k, bool external_arc) {
     return out;
 }
 
-// ---------------------------------------------------------------------------------------
-// Model in the representation the kernels use: i*P, i*Delta and the operator elements are
-// real for every model the kernels accept.
-// ---------------------------------------------------------------------------------------
-struct ScalarOp {  // SectorBlockMatrix: s_i -> (s_f, value of a 1x1 block, block id)
-    std::vector<int> sf;
-    std::vector<double> val;
-    std::vector<int> blk;   // index into HostModel::op_blocks
-};
-
-struct HostModel {
-    double beta = 0, h = 0;
-    int ntau = 0, ns = 0;
-    std::vector<int> dims;
-    std::vector<std::vector<double>> E;
-    double lambda0 = 0;
-    int ppgf_kind = QIW_GF_GRID_LINEAR;
-    int np = 0, ncorr = 0;
-    std::vector<ScalarOp> op_i, op_f, corr_A, corr_B;
-    std::vector<int> pair_table;  // pair -> distinct hybridization table
-    int ntab = 0;
-    std::vector<double> dcoef;    // [ntab][ntau-1][4]
-    std::vector<int32_t> dkind;
-    std::vector<std::vector<int>> attachable;
-    int bt = 0;
-    std::vector<int> bt_off;
-    bool scalar = true;          // every sector is one-dimensional
-    bool scalar_kernel = true;   // programs for k_eval_scalar_t (scalar model within its register budget)
-    int mmax = 1;
-    std::vector<double> op_data;    // all operator blocks, column-major, concatenated
-    std::vector<int32_t> op_desc;   // [n_blocks][3]: offset, m_f, m_i
-};
-
 }  // namespace
 
-// ---------------------------------------------------------------------------------------
-// Context
-// ---------------------------------------------------------------------------------------
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
-
-struct qiw_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    bool has_model = false;
-    HostModel m;
-    // Sobol state
-    int sob_dim = 0;
-    std::vector<uint32_t> sob_m, sob_x;
-    // propagator state (host mirror of the device table), i*P in BT layout
-    std::vector<double> ip;  // [ntau][bt]
-    int n_known = 1;
-    std::vector<cd> der0;      // incremental-spline derivative dP/dtau at 0 (bookkeeping of normalize!)
-    std::vector<double> lambdas;
-    // device buffers
-    std::map<std::string, DevBuf> bufs;
-    qiw_stats stats{};
-};
-
-namespace {
+namespace qiwi {
 
 int fail(qiw_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
 }
-
-#define HIPCHK(ctx, expr)                                                                  \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(ctx, QIW_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 void *dev(qiw_ctx *c, const std::string &name, size_t bytes, int *rc) {
     DevBuf &b = c->bufs[name];
@@ -179,6 +113,10 @@ void *dev(qiw_ctx *c, const std::string &name, size_t bytes, int *rc) {
     }
     return b.p;
 }
+
+}  // namespace qiwi
+
+namespace {
 
 // natural cubic spline of uniformly spaced real values -> per-interval local cubic
 // coefficients (src/spline_gf.jl:194-202: BSpline(Cubic(Line(OnGrid()))) on the grid)
@@ -242,6 +180,15 @@ int load_scalar_op(qiw_ctx *c, const qiw_sbm &in, const std::vector<int> &dims, 
 // ---------------------------------------------------------------------------------------
 enum NodeKind { N_PAIR_HEAD, N_PAIR_TAIL, N_IDENTITY, N_OPERATOR };
 
+// Program formats: the scalar structured walk, the LDS block walk, the vector walk.
+enum ProgFormat { FMT_SCALAR = 0, FMT_BLOCK = 1, FMT_VEC = 2 };
+
+int program_format(const HostModel &m, int max_order) {
+    if (m.scalar_kernel) return FMT_SCALAR;
+    if (m.vec_ok && max_order <= 4) return FMT_VEC;
+    return FMT_BLOCK;
+}
+
 struct Compiled {
     QiwGroup g{};
     std::vector<uint64_t> prog;
@@ -262,7 +209,7 @@ struct TrieNode {
 
 struct Compiler {
     const HostModel &m;
-    int kind, order, n_after, corr;
+    int kind, order, n_after, corr, fmt;
     int n = 0, pos_w = 0;
     std::vector<int> conf_kind, conf_arc;
     std::vector<int> selected;
@@ -270,8 +217,8 @@ struct Compiler {
     int s0 = 0, parity = 1;
     std::string err;
 
-    Compiler(const HostModel &m_, int kind_, int order_, int n_after_, int corr_)
-        : m(m_), kind(kind_), order(order_), n_after(n_after_), corr(corr_) {}
+    Compiler(const HostModel &m_, int kind_, int order_, int n_after_, int corr_, int fmt_)
+        : m(m_), kind(kind_), order(order_), n_after(n_after_), corr(corr_), fmt(fmt_) {}
 
     int add(int parent, int level, int s_in, int wslot, bool tail, int s_out = 0, int blk = 0xFFFF) {
         TrieNode t{level, (level - 2) * m.ns + s_in, wslot, parent, tail, false, 0.0, {}, s_in, s_out, blk};
@@ -281,52 +228,70 @@ struct Compiler {
         return id;
     }
 
-    bool dfs(int node, int pos, int s, double c) {
+    // Child `id` was just added under `node`: keep it if its subtree reaches a leaf, else
+    // drop it (a pair whose tail operator annihilates the sector reached by then).  Returns
+    // -1 on error, 0 for a dead subtree, 1 for a live one.
+    int keep_child(int node, int id, int r) {
+        if (r <= 0) {
+            trie.erase(trie.begin() + id, trie.end());
+            trie[node].children.pop_back();
+        }
+        return r;
+    }
+
+    int dfs(int node, int pos, int s, double c) {
         if (pos > n) {
             if (s != s0) {
                 err = "A block off-diagonal contribution to a pseudo-particle propagator detected, try passing "
                       "more 'symmetry_breakers' when constructing the local exact diagonalization object";
-                return false;
+                return -1;
             }
             trie[node].leaf = true;
             trie[node].coef = c * parity * ((order % 2 == 0) ? 1.0 : -1.0);
-            return true;
+            return 1;
         }
         int k = conf_kind[pos];
         // block programs keep the operator blocks as matrices: no constant folding
-        const bool fold = m.scalar_kernel;
+        const bool fold = (fmt == FMT_SCALAR);
+        int live = 0;
         if (k == N_PAIR_HEAD) {
             int a = conf_arc[pos];
             for (int p : m.attachable[s]) {
                 selected[a] = p;
                 int id = add(node, pos, s, 0, false, m.op_i[p].sf[s], m.op_i[p].blk[s]);
-                if (!dfs(id, pos + 1, m.op_i[p].sf[s], fold ? c * m.op_i[p].val[s] : c)) return false;
+                int r = keep_child(node, id, dfs(id, pos + 1, m.op_i[p].sf[s], fold ? c * m.op_i[p].val[s] : c));
+                if (r < 0) return -1;
+                live |= r;
             }
         } else if (k == N_PAIR_TAIL) {
             int a = conf_arc[pos];
             int p = selected[a];
             if (m.op_f[p].sf[s] >= 0) {
                 int id = add(node, pos, s, a * m.ntab + m.pair_table[p], true, m.op_f[p].sf[s], m.op_f[p].blk[s]);
-                if (!dfs(id, pos + 1, m.op_f[p].sf[s], fold ? c * m.op_f[p].val[s] : c)) return false;
+                live = keep_child(node, id, dfs(id, pos + 1, m.op_f[p].sf[s], fold ? c * m.op_f[p].val[s] : c));
             }
         } else if (k == N_OPERATOR) {
             const ScalarOp &A = m.corr_A[corr];
             if (A.sf[s] >= 0) {
                 int id = add(node, pos, s, 0, false, A.sf[s], A.blk[s]);
-                if (!dfs(id, pos + 1, A.sf[s], fold ? c * A.val[s] : c)) return false;
+                live = keep_child(node, id, dfs(id, pos + 1, A.sf[s], fold ? c * A.val[s] : c));
             }
         } else {
             int id = add(node, pos, s, 0, false, s, 0xFFFF);
-            if (!dfs(id, pos + 1, s, c)) return false;
+            live = keep_child(node, id, dfs(id, pos + 1, s, c));
         }
-        return true;
+        return live;
     }
 
     void emit(int id, Compiled &out) {
         const TrieNode &t = trie[id];
         uint64_t w;
-        if (m.scalar_kernel) {  // structured walk: sector after the node, child count
+        if (fmt == FMT_SCALAR) {  // structured walk: sector after the node, child count
             w = ((uint64_t)t.s_out << 8) | ((uint64_t)t.wslot << 16) | ((uint64_t)t.children.size() << 24);
+        } else if (fmt == FMT_VEC) {
+            w = (uint64_t)m.dims[t.s_out] | ((uint64_t)t.s_out << 8) | ((uint64_t)t.wslot << 16) |
+                ((uint64_t)t.children.size() << 24) | (op_offset(t.blk) << 32) |
+                ((uint64_t)m.bt_off[t.s_out] << 52);
         } else {
             w = (uint64_t)t.level | ((uint64_t)t.s_in << 8) | ((uint64_t)t.wslot << 16) |
                 ((uint64_t)(t.blk & 0xFFFF) << 24) | ((uint64_t)t.s_out << 40);
@@ -343,6 +308,9 @@ struct Compiler {
         }
         for (size_t i = 0; i < t.children.size(); ++i) emit(t.children[i], out);
     }
+
+    // offset of operator block blk in the operator data (0xFFFF: identity)
+    uint64_t op_offset(int blk) const { return blk == 0xFFFF ? 0xFFFF : (uint64_t)m.op_desc[3 * blk]; }
 
     bool compile(Compiled &out) {
         n = (kind == QIW_G_BARE) ? 2 * order + 2 : 2 * order + 3;
@@ -365,6 +333,8 @@ struct Compiler {
                                          : topologies_at_order(order, n_after, kind == QIW_G_CORR);
         selected.assign(std::max(order, 1), -1);
         out.n_top = (int)tops.size();
+        // vector format: segments per initial sector, collected first
+        std::vector<std::vector<uint64_t>> seg(fmt == FMT_VEC ? m.ns : 0);
         for (int ti = 0; ti < (int)tops.size(); ++ti) {
             const Topology &top = tops[ti];
             parity = top.parity;
@@ -379,7 +349,7 @@ struct Compiler {
                 out.arcs.push_back((uint8_t)tail);
                 topo_word |= ((uint64_t)head << (8 * a)) | ((uint64_t)tail << (8 * a + 4));
             }
-            if (order > 0) out.prog.push_back(topo_word);
+            if (order > 0 && fmt != FMT_VEC) out.prog.push_back(topo_word);
             for (s0 = 0; s0 < m.ns; ++s0) {
                 trie.clear();
                 // root: configuration position 1 (identity, or the B operator)
@@ -388,19 +358,31 @@ struct Compiler {
                 if (kind == QIW_G_CORR) {
                     const ScalarOp &B = m.corr_B[corr];
                     if (B.sf[s0] < 0) continue;
-                    c = m.scalar ? B.val[s0] : 1.0;
+                    c = (fmt == FMT_SCALAR) ? B.val[s0] : 1.0;  // other formats apply the B block at the root
                     s = B.sf[s0];
                     root_blk = B.blk[s0];
                 }
                 TrieNode root{1, 0, 0, -1, false, false, 0.0, {}};
                 trie.push_back(root);
-                if (!dfs(0, 2, s, c)) return false;
+                if (dfs(0, 2, s, c) < 0) return false;
                 if (trie[0].children.empty()) continue;
                 if (trie[0].children.size() > 255) {
                     err = "too many pair interactions attachable to one sector";
                     return false;
                 }
-                if (m.scalar_kernel) {
+                if (fmt == FMT_VEC) {
+                    std::vector<uint64_t> saved;
+                    saved.swap(out.prog);
+                    if (order > 0) out.prog.push_back(topo_word);
+                    out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | (uint64_t)m.dims[s] |
+                                       ((uint64_t)trie[0].children.size() << 24) | (op_offset(root_blk) << 32) |
+                                       ((uint64_t)m.bt_off[s] << 52));
+                    for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], out);
+                    seg[s0].insert(seg[s0].end(), out.prog.begin(), out.prog.end());
+                    saved.swap(out.prog);
+                    continue;
+                }
+                if (fmt == FMT_SCALAR) {
                     out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | ((uint64_t)s << 8) |
                                        ((uint64_t)trie[0].children.size() << 24));
                     const double one = 1.0;  // constants of position 1 are folded into the leaves
@@ -413,6 +395,14 @@ struct Compiler {
                 }
                 for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], out);
                 out.prog.push_back(((uint64_t)QIW_OP_FLUSH << 48) | ((uint64_t)s0 << 8));
+            }
+        }
+        if (fmt == FMT_VEC) {  // every initial sector gets a segment, so every output is written
+            for (int s = 0; s < m.ns; ++s) {
+                out.prog.push_back(((uint64_t)QIW_OP_SEG << 48) | (uint64_t)m.dims[s] | ((uint64_t)s << 8) |
+                                   ((uint64_t)m.bt_off[s] << 52));
+                out.prog.insert(out.prog.end(), seg[s].begin(), seg[s].end());
+                out.prog.push_back((uint64_t)QIW_OP_FLUSH << 48);
             }
         }
         out.prog.push_back((uint64_t)QIW_OP_END << 48);
@@ -433,8 +423,8 @@ struct Compiler {
 // LDS doubles of one evaluator wavefront.  Scalar programs keep their state in registers;
 // block programs hold per-sample state shared by the lanes (iP slots, W slots, stack, X,
 // acc, out).
-int lds_slots(const QiwGroup &g, const HostModel &m) {
-    if (m.scalar_kernel) return 0;  // register-resident walk
+int lds_slots(const QiwGroup &g, const HostModel &m, int fmt) {
+    if (fmt != FMT_BLOCK) return 0;  // register-resident walks
     int mm = m.mmax * m.mmax;
     return (g.n_nodes - 1) * m.bt + g.n_wslots + 1 + (g.n_nodes + 1) * mm + 2 * mm + m.bt;
 }
@@ -447,6 +437,7 @@ struct Batch {
     std::vector<QiwTask> tasks;
     std::vector<int32_t> g_first, g_count, g_slot;  // per group: task rows, order slot
     int lds = 0;
+    int fmt = FMT_SCALAR;  // program format of every group of the batch
     double flops_per_launch = 0;  // algorithmic FLOPs of one launch
     int64_t leaves_per_launch = 0, diagram_evals_per_launch = 0;
 };
@@ -461,13 +452,21 @@ struct DevBatch {
     int32_t *g_first = nullptr, *g_count = nullptr, *g_slot = nullptr;
 };
 
-int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d) {
+// Warp group index of a bold group (order k <= 4, n_after points after tau_w), else -1.
+int warp_gid(const QiwGroup &g) {
+    if (g.kind != QIW_G_BOLD || g.order < 1 || g.order > 4) return -1;
+    return (g.order - 1) * (g.order - 1) + g.n_after - 1;
+}
+
+// weighted: the sampled bold groups carry a per-sample weight row (learned warp);
+// per_sample: offsets of the groups' samples in a per-sample output are assigned.
+int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d, bool weighted = false) {
     int rc = 0;
     std::vector<uint64_t> prog;
     std::vector<uint8_t> arcs;
     std::vector<double> coefs;
     std::vector<QiwGroup> gs;
-    int64_t ut_total = 0;
+    int64_t ut_total = 0, ps_total = 0;
     for (auto &cg : b.groups) {
         QiwGroup g = cg.g;
         // programs start on a 64-word block boundary (the NOP placement assumes it)
@@ -477,8 +476,11 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d)
         g.coef_off = (int32_t)coefs.size();
         g.sampled = g.order > 0;
         g.n_samples = g.sampled ? (int32_t)N : 1;
+        g.weighted = (weighted && g.sampled && warp_gid(g) >= 0) ? 1 : 0;
+        g.ps_off = ps_total;
+        ps_total += g.n_samples;
         g.ut_off = ut_total;
-        if (g.sampled) ut_total += (int64_t)(g.d_before + g.d_after) * N;
+        if (g.sampled) ut_total += (int64_t)(g.d_before + g.d_after + g.weighted) * N;
         prog.insert(prog.end(), cg.prog.begin(), cg.prog.end());
         arcs.insert(arcs.end(), cg.arcs.begin(), cg.arcs.end());
         coefs.insert(coefs.end(), cg.coefs.begin(), cg.coefs.end());
@@ -552,6 +554,9 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
     const HostModel &m = c->m;
     int n_o = bare ? run->n_orders_bare : run->n_orders;
     const int32_t *ord = bare ? run->orders_bare : run->orders;
+    int max_order = 0;
+    for (int oi = 0; oi < n_o; ++oi) max_order = std::max(max_order, (int)ord[oi]);
+    b.fmt = program_format(m, max_order);
     for (int oi = 0; oi < n_o; ++oi) {
         int order = ord[oi];
         if (order > 0 && run->N_samples <= 0) continue;
@@ -564,13 +569,13 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             for (int na = 1; na <= 2 * order - 1; ++na)
                 if (run->n_pts_after_max <= 0 || na <= run->n_pts_after_max) nas.push_back(na);
         for (int na : nas) {
-            Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1);
+            Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1, b.fmt);
             Compiled cg;
             if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
             if (cg.n_top == 0) continue;
             if (cg.g.n_wslots > 255 || m.ns > 255)
                 return fail(c, QIW_E_LIMIT, "too many slots");
-            b.lds = std::max(b.lds, lds_slots(cg.g, m));
+            b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
             int slot = (int)(std::find(oall.begin(), oall.end(), order) - oall.begin());
             int64_t N = order > 0 ? run->N_samples : 1;
             int gi = (int)b.groups.size();
@@ -608,12 +613,42 @@ int ensure_model_buffers(qiw_ctx *c, double **dcoef, int32_t **dkind, double **e
 }
 
 // Launches the evaluator matching the model's sector structure.
-int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks) {
+int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt) {
     const HostModel &m = c->m;
     const size_t lds_bytes = (size_t)a.lds_slots * sizeof(double);
     if (lds_bytes > 160 * 1024) return fail(c, QIW_E_LIMIT, "evaluator state exceeds the 160 KiB of LDS of a CU");
-    if (m.scalar_kernel) {
+    if (fmt == FMT_SCALAR) {
         HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab));
+        return 0;
+    }
+    if (fmt == FMT_VEC) {
+        // operator blocks and E + lambda on the BT diagonal, both padded so that the
+        // fixed-length vector loops stay inside the allocations
+        int rc = 0;
+        const size_t pad = 64;
+        double *ops = (double *)dev(c, "vec.ops", (m.op_data.size() + pad) * 8, &rc);
+        double *ediag = (double *)dev(c, "vec.ediag", ((size_t)m.bt + 2 * pad) * 8, &rc);
+        double2 *pt = (double2 *)dev(c, "vec.pt", ((size_t)(m.ntau - 1) * m.bt + pad) * 16, &rc);
+        if (rc) return rc;
+        if (!c->vec_staged) {  // once per model
+            std::vector<double> o(m.op_data), ed((size_t)m.bt + 2 * pad, 0.0);
+            o.resize(m.op_data.size() + pad, 0.0);
+            for (int s = 0; s < m.ns; ++s)
+                for (int i = 0; i < m.dims[s]; ++i) ed[m.bt_off[s] + i + i * m.dims[s]] = m.E[s][i] + m.lambda0;
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipMemcpy(ops, o.data(), o.size() * 8, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(ediag, ed.data(), ed.size() * 8, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemset(pt, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 16));
+            c->vec_staged = true;
+        }
+        HIPCHK(c, qiw_launch_ptab_pairs(c->stream, a.ptab, m.ntau, m.bt, pt));
+        VecArgs v{};
+        v.e = a;
+        v.pt = pt;
+        v.ediag = ediag;
+        v.ops = ops;
+        v.bt = m.bt;
+        HIPCHK(c, qiw_launch_eval_vec(c->stream, v, n_tasks, m.mmax, a.max_order, m.ntab));
         return 0;
     }
     int rc = 0;
@@ -710,6 +745,7 @@ extern "C" void qiw_destroy(qiw_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    warp_destroy(c->warp);
     for (auto &kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -810,8 +846,11 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
     if (m.mmax > 32) return fail(c, QIW_E_LIMIT, "sector blocks larger than 32 are not supported");
     m.scalar_kernel = m.scalar && m.ns <= 8 && m.ntab <= 4;
     if (m.op_data.empty()) m.op_data.push_back(0.0);
+    m.vec_ok = !m.scalar_kernel && m.mmax <= 6 && m.ntab <= 4 && m.ns <= 255 && m.bt < 4096 &&
+               m.op_data.size() < 0xFFFF - 64;
     c->m = m;
     c->has_model = true;
+    c->vec_staged = false;
     atomic_ip(c->m, c->ip);
     c->n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? 1 : m.ntau;
     c->der0.assign(m.bt, cd(0.0));  // initial_ppgf_derivative: -i diag(-E - lambda), src/ppgf.jl:710-720
@@ -880,8 +919,18 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     DevBatch bare_d, bold_d;
     if (do_bare && (rc = build_step_batch(c, run, true, oall, bare_b))) return rc;
     if (last_w >= first_w && (rc = build_step_batch(c, run, false, oall, bold_b))) return rc;
+    const bool warped = warp_enabled(c->warp) && !bold_b.groups.empty();
+    if (warped && bold_b.fmt == FMT_BLOCK)
+        return fail(c, QIW_E_LIMIT, "the learned warp needs sector blocks <= 6 and orders <= 4");
     if (!bare_b.groups.empty() && (rc = upload(c, "bare", bare_b, run->N_samples, bare_d))) return rc;
-    if (!bold_b.groups.empty() && (rc = upload(c, "bold", bold_b, run->N_samples, bold_d))) return rc;
+    if (!bold_b.groups.empty() && (rc = upload(c, "bold", bold_b, run->N_samples, bold_d, warped))) return rc;
+    if (warped) {
+        std::vector<WarpGroup> wg;
+        for (auto &cg : bold_b.groups)
+            if (cg.g.weighted)
+                wg.push_back({cg.g.order, cg.g.n_after, cg.g.d_before, cg.g.d_after, warp_gid(cg.g), cg.g.ut_off});
+        if ((rc = warp_prepare(c, wg, run->N_samples))) return rc;
+    }
     double *dcoef, *eshift;
     int32_t *dkind;
     if ((rc = ensure_model_buffers(c, &dcoef, &dkind, &eshift))) return rc;
@@ -931,7 +980,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     if (!bare_b.groups.empty()) {
         EvalArgs a = make_eval_args(c, bare_b, bare_d, ptab, eshift, dcoef, dkind, partial);
         HIPCHK(c, mark());
-        if ((rc = launch_eval(c, a, (int)bare_b.tasks.size()))) return rc;
+        if ((rc = launch_eval(c, a, (int)bare_b.tasks.size(), bare_b.fmt))) return rc;
         HIPCHK(c, mark());
         fa.groups = bare_d.groups;
         fa.group_task_first = bare_d.g_first;
@@ -961,8 +1010,9 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         fa.bare = 0;
         for (int jw = first_w; jw <= last_w; ++jw) {
             a.step_w = jw;
+            if (warped && (rc = warp_points(c, jw * m.h / m.beta, bold_d.ut))) return rc;
             HIPCHK(c, mark());
-            if ((rc = launch_eval(c, a, (int)bold_b.tasks.size()))) return rc;
+            if ((rc = launch_eval(c, a, (int)bold_b.tasks.size(), bold_b.fmt))) return rc;
             HIPCHK(c, mark());
             fa.jf = jw + 1;
             fa.tau_w = jw * m.h;
@@ -1113,18 +1163,21 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
     for (int ci = 0; ci < m.ncorr; ++ci) {
         // groups (src/inchworm.jl:957-981): every n_pts_after of every order
         Batch b;
+        int max_order = 0;
+        for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
+        b.fmt = program_format(m, max_order);
         bool first_is_order0 = false;
         for (int oi = 0; oi < run->n_orders; ++oi) {
             int order = run->orders[oi];
             if (order > 0 && run->N_samples <= 0) continue;
             int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
             for (int na = lo; na <= hi; ++na) {
-                Compiler comp(m, QIW_G_CORR, order, na, ci);
+                Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt);
                 Compiled cg;
                 if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
                 if (cg.n_top == 0) continue;
                 if (b.groups.empty() && order == 0) first_is_order0 = true;
-                b.lds = std::max(b.lds, lds_slots(cg.g, m));
+                b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
                 b.groups.push_back(std::move(cg));
             }
         }
@@ -1159,7 +1212,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         HIPCHK(c, hipEventCreate(&e1));
         HIPCHK(c, hipEventRecord(e0, c->stream));
         EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
-        if ((rc = launch_eval(c, a, (int)b.tasks.size()))) return rc;
+        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt))) return rc;
         HIPCHK(c, hipEventRecord(e1, c->stream));
         CorrFinalizeArgs fa{};
         fa.groups = d.groups;
@@ -1272,3 +1325,54 @@ extern "C" int qiw_get_stats(qiw_ctx *c, qiw_stats *out) {
     *out = c->stats;
     return QIW_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// Training data of the learned warp
+// ---------------------------------------------------------------------------------------
+namespace qiwi {
+
+int warp_collect(qiw_ctx *c, const qiw_run *run, const std::vector<int> &steps, int64_t n,
+                 std::vector<WarpGroup> &gs, std::vector<double> &mag) {
+    const HostModel &m = c->m;
+    int rc = 0;
+    qiw_run r = *run;
+    r.N_samples = n;
+    r.n_orders_bare = 0;
+    std::vector<int> oall = orders_all(&r);
+    Batch b;
+    DevBatch d;
+    if ((rc = build_step_batch(c, &r, false, oall, b))) return rc;
+    if (b.fmt == FMT_BLOCK) return fail(c, QIW_E_LIMIT, "the learned warp needs sector blocks <= 6 and orders <= 4");
+    if (b.groups.empty()) return fail(c, QIW_E_ARG, "no sampled bold groups to train on");
+    if ((rc = upload(c, "wtrain", b, n, d))) return rc;
+    double *dcoef, *eshift;
+    int32_t *dkind;
+    if ((rc = ensure_model_buffers(c, &dcoef, &dkind, &eshift))) return rc;
+    int64_t ps_total = 0;
+    for (auto &cg : b.groups) ps_total += cg.g.n_samples;
+    double *partial = (double *)dev(c, "partial", std::max<size_t>(b.tasks.size(), 1) * m.bt * 8, &rc);
+    double *ptab = (double *)dev(c, "ptab", (size_t)m.ntau * m.bt * 8, &rc);
+    double *ps = (double *)dev(c, "wtrain.ps", (size_t)ps_total * 8, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
+    gs.clear();
+    for (auto &cg : b.groups)
+        if (cg.g.sampled && warp_gid(cg.g) >= 0)
+            gs.push_back({cg.g.order, cg.g.n_after, cg.g.d_before, cg.g.d_after, warp_gid(cg.g), cg.g.ps_off});
+    mag.assign(steps.size() * gs.size() * (size_t)n, 0.0);
+    std::vector<double> host((size_t)ps_total);
+    EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
+    a.per_sample = ps;
+    for (size_t si = 0; si < steps.size(); ++si) {
+        a.step_w = steps[si];
+        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt))) return rc;
+        HIPCHK(c, hipMemcpyAsync(host.data(), ps, (size_t)ps_total * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t gi = 0; gi < gs.size(); ++gi)
+            std::copy(host.begin() + gs[gi].off, host.begin() + gs[gi].off + n,
+                      mag.begin() + (si * gs.size() + gi) * (size_t)n);
+    }
+    return QIW_OK;
+}
+
+}  // namespace qiwi

@@ -65,6 +65,29 @@ __global__ void k_simplex_coords(const uint32_t *__restrict__ m, const uint32_t 
     for (int k = 0; k < d; ++k) out[(int64_t)k * n + r] = u[k];
 }
 
+// Points of the unit cube for the learned warp: Sobol coordinates 0..dim-1 of sample r in
+// row r (stride `stride`, unused coordinates 0).
+__global__ void k_sobol_cube(const uint32_t *__restrict__ m, const uint32_t *__restrict__ x0, int dim, int stride,
+                             int64_t n, double *__restrict__ out) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int k = 0; k < stride; ++k)
+        out[r * stride + k] = (k < dim) ? ldexp((double)sobol_word(m, x0[k], k, (uint64_t)r), -32) : 0.0;
+}
+
+// Warped points -> simplex coordinates (rows 0..d-1) and the per-sample weight
+// exp(log |dz/dx|) (row d) of a weighted group.
+__global__ void k_warp_coords(const double *__restrict__ z, int stride, const double *__restrict__ logj,
+                              int d_before, int d_after, int64_t n, double *__restrict__ out) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double u[2 * QIW_KMAX];
+    const int d = d_before + d_after;
+    root_map(z + r * stride, d_after, d_before, u);
+    for (int k = 0; k < d; ++k) out[(int64_t)k * n + r] = u[k];
+    out[(int64_t)d * n + r] = exp(logj[r]);
+}
+
 // Full transform to contour times (tests of the simplex map).
 __global__ void k_simplex_points(const uint32_t *__restrict__ m, const uint32_t *__restrict__ x0, int d_before,
                                  int d_after, double t_i, double t_w, double t_f, int64_t n,
@@ -209,6 +232,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
     }
     const int64_t sample = (int64_t)task.first_sample + lane;
     const bool active = sample < g.n_samples;
+    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples +
+                                        (active ? sample : 0)]
+                                 : 1.0;
 
     // ---- contour times of the configuration positions (1-based) ----
     double t[NM + 1];
@@ -305,10 +331,16 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
     }
 
     // ---- wavefront reduction of the per-s0 outputs ----
+    if (a.per_sample && active) {
+        double m = 0.0;
+#pragma unroll
+        for (int s = 0; s < SM; ++s) m += (s < ns) ? fabs(st.out[s]) : 0.0;
+        a.per_sample[g.ps_off + sample] = m;
+    }
 #pragma unroll
     for (int s = 0; s < SM; ++s) {
         if (s < ns) {
-            double v = active ? st.out[s] : 0.0;
+            double v = active ? st.out[s] * wt : 0.0;
 #pragma unroll
             for (int off = QIW_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, QIW_WAVE);
             if (lane == 0) a.partial[(int64_t)task.out_index * ns + s] = v;
@@ -503,6 +535,269 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
 }
 
 // ---------------------------------------------------------------------------------------
+// The diagram evaluator for models with small sector blocks (all m_s <= MB, MB <= 8).
+//
+// One lane = one qMC sample, as in the scalar walk.  A matrix chain X = O_n iP_n ... O_2
+// iP_2 B is evaluated one column at a time: the program of an initial sector s0 is a
+// segment that the wavefront replays once for every column j < m_s0, starting from the
+// unit vector e_j (or column j of the correlator operator B).  Along the trie every lane
+// keeps, per level, the vector y = iP_L[s_par] * x_par of its own sample in registers;
+// a child edge applies its operator block O (uniform: read through the scalar cache) and
+// passes x = O y one level down.  Products of the arc values i*Delta travel as a scalar
+// weight beside the vector and meet the leaf coefficient at the leaf.
+//
+// Propagator lookups read (value, slope) pairs of the grid table, so one 16-byte load and
+// one FMA give the linearly interpolated i*P entry at the lane's own time difference.
+// Vector entries beyond the current block size hold finite values that are never read.
+// ---------------------------------------------------------------------------------------
+
+// (T[j][e], T[j+1][e] - T[j][e]) for every grid interval j < n_tau - 1 and BT element e.
+__global__ void k_ptab_pairs(const double *__restrict__ T, int n_tau, int bt, double2 *__restrict__ out) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (int64_t)(n_tau - 1) * bt) return;
+    const double v0 = T[r], v1 = T[r + bt];
+    out[r] = make_double2(v0, v1 - v0);
+}
+
+template <int MB, int KM, int TM>
+struct VecState {
+    static constexpr int NM = 2 * KM + 3;
+    int row[NM - 1];      // BT offset of the grid interval of segment L (bold)
+    double f[NM - 1];     // interpolation fraction (bold) or time difference (bare)
+    double W[KM * TM];    // i*Delta_tab(t_tail - t_head) per arc and table
+    double acc[MB];       // column of the current initial sector
+    ProgStream ps;
+    const double2 *__restrict__ pt;
+    const double *__restrict__ ediag;
+    const double *__restrict__ ops;
+};
+
+// y = iP_L[s] x for the m x m block of sector s at BT offset off.
+template <int L, int MB, int KM, int TM, bool BARE>
+__device__ __forceinline__ void prop_apply(const VecState<MB, KM, TM> &st, const double (&x)[MB], int m, int off,
+                                           double (&y)[MB]) {
+    if constexpr (BARE) {  // exact atomic propagator: diagonal exp(-dt (E + lambda))
+        const double dt = st.f[L - 2];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) y[i] = (i < m) ? exp(-dt * st.ediag[off + i * (m + 1)]) * x[i] : 0.0;
+    } else {
+        const double2 *__restrict__ p = st.pt + st.row[L - 2] + off;
+        const double f = st.f[L - 2];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) y[i] = 0.0;
+#pragma unroll
+        for (int q = 0; q < MB; ++q) {
+            if (q < m) {
+                const double xq = x[q];
+#pragma unroll
+                for (int i = 0; i < MB; ++i) {
+                    const double2 e = p[q * m + i];
+                    y[i] = fma(fma(f, e.y, e.x), xq, y[i]);
+                }
+            }
+        }
+    }
+}
+
+template <int L, int MB, int KM, int TM, bool BARE>
+__device__ __forceinline__ void vwalk(VecState<MB, KM, TM> &st, int n, const double (&xp)[MB], double wp, int mp,
+                                      int offp, int nchild) {
+    if constexpr (L <= 2 * KM + 3) {
+        double y[MB];
+        prop_apply<L, MB, KM, TM, BARE>(st, xp, mp, offp, y);
+        for (int c = 0; c < nchild; ++c) {
+            const uint64_t w = st.ps.next();
+            const double wv = (w & QIW_F_TAIL) ? wp * pick(st.W, (int)((w >> 16) & 0xFF)) : wp;
+            const int mo = (int)(w & 0x3F);
+            const int blk = (int)((w >> 32) & 0xFFFF);
+            double x[MB];
+            if (blk == 0xFFFF) {
+#pragma unroll
+                for (int i = 0; i < MB; ++i) x[i] = y[i];
+            } else {
+                const double *__restrict__ O = st.ops + blk;  // m_o x m_p, column-major
+#pragma unroll
+                for (int i = 0; i < MB; ++i) x[i] = 0.0;
+#pragma unroll
+                for (int q = 0; q < MB; ++q) {
+                    if (q < mp) {
+#pragma unroll
+                        for (int i = 0; i < MB; ++i) x[i] = fma(O[q * mo + i], y[q], x[i]);
+                    }
+                }
+            }
+            if (w & QIW_F_LEAF) {
+                const double cw = __longlong_as_double((long long)st.ps.next()) * wv;
+#pragma unroll
+                for (int i = 0; i < MB; ++i) st.acc[i] = fma(cw, x[i], st.acc[i]);
+            } else if (L < n) {
+                vwalk<L + 1, MB, KM, TM, BARE>(st, n, x, wv, mo, (int)(w >> 52), (int)((w >> 24) & 0xFF));
+            }
+        }
+    }
+}
+
+template <int MB, int KM, int TM, bool BARE>
+__device__ __forceinline__ void vec_program(VecState<MB, KM, TM> &st, const EvalArgs &a, const QiwGroup &g,
+                                            const double (&t)[2 * KM + 4], double inv_h, bool active, double wt,
+                                            double &absum, double *__restrict__ prow) {
+    const int n = g.n_nodes, k = g.order;
+    const int ntab = a.dtab.n_tables;
+    const uint64_t *seg = nullptr;
+    int s_m0 = 1, s_off0 = 0, col = 0;
+    for (;;) {
+        const uint64_t w = st.ps.next();
+        const int op = (int)((w >> 48) & 0xF);
+        if (op == QIW_OP_ROOT) {
+            // position 1: e_col, or column col of the operator B (correlators)
+            const int m1 = (int)(w & 0x3F);
+            const int blk = (int)((w >> 32) & 0xFFFF);
+            double x[MB];
+            if (blk == 0xFFFF) {
+#pragma unroll
+                for (int i = 0; i < MB; ++i) x[i] = (i == col) ? 1.0 : 0.0;
+            } else {
+#pragma unroll
+                for (int i = 0; i < MB; ++i) x[i] = st.ops[blk + col * m1 + i];
+            }
+            vwalk<2, MB, KM, TM, BARE>(st, n, x, 1.0, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF));
+        } else if (op == QIW_OP_TOPO) {
+#pragma unroll
+            for (int ai = 0; ai < KM; ++ai) {
+                const int head = (int)((w >> (8 * ai)) & 0xF), tail = (int)((w >> (8 * ai + 4)) & 0xF);
+                double th = 0.0, tt = 0.0;
+#pragma unroll
+                for (int p = 1; p <= 2 * KM + 3; ++p) {
+                    th = (p == head) ? t[p] : th;
+                    tt = (p == tail) ? t[p] : tt;
+                }
+                double z = tt - th;
+                z = z > 0.0 ? z : 0.0;
+                const double xx = z * inv_h;
+                int j = (int)floor(xx);
+                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+                const double u = xx - (double)j;
+#pragma unroll
+                for (int tb = 0; tb < TM; ++tb) {
+                    if (ai < k && tb < ntab) {
+                        const double *c = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
+                        st.W[ai * TM + tb] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                    }
+                }
+            }
+        } else if (op == QIW_OP_SEG) {  // initial sector s0: m0 columns, BT offset
+            s_m0 = (int)(w & 0x3F);
+            s_off0 = (int)(w >> 52);
+            col = 0;
+            seg = st.ps.base + st.ps.idx;
+#pragma unroll
+            for (int i = 0; i < MB; ++i) st.acc[i] = 0.0;
+        } else if (op == QIW_OP_FLUSH) {  // column col of the s0 block is complete
+#pragma unroll
+            for (int i = 0; i < MB; ++i) {
+                if (i < s_m0) {
+                    absum += fabs(st.acc[i]);
+                    double v = active ? st.acc[i] * wt : 0.0;
+#pragma unroll
+                    for (int o = QIW_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, QIW_WAVE);
+                    if (threadIdx.x == 0) prow[s_off0 + col * s_m0 + i] = v;
+                }
+                st.acc[i] = 0.0;
+            }
+            if (++col < s_m0) st.ps.init(seg, threadIdx.x);
+        } else if (op == QIW_OP_END) {
+            break;
+        }
+    }
+}
+
+template <int MB, int KM, int TM>
+__global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
+    constexpr int NM = 2 * KM + 3;
+    const EvalArgs &a = va.e;
+    const int lane = threadIdx.x;
+    const QiwTask task = a.tasks[blockIdx.x];
+    const QiwGroup g = a.groups[task.group];
+    const int n = g.n_nodes;
+    const double inv_h = 1.0 / a.h;
+
+    double tau_i = 0.0, tau_w, tau_f;
+    if (g.kind == QIW_G_BARE) {
+        tau_w = 0.0;
+        tau_f = a.h;
+    } else if (g.kind == QIW_G_BOLD) {
+        tau_w = a.step_w * a.h;
+        tau_f = (a.step_w + 1) * a.h;
+    } else {
+        tau_w = task.tau_index * a.h;
+        tau_f = a.beta;
+    }
+    const int64_t sample = (int64_t)task.first_sample + lane;
+    const bool active = sample < g.n_samples;
+
+    double t[NM + 1];
+    const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
+#pragma unroll
+    for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
+    if (g.sampled) {
+        const int64_t N = g.n_samples;
+        const double *u = a.ut + g.ut_off + (active ? sample : 0);
+        const double dfw = tau_f - tau_w, dwi = tau_w - tau_i;
+#pragma unroll
+        for (int j = 0; j < 2 * KM; ++j) {
+            if (j < g.d_after) {
+                const double v = u[j * N] * dfw + tau_w;
+#pragma unroll
+                for (int p = 1; p <= NM; ++p)
+                    if (p == n - 1 - j) t[p] = v;
+            } else if (j < g.d_after + g.d_before) {
+                const double v = u[j * N] * dwi + tau_i;
+                const int pos = g.pos_w - 1 - (j - g.d_after);
+#pragma unroll
+                for (int p = 1; p <= NM; ++p)
+                    if (p == pos) t[p] = v;
+            }
+        }
+    }
+
+    VecState<MB, KM, TM> st;
+    st.pt = va.pt;
+    st.ediag = va.ediag;
+    st.ops = va.ops;
+#pragma unroll
+    for (int L = 2; L <= NM; ++L) {
+        double dt = t[L] - t[L - 1];
+        dt = dt > 0.0 ? dt : 0.0;
+        if (g.kind == QIW_G_BARE) {
+            st.f[L - 2] = dt;
+            st.row[L - 2] = 0;
+        } else {
+            const double x = dt * inv_h;
+            int j = (int)floor(x);
+            j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+            st.f[L - 2] = x - (double)j;
+            st.row[L - 2] = j * va.bt;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < KM * TM; ++q) st.W[q] = 0.0;
+#pragma unroll
+    for (int i = 0; i < MB; ++i) st.acc[i] = 0.0;
+
+    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples +
+                                        (active ? sample : 0)]
+                                 : 1.0;
+    double absum = 0.0;
+    double *prow = a.partial + (int64_t)task.out_index * va.bt;
+    st.ps.init(a.prog + g.prog_off, lane);
+    if (g.kind == QIW_G_BARE)
+        vec_program<MB, KM, TM, true>(st, a, g, t, inv_h, active, wt, absum, prow);
+    else
+        vec_program<MB, KM, TM, false>(st, a, g, t, inv_h, active, wt, absum, prow);
+    if (a.per_sample && active) a.per_sample[g.ps_off + sample] = absum;
+}
+
+// ---------------------------------------------------------------------------------------
 // Device-side step bookkeeping: reduce the partial sums of every group in fixed task order,
 // apply the integration prefactors, form the order-resolved and total step results
 // (inchworm_step, src/inchworm.jl:142-202), store i*P(tau_f) (set_ppgf!,
@@ -622,6 +917,20 @@ hipError_t qiw_launch_simplex_coords(hipStream_t st, const uint32_t *m, const ui
     return hipGetLastError();
 }
 
+hipError_t qiw_launch_sobol_cube(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int stride,
+                                 int64_t n, double *out) {
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_sobol_cube, dim3(blocks), dim3(256), 0, st, m, x0, dim, stride, n, out);
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_warp_coords(hipStream_t st, const double *z, int stride, const double *logj, int d_before,
+                                  int d_after, int64_t n, double *out) {
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_warp_coords, dim3(blocks), dim3(256), 0, st, z, stride, logj, d_before, d_after, n, out);
+    return hipGetLastError();
+}
+
 hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
                                      int d_after, double t_i, double t_w, double t_f, int64_t n, double *out) {
     int blocks = (int)((n + 255) / 256);
@@ -640,6 +949,23 @@ hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks
         hipLaunchKernelGGL((k_eval_scalar_t<QIW_KMAX, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// Register budget of the vector walk: MB = largest block, KM = largest order, TM = tables.
+hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm) {
+    if (mb <= 6 && km <= 4 && tm <= 2)
+        hipLaunchKernelGGL((k_eval_vec_t<6, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+    else if (mb <= 6 && km <= 4 && tm <= 4)
+        hipLaunchKernelGGL((k_eval_vec_t<6, 4, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out) {
+    int64_t n = (int64_t)(n_tau - 1) * bt;
+    hipLaunchKernelGGL(k_ptab_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n_tau, bt, out);
     return hipGetLastError();
 }
 

@@ -7,3 +7,5 @@ happens here; all sampling and diagram evaluation runs in the native library on 
 from . import abi, diagrammatics, model, sobol  # noqa: F401
 from .inchworm import (context, correlator_2p, density_matrix, inchworm, inchworm_step,  # noqa: F401
                        inchworm_step_bare, normalize, partition_function, stats)
+from .warp import (group_index, warp_apply, warp_enable, warp_init, warp_loss_grad,  # noqa: F401
+                   warp_params, warp_train)

@@ -22,6 +22,7 @@ QIW_PREC_FP64, QIW_PREC_FP32 = 0, 1
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
+_f32p = C.POINTER(C.c_float)
 
 
 class qiw_sbm(C.Structure):
@@ -41,6 +42,12 @@ class qiw_run(C.Structure):
     _fields_ = [("n_orders", C.c_int32), ("orders", _i32p), ("n_orders_bare", C.c_int32),
                 ("orders_bare", _i32p), ("N_samples", C.c_int64), ("n_pts_after_max", C.c_int32),
                 ("precision", C.c_int32)]
+
+
+class qiw_warp_config(C.Structure):
+    _fields_ = [("n_layers", C.c_int32), ("n_bins", C.c_int32), ("hidden", C.c_int32),
+                ("iters", C.c_int32), ("batch", C.c_int32), ("n_train_steps", C.c_int32),
+                ("n_train", C.c_int64), ("lr", C.c_double), ("seed", C.c_uint64)]
 
 
 class qiw_stats(C.Structure):
@@ -119,7 +126,9 @@ class RunStruct:
 EXPORTS = ["qiw_create", "qiw_destroy", "qiw_last_error", "qiw_abi_version", "qiw_set_model",
            "qiw_set_sobol", "qiw_set_ppgf", "qiw_get_ppgf", "qiw_inchworm", "qiw_inchworm_step",
            "qiw_inchworm_step_bare", "qiw_correlator_2p", "qiw_partition_function",
-           "qiw_density_matrix", "qiw_sobol_points", "qiw_simplex_points", "qiw_get_stats"]
+           "qiw_density_matrix", "qiw_sobol_points", "qiw_simplex_points", "qiw_get_stats",
+           "qiw_warp_init", "qiw_warp_params", "qiw_warp_apply", "qiw_warp_loss_grad", "qiw_warp_train",
+           "qiw_warp_enable"]
 
 _lib = None
 
@@ -154,6 +163,12 @@ def load_library(path=None):
     lib.qiw_simplex_points.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                        C.c_double, C.c_int64, _f64p]
     lib.qiw_get_stats.argtypes = [vp, C.POINTER(qiw_stats)]
+    lib.qiw_warp_init.argtypes = [vp, C.POINTER(qiw_warp_config)]
+    lib.qiw_warp_params.argtypes = [vp, C.POINTER(C.c_int64), _f32p, _f32p]
+    lib.qiw_warp_apply.argtypes = [vp, C.c_int32, C.c_int64, _f64p, _i32p, _i32p, _f64p, _f64p, _f64p]
+    lib.qiw_warp_loss_grad.argtypes = [vp, C.c_int64, _f64p, _i32p, _i32p, _f64p, _f64p, _f64p, _f32p]
+    lib.qiw_warp_train.argtypes = [vp, C.POINTER(qiw_run), _f64p]
+    lib.qiw_warp_enable.argtypes = [vp, C.c_int32]
     _lib = lib
     return lib
 
