@@ -12,7 +12,8 @@
 // Scalar programs (k_eval_scalar_t, all sectors 1x1), one word per trie node in preorder:
 // ROOT   bits  8-15 : sector after position 1, bits 24-31 : number of level-2 children;
 //        the next word is the constant of position 1 (float64 bits)
-// EDGE   bit   6    : LEAF (last position): the next word is the leaf coefficient (float64)
+// EDGE   bit   6    : LEAF (last position): the next word is the leaf coefficient (float64),
+//                     unless bit 5 (UNIT) says the coefficient is +1, or -1 with bit 4 (NEG)
 //        bit   7    : TAIL  multiply the arc value in W slot bits 16-23 (arc * n_tables + table)
 //        bits  8-15 : sector after this node's operator (propagator sector of its children)
 //        bits 24-31 : number of children
@@ -27,8 +28,8 @@
 //        bits 32-47 : operator block B (offset into the operator data; 0xFFFF identity),
 //        bits 52-63 : BT offset of that sector's block
 // EDGE   bits  0-5  : dimension m_o of the sector after the node's operator,
-//        bit 6 LEAF (coefficient word follows), bit 7 TAIL (W slot bits 16-23),
-//        bits  8-15 : that sector, bits 24-31 : children,
+//        bit 6 LEAF (coefficient word follows unless bit 9 UNIT: +1, or -1 with bit 8 NEG),
+//        bit 7 TAIL (W slot bits 16-23), bits 24-31 : children,
 //        bits 32-47 : operator block (offset into the operator data; 0xFFFF identity),
 //        bits 52-63 : BT offset of the sector's block
 // FLUSH  the column of s0 is complete
@@ -36,6 +37,12 @@ enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3, QIW_O
        QIW_OP_SEG = 6 };
 #define QIW_F_LEAF (1ull << 6)
 #define QIW_F_TAIL (1ull << 7)
+// Leaf coefficients of magnitude 1 carry no coefficient word: UNIT, with NEG for -1
+// (scalar programs: bits 4, 5; vector programs: bits 8, 9)
+#define QIW_FS_NEG (1ull << 4)
+#define QIW_FS_UNIT (1ull << 5)
+#define QIW_FV_NEG (1ull << 8)
+#define QIW_FV_UNIT (1ull << 9)
 
 // Kinds of integration group
 enum { QIW_G_BOLD = 0, QIW_G_BARE = 1, QIW_G_CORR = 2 };

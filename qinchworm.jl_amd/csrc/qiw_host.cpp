@@ -289,7 +289,7 @@ struct Compiler {
         if (fmt == FMT_SCALAR) {  // structured walk: sector after the node, child count
             w = ((uint64_t)t.s_out << 8) | ((uint64_t)t.wslot << 16) | ((uint64_t)t.children.size() << 24);
         } else if (fmt == FMT_VEC) {
-            w = (uint64_t)m.dims[t.s_out] | ((uint64_t)t.s_out << 8) | ((uint64_t)t.wslot << 16) |
+            w = (uint64_t)m.dims[t.s_out] | ((uint64_t)t.wslot << 16) |
                 ((uint64_t)t.children.size() << 24) | (op_offset(t.blk) << 32) |
                 ((uint64_t)m.bt_off[t.s_out] << 52);
         } else {
@@ -298,9 +298,12 @@ struct Compiler {
         }
         if (t.tail) w |= QIW_F_TAIL;
         if (t.leaf) w |= QIW_F_LEAF;
+        const bool unit = t.leaf && fmt != FMT_BLOCK && std::fabs(t.coef) == 1.0;
+        if (unit) w |= (fmt == FMT_SCALAR ? QIW_FS_UNIT : QIW_FV_UNIT) | (t.coef < 0 ? (fmt == FMT_SCALAR ? QIW_FS_NEG : QIW_FV_NEG) : 0);
         out.edges_per_sample++;
         out.prog.push_back(w);
-        if (t.leaf) {  // the leaf coefficient follows as a raw float64 word
+        if (unit) out.leaves_per_sample++;
+        if (t.leaf && !unit) {  // the leaf coefficient follows as a raw float64 word
             uint64_t cw;
             std::memcpy(&cw, &t.coef, 8);
             out.prog.push_back(cw);

@@ -201,7 +201,10 @@ __device__ __forceinline__ void walk(ScalarState<KM, SM, TM> &st, int n, double 
             const uint64_t w = st.ps.next();
             const double v = (w & QIW_F_TAIL) ? base * pick(st.W, (int)((w >> 16) & 0xFF)) : base;
             if (w & QIW_F_LEAF) {
-                st.acc += __longlong_as_double((long long)st.ps.next()) * v;
+                if (w & QIW_FS_UNIT)
+                    st.acc += (w & QIW_FS_NEG) ? -v : v;
+                else
+                    st.acc += __longlong_as_double((long long)st.ps.next()) * v;
             } else if (L < n) {
                 walk<L + 1, KM, SM, TM>(st, n, v, (int)((w >> 8) & 0xFF), (int)((w >> 24) & 0xFF));
             }
@@ -627,7 +630,8 @@ __device__ __forceinline__ void vwalk(VecState<MB, KM, TM> &st, int n, const dou
                 }
             }
             if (w & QIW_F_LEAF) {
-                const double cw = __longlong_as_double((long long)st.ps.next()) * wv;
+                const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -wv : wv)
+                                                    : __longlong_as_double((long long)st.ps.next()) * wv;
 #pragma unroll
                 for (int i = 0; i < MB; ++i) st.acc[i] = fma(cw, x[i], st.acc[i]);
             } else if (L < n) {
