@@ -160,8 +160,10 @@ struct BlockArgs {
 struct VecArgs {
     EvalArgs e;
     const double2 *pt;         // [n_tau-1][bt] (value, slope) of the i*P table (bold groups)
+    const float2 *pt32;        // the same in float32
     const double *ediag;       // [bt] E + lambda on the diagonal positions (bare groups)
     const double *ops;         // operator blocks, column-major, padded by 64 entries
+    const float *ops32;        // the same in float32
     int bt;
 };
 
@@ -178,7 +180,7 @@ hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const ui
                                      int d_after, double t_i, double t_w, double t_f, int64_t n, double *out);
 hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
 hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes);
-hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm);
-hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out);
+hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm, int fp32);
+hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, float2 *out32);
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);

@@ -441,6 +441,7 @@ struct Batch {
     std::vector<int32_t> g_first, g_count, g_slot;  // per group: task rows, order slot
     int lds = 0;
     int fmt = FMT_SCALAR;  // program format of every group of the batch
+    int precision = QIW_PREC_FP64;
     double flops_per_launch = 0;  // algorithmic FLOPs of one launch
     int64_t leaves_per_launch = 0, diagram_evals_per_launch = 0;
 };
@@ -544,6 +545,16 @@ int check_run(qiw_ctx *c, const qiw_run *run) {
     return 0;
 }
 
+// Heaviest groups first: workgroups are dispatched in task order, so the long tasks do
+// not form the tail of the launch.  Each task keeps its output row, so the summation
+// order of the results does not change.
+void order_tasks(Batch &b) {
+    std::stable_sort(b.tasks.begin(), b.tasks.end(), [&](const QiwTask &x, const QiwTask &y) {
+        const Compiled &gx = b.groups[x.group], &gy = b.groups[y.group];
+        return gx.edges_per_sample + gx.leaves_per_sample > gy.edges_per_sample + gy.leaves_per_sample;
+    });
+}
+
 std::vector<int> orders_all(const qiw_run *run) {
     std::vector<int> v(run->orders, run->orders + run->n_orders);
     v.insert(v.end(), run->orders_bare, run->orders_bare + run->n_orders_bare);
@@ -560,6 +571,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
     int max_order = 0;
     for (int oi = 0; oi < n_o; ++oi) max_order = std::max(max_order, (int)ord[oi]);
     b.fmt = program_format(m, max_order);
+    b.precision = run->precision;
     for (int oi = 0; oi < n_o; ++oi) {
         int order = ord[oi];
         if (order > 0 && run->N_samples <= 0) continue;
@@ -594,6 +606,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             b.groups.push_back(std::move(cg));
         }
     }
+    order_tasks(b);
     return 0;
 }
 
@@ -616,7 +629,7 @@ int ensure_model_buffers(qiw_ctx *c, double **dcoef, int32_t **dkind, double **e
 }
 
 // Launches the evaluator matching the model's sector structure.
-int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt) {
+int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precision) {
     const HostModel &m = c->m;
     const size_t lds_bytes = (size_t)a.lds_slots * sizeof(double);
     if (lds_bytes > 160 * 1024) return fail(c, QIW_E_LIMIT, "evaluator state exceeds the 160 KiB of LDS of a CU");
@@ -632,10 +645,15 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt) {
         double *ops = (double *)dev(c, "vec.ops", (m.op_data.size() + pad) * 8, &rc);
         double *ediag = (double *)dev(c, "vec.ediag", ((size_t)m.bt + 2 * pad) * 8, &rc);
         double2 *pt = (double2 *)dev(c, "vec.pt", ((size_t)(m.ntau - 1) * m.bt + pad) * 16, &rc);
+        float *ops32 = (float *)dev(c, "vec.ops32", (m.op_data.size() + pad) * 4, &rc);
+        float2 *pt32 = (float2 *)dev(c, "vec.pt32", ((size_t)(m.ntau - 1) * m.bt + pad) * 8, &rc);
         if (rc) return rc;
         if (!c->vec_staged) {  // once per model
             std::vector<double> o(m.op_data), ed((size_t)m.bt + 2 * pad, 0.0);
             o.resize(m.op_data.size() + pad, 0.0);
+            std::vector<float> o32(o.begin(), o.end());
+            HIPCHK(c, hipMemcpy(ops32, o32.data(), o32.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemset(pt32, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 8));
             for (int s = 0; s < m.ns; ++s)
                 for (int i = 0; i < m.dims[s]; ++i) ed[m.bt_off[s] + i + i * m.dims[s]] = m.E[s][i] + m.lambda0;
             HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -644,14 +662,17 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt) {
             HIPCHK(c, hipMemset(pt, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 16));
             c->vec_staged = true;
         }
-        HIPCHK(c, qiw_launch_ptab_pairs(c->stream, a.ptab, m.ntau, m.bt, pt));
+        HIPCHK(c, qiw_launch_ptab_pairs(c->stream, a.ptab, m.ntau, m.bt, pt, pt32));
         VecArgs v{};
         v.e = a;
         v.pt = pt;
+        v.pt32 = pt32;
         v.ediag = ediag;
         v.ops = ops;
+        v.ops32 = ops32;
         v.bt = m.bt;
-        HIPCHK(c, qiw_launch_eval_vec(c->stream, v, n_tasks, m.mmax, a.max_order, m.ntab));
+        HIPCHK(c, qiw_launch_eval_vec(c->stream, v, n_tasks, m.mmax, a.max_order, m.ntab,
+                                      precision == QIW_PREC_FP32 ? 1 : 0));
         return 0;
     }
     int rc = 0;
@@ -983,7 +1004,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     if (!bare_b.groups.empty()) {
         EvalArgs a = make_eval_args(c, bare_b, bare_d, ptab, eshift, dcoef, dkind, partial);
         HIPCHK(c, mark());
-        if ((rc = launch_eval(c, a, (int)bare_b.tasks.size(), bare_b.fmt))) return rc;
+        if ((rc = launch_eval(c, a, (int)bare_b.tasks.size(), bare_b.fmt, bare_b.precision))) return rc;
         HIPCHK(c, mark());
         fa.groups = bare_d.groups;
         fa.group_task_first = bare_d.g_first;
@@ -1015,7 +1036,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
             a.step_w = jw;
             if (warped && (rc = warp_points(c, jw * m.h / m.beta, bold_d.ut))) return rc;
             HIPCHK(c, mark());
-            if ((rc = launch_eval(c, a, (int)bold_b.tasks.size(), bold_b.fmt))) return rc;
+            if ((rc = launch_eval(c, a, (int)bold_b.tasks.size(), bold_b.fmt, bold_b.precision))) return rc;
             HIPCHK(c, mark());
             fa.jf = jw + 1;
             fa.tau_w = jw * m.h;
@@ -1169,6 +1190,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         int max_order = 0;
         for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
         b.fmt = program_format(m, max_order);
+        b.precision = run->precision;
         bool first_is_order0 = false;
         for (int oi = 0; oi < run->n_orders; ++oi) {
             int order = run->orders[oi];
@@ -1200,6 +1222,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
                 b.leaves_per_launch += N * b.groups[gi].leaves_per_sample;
                 b.diagram_evals_per_launch += N * b.groups[gi].n_top;
             }
+        order_tasks(b);
         DevBatch d;
         if ((rc = upload(c, "corr", b, run->N_samples, d))) return rc;
         double *partial = (double *)dev(c, "corr.partial", b.tasks.size() * m.bt * 8, &rc);
@@ -1215,7 +1238,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         HIPCHK(c, hipEventCreate(&e1));
         HIPCHK(c, hipEventRecord(e0, c->stream));
         EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
-        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt))) return rc;
+        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt, b.precision))) return rc;
         HIPCHK(c, hipEventRecord(e1, c->stream));
         CorrFinalizeArgs fa{};
         fa.groups = d.groups;
@@ -1368,7 +1391,7 @@ int warp_collect(qiw_ctx *c, const qiw_run *run, const std::vector<int> &steps, 
     a.per_sample = ps;
     for (size_t si = 0; si < steps.size(); ++si) {
         a.step_w = steps[si];
-        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt))) return rc;
+        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt, b.precision))) return rc;
         HIPCHK(c, hipMemcpyAsync(host.data(), ps, (size_t)ps_total * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (size_t gi = 0; gi < gs.size(); ++gi)

@@ -190,6 +190,19 @@ __device__ __forceinline__ double pick(const double (&v)[N], int i) {
     return r;
 }
 
+// pick for any register array type
+template <typename T, int N>
+__device__ __forceinline__ T pickv(const T (&v)[N], int i) {
+    T r = v[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) {
+        T x = v[q];
+        asm volatile("" : "+v"(x));
+        r = (i == q) ? x : r;
+    }
+    return r;
+}
+
 // Scalar trie, format of the structured walk (see qiw_device.h): every node word carries
 // the sector after its operator and the number of its children, so a node's children
 // share one propagator factor: base = parent * i*P_L[s_par].
@@ -547,45 +560,101 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
 // keeps, per level, the vector y = iP_L[s_par] * x_par of its own sample in registers;
 // a child edge applies its operator block O (uniform: read through the scalar cache) and
 // passes x = O y one level down.  Products of the arc values i*Delta travel as a scalar
-// weight beside the vector and meet the leaf coefficient at the leaf.
+// weight beside the vector and meet the leaf coefficient at the leaf.  The last propagator
+// iP_n[s0] is the same for every leaf of a column, so it is applied once, to the column
+// sum, when the column ends.
 //
-// Propagator lookups read (value, slope) pairs of the grid table, so one 16-byte load and
-// one FMA give the linearly interpolated i*P entry at the lane's own time difference.
+// Propagator lookups read (value, slope) pairs of the grid table, so one load and one FMA
+// give the linearly interpolated i*P entry at the lane's own time difference.
 // Vector entries beyond the current block size hold finite values that are never read.
+//
+// Value type V: float64, or float32 for the chain products (QIW_PREC_FP32: propagator
+// pairs, operator blocks, arc values and the per-level vectors in float32; the column
+// sums, the last propagator and everything after them in float64).
 // ---------------------------------------------------------------------------------------
 
-// (T[j][e], T[j+1][e] - T[j][e]) for every grid interval j < n_tau - 1 and BT element e.
-__global__ void k_ptab_pairs(const double *__restrict__ T, int n_tau, int bt, double2 *__restrict__ out) {
+// (T[j][e], T[j+1][e] - T[j][e]) for every grid interval j < n_tau - 1 and BT element e,
+// in float64 and float32.
+__global__ void k_ptab_pairs(const double *__restrict__ T, int n_tau, int bt, double2 *__restrict__ out,
+                             float2 *__restrict__ out32) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= (int64_t)(n_tau - 1) * bt) return;
     const double v0 = T[r], v1 = T[r + bt];
     out[r] = make_double2(v0, v1 - v0);
+    out32[r] = make_float2((float)v0, (float)(v1 - v0));
 }
 
-template <int MB, int KM, int TM>
+template <typename V>
+struct VecT;
+template <>
+struct VecT<double> {
+    typedef double2 P2;
+};
+template <>
+struct VecT<float> {
+    typedef float2 P2;
+};
+
+template <typename V, int MB, int KM, int TM>
 struct VecState {
     static constexpr int NM = 2 * KM + 3;
+    typedef typename VecT<V>::P2 P2;
     int row[NM - 1];      // BT offset of the grid interval of segment L (bold)
-    double f[NM - 1];     // interpolation fraction (bold) or time difference (bare)
-    double W[KM * TM];    // i*Delta_tab(t_tail - t_head) per arc and table
+    V f[NM - 1];          // interpolation fraction (bold)
+    double dtb[NM - 1];   // time difference (bare)
+    V W[KM * TM];         // i*Delta_tab(t_tail - t_head) per arc and table
     double acc[MB];       // column of the current initial sector
     ProgStream ps;
-    const double2 *__restrict__ pt;
+    const P2 *__restrict__ pt;
+    const double2 *__restrict__ pt64;
     const double *__restrict__ ediag;
-    const double *__restrict__ ops;
+    const V *__restrict__ ops;
+    int bt;
 };
 
 // y = iP_L[s] x for the m x m block of sector s at BT offset off.
-template <int L, int MB, int KM, int TM, bool BARE>
-__device__ __forceinline__ void prop_apply(const VecState<MB, KM, TM> &st, const double (&x)[MB], int m, int off,
-                                           double (&y)[MB]) {
+template <int L, typename V, int MB, int KM, int TM, bool BARE>
+__device__ __forceinline__ void prop_apply(const VecState<V, MB, KM, TM> &st, const V (&x)[MB], int m, int off,
+                                           V (&y)[MB]) {
     if constexpr (BARE) {  // exact atomic propagator: diagonal exp(-dt (E + lambda))
-        const double dt = st.f[L - 2];
+        const double dt = st.dtb[L - 2];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) y[i] = (i < m) ? (V)exp(-dt * st.ediag[off + i * (m + 1)]) * x[i] : (V)0;
+    } else {
+        const typename VecT<V>::P2 *__restrict__ p = st.pt + st.row[L - 2] + off;
+        const V f = st.f[L - 2];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) y[i] = (V)0;
+#pragma unroll
+        for (int q = 0; q < MB; ++q) {
+            if (q < m) {
+                const V xq = x[q];
+#pragma unroll
+                for (int i = 0; i < MB; ++i) {
+                    const auto e = p[q * m + i];
+                    y[i] = fma(fma(f, e.y, e.x), xq, y[i]);
+                }
+            }
+        }
+    }
+}
+
+// y = iP_n[s0] x for the last segment (runtime n), at the end of a column (float64)
+template <typename V, int MB, int KM, int TM, bool BARE>
+__device__ __forceinline__ void last_prop(const VecState<V, MB, KM, TM> &st, const double (&t)[2 * KM + 4], int n,
+                                          const double (&x)[MB], int m, int off, double inv_h, int n_tau,
+                                          double (&y)[MB]) {
+    double dt = pick(t, n) - pick(t, n - 1);
+    dt = dt > 0.0 ? dt : 0.0;
+    if constexpr (BARE) {
 #pragma unroll
         for (int i = 0; i < MB; ++i) y[i] = (i < m) ? exp(-dt * st.ediag[off + i * (m + 1)]) * x[i] : 0.0;
     } else {
-        const double2 *__restrict__ p = st.pt + st.row[L - 2] + off;
-        const double f = st.f[L - 2];
+        const double xx = dt * inv_h;
+        int j = (int)floor(xx);
+        j = j < 0 ? 0 : (j > n_tau - 2 ? n_tau - 2 : j);
+        const double f = xx - (double)j;
+        const double2 *__restrict__ p = st.pt64 + (int64_t)j * st.bt + off;
 #pragma unroll
         for (int i = 0; i < MB; ++i) y[i] = 0.0;
 #pragma unroll
@@ -602,25 +671,30 @@ __device__ __forceinline__ void prop_apply(const VecState<MB, KM, TM> &st, const
     }
 }
 
-template <int L, int MB, int KM, int TM, bool BARE>
-__device__ __forceinline__ void vwalk(VecState<MB, KM, TM> &st, int n, const double (&xp)[MB], double wp, int mp,
+template <int L, typename V, int MB, int KM, int TM, bool BARE>
+__device__ __forceinline__ void vwalk(VecState<V, MB, KM, TM> &st, int n, const V (&xp)[MB], V wp, int mp,
                                       int offp, int nchild) {
     if constexpr (L <= 2 * KM + 3) {
-        double y[MB];
-        prop_apply<L, MB, KM, TM, BARE>(st, xp, mp, offp, y);
+        V y[MB];
+        if (L == n) {  // leaves: iP_n[s0] is applied to the column sum
+#pragma unroll
+            for (int i = 0; i < MB; ++i) y[i] = xp[i];
+        } else {
+            prop_apply<L, V, MB, KM, TM, BARE>(st, xp, mp, offp, y);
+        }
         for (int c = 0; c < nchild; ++c) {
             const uint64_t w = st.ps.next();
-            const double wv = (w & QIW_F_TAIL) ? wp * pick(st.W, (int)((w >> 16) & 0xFF)) : wp;
+            const V wv = (w & QIW_F_TAIL) ? wp * pickv(st.W, (int)((w >> 16) & 0xFF)) : wp;
             const int mo = (int)(w & 0x3F);
             const int blk = (int)((w >> 32) & 0xFFFF);
-            double x[MB];
+            V x[MB];
             if (blk == 0xFFFF) {
 #pragma unroll
                 for (int i = 0; i < MB; ++i) x[i] = y[i];
             } else {
-                const double *__restrict__ O = st.ops + blk;  // m_o x m_p, column-major
+                const V *__restrict__ O = st.ops + blk;  // m_o x m_p, column-major
 #pragma unroll
-                for (int i = 0; i < MB; ++i) x[i] = 0.0;
+                for (int i = 0; i < MB; ++i) x[i] = (V)0;
 #pragma unroll
                 for (int q = 0; q < MB; ++q) {
                     if (q < mp) {
@@ -630,19 +704,19 @@ __device__ __forceinline__ void vwalk(VecState<MB, KM, TM> &st, int n, const dou
                 }
             }
             if (w & QIW_F_LEAF) {
-                const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -wv : wv)
-                                                    : __longlong_as_double((long long)st.ps.next()) * wv;
+                const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -(double)wv : (double)wv)
+                                                    : __longlong_as_double((long long)st.ps.next()) * (double)wv;
 #pragma unroll
-                for (int i = 0; i < MB; ++i) st.acc[i] = fma(cw, x[i], st.acc[i]);
+                for (int i = 0; i < MB; ++i) st.acc[i] = fma(cw, (double)x[i], st.acc[i]);
             } else if (L < n) {
-                vwalk<L + 1, MB, KM, TM, BARE>(st, n, x, wv, mo, (int)(w >> 52), (int)((w >> 24) & 0xFF));
+                vwalk<L + 1, V, MB, KM, TM, BARE>(st, n, x, wv, mo, (int)(w >> 52), (int)((w >> 24) & 0xFF));
             }
         }
     }
 }
 
-template <int MB, int KM, int TM, bool BARE>
-__device__ __forceinline__ void vec_program(VecState<MB, KM, TM> &st, const EvalArgs &a, const QiwGroup &g,
+template <typename V, int MB, int KM, int TM, bool BARE>
+__device__ __forceinline__ void vec_program(VecState<V, MB, KM, TM> &st, const EvalArgs &a, const QiwGroup &g,
                                             const double (&t)[2 * KM + 4], double inv_h, bool active, double wt,
                                             double &absum, double *__restrict__ prow) {
     const int n = g.n_nodes, k = g.order;
@@ -656,15 +730,15 @@ __device__ __forceinline__ void vec_program(VecState<MB, KM, TM> &st, const Eval
             // position 1: e_col, or column col of the operator B (correlators)
             const int m1 = (int)(w & 0x3F);
             const int blk = (int)((w >> 32) & 0xFFFF);
-            double x[MB];
+            V x[MB];
             if (blk == 0xFFFF) {
 #pragma unroll
-                for (int i = 0; i < MB; ++i) x[i] = (i == col) ? 1.0 : 0.0;
+                for (int i = 0; i < MB; ++i) x[i] = (i == col) ? (V)1 : (V)0;
             } else {
 #pragma unroll
                 for (int i = 0; i < MB; ++i) x[i] = st.ops[blk + col * m1 + i];
             }
-            vwalk<2, MB, KM, TM, BARE>(st, n, x, 1.0, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF));
+            vwalk<2, V, MB, KM, TM, BARE>(st, n, x, (V)1, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF));
         } else if (op == QIW_OP_TOPO) {
 #pragma unroll
             for (int ai = 0; ai < KM; ++ai) {
@@ -685,7 +759,7 @@ __device__ __forceinline__ void vec_program(VecState<MB, KM, TM> &st, const Eval
                 for (int tb = 0; tb < TM; ++tb) {
                     if (ai < k && tb < ntab) {
                         const double *c = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
-                        st.W[ai * TM + tb] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                        st.W[ai * TM + tb] = (V)(c[0] + u * (c[1] + u * (c[2] + u * c[3])));
                     }
                 }
             }
@@ -697,11 +771,13 @@ __device__ __forceinline__ void vec_program(VecState<MB, KM, TM> &st, const Eval
 #pragma unroll
             for (int i = 0; i < MB; ++i) st.acc[i] = 0.0;
         } else if (op == QIW_OP_FLUSH) {  // column col of the s0 block is complete
+            double y[MB];
+            last_prop<V, MB, KM, TM, BARE>(st, t, n, st.acc, s_m0, s_off0, inv_h, a.n_tau, y);
 #pragma unroll
             for (int i = 0; i < MB; ++i) {
                 if (i < s_m0) {
-                    absum += fabs(st.acc[i]);
-                    double v = active ? st.acc[i] * wt : 0.0;
+                    absum += fabs(y[i]);
+                    double v = active ? y[i] * wt : 0.0;
 #pragma unroll
                     for (int o = QIW_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, QIW_WAVE);
                     if (threadIdx.x == 0) prow[s_off0 + col * s_m0 + i] = v;
@@ -715,7 +791,7 @@ __device__ __forceinline__ void vec_program(VecState<MB, KM, TM> &st, const Eval
     }
 }
 
-template <int MB, int KM, int TM>
+template <typename V, int MB, int KM, int TM>
 __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
     constexpr int NM = 2 * KM + 3;
     const EvalArgs &a = va.e;
@@ -764,27 +840,35 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
         }
     }
 
-    VecState<MB, KM, TM> st;
-    st.pt = va.pt;
+    VecState<V, MB, KM, TM> st;
+    if constexpr (sizeof(V) == 8) {
+        st.pt = (const typename VecT<V>::P2 *)va.pt;
+        st.ops = (const V *)va.ops;
+    } else {
+        st.pt = (const typename VecT<V>::P2 *)va.pt32;
+        st.ops = (const V *)va.ops32;
+    }
+    st.pt64 = va.pt;
     st.ediag = va.ediag;
-    st.ops = va.ops;
+    st.bt = va.bt;
 #pragma unroll
     for (int L = 2; L <= NM; ++L) {
         double dt = t[L] - t[L - 1];
         dt = dt > 0.0 ? dt : 0.0;
+        st.dtb[L - 2] = dt;
         if (g.kind == QIW_G_BARE) {
-            st.f[L - 2] = dt;
+            st.f[L - 2] = (V)0;
             st.row[L - 2] = 0;
         } else {
             const double x = dt * inv_h;
             int j = (int)floor(x);
             j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-            st.f[L - 2] = x - (double)j;
+            st.f[L - 2] = (V)(x - (double)j);
             st.row[L - 2] = j * va.bt;
         }
     }
 #pragma unroll
-    for (int q = 0; q < KM * TM; ++q) st.W[q] = 0.0;
+    for (int q = 0; q < KM * TM; ++q) st.W[q] = (V)0;
 #pragma unroll
     for (int i = 0; i < MB; ++i) st.acc[i] = 0.0;
 
@@ -795,9 +879,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
     double *prow = a.partial + (int64_t)task.out_index * va.bt;
     st.ps.init(a.prog + g.prog_off, lane);
     if (g.kind == QIW_G_BARE)
-        vec_program<MB, KM, TM, true>(st, a, g, t, inv_h, active, wt, absum, prow);
+        vec_program<V, MB, KM, TM, true>(st, a, g, t, inv_h, active, wt, absum, prow);
     else
-        vec_program<MB, KM, TM, false>(st, a, g, t, inv_h, active, wt, absum, prow);
+        vec_program<V, MB, KM, TM, false>(st, a, g, t, inv_h, active, wt, absum, prow);
     if (a.per_sample && active) a.per_sample[g.ps_off + sample] = absum;
 }
 
@@ -956,20 +1040,28 @@ hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks
     return hipGetLastError();
 }
 
-// Register budget of the vector walk: MB = largest block, KM = largest order, TM = tables.
-hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm) {
-    if (mb <= 6 && km <= 4 && tm <= 2)
-        hipLaunchKernelGGL((k_eval_vec_t<6, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
-    else if (mb <= 6 && km <= 4 && tm <= 4)
-        hipLaunchKernelGGL((k_eval_vec_t<6, 4, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
-    else
+// Register budget of the vector walk: MB = largest block, KM = largest order, TM = tables;
+// fp32 selects float32 chain products.
+hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm, int fp32) {
+    if (mb <= 6 && km <= 4 && tm <= 2) {
+        if (fp32)
+            hipLaunchKernelGGL((k_eval_vec_t<float, 6, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+        else
+            hipLaunchKernelGGL((k_eval_vec_t<double, 6, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+    } else if (mb <= 6 && km <= 4 && tm <= 4) {
+        if (fp32)
+            hipLaunchKernelGGL((k_eval_vec_t<float, 6, 4, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+        else
+            hipLaunchKernelGGL((k_eval_vec_t<double, 6, 4, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+    } else {
         return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out) {
+hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, float2 *out32) {
     int64_t n = (int64_t)(n_tau - 1) * bt;
-    hipLaunchKernelGGL(k_ptab_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n_tau, bt, out);
+    hipLaunchKernelGGL(k_ptab_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n_tau, bt, out, out32);
     return hipGetLastError();
 }
 

@@ -82,14 +82,15 @@ def inchworm(expansion, grid, orders, orders_bare, N_samples, n_pts_after_max=No
     return {o: Po[i] for i, o in enumerate(oall)}, {o: Ps[i] for i, o in enumerate(oall)}
 
 
-def inchworm_step(expansion, n_w, orders, N_samples, n_pts_after_max=None, seq=None, ctx=None):
+def inchworm_step(expansion, n_w, orders, N_samples, n_pts_after_max=None, seq=None, ctx=None,
+                  precision=abi.QIW_PREC_FP64):
     """One regular step (src/inchworm.jl:123-203) with tau_i = grid[0], tau_w = grid[n_w],
     tau_f = grid[n_w + 1] on the current expansion.P.  Returns (value, per-order)."""
     ctx = ctx or context()
     orders = list(orders)
     _stage(ctx, expansion, max(orders + [0]), seq)
     _set_P(ctx, expansion)
-    run = abi.RunStruct(orders, [], N_samples, n_pts_after_max or 0)
+    run = abi.RunStruct(orders, [], N_samples, n_pts_after_max or 0, precision)
     val = np.zeros(expansion.bt_size, np.complex128)
     per = np.zeros((len(orders), expansion.bt_size), np.complex128)
     ctx.check(ctx.lib.qiw_inchworm_step(ctx.h, run.ref(), int(n_w), val.ctypes.data_as(abi._f64p),
@@ -110,14 +111,14 @@ def inchworm_step_bare(expansion, orders_bare, N_samples, seq=None, ctx=None):
     return val, per
 
 
-def correlator_2p(expansion, grid, orders, N_samples, seq=None, ctx=None):
+def correlator_2p(expansion, grid, orders, N_samples, seq=None, ctx=None, precision=abi.QIW_PREC_FP64):
     """correlator_2p (src/inchworm.jl:1075-1084): one array of n_tau complex values
     <A(tau) B(0)> per (A, B) pair of expansion.corr_operators."""
     ctx = ctx or context()
     orders = list(orders)
     _stage(ctx, expansion, max(orders + [0]), seq)
     _set_P(ctx, expansion)
-    run = abi.RunStruct(orders, [], N_samples)
+    run = abi.RunStruct(orders, [], N_samples, 0, precision)
     out = np.zeros((len(expansion.corr_operators), grid.n_tau), np.complex128)
     ctx.check(ctx.lib.qiw_correlator_2p(ctx.h, run.ref(), out.ctypes.data_as(abi._f64p)))
     return [out[i] for i in range(out.shape[0])]

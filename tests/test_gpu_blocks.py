@@ -61,3 +61,17 @@ def test_block_correlator():
     gc = O.correlator_2p(kc, range(0, 3), 2 ** 5)
     for a, b in zip(gg, gc):
         assert rel(a, b) < RTOL
+
+
+def test_block_fp32_chain_products():
+    """QIW_PREC_FP32: float32 chain products, float64 column sums and bookkeeping.  The
+    error against the float64 oracle is stated here: below 2e-6 relative for P and Z."""
+    kg, kc = kanamori2(n_tau=10), kanamori2(n_tau=10)
+    P_c, _, _ = O.inchworm(kc, range(0, 4), range(0, 4), 2 ** 6)
+    qiw.inchworm(kg, kg.grid, range(0, 4), range(0, 4), 2 ** 6, precision=qiw.abi.QIW_PREC_FP32)
+    err_P = rel(kg.P, P_c)
+    Zg, Zc = qiw.partition_function(kg), sum(np.trace(kc.block(P_c[-1], s)) * 1j for s in range(5))
+    err_Z = abs(Zg - Zc) / abs(Zc)
+    print(f"fp32 chain products: rel err P {err_P:.2e}, Z {err_Z:.2e}")
+    assert 0 < err_P < 2e-6 and err_Z < 2e-6
+    assert rel(kg.P, P_c) > 1e-13  # really float32 (differs from the float64 path)
