@@ -21,6 +21,62 @@ sys.path.insert(0, os.path.join(REPO, "qinchworm.jl_amd"))
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_eval_scalar_c1_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per evaluator launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (copy-calibrated), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def c2_expansion():
+    from qiw import model
+    return model.kanamori(2, 2.0, 0.2, -1.0, 10.0, 200, 0.5)
+
+
+def two_orbital(log2n=14, cpu_n=16, with_cpu=True):
+    """One bold step of the two-orbital Kanamori model (blocks 1,4,6,4,1; 8 pairs; beta=10,
+    n_tau=200; tau_w = grid[100]; orders 0:4) on the GPU in float64 and with float32 chain
+    products, and the float64 oracle on a bounded sample of the same step."""
+    import numpy as np
+    import qiw
+    from qiw import abi
+    k = c2_expansion()
+    N = 2 ** log2n
+    out = {"workload": f"two-orbital Kanamori (C2 model), one bold step tau_w=grid[100], orders 0:4, N=2^{log2n}"}
+    vals = {}
+    for name, prec in (("fp64", abi.QIW_PREC_FP64), ("fp32", abi.QIW_PREC_FP32)):
+        t0 = time.perf_counter()
+        v, _ = qiw.inchworm_step(k, 100, range(0, 5), N, precision=prec)
+        dt = time.perf_counter() - t0
+        st = qiw.stats()
+        vals[name] = v
+        out[f"gpu_{name}_s"] = dt
+        out[f"gpu_{name}_evals_per_s"] = st["diagram_evals"] / dt
+    out["fp32_rel_err_vs_fp64"] = float(np.linalg.norm(vals["fp32"] - vals["fp64"]) / np.linalg.norm(vals["fp64"]))
+    if with_cpu:
+        import oracle_lib as O
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        O.reset_counters()
+        kc = c2_expansion()
+        t0 = time.perf_counter()
+        O.inchworm_step(kc, range(0, 5), cpu_n, 100)
+        dt = time.perf_counter() - t0
+        _, evals, _ = O.counters()
+        out["cpu_evals_per_s"] = evals / dt
+        out["cpu_cores"] = threads
+        out["cpu_sample"] = f"same step at N={cpu_n}, {evals} diagram evals in {dt:.1f} s, float64 oracle"
+        out["speedup_fp64"] = out["gpu_fp64_evals_per_s"] / out["cpu_evals_per_s"]
+        out["speedup_fp32"] = out["gpu_fp32_evals_per_s"] / out["cpu_evals_per_s"]
+    return out
 
 
 def c1_expansion():
@@ -55,6 +111,58 @@ def cpu_baseline(N, steps=2):
                       f"{evals} diagram evals in {dt:.2f} s, OMP threads={threads}"}
 
 
+def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10, 17)), steps=(10, 25, 45),
+               iters=1500):
+    """Error versus samples of the learned warp against plain qMC on the two-orbital model
+    (C2 physics at n_tau=50, orders 0:2).  The propagator comes from a plain float64 run;
+    the flow is trained on |integrand| samples of that run; for three bold steps the judge is
+    the float64 step at 2^judge_log2n plain points (the GPU float64 path, equal to the
+    oracle to 1e-10 in the parity tests)."""
+    import numpy as np
+    import qiw
+    from qiw import model
+    k = model.kanamori(2, 2.0, 0.2, -1.0, 10.0, n_tau, 0.5)
+    orders = list(orders)
+    qiw.inchworm(k, k.grid, orders, orders, 2 ** 12)
+    P = k.P.copy()
+    qiw.warp_init(iters=iters, batch=8192, n_train_steps=8, n_train=2 ** 12, lr=2e-3, seed=1)
+    t0 = time.perf_counter()
+    hist = qiw.warp_train(k, orders)
+    train_s = time.perf_counter() - t0
+    err = {"plain": np.zeros((len(steps), len(log2ns))), "warped": np.zeros((len(steps), len(log2ns)))}
+    for si, jw in enumerate(steps):
+        k.P = P.copy()
+        qiw.warp_enable(False)
+        judge, _ = qiw.inchworm_step(k, jw, orders, 2 ** judge_log2n)
+        for ni, ln in enumerate(log2ns):
+            for name, on in (("plain", False), ("warped", True)):
+                qiw.warp_enable(on)
+                v, _ = qiw.inchworm_step(k, jw, orders, 2 ** ln)
+                err[name][si, ni] = np.linalg.norm(v - judge) / np.linalg.norm(judge)
+    qiw.warp_enable(False)
+    e_plain, e_warp = err["plain"].mean(axis=0), err["warped"].mean(axis=0)
+    # plain samples needed to reach the warped error, by log-log interpolation
+    lp = np.log2(e_plain)
+    gains = []
+    for ni, ln in enumerate(log2ns):
+        target = np.log2(e_warp[ni])
+        if target >= lp[0]:
+            gains.append(None)
+            continue
+        if target <= lp[-1]:
+            gains.append(float(2 ** (log2ns[-1] + (lp[-1] - target) / max(lp[-2] - lp[-1], 1e-9))
+                               / 2 ** ln))
+            continue
+        j = int(np.nonzero(lp <= target)[0][0])
+        x = log2ns[j - 1] + (lp[j - 1] - target) / (lp[j - 1] - lp[j])
+        gains.append(float(2 ** x / 2 ** ln))
+    return {"workload": f"two-orbital Kanamori, n_tau={n_tau}, orders {orders[0]}:{orders[-1]}, bold steps "
+                        f"{list(steps)}; judge: plain float64 at 2^{judge_log2n}",
+            "N": [2 ** ln for ln in log2ns], "rel_err_plain": e_plain.tolist(), "rel_err_warped": e_warp.tolist(),
+            "sample_gain": gains, "train_iters": iters, "train_s": train_s,
+            "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: 2 x mib MiB of HBM traffic per launch."""
     import torch
@@ -73,12 +181,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log2n", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c2", action="store_true", help="skip the two-orbital step measurement")
+    ap.add_argument("--warp-study", action="store_true",
+                    help="only run the learned-warp error-versus-samples study and print its JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE hardware counters) and exit")
     args = ap.parse_args()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
+        return
+    if args.warp_study:
+        print(json.dumps(warp_study()), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
@@ -126,12 +240,14 @@ def main():
                    "n_tau": 200, "orders": "0:4", "N_samples": N},
         "time_to_solution_s": ms_per_step / 1e3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "k_eval_scalar", "avg_launch_ms": avg_launch_ms,
                      "launches_per_step": launches / args.steps},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(N)
+    if not args.no_c2:
+        line["two_orbital_step"] = two_orbital(with_cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

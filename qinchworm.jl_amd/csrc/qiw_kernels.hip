@@ -209,7 +209,8 @@ __device__ __forceinline__ T pickv(const T (&v)[N], int i) {
 template <int L, int KM, int SM, int TM>
 __device__ __forceinline__ void walk(ScalarState<KM, SM, TM> &st, int n, double vpar, int s_par, int nchild) {
     if constexpr (L <= 2 * KM + 3) {
-        const double base = vpar * pick(st.P[L - 2], s_par);
+        // the last propagator iP_n[s0] is common to all leaves and applied at FLUSH
+        const double base = (L == n) ? vpar : vpar * pick(st.P[L - 2], s_par);
         for (int c = 0; c < nchild; ++c) {
             const uint64_t w = st.ps.next();
             const double v = (w & QIW_F_TAIL) ? base * pick(st.W, (int)((w >> 16) & 0xFF)) : base;
@@ -313,8 +314,11 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
                                 (int)((w >> 24) & 0xFF));
         } else if (op == QIW_OP_FLUSH) {
             const int s0 = (int)((w >> 8) & 0xFF);
+            double pn = 0.0;  // iP_n[s0]
 #pragma unroll
-            for (int s = 0; s < SM; ++s) st.out[s] += (s == s0) ? st.acc : 0.0;
+            for (int L = 2; L <= NM; ++L) pn = (L == n) ? pick(st.P[L - 2], s0) : pn;
+#pragma unroll
+            for (int s = 0; s < SM; ++s) st.out[s] += (s == s0) ? st.acc * pn : 0.0;
             st.acc = 0.0;
         } else if (op == QIW_OP_TOPO) {
             // arc values of one topology: arc ai has head at bits 8ai..8ai+3, tail at 8ai+4..
