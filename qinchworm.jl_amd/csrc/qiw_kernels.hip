@@ -162,18 +162,41 @@ struct ProgStream {
     }
 };
 
-// Register-resident state of one lane (one qMC sample).  All array indices are
-// compile-time constants inside the walk; values chosen by a uniform program field are
-// picked with select chains.
+// Register-resident state of one lane (one qMC sample).  Array indices are compile-time
+// constants inside the walk, or uniform program fields read through the index mode.
+// Register vectors indexed by a uniform (SGPR) index: the compiler reads them with
+// s_set_gpr_idx_on / v_mov / s_set_gpr_idx_off, a few instructions per pick instead of a
+// compare-and-select chain over every element.
+template <typename T, int N>
+struct RegVec;
+template <typename T>
+struct RegVec<T, 8> {
+    typedef T type __attribute__((ext_vector_type(8)));
+};
+template <typename T>
+struct RegVec<T, 16> {
+    typedef T type __attribute__((ext_vector_type(16)));
+};
+constexpr int regvec_size(int n) { return n <= 8 ? 8 : 16; }
+
 template <int KM, int SM, int TM>
 struct ScalarState {
     static constexpr int NM = 2 * KM + 3;
-    double P[NM - 1][SM];   // i*P_s(t_L - t_{L-1}) for L = 2..n
-    double W[KM * TM];      // i*Delta_tab(t_tail - t_head) per arc and table
+    static constexpr int LPC = 16 / SM;                      // segments per P chunk
+    static constexpr int NCH = (NM - 1 + LPC - 1) / LPC;     // P chunks
+    typename RegVec<double, 16>::type P[NCH];                // i*P_s(t_L - t_{L-1}), L = 2..n
+    typename RegVec<double, regvec_size(KM * TM)>::type W;   // i*Delta_tab per arc and table
     double out[SM];         // per initial sector sums
     double acc;
     ProgStream ps;
 };
+
+// i*P_s of segment L (compile time) for a uniform sector s
+template <int L, int KM, int SM, int TM>
+__device__ __forceinline__ double pget(const ScalarState<KM, SM, TM> &st, int s) {
+    using S = ScalarState<KM, SM, TM>;
+    return st.P[(L - 2) / S::LPC][((L - 2) % S::LPC) * SM + s];
+}
 
 // v[i] for a uniform i as a v_cndmask chain over registers.  The empty asm makes each
 // element an opaque register value, so the select chain is not rewritten into an indexed
@@ -210,10 +233,10 @@ template <int L, int KM, int SM, int TM>
 __device__ __forceinline__ void walk(ScalarState<KM, SM, TM> &st, int n, double vpar, int s_par, int nchild) {
     if constexpr (L <= 2 * KM + 3) {
         // the last propagator iP_n[s0] is common to all leaves and applied at FLUSH
-        const double base = (L == n) ? vpar : vpar * pick(st.P[L - 2], s_par);
+        const double base = (L == n) ? vpar : vpar * pget<L>(st, s_par);
         for (int c = 0; c < nchild; ++c) {
             const uint64_t w = st.ps.next();
-            const double v = (w & QIW_F_TAIL) ? base * pick(st.W, (int)((w >> 16) & 0xFF)) : base;
+            const double v = (w & QIW_F_TAIL) ? base * st.W[(int)((w >> 16) & 0xFF)] : base;
             if (w & QIW_F_LEAF) {
                 if (w & QIW_FS_UNIT)
                     st.acc += (w & QIW_FS_NEG) ? -v : v;
@@ -279,7 +302,12 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
         }
     }
 
-    ScalarState<KM, SM, TM> st;
+    using S = ScalarState<KM, SM, TM>;
+    S st;
+#pragma unroll
+    for (int c = 0; c < S::NCH; ++c)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) st.P[c][e] = 0.0;
     // ---- propagator values i*P_s(t_L - t_{L-1}), L = 2..n (ordering clamp) ----
 #pragma unroll
     for (int L = 2; L <= NM; ++L) {
@@ -287,7 +315,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
         dt = dt > 0.0 ? dt : 0.0;
         if (g.kind == QIW_G_BARE) {
 #pragma unroll
-            for (int s = 0; s < SM; ++s) st.P[L - 2][s] = (s < ns && L <= n) ? exp(-dt * a.e_shift[s < ns ? s : 0]) : 0.0;
+            for (int s = 0; s < SM; ++s)
+                st.P[(L - 2) / S::LPC][((L - 2) % S::LPC) * SM + s] =
+                    (s < ns && L <= n) ? exp(-dt * a.e_shift[s < ns ? s : 0]) : 0.0;
         } else {
             const double x = dt * inv_h;
             int j = (int)floor(x);
@@ -295,13 +325,15 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
             const double f = x - (double)j;
             const double *r0 = a.ptab + (int64_t)j * ns, *r1 = r0 + ns;
 #pragma unroll
-            for (int s = 0; s < SM; ++s) st.P[L - 2][s] = (s < ns && L <= n) ? (1.0 - f) * r0[s] + f * r1[s] : 0.0;
+            for (int s = 0; s < SM; ++s)
+                st.P[(L - 2) / S::LPC][((L - 2) % S::LPC) * SM + s] =
+                    (s < ns && L <= n) ? (1.0 - f) * r0[s] + f * r1[s] : 0.0;
         }
     }
 #pragma unroll
     for (int s = 0; s < SM; ++s) st.out[s] = 0.0;
 #pragma unroll
-    for (int q = 0; q < KM * TM; ++q) st.W[q] = 0.0;
+    for (int q = 0; q < regvec_size(KM * TM); ++q) st.W[q] = 0.0;
     st.acc = 0.0;
 
     // ---- walk the program ----
@@ -316,7 +348,8 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
             const int s0 = (int)((w >> 8) & 0xFF);
             double pn = 0.0;  // iP_n[s0]
 #pragma unroll
-            for (int L = 2; L <= NM; ++L) pn = (L == n) ? pick(st.P[L - 2], s0) : pn;
+            for (int c = 0; c < S::NCH; ++c)
+                pn = (c == (n - 2) / S::LPC) ? st.P[c][((n - 2) % S::LPC) * SM + s0] : pn;
 #pragma unroll
             for (int s = 0; s < SM; ++s) st.out[s] += (s == s0) ? st.acc * pn : 0.0;
             st.acc = 0.0;
