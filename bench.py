@@ -112,7 +112,7 @@ def cpu_baseline(N, steps=2):
 
 
 def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10, 17)), steps=(10, 25, 45),
-               iters=1500):
+               iters=4000, n_train=2 ** 14):
     """Error versus samples of the learned warp against plain qMC on the two-orbital model
     (C2 physics at n_tau=50, orders 0:2).  The propagator comes from a plain float64 run;
     the flow is trained on |integrand| samples of that run; for three bold steps the judge is
@@ -125,7 +125,7 @@ def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10
     orders = list(orders)
     qiw.inchworm(k, k.grid, orders, orders, 2 ** 12)
     P = k.P.copy()
-    qiw.warp_init(iters=iters, batch=8192, n_train_steps=8, n_train=2 ** 12, lr=2e-3, seed=1)
+    qiw.warp_init(iters=iters, batch=8192, n_train_steps=8, n_train=n_train, lr=2e-3, seed=1)
     t0 = time.perf_counter()
     hist = qiw.warp_train(k, orders)
     train_s = time.perf_counter() - t0
@@ -159,7 +159,8 @@ def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10
     return {"workload": f"two-orbital Kanamori, n_tau={n_tau}, orders {orders[0]}:{orders[-1]}, bold steps "
                         f"{list(steps)}; judge: plain float64 at 2^{judge_log2n}",
             "N": [2 ** ln for ln in log2ns], "rel_err_plain": e_plain.tolist(), "rel_err_warped": e_warp.tolist(),
-            "sample_gain": gains, "train_iters": iters, "train_s": train_s,
+            "sample_gain": gains, "train_iters": iters, "train_samples_per_step_group": n_train,
+            "train_s": train_s,
             "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
 
 
