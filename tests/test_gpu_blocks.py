@@ -75,3 +75,19 @@ def test_block_fp32_chain_products():
     print(f"fp32 chain products: rel err P {err_P:.2e}, Z {err_Z:.2e}")
     assert 0 < err_P < 2e-6 and err_Z < 2e-6
     assert rel(kg.P, P_c) > 1e-13  # really float32 (differs from the float64 path)
+
+
+def kanamori3(n_tau=5):
+    # three-orbital Kanamori model (C3 physics): blocks 1, 6, 15, 20, 15, 6, 1, 12 pairs
+    return model.kanamori(3, 3.0, 0.3, 0.0, 20.0, n_tau, 0.4)
+
+
+def test_three_orbital_blocks():
+    """Blocks wider than the vector walk (up to 20) run the LDS block walk."""
+    kg, kc = kanamori3(), kanamori3()
+    v_gpu, per_gpu = qiw.inchworm_step_bare(kg, [0, 1, 2], 2 ** 3)
+    v_cpu, per_cpu = O.inchworm_step_bare(kc, [0, 1, 2], 2 ** 3)
+    assert rel(v_gpu, v_cpu) < RTOL
+    P_c, _, _ = O.inchworm(kc, range(0, 3), range(0, 3), 2 ** 3)
+    qiw.inchworm(kg, kg.grid, range(0, 3), range(0, 3), 2 ** 3)
+    assert rel(kg.P, P_c) < RTOL
