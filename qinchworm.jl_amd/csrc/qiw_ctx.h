@@ -44,6 +44,7 @@ struct HostModel {
     bool scalar = true;          // every sector is one-dimensional
     bool scalar_kernel = true;   // programs for k_eval_scalar_t (scalar model within its register budget)
     bool vec_ok = false;         // programs for k_eval_vec_t up to order 4 (blocks <= 6, <= 4 tables)
+    bool mfma_ok = false;        // the same programs on k_eval_mfma_t (blocks <= 32, <= 2 tables)
     int mmax = 1;
     std::vector<double> op_data;    // all operator blocks, column-major, concatenated
     std::vector<int32_t> op_desc;   // [n_blocks][3]: offset, m_f, m_i
@@ -78,7 +79,8 @@ struct qiw_ctx {
     std::string err;
     bool has_model = false;
     qiwi::HostModel m;
-    bool vec_staged = false;  // operator blocks of the vector walk are on the device
+    bool vec_staged = false;   // operator blocks of the vector walk are on the device
+    bool mfma_staged = false;  // operator blocks of the matrix-core walk are on the device
     // Sobol state
     int sob_dim = 0;
     std::vector<uint32_t> sob_m, sob_x;

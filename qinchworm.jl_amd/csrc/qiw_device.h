@@ -103,6 +103,7 @@ struct EvalArgs {
     int max_order;             // largest order of the launch's groups
     double *partial;           // [n_out][ns]
     double *per_sample;        // optional: sum of |integrand| over the outputs, per sample
+    int spt;                   // samples per task (64 for the lane-per-sample walks)
 };
 
 // Arguments of the step bookkeeping kernel (k_finalize_step)
@@ -167,6 +168,15 @@ struct VecArgs {
     int bt;
 };
 
+// Arguments of the matrix-core evaluator (k_eval_mfma_t)
+struct MfmaArgs {
+    EvalArgs e;
+    const float *ops;          // operator blocks, column-major, float32
+    const double *ediag;       // [bt] E + lambda on the diagonal positions (bare groups)
+    const uint8_t *emask;      // [bt] 1 on the diagonal positions
+    int bt;
+};
+
 // Launch wrappers (qiw_kernels.hip)
 hipError_t qiw_launch_sobol_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int64_t first,
                                    int64_t n, double *out);
@@ -180,6 +190,7 @@ hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const ui
                                      int d_after, double t_i, double t_w, double t_f, int64_t n, double *out);
 hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
 hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes);
+hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes);
 hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm, int fp32);
 hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, float2 *out32);
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);

@@ -183,9 +183,11 @@ enum NodeKind { N_PAIR_HEAD, N_PAIR_TAIL, N_IDENTITY, N_OPERATOR };
 // Program formats: the scalar structured walk, the LDS block walk, the vector walk.
 enum ProgFormat { FMT_SCALAR = 0, FMT_BLOCK = 1, FMT_VEC = 2 };
 
-int program_format(const HostModel &m, int max_order) {
+// Vector programs also drive the matrix-core walk of wide blocks (float32 chain products).
+int program_format(const HostModel &m, int max_order, int precision) {
     if (m.scalar_kernel) return FMT_SCALAR;
     if (m.vec_ok && max_order <= 4) return FMT_VEC;
+    if (m.mfma_ok && max_order <= 4 && precision == QIW_PREC_FP32) return FMT_VEC;
     return FMT_BLOCK;
 }
 
@@ -441,6 +443,7 @@ struct Batch {
     std::vector<int32_t> g_first, g_count, g_slot;  // per group: task rows, order slot
     int lds = 0;
     int fmt = FMT_SCALAR;  // program format of every group of the batch
+    int spt = QIW_WAVE;    // samples per task
     int precision = QIW_PREC_FP64;
     double flops_per_launch = 0;  // algorithmic FLOPs of one launch
     int64_t leaves_per_launch = 0, diagram_evals_per_launch = 0;
@@ -555,6 +558,16 @@ void order_tasks(Batch &b) {
     });
 }
 
+// Samples per task.  The scalar and vector walks map one sample to a lane (64 per task);
+// the block and matrix-core walks evaluate one sample at a time per wavefront, so their
+// tasks are cut smaller until a launch holds at least 4 tasks per SIMD of the device.
+int samples_per_task(const HostModel &m, int fmt, int64_t total_samples) {
+    if (fmt == FMT_SCALAR || (fmt == FMT_VEC && m.mmax <= 6)) return QIW_WAVE;
+    int spt = QIW_WAVE;
+    while (spt > 1 && total_samples / spt < 4096) spt /= 2;
+    return spt;
+}
+
 std::vector<int> orders_all(const qiw_run *run) {
     std::vector<int> v(run->orders, run->orders + run->n_orders);
     v.insert(v.end(), run->orders_bare, run->orders_bare + run->n_orders_bare);
@@ -569,8 +582,13 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
     int n_o = bare ? run->n_orders_bare : run->n_orders;
     const int32_t *ord = bare ? run->orders_bare : run->orders;
     int max_order = 0;
-    for (int oi = 0; oi < n_o; ++oi) max_order = std::max(max_order, (int)ord[oi]);
-    b.fmt = program_format(m, max_order);
+    int64_t n_groups = 0;
+    for (int oi = 0; oi < n_o; ++oi) {
+        max_order = std::max(max_order, (int)ord[oi]);
+        if (ord[oi] > 0) n_groups += bare ? 1 : 2 * ord[oi] - 1;
+    }
+    b.fmt = program_format(m, max_order, run->precision);
+    b.spt = samples_per_task(m, b.fmt, n_groups * std::max<int64_t>(run->N_samples, 1));
     b.precision = run->precision;
     for (int oi = 0; oi < n_o; ++oi) {
         int order = ord[oi];
@@ -595,9 +613,9 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             int64_t N = order > 0 ? run->N_samples : 1;
             int gi = (int)b.groups.size();
             b.g_first.push_back((int32_t)b.tasks.size());
-            int nt = (int)((N + QIW_WAVE - 1) / QIW_WAVE);
+            int nt = (int)((N + b.spt - 1) / b.spt);
             for (int t = 0; t < nt; ++t)
-                b.tasks.push_back({gi, t * QIW_WAVE, 0, (int32_t)b.tasks.size()});
+                b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size()});
             b.g_count.push_back(nt);
             b.g_slot.push_back(slot);
             b.flops_per_launch += (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
@@ -635,6 +653,41 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
     if (lds_bytes > 160 * 1024) return fail(c, QIW_E_LIMIT, "evaluator state exceeds the 160 KiB of LDS of a CU");
     if (fmt == FMT_SCALAR) {
         HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab));
+        return 0;
+    }
+    if (fmt == FMT_VEC && m.mmax > 6) {  // wide blocks: the matrix-core walk
+        int rc = 0;
+        const size_t pad = 64;
+        float *ops32 = (float *)dev(c, "mfma.ops32", (m.op_data.size() + pad) * 4, &rc);
+        double *ediag = (double *)dev(c, "mfma.ediag", ((size_t)m.bt + pad) * 8, &rc);
+        uint8_t *emask = (uint8_t *)dev(c, "mfma.emask", (size_t)m.bt + pad, &rc);
+        if (rc) return rc;
+        if (!c->mfma_staged) {  // once per model
+            std::vector<float> o32(m.op_data.begin(), m.op_data.end());
+            o32.resize(m.op_data.size() + pad, 0.0f);
+            std::vector<double> ed((size_t)m.bt + pad, 0.0);
+            std::vector<uint8_t> mk((size_t)m.bt + pad, 0);
+            for (int s = 0; s < m.ns; ++s)
+                for (int i = 0; i < m.dims[s]; ++i) {
+                    ed[m.bt_off[s] + i + i * m.dims[s]] = m.E[s][i] + m.lambda0;
+                    mk[m.bt_off[s] + i + i * m.dims[s]] = 1;
+                }
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipMemcpy(ops32, o32.data(), o32.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(ediag, ed.data(), ed.size() * 8, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(emask, mk.data(), mk.size(), hipMemcpyHostToDevice));
+            c->mfma_staged = true;
+        }
+        const int KM = a.max_order <= 3 ? 3 : 4, NM = 2 * KM + 3;
+        const size_t lds = ((((size_t)(NM - 1) * m.bt + 3) & ~(size_t)3) + 1024) * 4;
+        if (lds > 160 * 1024) return fail(c, QIW_E_LIMIT, "propagator blocks exceed the 160 KiB of LDS of a CU");
+        MfmaArgs mf{};
+        mf.e = a;
+        mf.ops = ops32;
+        mf.ediag = ediag;
+        mf.emask = emask;
+        mf.bt = m.bt;
+        HIPCHK(c, qiw_launch_eval_mfma(c->stream, mf, n_tasks, a.max_order, m.ntab, lds));
         return 0;
     }
     if (fmt == FMT_VEC) {
@@ -727,6 +780,7 @@ EvalArgs make_eval_args(qiw_ctx *c, const Batch &bt, const DevBatch &d, double *
     a.lds_slots = lds;
     a.step_w = 0;
     a.partial = partial;
+    a.spt = bt.spt;
     return a;
 }
 
@@ -872,9 +926,12 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
     if (m.op_data.empty()) m.op_data.push_back(0.0);
     m.vec_ok = !m.scalar_kernel && m.mmax <= 6 && m.ntab <= 4 && m.ns <= 255 && m.bt < 4096 &&
                m.op_data.size() < 0xFFFF - 64;
+    m.mfma_ok = !m.scalar_kernel && m.mmax <= 32 && m.ntab <= 2 && m.ns <= 255 && m.bt < 4096 &&
+                m.op_data.size() < 0xFFFF - 64;
     c->m = m;
     c->has_model = true;
     c->vec_staged = false;
+    c->mfma_staged = false;
     atomic_ip(c->m, c->ip);
     c->n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? 1 : m.ntau;
     c->der0.assign(m.bt, cd(0.0));  // initial_ppgf_derivative: -i diag(-E - lambda), src/ppgf.jl:710-720
@@ -1189,7 +1246,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         Batch b;
         int max_order = 0;
         for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
-        b.fmt = program_format(m, max_order);
+        b.fmt = program_format(m, max_order, run->precision);
         b.precision = run->precision;
         bool first_is_order0 = false;
         for (int oi = 0; oi < run->n_orders; ++oi) {
@@ -1208,16 +1265,17 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         }
         if (b.groups.empty()) continue;
         int ng = (int)b.groups.size();
+        b.spt = samples_per_task(m, b.fmt, (int64_t)m.ntau * ng * std::max<int64_t>(run->N_samples, 1));
         std::vector<int32_t> rf((size_t)m.ntau * ng, 0), rcnt((size_t)m.ntau * ng, 0);
         for (int j = 0; j < m.ntau; ++j)
             for (int gi = 0; gi < ng; ++gi) {
                 // tau_A = tau_B: only the leading order-0 group (src/inchworm.jl:1014-1024)
                 if (j == 0 && !(gi == 0 && first_is_order0)) continue;
                 int64_t N = b.groups[gi].g.order > 0 ? run->N_samples : 1;
-                int nt = (int)((N + QIW_WAVE - 1) / QIW_WAVE);
+                int nt = (int)((N + b.spt - 1) / b.spt);
                 rf[(size_t)j * ng + gi] = (int32_t)b.tasks.size();
                 rcnt[(size_t)j * ng + gi] = nt;
-                for (int t = 0; t < nt; ++t) b.tasks.push_back({gi, t * QIW_WAVE, j, (int32_t)b.tasks.size()});
+                for (int t = 0; t < nt; ++t) b.tasks.push_back({gi, t * b.spt, j, (int32_t)b.tasks.size()});
                 b.flops_per_launch += (double)N * (2.0 * b.groups[gi].edges_per_sample + 2.0 * b.groups[gi].leaves_per_sample);
                 b.leaves_per_launch += N * b.groups[gi].leaves_per_sample;
                 b.diagram_evals_per_launch += N * b.groups[gi].n_top;

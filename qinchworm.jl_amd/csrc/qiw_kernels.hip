@@ -445,7 +445,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
         tau_f = a.beta;
     }
     const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
-    const int n_here = g.sampled ? min(QIW_WAVE, g.n_samples - task.first_sample) : 1;
+    const int n_here = g.sampled ? min(a.spt, g.n_samples - task.first_sample) : 1;
     for (int r = 0; r < n_here; ++r) {
         const int64_t sample = (int64_t)task.first_sample + r;
         // times of the configuration positions (identical in every lane)
@@ -923,6 +923,278 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
 }
 
 // ---------------------------------------------------------------------------------------
+// The diagram evaluator for wide sector blocks (up to 32) on the matrix cores.
+//
+// One wavefront evaluates one sample at a time (all lanes share the sample's times, so
+// propagators and arc values are uniform).  Partial products X (m_L x m_s0, all columns at
+// once) are kept as 16x16 tiles in the accumulator layout of v_mfma_f32_16x16x4f32 (lane l,
+// element r of tile (ti, tj): row 16 ti + 4 (l / 16) + r, column 16 tj + l % 16); a product
+// C = A X streams A (an i*P block from LDS or an operator block from global memory) as the
+// MFMA A operand, one column-major 16x4 slice per instruction, and passes X through a
+// 32x32 LDS scratch that turns the accumulator layout into B-operand slices.  The walk
+// follows the vector programs (one segment per initial sector) with all columns of s0 at
+// once; products of arc values travel as a float64 weight, leaves accumulate into float64
+// accumulators, and the last propagator iP_n[s0] is applied once per segment.
+// float32 products (the QIW_PREC_FP32 mode); sums over samples in float64.
+// ---------------------------------------------------------------------------------------
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+template <int TB>
+struct MTile {
+    mf4 t[TB][TB];
+};
+
+template <int KM, int TM, int TB>
+struct MState {
+    typename RegVec<double, regvec_size(KM * TM)>::type W;  // arc values of the sample
+    double acc[TB][TB][4];                                  // leaf sums, accumulator layout
+    ProgStream ps;
+    const float *__restrict__ pl;   // i*P blocks of the sample in LDS: [(L - 2) * bt + BT offset]
+    float *__restrict__ scr;        // 32 x 32 column-major scratch
+    const float *__restrict__ ops;  // operator blocks, column-major
+    int bt;
+};
+
+// C = A X for A (m_a x K, column-major, leading dimension lda) and X (K x n).
+template <int TB>
+__device__ __forceinline__ MTile<TB> mmul(const float *__restrict__ A, int lda, int m_a, int K, const MTile<TB> &X,
+                                          int n, float *__restrict__ scr) {
+    const int l = threadIdx.x, q = l >> 4, c = l & 15;
+#pragma unroll
+    for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TB; ++tj) *(mf4 *)&scr[(16 * tj + c) * 32 + 16 * ti + 4 * q] = X.t[ti][tj];
+    MTile<TB> C;
+#pragma unroll
+    for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TB; ++tj) C.t[ti][tj] = mf4{0.f, 0.f, 0.f, 0.f};
+    const int mt = (m_a + 15) >> 4, nt = (n + 15) >> 4, nk = (K + 3) >> 2;
+#pragma unroll
+    for (int kc = 0; kc < 4 * TB; ++kc) {
+        if (kc < nk) {
+            const int k = 4 * kc + q;
+#pragma unroll
+            for (int ti = 0; ti < TB; ++ti) {
+                if (ti < mt) {
+                    const int row = 16 * ti + c;
+                    const float a = (k < K && row < m_a) ? A[k * lda + row] : 0.f;
+#pragma unroll
+                    for (int tj = 0; tj < TB; ++tj) {
+                        if (tj < nt) {
+                            const float b = scr[(16 * tj + c) * 32 + k];
+                            C.t[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, C.t[ti][tj], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return C;
+}
+
+template <int L, int KM, int TM, int TB>
+__device__ __forceinline__ void mwalk(MState<KM, TM, TB> &st, int n, const MTile<TB> &Xp, double wp, int mp, int offp,
+                                      int nchild, int m0) {
+    if constexpr (L <= 2 * KM + 3) {
+        const MTile<TB> Y = (L == n) ? Xp : mmul<TB>(st.pl + (L - 2) * st.bt + offp, mp, mp, mp, Xp, m0, st.scr);
+        for (int c = 0; c < nchild; ++c) {
+            const uint64_t w = st.ps.next();
+            const double wv = (w & QIW_F_TAIL) ? wp * st.W[(int)((w >> 16) & 0xFF)] : wp;
+            const int mo = (int)(w & 0x3F);
+            const int blk = (int)((w >> 32) & 0xFFFF);
+            const MTile<TB> X = (blk == 0xFFFF) ? Y : mmul<TB>(st.ops + blk, mo, mo, mp, Y, m0, st.scr);
+            if (w & QIW_F_LEAF) {
+                const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -wv : wv)
+                                                    : __longlong_as_double((long long)st.ps.next()) * wv;
+#pragma unroll
+                for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) st.acc[ti][tj][r] = fma(cw, (double)X.t[ti][tj][r], st.acc[ti][tj][r]);
+            } else if (L < n) {
+                mwalk<L + 1, KM, TM, TB>(st, n, X, wv, mo, (int)(w >> 52), (int)((w >> 24) & 0xFF), m0);
+            }
+        }
+    }
+}
+
+template <int KM, int TM, int TB>
+__global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
+    constexpr int NM = 2 * KM + 3;
+    extern __shared__ float mlds[];  // [(NM - 1) * bt] i*P blocks, then the 32 x 32 scratch
+    const EvalArgs &a = ma.e;
+    const int lane = threadIdx.x, q = lane >> 4, cl = lane & 15;
+    const QiwTask task = a.tasks[blockIdx.x];
+    const QiwGroup g = a.groups[task.group];
+    const int n = g.n_nodes, k = g.order, bt = ma.bt;
+    const int ntab = a.dtab.n_tables;
+    const double inv_h = 1.0 / a.h;
+    float *pl = mlds;
+    MState<KM, TM, TB> st;
+    st.pl = pl;
+    st.scr = mlds + (((NM - 1) * bt + 3) & ~3);
+    st.ops = ma.ops;
+    st.bt = bt;
+#pragma unroll
+    for (int e = 0; e < regvec_size(KM * TM); ++e) st.W[e] = 0.0;
+
+    double tau_i = 0.0, tau_w, tau_f;
+    if (g.kind == QIW_G_BARE) {
+        tau_w = 0.0;
+        tau_f = a.h;
+    } else if (g.kind == QIW_G_BOLD) {
+        tau_w = a.step_w * a.h;
+        tau_f = (a.step_w + 1) * a.h;
+    } else {
+        tau_w = task.tau_index * a.h;
+        tau_f = a.beta;
+    }
+    double *prow = a.partial + (int64_t)task.out_index * bt;
+    for (int e = lane; e < bt; e += QIW_WAVE) prow[e] = 0.0;
+    __threadfence_block();
+    __syncthreads();
+    const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
+    const int n_here = g.sampled ? min(ma.e.spt, g.n_samples - task.first_sample) : 1;
+    for (int sr = 0; sr < n_here; ++sr) {
+        const int64_t sample = (int64_t)task.first_sample + sr;
+        // ---- times of the configuration positions (the same in every lane) ----
+        double t[NM + 1];
+#pragma unroll
+        for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
+        if (g.sampled) {
+            const int64_t N = g.n_samples;
+            const double *u = a.ut + g.ut_off + sample;
+#pragma unroll
+            for (int j = 0; j < 2 * KM; ++j) {
+                if (j < g.d_after) {
+                    const double v = u[j * N] * (tau_f - tau_w) + tau_w;
+#pragma unroll
+                    for (int p = 1; p <= NM; ++p)
+                        if (p == n - 1 - j) t[p] = v;
+                } else if (j < g.d_after + g.d_before) {
+                    const double v = u[j * N] * (tau_w - tau_i) + tau_i;
+                    const int pos = g.pos_w - 1 - (j - g.d_after);
+#pragma unroll
+                    for (int p = 1; p <= NM; ++p)
+                        if (p == pos) t[p] = v;
+                }
+            }
+        }
+        const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples + sample] : 1.0;
+        // ---- i*P blocks of every segment into LDS (lanes over BT elements) ----
+        for (int L = 2; L <= n; ++L) {
+            double dt = pick(t, L) - pick(t, L - 1);
+            dt = dt > 0.0 ? dt : 0.0;
+            float *dst = pl + (L - 2) * bt;
+            if (g.kind == QIW_G_BARE) {
+                for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = ma.emask[e] ? (float)exp(-dt * ma.ediag[e]) : 0.f;
+            } else {
+                const double x = dt * inv_h;
+                int j = (int)floor(x);
+                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+                const double f = x - (double)j;
+                const double *r0 = a.ptab + (int64_t)j * bt, *r1 = r0 + bt;
+                for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = (float)((1.0 - f) * r0[e] + f * r1[e]);
+            }
+        }
+        __syncthreads();
+        // ---- walk the program: one segment per initial sector ----
+        double mag = 0.0;
+        int m0 = 1, off0 = 0;
+        st.ps.init(a.prog + g.prog_off, lane);
+        for (;;) {
+            const uint64_t w = st.ps.next();
+            const int op = (int)((w >> 48) & 0xF);
+            if (op == QIW_OP_ROOT) {
+                const int m1 = (int)(w & 0x3F);
+                const int blk = (int)((w >> 32) & 0xFFFF);
+                MTile<TB> X;
+#pragma unroll
+                for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = 16 * ti + 4 * q + r, col = 16 * tj + cl;
+                            float v = 0.f;
+                            if (blk == 0xFFFF)
+                                v = (row == col && row < m0) ? 1.f : 0.f;
+                            else if (row < m1 && col < m0)
+                                v = st.ops[blk + col * m1 + row];
+                            X.t[ti][tj][r] = v;
+                        }
+                mwalk<2, KM, TM, TB>(st, n, X, 1.0, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF), m0);
+            } else if (op == QIW_OP_TOPO) {
+#pragma unroll
+                for (int ai = 0; ai < KM; ++ai) {
+                    const int head = (int)((w >> (8 * ai)) & 0xF), tail = (int)((w >> (8 * ai + 4)) & 0xF);
+                    double th = 0.0, tt = 0.0;
+#pragma unroll
+                    for (int p = 1; p <= NM; ++p) {
+                        th = (p == head) ? t[p] : th;
+                        tt = (p == tail) ? t[p] : tt;
+                    }
+                    double z = tt - th;
+                    z = z > 0.0 ? z : 0.0;
+                    const double xx = z * inv_h;
+                    int j = (int)floor(xx);
+                    j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+                    const double u = xx - (double)j;
+#pragma unroll
+                    for (int tb = 0; tb < TM; ++tb) {
+                        if (ai < k && tb < ntab) {
+                            const double *cc = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
+                            st.W[ai * TM + tb] = cc[0] + u * (cc[1] + u * (cc[2] + u * cc[3]));
+                        }
+                    }
+                }
+            } else if (op == QIW_OP_SEG) {
+                m0 = (int)(w & 0x3F);
+                off0 = (int)(w >> 52);
+#pragma unroll
+                for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) st.acc[ti][tj][r] = 0.0;
+            } else if (op == QIW_OP_FLUSH) {  // the s0 block of this sample: iP_n[s0] times the leaf sum
+                MTile<TB> S;
+#pragma unroll
+                for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) S.t[ti][tj][r] = (float)st.acc[ti][tj][r];
+                const MTile<TB> R = mmul<TB>(pl + (n - 2) * bt + off0, m0, m0, m0, S, m0, st.scr);
+#pragma unroll
+                for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = 16 * ti + 4 * q + r, col = 16 * tj + cl;
+                            if (row < m0 && col < m0) {
+                                const double v = (double)R.t[ti][tj][r];
+                                prow[off0 + row + col * m0] += wt * v;
+                                mag += fabs(v);
+                            }
+                        }
+            } else if (op == QIW_OP_END) {
+                break;
+            }
+        }
+        if (a.per_sample) {
+#pragma unroll
+            for (int o = QIW_WAVE / 2; o > 0; o >>= 1) mag += __shfl_xor(mag, o, QIW_WAVE);
+            if (lane == 0) a.per_sample[g.ps_off + sample] = mag;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Device-side step bookkeeping: reduce the partial sums of every group in fixed task order,
 // apply the integration prefactors, form the order-resolved and total step results
 // (inchworm_step, src/inchworm.jl:142-202), store i*P(tau_f) (set_ppgf!,
@@ -1099,6 +1371,18 @@ hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, in
 hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, float2 *out32) {
     int64_t n = (int64_t)(n_tau - 1) * bt;
     hipLaunchKernelGGL(k_ptab_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n_tau, bt, out, out32);
+    return hipGetLastError();
+}
+
+// Wide blocks on the matrix cores: orders up to 3 or 4, blocks up to 32, <= 2 tables.
+hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes) {
+    if (tm > 2) return hipErrorInvalidValue;
+    if (km <= 3)
+        hipLaunchKernelGGL((k_eval_mfma_t<3, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+    else if (km <= 4)
+        hipLaunchKernelGGL((k_eval_mfma_t<4, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
