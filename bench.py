@@ -111,6 +111,45 @@ def cpu_baseline(N, steps=2):
                       f"{evals} diagram evals in {dt:.2f} s, OMP threads={threads}"}
 
 
+def c3_expansion():
+    from qiw import model
+    return model.kanamori(3, 3.0, 0.3, 0.0, 20.0, 400, 0.4)
+
+
+def three_orbital(log2n=12, cpu_n=16, with_cpu=True):
+    """One bold step of the three-orbital Kanamori model (blocks 1,6,15,20,15,6,1; 12 pairs;
+    beta=20, n_tau=400; tau_w = grid[200]; orders 0:3) with float32 chain products on the
+    matrix cores, and the float64 oracle on a bounded sample of the same step."""
+    import qiw
+    from qiw import abi
+    k = c3_expansion()
+    N = 2 ** log2n
+    out = {"workload": f"three-orbital Kanamori (C3 model), one bold step tau_w=grid[200], orders 0:3, "
+                       f"N=2^{log2n}, float32 chain products on MFMA"}
+    t0 = time.perf_counter()
+    qiw.inchworm_step(k, 200, range(0, 4), N, precision=abi.QIW_PREC_FP32)
+    dt = time.perf_counter() - t0
+    st = qiw.stats()
+    out["gpu_fp32_s"] = dt
+    out["gpu_fp32_evals_per_s"] = st["diagram_evals"] / dt
+    out["gpu_leaves_per_s"] = st["leaves"] / dt
+    if with_cpu:
+        import oracle_lib as O
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        O.reset_counters()
+        kc = c3_expansion()
+        t0 = time.perf_counter()
+        O.inchworm_step(kc, range(0, 4), cpu_n, 200)
+        dt = time.perf_counter() - t0
+        _, evals, _ = O.counters()
+        out["cpu_evals_per_s"] = evals / dt
+        out["cpu_cores"] = threads
+        out["cpu_sample"] = f"same step at N={cpu_n}, {evals} diagram evals in {dt:.1f} s, float64 oracle"
+        out["speedup_fp32"] = out["gpu_fp32_evals_per_s"] / out["cpu_evals_per_s"]
+    return out
+
+
 def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10, 17)), steps=(10, 25, 45),
                iters=4000, n_train=2 ** 14):
     """Error versus samples of the learned warp against plain qMC on the two-orbital model
@@ -182,7 +221,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log2n", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c2", action="store_true", help="skip the two-orbital step measurement")
+    ap.add_argument("--no-c2", action="store_true", help="skip the two- and three-orbital step measurements")
     ap.add_argument("--warp-study", action="store_true",
                     help="only run the learned-warp error-versus-samples study and print its JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -249,6 +288,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline(N)
     if not args.no_c2:
         line["two_orbital_step"] = two_orbital(with_cpu=not args.no_cpu)
+        line["three_orbital_step"] = three_orbital(with_cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -919,7 +919,8 @@ static std::vector<CMat> integrate_group(const Model &m, const PPGF *P, bool bol
     double J = tr.jacobian();
     cd dir = 1.0;
     for (int i = 0; i < d; ++i) dir *= -I;  // prod(branch_direction) over the imaginary branch
-    const int64_t CH = 64;
+    // chunks of up to 64 samples, smaller when N is small so that every thread has work
+    const int64_t CH = std::max<int64_t>(1, std::min<int64_t>(64, sample_count / std::max(g_threads, 1)));
     int64_t nch = (sample_count + CH - 1) / CH;
     std::vector<std::vector<CMat>> partial(nch);
     int64_t leaves_total = 0;
