@@ -18,6 +18,8 @@ using QInchworm.randomization: RandomizationParams
 
 const libqiw = get(ENV, "QIW_LIBRARY", joinpath(@__DIR__, "..", "lib", "libqiw.so"))
 
+const QIW_PREC_FP64 = Int32(0)
+const QIW_PREC_FP32 = Int32(1)
 const QIW_GF_GRID_LINEAR = Int32(0)
 const QIW_GF_CUBIC_SPLINE = Int32(1)
 const QIW_GF_INC_SPLINE = Int32(2)
@@ -56,6 +58,22 @@ struct QiwRun
     n_pts_after_max::Int32
     precision::Int32
 end
+
+# Learned warp of the unit cube (qiw_warp_config); no counterpart in the reference
+struct QiwWarpConfig
+    n_layers::Int32
+    n_bins::Int32
+    hidden::Int32
+    iters::Int32
+    batch::Int32
+    n_train_steps::Int32
+    n_train::Int64
+    lr::Float64
+    seed::UInt64
+end
+QiwWarpConfig(; n_layers = 4, n_bins = 8, hidden = 64, iters = 4000, batch = 8192, n_train_steps = 8,
+              n_train = 2^14, lr = 2e-3, seed = 1) =
+    QiwWarpConfig(n_layers, n_bins, hidden, iters, batch, n_train_steps, n_train, lr, seed)
 
 mutable struct Context
     h::Ptr{Cvoid}
@@ -162,7 +180,7 @@ Dict{Int, Matrix{ComplexF64}} in BT layout (sum m^2 x n_tau) and writes expansio
 """
 function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_bare, N_samples::Int64;
                    n_pts_after_max::Int64 = typemax(Int64), rand_params::RandomizationParams = RandomizationParams(),
-                   ctx::Context = context())
+                   precision::Int32 = QIW_PREC_FP64, ctx::Context = context())
     @assert N_samples == 0 || ispow2(N_samples)
     @assert rand_params.N_seqs == 1 && rand_params.rng === nothing "randomized qMC is not offloaded"
     keep = Any[]
@@ -175,7 +193,7 @@ function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_ba
     Po = zeros(ComplexF64, bt, n_tau, length(oall))
     Ps = zeros(ComplexF64, bt, n_tau, length(oall))
     run = Ref(QiwRun(length(orders_v), pointer(orders_v), length(bare_v), pointer(bare_v), N_samples,
-                     n_pts_after_max == typemax(Int64) ? 0 : Int32(n_pts_after_max), 0))
+                     n_pts_after_max == typemax(Int64) ? 0 : Int32(n_pts_after_max), precision))
     GC.@preserve keep orders_v bare_v begin
         check(ctx, ccall((:qiw_inchworm, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
@@ -211,5 +229,42 @@ function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_sam
     end
     return [out[:, c] for c in 1:size(out, 2)]
 end
+
+"""
+    train_warp!(expansion, grid, orders; config = QiwWarpConfig(), ctx)
+
+Fit the learned warp to |integrand| samples of the plain method at `expansion.P` (run a
+plain `inchworm!` first) and enable it for the following `inchworm!` calls on `ctx`.
+Returns the loss of every Adam iteration.
+"""
+function train_warp!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders; config::QiwWarpConfig = QiwWarpConfig(),
+                     ctx::Context = context())
+    keep = Any[]
+    orders_v = Int32.(collect(orders))
+    stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, Int32[0]))), keep)
+    n_tau = length(grid)
+    T = ppgf_table(exp, n_tau)
+    der0 = zeros(ComplexF64, size(T, 1))
+    check(ctx, ccall((:qiw_set_ppgf, libqiw), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                     ctx.h, n_tau, T, der0))
+    cfg = Ref(config)
+    check(ctx, ccall((:qiw_warp_init, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwWarpConfig}), ctx.h, cfg))
+    loss = zeros(Float64, max(config.iters, 1))
+    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, 0, 0, QIW_PREC_FP64))
+    GC.@preserve keep orders_v begin
+        check(ctx, ccall((:qiw_warp_train, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwRun}, Ptr{Float64}),
+                         ctx.h, run, loss))
+    end
+    check(ctx, ccall((:qiw_warp_enable, libqiw), Cint, (Ptr{Cvoid}, Int32), ctx.h, 1))
+    return loss[1:config.iters]
+end
+
+"""
+    disable_warp!(ctx)
+
+Return to plain quasi Monte Carlo points.
+"""
+disable_warp!(ctx::Context = context()) =
+    check(ctx, ccall((:qiw_warp_enable, libqiw), Cint, (Ptr{Cvoid}, Int32), ctx.h, 0))
 
 end # module QInchwormAMD

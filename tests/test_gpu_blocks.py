@@ -96,15 +96,16 @@ def test_three_orbital_blocks():
 def test_three_orbital_matrix_cores_fp32():
     """Wide blocks with float32 chain products run on the matrix cores (k_eval_mfma_t); the
     error against the float64 oracle is stated here: below 2e-6 relative."""
-    kg, kc = kanamori3(), kanamori3()
+    kg, kc = kanamori3(n_tau=4), kanamori3(n_tau=4)
     fp32 = qiw.abi.QIW_PREC_FP32
-    P_c, _, _ = O.inchworm(kc, range(0, 4), range(0, 4), 2 ** 3)
-    qiw.inchworm(kg, kg.grid, range(0, 4), range(0, 4), 2 ** 3, precision=fp32)
+    O.set_threads(16)
+    P_c, _, _ = O.inchworm(kc, range(0, 4), range(0, 4), 2 ** 2)
+    qiw.inchworm(kg, kg.grid, range(0, 4), range(0, 4), 2 ** 2, precision=fp32)
     err = rel(kg.P, P_c)
     print(f"three-orbital, matrix cores (float32): rel err P {err:.2e}")
     assert 0 < err < 2e-6
     kc.P, kc.P_n_known = P_c, kc.grid.n_tau
     kg.P, kg.P_n_known = P_c.copy(), kg.grid.n_tau
-    v_c, _ = O.inchworm_step(kc, range(0, 4), 2 ** 4, 2)
-    v_g, _ = qiw.inchworm_step(kg, 2, range(0, 4), 2 ** 4, precision=fp32)
+    v_c, _ = O.inchworm_step(kc, range(0, 4), 2 ** 3, 2)
+    v_g, _ = qiw.inchworm_step(kg, 2, range(0, 4), 2 ** 3, precision=fp32)
     assert rel(v_g, v_c) < 2e-6
