@@ -1,13 +1,13 @@
 // qiw_kernels.hip -- gfx950 kernels of libqiw: Sobol points and simplex maps, the
 // trie-walking diagram evaluator, and the device-side step bookkeeping.
 //
-// Work mapping of the evaluator: one wavefront of 64 lanes = 64 consecutive qMC samples of
-// one (order, n_pts_after) group.  All lanes walk the same precompiled configuration trie
-// (the depth-first traversal of src/topology_eval.jl:454-556, flattened on the host), so
-// control flow is uniform: program words are read through the scalar cache and only the
-// per-sample values differ between lanes.  Per-lane operands (propagator values per
-// segment and sector, hybridization values per arc, the partial-product stack) live in LDS
-// as [slot][lane] columns, so every access is a conflict-free contiguous 512-byte row.
+// Work mapping of the evaluators: all lanes of a wavefront walk the same precompiled
+// configuration trie (the depth-first traversal of src/topology_eval.jl:454-556, compiled on
+// the host into program words), so control flow is uniform and program words are broadcast
+// from one coalesced block load.  The scalar and vector walks give each lane one qMC sample
+// (64 consecutive samples of one (order, n_pts_after) group per wavefront) and keep the
+// sample's operands in registers; the matrix-core and LDS block walks give the wavefront
+// one sample at a time and spread its matrix products over the lanes.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
