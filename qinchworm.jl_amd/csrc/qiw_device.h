@@ -44,6 +44,13 @@ enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3, QIW_O
 #define QIW_FV_NEG (1ull << 8)
 #define QIW_FV_UNIT (1ull << 9)
 
+// Table stride of the W slots (slot = arc * stride + table) of the register walks: the
+// table count TM of the kernel instantiation that runs the program.  The host compiler and
+// the launchers choose it by the same rule, from the largest order km of the launch, the
+// sector count sm and the model's table count tm.  The LDS block walk uses tm itself.
+static inline int qiw_scalar_tm(int km, int sm, int tm) { return (km <= 4 && sm <= 4 && tm <= 2) ? 2 : 4; }
+static inline int qiw_vec_tm(int tm) { return tm <= 2 ? 2 : 4; }
+
 // Kinds of integration group
 enum { QIW_G_BOLD = 0, QIW_G_BARE = 1, QIW_G_CORR = 2 };
 

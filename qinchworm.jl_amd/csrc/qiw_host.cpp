@@ -191,6 +191,14 @@ int program_format(const HostModel &m, int max_order, int precision) {
     return FMT_BLOCK;
 }
 
+// Stride of the W slots: the table count of the register walk instantiation that runs the
+// program, or the model's table count for the LDS block walk.
+int w_stride(const HostModel &m, int fmt, int max_order) {
+    if (fmt == FMT_BLOCK) return m.ntab;
+    if (fmt == FMT_SCALAR) return qiw_scalar_tm(max_order, m.ns, m.ntab);
+    return qiw_vec_tm(m.ntab);
+}
+
 struct Compiled {
     QiwGroup g{};
     std::vector<uint64_t> prog;
@@ -212,6 +220,7 @@ struct TrieNode {
 struct Compiler {
     const HostModel &m;
     int kind, order, n_after, corr, fmt;
+    int wstride;  // W slot of arc a, table t: a * wstride + t
     int n = 0, pos_w = 0;
     std::vector<int> conf_kind, conf_arc;
     std::vector<int> selected;
@@ -219,8 +228,9 @@ struct Compiler {
     int s0 = 0, parity = 1;
     std::string err;
 
-    Compiler(const HostModel &m_, int kind_, int order_, int n_after_, int corr_, int fmt_)
-        : m(m_), kind(kind_), order(order_), n_after(n_after_), corr(corr_), fmt(fmt_) {}
+    Compiler(const HostModel &m_, int kind_, int order_, int n_after_, int corr_, int fmt_, int max_order)
+        : m(m_), kind(kind_), order(order_), n_after(n_after_), corr(corr_), fmt(fmt_),
+          wstride(w_stride(m_, fmt_, max_order)) {}
 
     int add(int parent, int level, int s_in, int wslot, bool tail, int s_out = 0, int blk = 0xFFFF) {
         TrieNode t{level, (level - 2) * m.ns + s_in, wslot, parent, tail, false, 0.0, {}, s_in, s_out, blk};
@@ -269,7 +279,7 @@ struct Compiler {
             int a = conf_arc[pos];
             int p = selected[a];
             if (m.op_f[p].sf[s] >= 0) {
-                int id = add(node, pos, s, a * m.ntab + m.pair_table[p], true, m.op_f[p].sf[s], m.op_f[p].blk[s]);
+                int id = add(node, pos, s, a * wstride + m.pair_table[p], true, m.op_f[p].sf[s], m.op_f[p].blk[s]);
                 live = keep_child(node, id, dfs(id, pos + 1, m.op_f[p].sf[s], fold ? c * m.op_f[p].val[s] : c));
             }
         } else if (k == N_OPERATOR) {
@@ -420,7 +430,7 @@ struct Compiler {
         g.d_after = d_after;
         g.pos_w = pos_w;
         g.n_pslots = (n - 1) * m.ns;
-        g.n_wslots = order * m.ntab;
+        g.n_wslots = order * wstride;
         return true;
     }
 };
@@ -445,6 +455,7 @@ struct Batch {
     int fmt = FMT_SCALAR;  // program format of every group of the batch
     int spt = QIW_WAVE;    // samples per task
     int precision = QIW_PREC_FP64;
+    int max_order = 0;  // largest order of the batch: selects the kernel instantiation
     double flops_per_launch = 0;  // algorithmic FLOPs of one launch
     int64_t leaves_per_launch = 0, diagram_evals_per_launch = 0;
 };
@@ -588,6 +599,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
         if (ord[oi] > 0) n_groups += bare ? 1 : 2 * ord[oi] - 1;
     }
     b.fmt = program_format(m, max_order, run->precision);
+    b.max_order = max_order;
     b.spt = samples_per_task(m, b.fmt, n_groups * std::max<int64_t>(run->N_samples, 1));
     b.precision = run->precision;
     for (int oi = 0; oi < n_o; ++oi) {
@@ -602,7 +614,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             for (int na = 1; na <= 2 * order - 1; ++na)
                 if (run->n_pts_after_max <= 0 || na <= run->n_pts_after_max) nas.push_back(na);
         for (int na : nas) {
-            Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1, b.fmt);
+            Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1, b.fmt, b.max_order);
             Compiled cg;
             if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
             if (cg.n_top == 0) continue;
@@ -757,8 +769,7 @@ EvalArgs make_eval_args(qiw_ctx *c, const Batch &bt, const DevBatch &d, double *
     const HostModel &m = c->m;
     EvalArgs a{};
     int lds = bt.lds;
-    a.max_order = 0;
-    for (auto &cg : bt.groups) a.max_order = std::max(a.max_order, cg.g.order);
+    a.max_order = bt.max_order;
     a.groups = d.groups;
     a.tasks = d.tasks;
     a.prog = d.prog;
@@ -1247,6 +1258,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         int max_order = 0;
         for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
         b.fmt = program_format(m, max_order, run->precision);
+        b.max_order = max_order;
         b.precision = run->precision;
         bool first_is_order0 = false;
         for (int oi = 0; oi < run->n_orders; ++oi) {
@@ -1254,7 +1266,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
             if (order > 0 && run->N_samples <= 0) continue;
             int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
             for (int na = lo; na <= hi; ++na) {
-                Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt);
+                Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
                 Compiled cg;
                 if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
                 if (cg.n_top == 0) continue;

@@ -109,3 +109,22 @@ def test_three_orbital_matrix_cores_fp32():
     v_c, _ = O.inchworm_step(kc, range(0, 4), 2 ** 3, 2)
     v_g, _ = qiw.inchworm_step(kg, 2, range(0, 4), 2 ** 3, precision=fp32)
     assert rel(v_g, v_c) < 2e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_block_one_table(precision):
+    """Two-orbital model whose 4 pairs share one hybridization table (W slot stride 2)."""
+    def build():
+        k = kanamori2(n_tau=8)
+        f = k.ed.fock
+        labels = [(a, s) for a in range(2) for s in ("up", "dn")]
+        delta = k.pairs[0].propagator
+        return model.Expansion(k.ed, k.grid, [model.InteractionPair(f.c_dag(l), f.c(l), delta) for l in labels])
+    kg, kc = build(), build()
+    P_c, _, _ = O.inchworm(kc, range(0, 3), range(0, 3), 2 ** 5, n_steps_max=4)
+    for e in (kg, kc):
+        e.P, e.P_n_known = P_c, e.grid.n_tau
+    prec = qiw.abi.QIW_PREC_FP32 if precision == "fp32" else qiw.abi.QIW_PREC_FP64
+    v_gpu, _ = qiw.inchworm_step(kg, 4, range(0, 4), 2 ** 5, precision=prec)
+    v_cpu, _ = O.inchworm_step(kc, range(0, 4), 2 ** 5, 4)
+    assert rel(v_gpu, v_cpu) < (RTOL if precision == "fp64" else 2e-6)

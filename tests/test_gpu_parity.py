@@ -147,3 +147,35 @@ def test_error_codes(ctx):
     with pytest.raises(abi.QiwError) as e:
         qiw.inchworm(k, k.grid, [0, 7], [0, 1], 2 ** 4)   # order beyond the compiled limit
     assert e.value.code in (abi.QIW_E_LIMIT, abi.QIW_E_DIM)
+
+
+def anderson_tables(n_tables, n_tau=16):
+    """Single-orbital Anderson model whose pairs use 1 or 3 distinct hybridization tables,
+    so the W slots of the walk (arc * stride + table) differ from arc * n_tables."""
+    fock = model.FockSpace(["up", "dn"])
+    H = 0.1 * (fock.n("up") + fock.n("dn")) + 1.0 * fock.n("up") @ fock.n("dn")
+    ed = model.EDCore(H, fock, lambda b: (fock.occupation(b, "up"), fock.occupation(b, "dn")))
+    grid = model.ImaginaryTimeGrid(10.0, n_tau)
+    delta = 0.25 * model.bethe_gf(1.0, 10.0, grid)
+    if n_tables == 1:
+        pairs = [model.InteractionPair(fock.c_dag(s), fock.c(s), delta) for s in ("up", "dn")]
+    else:
+        pairs = [model.InteractionPair(fock.c_dag("up"), fock.c("up"), delta),
+                 model.InteractionPair(fock.c("up"), fock.c_dag("up"), model.ph_conj(delta)),
+                 model.InteractionPair(fock.c_dag("dn"), fock.c("dn"), 0.5 * delta)]
+    return model.Expansion(ed, grid, pairs)
+
+
+@pytest.mark.parametrize("n_tables", [1, 3])
+def test_table_count_other_than_stride(ctx, n_tables):
+    """One and three distinct tables: W slots are laid out with the kernel's table stride."""
+    exp_g, exp_c = anderson_tables(n_tables), anderson_tables(n_tables)
+    P, _, _ = O.inchworm(exp_c, range(0, 4), range(0, 4), 2 ** 7, n_steps_max=6)
+    for e in (exp_g, exp_c):
+        e.P, e.P_n_known = P, e.grid.n_tau
+    v_gpu, per_gpu = qiw.inchworm_step(exp_g, 5, range(0, 5), 2 ** 8)
+    v_cpu, per_cpu = O.inchworm_step(exp_c, range(0, 5), 2 ** 8, 5)
+    assert rel(v_gpu, v_cpu) < RTOL
+    for o in range(1, 5):
+        assert np.abs(per_cpu[o]).max() > 0
+        assert np.abs(per_gpu[o] - per_cpu[o]).max() < RTOL * np.abs(v_cpu).max()
