@@ -51,6 +51,43 @@ enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3, QIW_O
 static inline int qiw_scalar_tm(int km, int sm, int tm) { return (km <= 4 && sm <= 4 && tm <= 2) ? 2 : 4; }
 static inline int qiw_vec_tm(int tm) { return tm <= 2 ? 2 : 4; }
 
+// Flat scalar programs (k_eval_flat_t, all sectors 1x1): 32-bit words, one per trie node in
+// preorder, the leaf (the last position, always an only child) folded into its parent.  A
+// running product flows down each chain; a node with several children saves it in the slot
+// of its children's level (PUSH) and every child but the first restores it (POP).
+// EDGE   (a node at level L >= 2, its configuration position)
+//        bits  0-7  : inner propagator entry multiplied after the node, (NM - 1 - (L + 1)) * SM
+//                     + sector after the node's operator (NM = 2 KM + 3, SM: sector stride;
+//                     qiw_scalar_km / _sm), or for LEAF without UNIT the index of the
+//                     coefficient in the group's table
+//        bits  8-15 : W slot of the arc value (TAIL)
+//        bits 16-19 : level L
+//        bit  20 TAIL : multiply the arc value
+//        bit  21 LEAF : the only child is the last position: acc += coefficient * product
+//        bit  22 PUSH : after the propagator, save the product in slot L + 1
+//        bit  23 POP  : first restore the product from slot L
+//        bit  24 UNIT : coefficient +1, or -1 with bit 25 NEG
+//        bits 28-31 : op, 0 for EDGE
+// ROOT   op 4, position 1: product = iP_2[s], s in bits 0-7 (PUSH: save it in slot 2); with
+//        LEAF (two positions only): acc += coefficient (bits 0-7, UNIT, NEG as for EDGE)
+// TOPO   op 1, bits 0-23 : topology index into the group's table of arc head/tail nibbles
+// FLUSH  op 2, bits 0-7 : s0 (out[s0] += acc * iP_n[s0]; acc = 0)
+// END    op 3
+#define QIW_FL_TAIL (1u << 20)
+#define QIW_FL_LEAF (1u << 21)
+#define QIW_FL_PUSH (1u << 22)
+#define QIW_FL_POP (1u << 23)
+#define QIW_FL_UNIT (1u << 24)
+#define QIW_FL_NEG (1u << 25)
+#define QIW_FL_OP (1u << 28)
+enum { QIW_FL_OP_TOPO = 1, QIW_FL_OP_FLUSH = 2, QIW_FL_OP_END = 3, QIW_FL_OP_ROOT = 4 };
+
+// Sector stride SM and largest order KM of the flat scalar walk instantiation (see
+// qiw_scalar_tm); its inner propagator entries are (2*KM + 2 - L) * SM + s for level L,
+// sector s.
+static inline int qiw_scalar_sm(int km, int sm, int tm) { return qiw_scalar_tm(km, sm, tm) == 2 ? 4 : 8; }
+static inline int qiw_scalar_km(int km, int sm, int tm) { return (km <= 4 && sm <= 8 && tm <= 4) ? 4 : QIW_KMAX; }
+
 // Kinds of integration group
 enum { QIW_G_BOLD = 0, QIW_G_BARE = 1, QIW_G_CORR = 2 };
 
@@ -64,8 +101,8 @@ struct QiwGroup {
     int32_t pos_w;         // position of the inch / A operator node (0 for bare)
     int32_t n_pslots, n_wslots;
     int32_t prog_off;      // offset into the program word array
-    int32_t topo_off;      // offset into the arc table: [n_top][k] x (head, tail)
-    int32_t coef_off;      // offset into the coefficient array
+    int32_t topo_off;      // offset into the topology table: [n_top][2] words of head/tail nibbles
+    int32_t coef_off;      // offset into the leaf coefficient table (scalar programs)
     int64_t ut_off;        // offset (in doubles) of this group's simplex coordinates
     int32_t n_samples;     // qMC points (1 for order 0: exact evaluation)
     int32_t sampled;       // 0 for order 0
@@ -96,7 +133,7 @@ struct EvalArgs {
     const QiwGroup *groups;
     const QiwTask *tasks;
     const uint64_t *prog;
-    const uint8_t *arcs;
+    const uint32_t *topo;      // per topology two words of arc head/tail nibbles
     const double *coefs;
     const double *ut;          // simplex coordinates of all groups
     const double *ptab;        // [n_tau][ns] i*P on the grid (bold groups)
@@ -196,6 +233,7 @@ hipError_t qiw_launch_warp_coords(hipStream_t st, const double *z, int stride, c
 hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
                                      int d_after, double t_i, double t_w, double t_f, int64_t n, double *out);
 hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
+hipError_t qiw_launch_eval_flat(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
 hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes);
 hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes);
 hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm, int fp32);

@@ -202,8 +202,8 @@ int w_stride(const HostModel &m, int fmt, int max_order) {
 struct Compiled {
     QiwGroup g{};
     std::vector<uint64_t> prog;
-    std::vector<uint8_t> arcs;
-    std::vector<double> coefs;
+    std::vector<uint32_t> topo;   // per topology: head/tail nibbles of arcs 0-3, of arcs 4-5
+    std::vector<double> coefs;    // scalar programs: leaf coefficients other than +-1
     int64_t leaves_per_sample = 0;
     int64_t edges_per_sample = 0;
     int n_top = 0;
@@ -221,6 +221,9 @@ struct Compiler {
     const HostModel &m;
     int kind, order, n_after, corr, fmt;
     int wstride;  // W slot of arc a, table t: a * wstride + t
+    int pstride;  // scalar programs: inner P entry of level L, sector s: (nm - 1 - L) * pstride + s
+    int nm;       // scalar programs: positions of the largest order of the instantiation
+    std::vector<uint32_t> p32;  // scalar program words
     int n = 0, pos_w = 0;
     std::vector<int> conf_kind, conf_arc;
     std::vector<int> selected;
@@ -230,7 +233,8 @@ struct Compiler {
 
     Compiler(const HostModel &m_, int kind_, int order_, int n_after_, int corr_, int fmt_, int max_order)
         : m(m_), kind(kind_), order(order_), n_after(n_after_), corr(corr_), fmt(fmt_),
-          wstride(w_stride(m_, fmt_, max_order)) {}
+          wstride(w_stride(m_, fmt_, max_order)), pstride(qiw_scalar_sm(max_order, m_.ns, m_.ntab)),
+          nm(2 * qiw_scalar_km(max_order, m_.ns, m_.ntab) + 3) {}
 
     int add(int parent, int level, int s_in, int wslot, bool tail, int s_out = 0, int blk = 0xFFFF) {
         TrieNode t{level, (level - 2) * m.ns + s_in, wslot, parent, tail, false, 0.0, {}, s_in, s_out, blk};
@@ -298,9 +302,7 @@ struct Compiler {
     void emit(int id, Compiled &out) {
         const TrieNode &t = trie[id];
         uint64_t w;
-        if (fmt == FMT_SCALAR) {  // structured walk: sector after the node, child count
-            w = ((uint64_t)t.s_out << 8) | ((uint64_t)t.wslot << 16) | ((uint64_t)t.children.size() << 24);
-        } else if (fmt == FMT_VEC) {
+        if (fmt == FMT_VEC) {
             w = (uint64_t)m.dims[t.s_out] | ((uint64_t)t.wslot << 16) |
                 ((uint64_t)t.children.size() << 24) | (op_offset(t.blk) << 32) |
                 ((uint64_t)m.bt_off[t.s_out] << 52);
@@ -311,7 +313,7 @@ struct Compiler {
         if (t.tail) w |= QIW_F_TAIL;
         if (t.leaf) w |= QIW_F_LEAF;
         const bool unit = t.leaf && fmt != FMT_BLOCK && std::fabs(t.coef) == 1.0;
-        if (unit) w |= (fmt == FMT_SCALAR ? QIW_FS_UNIT : QIW_FV_UNIT) | (t.coef < 0 ? (fmt == FMT_SCALAR ? QIW_FS_NEG : QIW_FV_NEG) : 0);
+        if (unit) w |= QIW_FV_UNIT | (t.coef < 0 ? QIW_FV_NEG : 0);
         out.edges_per_sample++;
         out.prog.push_back(w);
         if (unit) out.leaves_per_sample++;
@@ -322,6 +324,54 @@ struct Compiler {
             out.leaves_per_sample++;
         }
         for (size_t i = 0; i < t.children.size(); ++i) emit(t.children[i], out);
+    }
+
+    // Leaf coefficient bits of a scalar program word: UNIT/NEG, or the index into the
+    // group's table of other coefficients.
+    bool coef_bits(double coef, uint32_t &w, Compiled &out) {
+        if (std::fabs(coef) == 1.0) {
+            w |= QIW_FL_UNIT | (coef < 0 ? QIW_FL_NEG : 0u);
+            return true;
+        }
+        size_t ci = std::find(out.coefs.begin(), out.coefs.end(), coef) - out.coefs.begin();
+        if (ci == out.coefs.size()) out.coefs.push_back(coef);
+        if (ci > 255) {
+            err = "more than 256 distinct leaf coefficients in one group";
+            return false;
+        }
+        w |= (uint32_t)ci;
+        return true;
+    }
+
+    // Scalar program words of node `id` at level L and its subtree (format in qiw_device.h).
+    // `pop`: the node is not its parent's first child, so it restores the parent's product.
+    bool emit_flat(int id, int L, bool pop, Compiled &out) {
+        const TrieNode &t = trie[id];
+        const bool leaf = (L + 1 == n);  // the only child is the leaf at the last position
+        uint32_t w;
+        if (id == 0) {  // root, position 1
+            w = (uint32_t)QIW_FL_OP_ROOT << 28;
+            if (!leaf) w |= (uint32_t)t.s_out;
+        } else {
+            w = (uint32_t)L << 16;
+            if (pop) w |= QIW_FL_POP;
+            if (t.tail) w |= QIW_FL_TAIL | ((uint32_t)t.wslot << 8);
+            out.edges_per_sample++;
+        }
+        if (leaf) {
+            w |= QIW_FL_LEAF;
+            out.edges_per_sample++;
+            out.leaves_per_sample++;
+            if (!coef_bits(trie[t.children.at(0)].coef, w, out)) return false;
+            p32.push_back(w);
+            return true;
+        }
+        if (id != 0) w |= (uint32_t)((nm - 1 - (L + 1)) * pstride + t.s_out);
+        if (t.children.size() > 1) w |= QIW_FL_PUSH;
+        p32.push_back(w);
+        for (size_t i = 0; i < t.children.size(); ++i)
+            if (!emit_flat(t.children[i], L + 1, i > 0, out)) return false;
+        return true;
     }
 
     // offset of operator block blk in the operator data (0xFFFF: identity)
@@ -354,17 +404,20 @@ struct Compiler {
             const Topology &top = tops[ti];
             parity = top.parity;
             uint64_t topo_word = (uint64_t)QIW_OP_TOPO << 48;
+            uint32_t tw[2] = {0, 0};
             for (int a = 0; a < order; ++a) {
                 int head = top_to_conf[top.pairs[a].second - 1];
                 int tail = top_to_conf[top.pairs[a].first - 1];
                 conf_kind[head] = N_PAIR_HEAD;
                 conf_kind[tail] = N_PAIR_TAIL;
                 conf_arc[head] = conf_arc[tail] = a;
-                out.arcs.push_back((uint8_t)head);
-                out.arcs.push_back((uint8_t)tail);
                 topo_word |= ((uint64_t)head << (8 * a)) | ((uint64_t)tail << (8 * a + 4));
+                tw[a / 4] |= (uint32_t)(head | (tail << 4)) << (8 * (a % 4));
             }
-            if (order > 0 && fmt != FMT_VEC) out.prog.push_back(topo_word);
+            out.topo.push_back(tw[0]);
+            out.topo.push_back(tw[1]);
+            if (order > 0 && fmt == FMT_SCALAR) p32.push_back(((uint32_t)QIW_FL_OP_TOPO << 28) | (uint32_t)ti);
+            if (order > 0 && fmt == FMT_BLOCK) out.prog.push_back(topo_word);
             for (s0 = 0; s0 < m.ns; ++s0) {
                 trie.clear();
                 // root: configuration position 1 (identity, or the B operator)
@@ -381,6 +434,12 @@ struct Compiler {
                 trie.push_back(root);
                 if (dfs(0, 2, s, c) < 0) return false;
                 if (trie[0].children.empty()) continue;
+                if (fmt == FMT_SCALAR) {  // position 1 is the root, level 1
+                    trie[0].s_out = s;
+                    if (!emit_flat(0, 1, true, out)) return false;
+                    p32.push_back(((uint32_t)QIW_FL_OP_FLUSH << 28) | (uint32_t)s0);
+                    continue;
+                }
                 if (trie[0].children.size() > 255) {
                     err = "too many pair interactions attachable to one sector";
                     return false;
@@ -397,17 +456,8 @@ struct Compiler {
                     saved.swap(out.prog);
                     continue;
                 }
-                if (fmt == FMT_SCALAR) {
-                    out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | ((uint64_t)s << 8) |
-                                       ((uint64_t)trie[0].children.size() << 24));
-                    const double one = 1.0;  // constants of position 1 are folded into the leaves
-                    uint64_t cw;
-                    std::memcpy(&cw, &one, 8);
-                    out.prog.push_back(cw);
-                } else {
-                    out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | ((uint64_t)s0 << 8) |
-                                       ((uint64_t)(root_blk & 0xFFFF) << 24) | ((uint64_t)s << 40));
-                }
+                out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | ((uint64_t)s0 << 8) |
+                                   ((uint64_t)(root_blk & 0xFFFF) << 24) | ((uint64_t)s << 40));
                 for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], out);
                 out.prog.push_back(((uint64_t)QIW_OP_FLUSH << 48) | ((uint64_t)s0 << 8));
             }
@@ -420,7 +470,13 @@ struct Compiler {
                 out.prog.push_back((uint64_t)QIW_OP_FLUSH << 48);
             }
         }
-        out.prog.push_back((uint64_t)QIW_OP_END << 48);
+        if (fmt == FMT_SCALAR) {  // 32-bit words, two to a 64-bit program element
+            p32.push_back((uint32_t)QIW_FL_OP_END << 28);
+            if (p32.size() % 2) p32.push_back((uint32_t)QIW_FL_OP_END << 28);
+            for (size_t i = 0; i < p32.size(); i += 2) out.prog.push_back(p32[i] | ((uint64_t)p32[i + 1] << 32));
+        } else {
+            out.prog.push_back((uint64_t)QIW_OP_END << 48);
+        }
         QiwGroup &g = out.g;
         g.kind = kind;
         g.order = order;
@@ -464,7 +520,7 @@ struct DevBatch {
     QiwGroup *groups = nullptr;
     QiwTask *tasks = nullptr;
     uint64_t *prog = nullptr;
-    uint8_t *arcs = nullptr;
+    uint32_t *topo = nullptr;
     double *coefs = nullptr;
     double *ut = nullptr;
     int32_t *g_first = nullptr, *g_count = nullptr, *g_slot = nullptr;
@@ -481,16 +537,19 @@ int warp_gid(const QiwGroup &g) {
 int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d, bool weighted = false) {
     int rc = 0;
     std::vector<uint64_t> prog;
-    std::vector<uint8_t> arcs;
+    std::vector<uint32_t> topo;
     std::vector<double> coefs;
     std::vector<QiwGroup> gs;
+    // END words between and after the programs (never executed: prefetch and alignment)
+    const uint64_t end_fl = (uint64_t)QIW_FL_OP_END << 28;
+    const uint64_t pad = (b.fmt == FMT_SCALAR) ? (end_fl | (end_fl << 32)) : (uint64_t)QIW_OP_END << 48;
     int64_t ut_total = 0, ps_total = 0;
     for (auto &cg : b.groups) {
         QiwGroup g = cg.g;
         // programs start on a 64-word block boundary (the NOP placement assumes it)
-        while (prog.size() % QIW_WAVE) prog.push_back((uint64_t)QIW_OP_END << 48);
+        while (prog.size() % QIW_WAVE) prog.push_back(pad);
         g.prog_off = (int32_t)prog.size();
-        g.topo_off = (int32_t)arcs.size();
+        g.topo_off = (int32_t)topo.size();
         g.coef_off = (int32_t)coefs.size();
         g.sampled = g.order > 0;
         g.n_samples = g.sampled ? (int32_t)N : 1;
@@ -500,18 +559,18 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
         g.ut_off = ut_total;
         if (g.sampled) ut_total += (int64_t)(g.d_before + g.d_after + g.weighted) * N;
         prog.insert(prog.end(), cg.prog.begin(), cg.prog.end());
-        arcs.insert(arcs.end(), cg.arcs.begin(), cg.arcs.end());
+        topo.insert(topo.end(), cg.topo.begin(), cg.topo.end());
         coefs.insert(coefs.end(), cg.coefs.begin(), cg.coefs.end());
         cg.g = g;
         gs.push_back(g);
     }
     // two blocks of END words after the last program: the evaluator prefetches one block ahead
-    prog.insert(prog.end(), 2 * QIW_WAVE, (uint64_t)QIW_OP_END << 48);
-    if (arcs.empty()) arcs.push_back(0);
+    prog.insert(prog.end(), 2 * QIW_WAVE, pad);
+    if (topo.empty()) topo.push_back(0);
     if (coefs.empty()) coefs.push_back(0.0);
     d.groups = (QiwGroup *)dev(c, tag + ".groups", gs.size() * sizeof(QiwGroup), &rc);
     d.prog = (uint64_t *)dev(c, tag + ".prog", prog.size() * 8, &rc);
-    d.arcs = (uint8_t *)dev(c, tag + ".arcs", arcs.size(), &rc);
+    d.topo = (uint32_t *)dev(c, tag + ".topo", topo.size() * 4, &rc);
     d.coefs = (double *)dev(c, tag + ".coefs", coefs.size() * 8, &rc);
     d.ut = (double *)dev(c, tag + ".ut", std::max<int64_t>(ut_total, 1) * 8, &rc);
     d.tasks = (QiwTask *)dev(c, tag + ".tasks", b.tasks.size() * sizeof(QiwTask), &rc);
@@ -521,7 +580,7 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
     if (rc) return rc;
     HIPCHK(c, hipMemcpy(d.groups, gs.data(), gs.size() * sizeof(QiwGroup), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.prog, prog.data(), prog.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(d.arcs, arcs.data(), arcs.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.topo, topo.data(), topo.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.coefs, coefs.data(), coefs.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.tasks, b.tasks.data(), b.tasks.size() * sizeof(QiwTask), hipMemcpyHostToDevice));
     if (!b.g_first.empty()) {
@@ -664,7 +723,7 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
     const size_t lds_bytes = (size_t)a.lds_slots * sizeof(double);
     if (lds_bytes > 160 * 1024) return fail(c, QIW_E_LIMIT, "evaluator state exceeds the 160 KiB of LDS of a CU");
     if (fmt == FMT_SCALAR) {
-        HIPCHK(c, qiw_launch_eval_scalar(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab));
+        HIPCHK(c, qiw_launch_eval_flat(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab));
         return 0;
     }
     if (fmt == FMT_VEC && m.mmax > 6) {  // wide blocks: the matrix-core walk
@@ -773,7 +832,7 @@ EvalArgs make_eval_args(qiw_ctx *c, const Batch &bt, const DevBatch &d, double *
     a.groups = d.groups;
     a.tasks = d.tasks;
     a.prog = d.prog;
-    a.arcs = d.arcs;
+    a.topo = d.topo;
     a.coefs = d.coefs;
     a.ut = d.ut;
     a.ptab = ptab;

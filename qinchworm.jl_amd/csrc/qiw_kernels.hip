@@ -402,6 +402,265 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------
+// The flat walk of scalar programs (all sectors 1x1).
+//
+// One loop over 32-bit program words, no recursion: every word is one trie node with its
+// level L stated in the word.  The running product `cur` flows down each chain of the
+// trie; a node with several children saves it in the slot of its children's level (PUSH)
+// and every child but the first restores it (POP).  The propagator values of the inner
+// segments (levels 3..NM-1, read by every edge), the arc values and the outputs of the lane's
+// sample stay in registers, read with uniform indices through the VGPR index mode.  LDS
+// holds what is read once per topology or per restored branch: the contour times and the
+// saved products.
+// ---------------------------------------------------------------------------------------
+
+// 32-bit program words in 64-word blocks: lane l loads word base+l of the block with one
+// coalesced load and each word is broadcast with v_readlane; the next block is loaded when
+// the current one is used up.
+struct ProgStream32 {
+    const uint32_t *__restrict__ p;
+    uint32_t blk;
+    int idx, lane;
+    __device__ __forceinline__ void init(const uint32_t *p0, int l) {
+        p = p0;
+        lane = l;
+        blk = p0[l];
+        idx = 0;
+    }
+    __device__ __forceinline__ uint32_t next() {
+        if (idx == QIW_WAVE) {
+            p += QIW_WAVE;
+            blk = p[lane];
+            idx = 0;
+        }
+        const uint32_t w = __builtin_amdgcn_readlane(blk, idx);
+        ++idx;
+        return w;
+    }
+};
+
+template <int KM, int SM>
+struct FlatShape {
+    static constexpr int NM = 2 * KM + 3;
+    static constexpr int NH = (NM - 3) * SM;   // inner entries (NM-1-L)*SM + s, L = 3..NM-1
+    static constexpr int NC = (NH + 15) / 16;  // register vectors of 16 entries
+    static_assert(NC >= 1 && NC <= 6, "inner propagator entries beyond 6 register vectors");
+};
+
+// The inner propagator entries of one lane live in up to six register vectors of 16 doubles
+// (the largest VGPR tuple), declared as separate locals P0..P5: a uniform choice of vector
+// then selects between register values and never becomes a dynamically indexed (scratch)
+// address.  Entry e is element e % 16 of vector e / 16 (e uniform; constant in unrolled
+// loops).  NC_ is the number of vectors in use.
+#define QIW_PIN_SET(NC_, e, x)                              \
+    do {                                                    \
+        const int c_ = (e) >> 4, i_ = (e)&15;               \
+        if (c_ == 0) P0[i_] = (x);                          \
+        else if ((NC_) > 1 && c_ == 1) P1[i_] = (x);        \
+        else if ((NC_) > 2 && c_ == 2) P2[i_] = (x);        \
+        else if ((NC_) > 3 && c_ == 3) P3[i_] = (x);        \
+        else if ((NC_) > 4 && c_ == 4) P4[i_] = (x);        \
+        else if ((NC_) > 5) P5[i_] = (x);                   \
+    } while (0)
+#define QIW_PIN_GET(NC_, e, out)                            \
+    do {                                                    \
+        const int c_ = (e) >> 4, i_ = (e)&15;               \
+        if ((NC_) == 1 || c_ == 0) out = P0[i_];            \
+        else if ((NC_) == 2 || c_ == 1) out = P1[i_];       \
+        else if ((NC_) == 3 || c_ == 2) out = P2[i_];       \
+        else if ((NC_) == 4 || c_ == 3) out = P3[i_];       \
+        else if ((NC_) == 5 || c_ == 4) out = P4[i_];       \
+        else out = P5[i_];                                  \
+    } while (0)
+
+template <int KM, int SM, int TM>
+__global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
+    using F = FlatShape<KM, SM>;
+    constexpr int NM = F::NM, NC = F::NC;
+    constexpr int NW = regvec_size(KM * TM);
+    __shared__ double tl[NM * QIW_WAVE];         // contour time of position p: tl[(p - 1) * 64 + lane]
+    __shared__ double sk[(NM - 2) * QIW_WAVE];   // saved product of level L: sk[(L - 2) * 64 + lane]
+    __shared__ double so[SM * QIW_WAVE];         // output of initial sector s: so[s * 64 + lane]
+    const int lane = threadIdx.x;
+    const QiwTask task = a.tasks[blockIdx.x];
+    const QiwGroup g = a.groups[task.group];
+    const int ns = a.ns, n = g.n_nodes, k = g.order;
+    const int ntab = a.dtab.n_tables;
+    const double inv_h = 1.0 / a.h;
+
+    double tau_i = 0.0, tau_w, tau_f;
+    if (g.kind == QIW_G_BARE) {
+        tau_w = 0.0;
+        tau_f = a.h;
+    } else if (g.kind == QIW_G_BOLD) {
+        tau_w = a.step_w * a.h;
+        tau_f = (a.step_w + 1) * a.h;
+    } else {
+        tau_w = task.tau_index * a.h;
+        tau_f = a.beta;
+    }
+    const int64_t sample = (int64_t)task.first_sample + lane;
+    const bool active = sample < g.n_samples;
+    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples +
+                                        (active ? sample : 0)]
+                                 : 1.0;
+
+    // ---- contour times of the configuration positions (1-based) ----
+    double t[NM + 1];
+    const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
+#pragma unroll
+    for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
+    if (g.sampled) {
+        const int64_t N = g.n_samples;
+        const double *u = a.ut + g.ut_off + (active ? sample : 0);
+        const double dfw = tau_f - tau_w, dwi = tau_w - tau_i;
+#pragma unroll
+        for (int j = 0; j < 2 * KM; ++j) {
+            if (j < g.d_after) {
+                const double v = u[j * N] * dfw + tau_w;
+#pragma unroll
+                for (int p = 1; p <= NM; ++p)
+                    if (p == n - 1 - j) t[p] = v;
+            } else if (j < g.d_after + g.d_before) {
+                const double v = u[j * N] * dwi + tau_i;
+                const int pos = g.pos_w - 1 - (j - g.d_after);
+#pragma unroll
+                for (int p = 1; p <= NM; ++p)
+                    if (p == pos) t[p] = v;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 1; p <= NM; ++p) tl[(p - 1) * QIW_WAVE + lane] = t[p];
+
+    // ---- propagator values i*P_s(t_L - t_{L-1}), L = 2..n (ordering clamp) ----
+    typename RegVec<double, 16>::type P0 = 0.0, P1 = 0.0, P2 = 0.0, P3 = 0.0, P4 = 0.0, P5 = 0.0;
+    double p_first[SM], p_last[SM];  // iP_s of segment 2 and of segment n
+#pragma unroll
+    for (int L = 2; L <= NM; ++L) {
+        if (L <= n) {
+            double dt = t[L] - t[L - 1];
+            dt = dt > 0.0 ? dt : 0.0;
+            const double x = dt * inv_h;
+            int j = (int)floor(x);
+            j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+            const double f = x - (double)j;
+            const double *r0 = a.ptab + (int64_t)j * ns, *r1 = r0 + ns;
+#pragma unroll
+            for (int s = 0; s < SM; ++s) {
+                double v = 0.0;
+                if (s < ns)
+                    v = (g.kind == QIW_G_BARE) ? exp(-dt * a.e_shift[s]) : (1.0 - f) * r0[s] + f * r1[s];
+                if (L == 2) p_first[s] = v;
+                if (L == n) p_last[s] = v;
+                if (L >= 3 && L <= NM - 1) QIW_PIN_SET(NC, (NM - 1 - L) * SM + s, v);
+            }
+        }
+    }
+
+    typename RegVec<double, NW>::type W;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) W[q] = 0.0;
+#pragma unroll
+    for (int s = 0; s < SM; ++s) so[s * QIW_WAVE + lane] = 0.0;
+    double acc = 0.0, cur = 0.0;
+
+    // ---- walk the program: one word per iteration, until END ----
+    const double *coefs = a.coefs + g.coef_off;
+    const uint32_t *topo = a.topo + g.topo_off;
+    ProgStream32 ps;
+    ps.init(reinterpret_cast<const uint32_t *>(a.prog + g.prog_off), lane);
+    uint32_t w = ps.next();
+    for (;;) {
+        // A run of EDGE words: the inner loop reads the arc values and the propagators but
+        // never writes them, so they stay in place across its iterations.
+        while (w < QIW_FL_OP) {
+            // The empty volatile asm statements keep the rare cases as uniform branches: the
+            // compiler would otherwise evaluate them on every word and select the result.
+            const int L = (int)((w >> 16) & 15);
+            if (w & QIW_FL_POP) cur = sk[(L - 2) * QIW_WAVE + lane];
+            if (w & QIW_FL_TAIL) {
+                asm volatile("");
+                cur *= W[(int)((w >> 8) & (NW - 1))];
+            }
+            if (w & QIW_FL_LEAF) {
+                if (w & QIW_FL_UNIT) {
+                    acc += (w & QIW_FL_NEG) ? -cur : cur;
+                } else {
+                    asm volatile("");
+                    acc += coefs[w & 0xFF] * cur;
+                }
+            } else {
+                double pv;
+                QIW_PIN_GET(NC, (int)(w & 0xFF), pv);
+                cur *= pv;
+                if (w & QIW_FL_PUSH) sk[(L - 1) * QIW_WAVE + lane] = cur;
+            }
+            w = ps.next();
+        }
+        const uint32_t op = w >> 28;
+        if (op == QIW_FL_OP_END) break;
+        if (op == QIW_FL_OP_ROOT) {  // position 1: cur = iP_2[s], or the leaf of n = 2
+            if (w & QIW_FL_LEAF) {
+                acc += (w & QIW_FL_UNIT) ? ((w & QIW_FL_NEG) ? -1.0 : 1.0) : coefs[w & 0xFF];
+            } else {
+                cur = pick(p_first, (int)(w & 0xFF));
+                if (w & QIW_FL_PUSH) sk[lane] = cur;
+            }
+        } else if (op == QIW_FL_OP_TOPO) {
+            const uint32_t *tw = topo + 2 * (w & 0xFFFFFF);
+            const uint32_t lo = tw[0], hi = tw[1];
+            // every slot is rewritten, so the old values die here: arcs beyond the order
+            // read position 1, tables beyond the model's count read its last table
+#pragma unroll
+            for (int ai = 0; ai < KM; ++ai) {
+                const uint32_t b = ai >= k ? 0x11u : ai < 4 ? (lo >> (8 * ai)) : (hi >> (8 * (ai - 4)));
+                const int head = (int)(b & 15), tail = (int)((b >> 4) & 15);
+                double z = tl[(tail - 1) * QIW_WAVE + lane] - tl[(head - 1) * QIW_WAVE + lane];
+                z = z > 0.0 ? z : 0.0;
+                const double x = z * inv_h;
+                int j = (int)floor(x);
+                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
+                const double u = x - (double)j;
+#pragma unroll
+                for (int tb = 0; tb < TM; ++tb) {
+                    const int tt = tb < ntab ? tb : (ntab > 0 ? ntab - 1 : 0);
+                    const double *c = a.dtab.coef + ((int64_t)tt * (a.n_tau - 1) + j) * 4;
+                    W[ai * TM + tb] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                }
+            }
+#pragma unroll
+            for (int q = KM * TM; q < NW; ++q) W[q] = 0.0;
+        } else {  // FLUSH: out[s0] += acc * iP_n[s0]
+            const int s0 = (int)(w & 0xFF);
+            so[s0 * QIW_WAVE + lane] += acc * pick(p_last, s0);
+            acc = 0.0;
+        }
+        w = ps.next();
+    }
+
+    // ---- wavefront reduction of the per-s0 outputs ----
+    double out[SM];
+#pragma unroll
+    for (int s = 0; s < SM; ++s) out[s] = so[s * QIW_WAVE + lane];
+    if (a.per_sample && active) {
+        double m = 0.0;
+#pragma unroll
+        for (int s = 0; s < SM; ++s) m += (s < ns) ? fabs(out[s]) : 0.0;
+        a.per_sample[g.ps_off + sample] = m;
+    }
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+        if (s < ns) {
+            double v = active ? out[s] * wt : 0.0;
+#pragma unroll
+            for (int off = QIW_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, QIW_WAVE);
+            if (lane == 0) a.partial[(int64_t)task.out_index * ns + s] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // The diagram evaluator for models with sector blocks larger than 1x1.
 //
 // One wavefront walks the trie for the 64 samples of its task one after the other; for
@@ -1337,6 +1596,20 @@ hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const ui
 }
 
 // Register budget of the scalar walk: KM = largest order, SM = sectors, TM = tables.
+// The instantiation (KM, SM, TM) is the one the host compiler laid the program out for:
+// qiw_scalar_km / _sm / _tm of the launch's largest order, sectors and tables.
+hipError_t qiw_launch_eval_flat(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm) {
+    if (km > QIW_KMAX || sm > 8 || tm > 4) return hipErrorInvalidValue;
+    const int KM = qiw_scalar_km(km, sm, tm), SM = qiw_scalar_sm(km, sm, tm), TM = qiw_scalar_tm(km, sm, tm);
+    if (KM == 4 && SM == 4 && TM == 2)
+        hipLaunchKernelGGL((k_eval_flat_t<4, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    else if (KM == 4)
+        hipLaunchKernelGGL((k_eval_flat_t<4, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_eval_flat_t<QIW_KMAX, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm) {
     if (km <= 4 && sm <= 4 && tm <= 2)
         hipLaunchKernelGGL((k_eval_scalar_t<4, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
