@@ -42,7 +42,7 @@ struct HostModel {
     int bt = 0;
     std::vector<int> bt_off;
     bool scalar = true;          // every sector is one-dimensional
-    bool scalar_kernel = true;   // programs for k_eval_scalar_t (scalar model within its register budget)
+    bool scalar_kernel = true;   // flat scalar programs for k_eval_flat_t (scalar model, <= 8 sectors, <= 4 tables)
     bool vec_ok = false;         // programs for k_eval_vec_t up to order 4 (blocks <= 6, <= 4 tables)
     bool mfma_ok = false;        // the same programs on k_eval_mfma_t (blocks <= 32, <= 2 tables)
     int mmax = 1;
@@ -55,6 +55,9 @@ struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
 };
+
+struct PlanStore;  // compiled batches resident on the device, by buffer tag (qiw_host.cpp)
+void plans_destroy(PlanStore *p);
 
 struct WarpState;  // learned warp of the unit cube (qiw_warp.cpp)
 void warp_destroy(WarpState *w);
@@ -79,6 +82,7 @@ struct qiw_ctx {
     std::string err;
     bool has_model = false;
     qiwi::HostModel m;
+    std::string model_fp;      // bytes of the model content the compiled programs depend on
     bool vec_staged = false;   // operator blocks of the vector walk are on the device
     bool mfma_staged = false;  // operator blocks of the matrix-core walk are on the device
     // Sobol state
@@ -93,6 +97,7 @@ struct qiw_ctx {
     std::map<std::string, qiwi::DevBuf> bufs;
     qiw_stats stats{};
     qiwi::WarpState *warp = nullptr;  // learned warp, or null
+    qiwi::PlanStore *plans = nullptr;  // compiled batches kept between calls
 };
 
 namespace qiwi {

@@ -9,17 +9,8 @@
 #define QIW_WAVE 64
 
 // ---- program words ----------------------------------------------------------------------
-// Scalar programs (k_eval_scalar_t, all sectors 1x1), one word per trie node in preorder:
-// ROOT   bits  8-15 : sector after position 1, bits 24-31 : number of level-2 children;
-//        the next word is the constant of position 1 (float64 bits)
-// EDGE   bit   6    : LEAF (last position): the next word is the leaf coefficient (float64),
-//                     unless bit 5 (UNIT) says the coefficient is +1, or -1 with bit 4 (NEG)
-//        bit   7    : TAIL  multiply the arc value in W slot bits 16-23 (arc * n_tables + table)
-//        bits  8-15 : sector after this node's operator (propagator sector of its children)
-//        bits 24-31 : number of children
-// TOPO   bits 8a..8a+3 / 8a+4..8a+7 : head / tail position of arc a (a < order)
-// FLUSH  bits  8-15 : initial sector s0 (out[s0] += acc; acc = 0)
-// all    bits 48-51 : op 0 EDGE, 1 TOPO, 2 FLUSH, 3 END, 4 NOP, 5 ROOT, 6 SEG
+// Block programs (k_eval_block) and vector programs carry 64-bit words; all formats use
+// bits 48-51 for the op: 0 EDGE, 1 TOPO, 2 FLUSH, 3 END, 4 NOP, 5 ROOT, 6 SEG.
 //
 // Vector programs (k_eval_vec_t, blocks m_s <= MB): one segment per initial sector s0,
 // SEG, then TOPO ROOT EDGE... per topology, then FLUSH (replayed for every column of s0):
@@ -38,9 +29,7 @@ enum { QIW_OP_EDGE = 0, QIW_OP_TOPO = 1, QIW_OP_FLUSH = 2, QIW_OP_END = 3, QIW_O
 #define QIW_F_LEAF (1ull << 6)
 #define QIW_F_TAIL (1ull << 7)
 // Leaf coefficients of magnitude 1 carry no coefficient word: UNIT, with NEG for -1
-// (scalar programs: bits 4, 5; vector programs: bits 8, 9)
-#define QIW_FS_NEG (1ull << 4)
-#define QIW_FS_UNIT (1ull << 5)
+// (vector programs: bits 8, 9)
 #define QIW_FV_NEG (1ull << 8)
 #define QIW_FV_UNIT (1ull << 9)
 
@@ -56,10 +45,10 @@ static inline int qiw_vec_tm(int tm) { return tm <= 2 ? 2 : 4; }
 // running product flows down each chain; a node with several children saves it in the slot
 // of its children's level (PUSH) and every child but the first restores it (POP).
 // EDGE   (a node at level L >= 2, its configuration position)
-//        bits  0-7  : inner propagator entry multiplied after the node, (NM - 1 - (L + 1)) * SM
-//                     + sector after the node's operator (NM = 2 KM + 3, SM: sector stride;
-//                     qiw_scalar_km / _sm), or for LEAF without UNIT the index of the
-//                     coefficient in the group's table
+//        bits  0-7  : twice the inner propagator entry multiplied after the node, entry
+//                     (NM - 1 - (L + 1)) * SM + sector after the node's operator (NM = 2 KM + 3,
+//                     SM: sector stride; qiw_scalar_km / _sm), or for LEAF without UNIT the
+//                     index of the coefficient in the group's table
 //        bits  8-15 : W slot of the arc value (TAIL)
 //        bits 16-19 : level L
 //        bit  20 TAIL : multiply the arc value
@@ -73,6 +62,7 @@ static inline int qiw_vec_tm(int tm) { return tm <= 2 ? 2 : 4; }
 // TOPO   op 1, bits 0-23 : topology index into the group's table of arc head/tail nibbles
 // FLUSH  op 2, bits 0-7 : s0 (out[s0] += acc * iP_n[s0]; acc = 0)
 // END    op 3
+// NEXT   op 5, the last word of every 64-word block: continue with the next block
 #define QIW_FL_TAIL (1u << 20)
 #define QIW_FL_LEAF (1u << 21)
 #define QIW_FL_PUSH (1u << 22)
@@ -80,7 +70,7 @@ static inline int qiw_vec_tm(int tm) { return tm <= 2 ? 2 : 4; }
 #define QIW_FL_UNIT (1u << 24)
 #define QIW_FL_NEG (1u << 25)
 #define QIW_FL_OP (1u << 28)
-enum { QIW_FL_OP_TOPO = 1, QIW_FL_OP_FLUSH = 2, QIW_FL_OP_END = 3, QIW_FL_OP_ROOT = 4 };
+enum { QIW_FL_OP_TOPO = 1, QIW_FL_OP_FLUSH = 2, QIW_FL_OP_END = 3, QIW_FL_OP_ROOT = 4, QIW_FL_OP_NEXT = 5 };
 
 // Sector stride SM and largest order KM of the flat scalar walk instantiation (see
 // qiw_scalar_tm); its inner propagator entries are (2*KM + 2 - L) * SM + s for level L,
@@ -128,7 +118,7 @@ struct QiwDeltaTables {
     const double *coef;    // [n_tables][n_tau-1][4] local cubic (linear: c2 = c3 = 0)
 };
 
-// Arguments of the diagram evaluator (k_eval_scalar)
+// Arguments of the diagram evaluators (k_eval_flat_t; the other walks embed them)
 struct EvalArgs {
     const QiwGroup *groups;
     const QiwTask *tasks;
@@ -232,7 +222,6 @@ hipError_t qiw_launch_warp_coords(hipStream_t st, const double *z, int stride, c
                                   int d_after, int64_t n, double *out);
 hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,
                                      int d_after, double t_i, double t_w, double t_f, int64_t n, double *out);
-hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
 hipError_t qiw_launch_eval_flat(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm);
 hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes);
 hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes);

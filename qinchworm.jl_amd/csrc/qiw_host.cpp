@@ -224,6 +224,12 @@ struct Compiler {
     int pstride;  // scalar programs: inner P entry of level L, sector s: (nm - 1 - L) * pstride + s
     int nm;       // scalar programs: positions of the largest order of the instantiation
     std::vector<uint32_t> p32;  // scalar program words
+
+    // appends a scalar program word; the last word of every 64-word block is NEXT
+    void put32(uint32_t w) {
+        if (p32.size() % QIW_WAVE == QIW_WAVE - 1) p32.push_back((uint32_t)QIW_FL_OP_NEXT << 28);
+        p32.push_back(w);
+    }
     int n = 0, pos_w = 0;
     std::vector<int> conf_kind, conf_arc;
     std::vector<int> selected;
@@ -363,12 +369,12 @@ struct Compiler {
             out.edges_per_sample++;
             out.leaves_per_sample++;
             if (!coef_bits(trie[t.children.at(0)].coef, w, out)) return false;
-            p32.push_back(w);
+            put32(w);
             return true;
         }
-        if (id != 0) w |= (uint32_t)((nm - 1 - (L + 1)) * pstride + t.s_out);
+        if (id != 0) w |= (uint32_t)(2 * ((nm - 1 - (L + 1)) * pstride + t.s_out));
         if (t.children.size() > 1) w |= QIW_FL_PUSH;
-        p32.push_back(w);
+        put32(w);
         for (size_t i = 0; i < t.children.size(); ++i)
             if (!emit_flat(t.children[i], L + 1, i > 0, out)) return false;
         return true;
@@ -416,7 +422,7 @@ struct Compiler {
             }
             out.topo.push_back(tw[0]);
             out.topo.push_back(tw[1]);
-            if (order > 0 && fmt == FMT_SCALAR) p32.push_back(((uint32_t)QIW_FL_OP_TOPO << 28) | (uint32_t)ti);
+            if (order > 0 && fmt == FMT_SCALAR) put32(((uint32_t)QIW_FL_OP_TOPO << 28) | (uint32_t)ti);
             if (order > 0 && fmt == FMT_BLOCK) out.prog.push_back(topo_word);
             for (s0 = 0; s0 < m.ns; ++s0) {
                 trie.clear();
@@ -437,7 +443,7 @@ struct Compiler {
                 if (fmt == FMT_SCALAR) {  // position 1 is the root, level 1
                     trie[0].s_out = s;
                     if (!emit_flat(0, 1, true, out)) return false;
-                    p32.push_back(((uint32_t)QIW_FL_OP_FLUSH << 28) | (uint32_t)s0);
+                    put32(((uint32_t)QIW_FL_OP_FLUSH << 28) | (uint32_t)s0);
                     continue;
                 }
                 if (trie[0].children.size() > 255) {
@@ -471,7 +477,7 @@ struct Compiler {
             }
         }
         if (fmt == FMT_SCALAR) {  // 32-bit words, two to a 64-bit program element
-            p32.push_back((uint32_t)QIW_FL_OP_END << 28);
+            put32((uint32_t)QIW_FL_OP_END << 28);
             if (p32.size() % 2) p32.push_back((uint32_t)QIW_FL_OP_END << 28);
             for (size_t i = 0; i < p32.size(); i += 2) out.prog.push_back(p32[i] | ((uint64_t)p32[i + 1] << 32));
         } else {
@@ -620,12 +626,23 @@ int check_run(qiw_ctx *c, const qiw_run *run) {
 
 // Heaviest groups first: workgroups are dispatched in task order, so the long tasks do
 // not form the tail of the launch.  Each task keeps its output row, so the summation
-// order of the results does not change.
+// order of the results does not change.  Tasks are bucketed by group (linear time).
 void order_tasks(Batch &b) {
-    std::stable_sort(b.tasks.begin(), b.tasks.end(), [&](const QiwTask &x, const QiwTask &y) {
-        const Compiled &gx = b.groups[x.group], &gy = b.groups[y.group];
-        return gx.edges_per_sample + gx.leaves_per_sample > gy.edges_per_sample + gy.leaves_per_sample;
-    });
+    const size_t ng = b.groups.size();
+    std::vector<int> go(ng);
+    for (size_t i = 0; i < ng; ++i) go[i] = (int)i;
+    auto weight = [&](int gi) { return b.groups[gi].edges_per_sample + b.groups[gi].leaves_per_sample; };
+    std::stable_sort(go.begin(), go.end(), [&](int x, int y) { return weight(x) > weight(y); });
+    std::vector<size_t> count(ng, 0), start(ng, 0);
+    for (const QiwTask &t : b.tasks) count[t.group]++;
+    size_t pos = 0;
+    for (int gi : go) {
+        start[gi] = pos;
+        pos += count[gi];
+    }
+    std::vector<QiwTask> out(b.tasks.size());
+    for (const QiwTask &t : b.tasks) out[start[t.group]++] = t;
+    b.tasks.swap(out);
 }
 
 // Samples per task.  The scalar and vector walks map one sample to a lane (64 per task);
@@ -696,6 +713,73 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
         }
     }
     order_tasks(b);
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Plans: a compiled batch and its device copy (programs, tasks, simplex coordinates) kept
+// under its buffer tag, and reused while the model content, the Sobol tables and the run
+// parameters it was built from stay the same.  Warped coordinates are rewritten on every
+// step, so a plan of warped groups is never reused.
+// ---------------------------------------------------------------------------------------
+namespace qiwi {
+struct Plan {
+    std::string key;
+    Batch b;
+    DevBatch d;
+    bool first_is_order0 = false;  // correlator plans: the leading group is order 0
+    int32_t *rf = nullptr, *rcnt = nullptr;  // correlator plans: per (tau row, group) task ranges
+};
+struct PlanStore {
+    std::map<std::string, Plan> by_tag;
+};
+void plans_destroy(PlanStore *p) { delete p; }
+}  // namespace qiwi
+
+namespace {
+
+Plan &plan_slot(qiw_ctx *c, const std::string &tag) {
+    if (!c->plans) c->plans = new PlanStore;
+    return c->plans->by_tag[tag];
+}
+
+std::string plan_key(const qiw_ctx *c, const qiw_run *run, const std::vector<int> &oall, const std::string &kind) {
+    std::string k = kind;
+    auto put = [&](const void *p, size_t n) { k.append((const char *)p, n); };
+    put(&run->N_samples, 8);
+    put(&run->n_pts_after_max, 4);
+    put(&run->precision, 4);
+    put(&run->n_orders, 4);
+    put(run->orders, 4 * (size_t)run->n_orders);
+    put(&run->n_orders_bare, 4);
+    put(run->orders_bare, 4 * (size_t)run->n_orders_bare);
+    const int no = (int)oall.size();
+    put(&no, 4);
+    put(oall.data(), 4 * (size_t)no);
+    put(&c->sob_dim, 4);
+    put(c->sob_m.data(), 4 * c->sob_m.size());
+    put(c->sob_x.data(), 4 * c->sob_x.size());
+    return k + c->model_fp;
+}
+
+// The plan of a bare or bold step batch (tag "bare" / "bold"), built and uploaded unless
+// the resident one matches.
+int step_plan(qiw_ctx *c, const qiw_run *run, const std::vector<int> &oall, bool bare, bool warp, Plan *&out) {
+    const std::string tag = bare ? "bare" : "bold";
+    Plan &p = plan_slot(c, tag);
+    out = &p;
+    const std::string key = plan_key(c, run, oall, tag);
+    if (!warp && p.key == key) return 0;
+    p = Plan{};
+    int rc = build_step_batch(c, run, bare, oall, p.b);
+    if (rc) return rc;
+    const bool weighted = warp && !p.b.groups.empty();
+    if (weighted && p.b.fmt == FMT_BLOCK)
+        return fail(c, QIW_E_LIMIT, "the learned warp needs sector blocks <= 6 and orders <= 4");
+    if (!p.b.groups.empty() && (rc = upload(c, tag, p.b, run->N_samples, p.d, weighted))) return rc;
+    if (!weighted) p.key = key;
     return 0;
 }
 
@@ -889,11 +973,43 @@ extern "C" int qiw_create(qiw_ctx **out, int device) {
     return QIW_OK;
 }
 
+namespace {
+// Bytes of everything the compiled programs and task lists depend on: sector structure,
+// operator elements and blocks, pair tables, correlator operators, grid size.
+std::string model_fingerprint(const HostModel &m) {
+    std::string k;
+    auto put = [&](const void *p, size_t n) { k.append((const char *)p, n); };
+    auto vec = [&](const auto &v) {
+        const size_t n = v.size();
+        put(&n, sizeof n);
+        if (n) put(v.data(), n * sizeof(v[0]));
+    };
+    put(&m.ntau, sizeof m.ntau);
+    put(&m.ns, sizeof m.ns);
+    put(&m.ntab, sizeof m.ntab);
+    vec(m.dims);
+    vec(m.pair_table);
+    vec(m.op_data);
+    vec(m.op_desc);
+    for (const auto *ops : {&m.op_i, &m.op_f, &m.corr_A, &m.corr_B}) {
+        const size_t n = ops->size();
+        put(&n, sizeof n);
+        for (const ScalarOp &o : *ops) {
+            vec(o.sf);
+            vec(o.val);
+            vec(o.blk);
+        }
+    }
+    return k;
+}
+}  // namespace
+
 extern "C" void qiw_destroy(qiw_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     warp_destroy(c->warp);
+    plans_destroy(c->plans);
     for (auto &kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1000,6 +1116,7 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
                 m.op_data.size() < 0xFFFF - 64;
     c->m = m;
     c->has_model = true;
+    c->model_fp = model_fingerprint(m);
     c->vec_staged = false;
     c->mfma_staged = false;
     atomic_ip(c->m, c->ip);
@@ -1066,15 +1183,13 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     }
     int n_slots = std::max<int>((int)oall.size(), 1);
     int rc = 0;
-    Batch bare_b, bold_b;
-    DevBatch bare_d, bold_d;
-    if (do_bare && (rc = build_step_batch(c, run, true, oall, bare_b))) return rc;
-    if (last_w >= first_w && (rc = build_step_batch(c, run, false, oall, bold_b))) return rc;
-    const bool warped = warp_enabled(c->warp) && !bold_b.groups.empty();
-    if (warped && bold_b.fmt == FMT_BLOCK)
-        return fail(c, QIW_E_LIMIT, "the learned warp needs sector blocks <= 6 and orders <= 4");
-    if (!bare_b.groups.empty() && (rc = upload(c, "bare", bare_b, run->N_samples, bare_d))) return rc;
-    if (!bold_b.groups.empty() && (rc = upload(c, "bold", bold_b, run->N_samples, bold_d, warped))) return rc;
+    Plan none, *bp = &none, *sp = &none;
+    const bool want_warp = warp_enabled(c->warp);
+    if (do_bare && (rc = step_plan(c, run, oall, true, false, bp))) return rc;
+    if (last_w >= first_w && (rc = step_plan(c, run, oall, false, want_warp, sp))) return rc;
+    Batch &bare_b = bp->b, &bold_b = sp->b;
+    DevBatch &bare_d = bp->d, &bold_d = sp->d;
+    const bool warped = want_warp && !bold_b.groups.empty();
     if (warped) {
         std::vector<WarpGroup> wg;
         for (auto &cg : bold_b.groups)
@@ -1311,57 +1426,71 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
     double *ptab = (double *)dev(c, "ptab", (size_t)m.ntau * m.bt * 8, &rc);
     if (rc) return rc;
     HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
+    std::vector<int> oall(run->orders, run->orders + run->n_orders);
     for (int ci = 0; ci < m.ncorr; ++ci) {
         // groups (src/inchworm.jl:957-981): every n_pts_after of every order
-        Batch b;
-        int max_order = 0;
-        for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
-        b.fmt = program_format(m, max_order, run->precision);
-        b.max_order = max_order;
-        b.precision = run->precision;
-        bool first_is_order0 = false;
-        for (int oi = 0; oi < run->n_orders; ++oi) {
-            int order = run->orders[oi];
-            if (order > 0 && run->N_samples <= 0) continue;
-            int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
-            for (int na = lo; na <= hi; ++na) {
-                Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
-                Compiled cg;
-                if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
-                if (cg.n_top == 0) continue;
-                if (b.groups.empty() && order == 0) first_is_order0 = true;
-                b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
-                b.groups.push_back(std::move(cg));
+        const std::string tag = "corr" + std::to_string(ci);
+        Plan &p = plan_slot(c, tag);
+        const std::string key = plan_key(c, run, oall, tag);
+        if (p.key != key) {
+            p = Plan{};
+            Batch &b = p.b;
+            int max_order = 0;
+            for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
+            b.fmt = program_format(m, max_order, run->precision);
+            b.max_order = max_order;
+            b.precision = run->precision;
+            for (int oi = 0; oi < run->n_orders; ++oi) {
+                int order = run->orders[oi];
+                if (order > 0 && run->N_samples <= 0) continue;
+                int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
+                for (int na = lo; na <= hi; ++na) {
+                    Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
+                    Compiled cg;
+                    if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
+                    if (cg.n_top == 0) continue;
+                    if (b.groups.empty() && order == 0) p.first_is_order0 = true;
+                    b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
+                    b.groups.push_back(std::move(cg));
+                }
             }
+            if (!b.groups.empty()) {
+                int ng = (int)b.groups.size();
+                b.spt = samples_per_task(m, b.fmt, (int64_t)m.ntau * ng * std::max<int64_t>(run->N_samples, 1));
+                std::vector<int32_t> rf((size_t)m.ntau * ng, 0), rcnt((size_t)m.ntau * ng, 0);
+                for (int j = 0; j < m.ntau; ++j)
+                    for (int gi = 0; gi < ng; ++gi) {
+                        // tau_A = tau_B: only the leading order-0 group (src/inchworm.jl:1014-1024)
+                        if (j == 0 && !(gi == 0 && p.first_is_order0)) continue;
+                        int64_t N = b.groups[gi].g.order > 0 ? run->N_samples : 1;
+                        int nt = (int)((N + b.spt - 1) / b.spt);
+                        rf[(size_t)j * ng + gi] = (int32_t)b.tasks.size();
+                        rcnt[(size_t)j * ng + gi] = nt;
+                        for (int t = 0; t < nt; ++t) b.tasks.push_back({gi, t * b.spt, j, (int32_t)b.tasks.size()});
+                        b.flops_per_launch +=
+                            (double)N * (2.0 * b.groups[gi].edges_per_sample + 2.0 * b.groups[gi].leaves_per_sample);
+                        b.leaves_per_launch += N * b.groups[gi].leaves_per_sample;
+                        b.diagram_evals_per_launch += N * b.groups[gi].n_top;
+                    }
+                order_tasks(b);
+                if ((rc = upload(c, tag, b, run->N_samples, p.d))) return rc;
+                p.rf = (int32_t *)dev(c, tag + ".rf", rf.size() * 4, &rc);
+                p.rcnt = (int32_t *)dev(c, tag + ".rc", rcnt.size() * 4, &rc);
+                if (rc) return rc;
+                HIPCHK(c, hipMemcpy(p.rf, rf.data(), rf.size() * 4, hipMemcpyHostToDevice));
+                HIPCHK(c, hipMemcpy(p.rcnt, rcnt.data(), rcnt.size() * 4, hipMemcpyHostToDevice));
+            }
+            p.key = key;
         }
+        Batch &b = p.b;
+        DevBatch &d = p.d;
         if (b.groups.empty()) continue;
         int ng = (int)b.groups.size();
-        b.spt = samples_per_task(m, b.fmt, (int64_t)m.ntau * ng * std::max<int64_t>(run->N_samples, 1));
-        std::vector<int32_t> rf((size_t)m.ntau * ng, 0), rcnt((size_t)m.ntau * ng, 0);
-        for (int j = 0; j < m.ntau; ++j)
-            for (int gi = 0; gi < ng; ++gi) {
-                // tau_A = tau_B: only the leading order-0 group (src/inchworm.jl:1014-1024)
-                if (j == 0 && !(gi == 0 && first_is_order0)) continue;
-                int64_t N = b.groups[gi].g.order > 0 ? run->N_samples : 1;
-                int nt = (int)((N + b.spt - 1) / b.spt);
-                rf[(size_t)j * ng + gi] = (int32_t)b.tasks.size();
-                rcnt[(size_t)j * ng + gi] = nt;
-                for (int t = 0; t < nt; ++t) b.tasks.push_back({gi, t * b.spt, j, (int32_t)b.tasks.size()});
-                b.flops_per_launch += (double)N * (2.0 * b.groups[gi].edges_per_sample + 2.0 * b.groups[gi].leaves_per_sample);
-                b.leaves_per_launch += N * b.groups[gi].leaves_per_sample;
-                b.diagram_evals_per_launch += N * b.groups[gi].n_top;
-            }
-        order_tasks(b);
-        DevBatch d;
-        if ((rc = upload(c, "corr", b, run->N_samples, d))) return rc;
         double *partial = (double *)dev(c, "corr.partial", b.tasks.size() * m.bt * 8, &rc);
         int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
-        int32_t *drf = (int32_t *)dev(c, "corr.rf", rf.size() * 4, &rc);
-        int32_t *drc = (int32_t *)dev(c, "corr.rc", rcnt.size() * 4, &rc);
+        int32_t *drf = p.rf, *drc = p.rcnt;
         double *dout = (double *)dev(c, "corr.out", (size_t)m.ntau * 8, &rc);
         if (rc) return rc;
-        HIPCHK(c, hipMemcpy(drf, rf.data(), rf.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(drc, rcnt.data(), rcnt.size() * 4, hipMemcpyHostToDevice));
         hipEvent_t e0, e1;
         HIPCHK(c, hipEventCreate(&e0));
         HIPCHK(c, hipEventCreate(&e1));

@@ -4,10 +4,10 @@
 // Work mapping of the evaluators: all lanes of a wavefront walk the same precompiled
 // configuration trie (the depth-first traversal of src/topology_eval.jl:454-556, compiled on
 // the host into program words), so control flow is uniform and program words are broadcast
-// from one coalesced block load.  The scalar and vector walks give each lane one qMC sample
-// (64 consecutive samples of one (order, n_pts_after) group per wavefront) and keep the
-// sample's operands in registers; the matrix-core and LDS block walks give the wavefront
-// one sample at a time and spread its matrix products over the lanes.
+// from one coalesced block load.  The flat scalar walk and the vector walk give each lane
+// one qMC sample (64 consecutive samples of one (order, n_pts_after) group per wavefront)
+// and keep the sample's hot operands in registers; the matrix-core and LDS block walks give
+// the wavefront one sample at a time and spread its matrix products over the lanes.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -179,25 +179,6 @@ struct RegVec<T, 16> {
 };
 constexpr int regvec_size(int n) { return n <= 8 ? 8 : 16; }
 
-template <int KM, int SM, int TM>
-struct ScalarState {
-    static constexpr int NM = 2 * KM + 3;
-    static constexpr int LPC = 16 / SM;                      // segments per P chunk
-    static constexpr int NCH = (NM - 1 + LPC - 1) / LPC;     // P chunks
-    typename RegVec<double, 16>::type P[NCH];                // i*P_s(t_L - t_{L-1}), L = 2..n
-    typename RegVec<double, regvec_size(KM * TM)>::type W;   // i*Delta_tab per arc and table
-    double out[SM];         // per initial sector sums
-    double acc;
-    ProgStream ps;
-};
-
-// i*P_s of segment L (compile time) for a uniform sector s
-template <int L, int KM, int SM, int TM>
-__device__ __forceinline__ double pget(const ScalarState<KM, SM, TM> &st, int s) {
-    using S = ScalarState<KM, SM, TM>;
-    return st.P[(L - 2) / S::LPC][((L - 2) % S::LPC) * SM + s];
-}
-
 // v[i] for a uniform i as a v_cndmask chain over registers.  The empty asm makes each
 // element an opaque register value, so the select chain is not rewritten into an indexed
 // load from a private (scratch) copy of the array.
@@ -226,181 +207,6 @@ __device__ __forceinline__ T pickv(const T (&v)[N], int i) {
     return r;
 }
 
-// Scalar trie, format of the structured walk (see qiw_device.h): every node word carries
-// the sector after its operator and the number of its children, so a node's children
-// share one propagator factor: base = parent * i*P_L[s_par].
-template <int L, int KM, int SM, int TM>
-__device__ __forceinline__ void walk(ScalarState<KM, SM, TM> &st, int n, double vpar, int s_par, int nchild) {
-    if constexpr (L <= 2 * KM + 3) {
-        // the last propagator iP_n[s0] is common to all leaves and applied at FLUSH
-        const double base = (L == n) ? vpar : vpar * pget<L>(st, s_par);
-        for (int c = 0; c < nchild; ++c) {
-            const uint64_t w = st.ps.next();
-            const double v = (w & QIW_F_TAIL) ? base * st.W[(int)((w >> 16) & 0xFF)] : base;
-            if (w & QIW_F_LEAF) {
-                if (w & QIW_FS_UNIT)
-                    st.acc += (w & QIW_FS_NEG) ? -v : v;
-                else
-                    st.acc += __longlong_as_double((long long)st.ps.next()) * v;
-            } else if (L < n) {
-                walk<L + 1, KM, SM, TM>(st, n, v, (int)((w >> 8) & 0xFF), (int)((w >> 24) & 0xFF));
-            }
-        }
-    }
-}
-
-template <int KM, int SM, int TM>
-__global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
-    constexpr int NM = 2 * KM + 3;
-    const int lane = threadIdx.x;
-    const QiwTask task = a.tasks[blockIdx.x];
-    const QiwGroup g = a.groups[task.group];
-    const int ns = a.ns, n = g.n_nodes, k = g.order;
-    const int ntab = a.dtab.n_tables;
-    const double inv_h = 1.0 / a.h;
-
-    double tau_i = 0.0, tau_w, tau_f;
-    if (g.kind == QIW_G_BARE) {
-        tau_w = 0.0;
-        tau_f = a.h;
-    } else if (g.kind == QIW_G_BOLD) {
-        tau_w = a.step_w * a.h;
-        tau_f = (a.step_w + 1) * a.h;
-    } else {
-        tau_w = task.tau_index * a.h;
-        tau_f = a.beta;
-    }
-    const int64_t sample = (int64_t)task.first_sample + lane;
-    const bool active = sample < g.n_samples;
-    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples +
-                                        (active ? sample : 0)]
-                                 : 1.0;
-
-    // ---- contour times of the configuration positions (1-based) ----
-    double t[NM + 1];
-    const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
-#pragma unroll
-    for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
-    if (g.sampled) {
-        const int64_t N = g.n_samples;
-        const double *u = a.ut + g.ut_off + (active ? sample : 0);
-        const double dfw = tau_f - tau_w, dwi = tau_w - tau_i;
-#pragma unroll
-        for (int j = 0; j < 2 * KM; ++j) {
-            if (j < g.d_after) {
-                const double v = u[j * N] * dfw + tau_w;
-#pragma unroll
-                for (int p = 1; p <= NM; ++p)
-                    if (p == n - 1 - j) t[p] = v;
-            } else if (j < g.d_after + g.d_before) {
-                const double v = u[j * N] * dwi + tau_i;
-                const int pos = g.pos_w - 1 - (j - g.d_after);
-#pragma unroll
-                for (int p = 1; p <= NM; ++p)
-                    if (p == pos) t[p] = v;
-            }
-        }
-    }
-
-    using S = ScalarState<KM, SM, TM>;
-    S st;
-#pragma unroll
-    for (int c = 0; c < S::NCH; ++c)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) st.P[c][e] = 0.0;
-    // ---- propagator values i*P_s(t_L - t_{L-1}), L = 2..n (ordering clamp) ----
-#pragma unroll
-    for (int L = 2; L <= NM; ++L) {
-        double dt = t[L] - t[L - 1];
-        dt = dt > 0.0 ? dt : 0.0;
-        if (g.kind == QIW_G_BARE) {
-#pragma unroll
-            for (int s = 0; s < SM; ++s)
-                st.P[(L - 2) / S::LPC][((L - 2) % S::LPC) * SM + s] =
-                    (s < ns && L <= n) ? exp(-dt * a.e_shift[s < ns ? s : 0]) : 0.0;
-        } else {
-            const double x = dt * inv_h;
-            int j = (int)floor(x);
-            j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-            const double f = x - (double)j;
-            const double *r0 = a.ptab + (int64_t)j * ns, *r1 = r0 + ns;
-#pragma unroll
-            for (int s = 0; s < SM; ++s)
-                st.P[(L - 2) / S::LPC][((L - 2) % S::LPC) * SM + s] =
-                    (s < ns && L <= n) ? (1.0 - f) * r0[s] + f * r1[s] : 0.0;
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < SM; ++s) st.out[s] = 0.0;
-#pragma unroll
-    for (int q = 0; q < regvec_size(KM * TM); ++q) st.W[q] = 0.0;
-    st.acc = 0.0;
-
-    // ---- walk the program ----
-    st.ps.init(a.prog + g.prog_off, lane);
-    for (;;) {
-        const uint64_t w = st.ps.next();
-        const int op = (int)((w >> 48) & 0xF);
-        if (op == QIW_OP_ROOT) {
-            walk<2, KM, SM, TM>(st, n, __longlong_as_double((long long)st.ps.next()), (int)((w >> 8) & 0xFF),
-                                (int)((w >> 24) & 0xFF));
-        } else if (op == QIW_OP_FLUSH) {
-            const int s0 = (int)((w >> 8) & 0xFF);
-            double pn = 0.0;  // iP_n[s0]
-#pragma unroll
-            for (int c = 0; c < S::NCH; ++c)
-                pn = (c == (n - 2) / S::LPC) ? st.P[c][((n - 2) % S::LPC) * SM + s0] : pn;
-#pragma unroll
-            for (int s = 0; s < SM; ++s) st.out[s] += (s == s0) ? st.acc * pn : 0.0;
-            st.acc = 0.0;
-        } else if (op == QIW_OP_TOPO) {
-            // arc values of one topology: arc ai has head at bits 8ai..8ai+3, tail at 8ai+4..
-#pragma unroll
-            for (int ai = 0; ai < KM; ++ai) {
-                const int head = (int)((w >> (8 * ai)) & 0xF), tail = (int)((w >> (8 * ai + 4)) & 0xF);
-                double th = 0.0, tt = 0.0;
-#pragma unroll
-                for (int p = 1; p <= NM; ++p) {
-                    th = (p == head) ? t[p] : th;
-                    tt = (p == tail) ? t[p] : tt;
-                }
-                double z = tt - th;
-                z = z > 0.0 ? z : 0.0;
-                const double x = z * inv_h;
-                int j = (int)floor(x);
-                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-                const double u = x - (double)j;
-#pragma unroll
-                for (int tb = 0; tb < TM; ++tb) {
-                    if (ai < k && tb < ntab) {
-                        const double *c = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
-                        st.W[ai * TM + tb] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
-                    }
-                }
-            }
-        } else if (op == QIW_OP_END) {
-            break;
-        }
-    }
-
-    // ---- wavefront reduction of the per-s0 outputs ----
-    if (a.per_sample && active) {
-        double m = 0.0;
-#pragma unroll
-        for (int s = 0; s < SM; ++s) m += (s < ns) ? fabs(st.out[s]) : 0.0;
-        a.per_sample[g.ps_off + sample] = m;
-    }
-#pragma unroll
-    for (int s = 0; s < SM; ++s) {
-        if (s < ns) {
-            double v = active ? st.out[s] * wt : 0.0;
-#pragma unroll
-            for (int off = QIW_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, QIW_WAVE);
-            if (lane == 0) a.partial[(int64_t)task.out_index * ns + s] = v;
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------------------
 // The flat walk of scalar programs (all sectors 1x1).
 //
@@ -410,13 +216,13 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_scalar_t(EvalArgs a) {
 // and every child but the first restores it (POP).  The propagator values of the inner
 // segments (levels 3..NM-1, read by every edge), the arc values and the outputs of the lane's
 // sample stay in registers, read with uniform indices through the VGPR index mode.  LDS
-// holds what is read once per topology or per restored branch: the contour times and the
-// saved products.
+// holds what is read once per topology, per restored branch or per initial sector: the
+// sampled contour times, the saved products and the outputs.
 // ---------------------------------------------------------------------------------------
 
 // 32-bit program words in 64-word blocks: lane l loads word base+l of the block with one
-// coalesced load and each word is broadcast with v_readlane; the next block is loaded when
-// the current one is used up.
+// coalesced load and each word is broadcast with v_readlane.  The last word of every block
+// is NEXT (the host compiler puts it there), so reading a word needs no bounds test.
 struct ProgStream32 {
     const uint32_t *__restrict__ p;
     uint32_t blk;
@@ -427,15 +233,11 @@ struct ProgStream32 {
         blk = p0[l];
         idx = 0;
     }
-    __device__ __forceinline__ uint32_t next() {
-        if (idx == QIW_WAVE) {
-            p += QIW_WAVE;
-            blk = p[lane];
-            idx = 0;
-        }
-        const uint32_t w = __builtin_amdgcn_readlane(blk, idx);
-        ++idx;
-        return w;
+    __device__ __forceinline__ uint32_t next() { return __builtin_amdgcn_readlane(blk, idx++); }
+    __device__ __forceinline__ void next_block() {
+        p += QIW_WAVE;
+        blk = p[lane];
+        idx = 0;
     }
 };
 
@@ -478,7 +280,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     using F = FlatShape<KM, SM>;
     constexpr int NM = F::NM, NC = F::NC;
     constexpr int NW = regvec_size(KM * TM);
-    __shared__ double tl[NM * QIW_WAVE];         // contour time of position p: tl[(p - 1) * 64 + lane]
+    __shared__ double tl[(NM - 2) * QIW_WAVE];   // time of sampled position p: tl[(p - 2) * 64 + lane]
     __shared__ double sk[(NM - 2) * QIW_WAVE];   // saved product of level L: sk[(L - 2) * 64 + lane]
     __shared__ double so[SM * QIW_WAVE];         // output of initial sector s: so[s * 64 + lane]
     const int lane = threadIdx.x;
@@ -531,7 +333,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
         }
     }
 #pragma unroll
-    for (int p = 1; p <= NM; ++p) tl[(p - 1) * QIW_WAVE + lane] = t[p];
+    for (int p = 2; p <= NM - 1; ++p) tl[(p - 2) * QIW_WAVE + lane] = t[p];
 
     // ---- propagator values i*P_s(t_L - t_{L-1}), L = 2..n (ordering clamp) ----
     typename RegVec<double, 16>::type P0 = 0.0, P1 = 0.0, P2 = 0.0, P3 = 0.0, P4 = 0.0, P5 = 0.0;
@@ -592,15 +394,18 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
                 }
             } else {
                 double pv;
-                QIW_PIN_GET(NC, (int)(w & 0xFF), pv);
+                QIW_PIN_GET(NC, (int)((w & 0xFF) >> 1), pv);
                 cur *= pv;
                 if (w & QIW_FL_PUSH) sk[(L - 1) * QIW_WAVE + lane] = cur;
             }
             w = ps.next();
         }
         const uint32_t op = w >> 28;
-        if (op == QIW_FL_OP_END) break;
-        if (op == QIW_FL_OP_ROOT) {  // position 1: cur = iP_2[s], or the leaf of n = 2
+        if (op == QIW_FL_OP_NEXT) {
+            ps.next_block();
+        } else if (op == QIW_FL_OP_END) {
+            break;
+        } else if (op == QIW_FL_OP_ROOT) {  // position 1: cur = iP_2[s], or the leaf of n = 2
             if (w & QIW_FL_LEAF) {
                 acc += (w & QIW_FL_UNIT) ? ((w & QIW_FL_NEG) ? -1.0 : 1.0) : coefs[w & 0xFF];
             } else {
@@ -611,12 +416,13 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
             const uint32_t *tw = topo + 2 * (w & 0xFFFFFF);
             const uint32_t lo = tw[0], hi = tw[1];
             // every slot is rewritten, so the old values die here: arcs beyond the order
-            // read position 1, tables beyond the model's count read its last table
+            // read position 2, tables beyond the model's count read its last table (arc
+            // ends are sampled positions 2..n-1)
 #pragma unroll
             for (int ai = 0; ai < KM; ++ai) {
-                const uint32_t b = ai >= k ? 0x11u : ai < 4 ? (lo >> (8 * ai)) : (hi >> (8 * (ai - 4)));
+                const uint32_t b = ai >= k ? 0x22u : ai < 4 ? (lo >> (8 * ai)) : (hi >> (8 * (ai - 4)));
                 const int head = (int)(b & 15), tail = (int)((b >> 4) & 15);
-                double z = tl[(tail - 1) * QIW_WAVE + lane] - tl[(head - 1) * QIW_WAVE + lane];
+                double z = tl[(tail - 2) * QIW_WAVE + lane] - tl[(head - 2) * QIW_WAVE + lane];
                 z = z > 0.0 ? z : 0.0;
                 const double x = z * inv_h;
                 int j = (int)floor(x);
@@ -1607,18 +1413,6 @@ hipError_t qiw_launch_eval_flat(hipStream_t st, const EvalArgs &a, int n_tasks, 
         hipLaunchKernelGGL((k_eval_flat_t<4, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
     else
         hipLaunchKernelGGL((k_eval_flat_t<QIW_KMAX, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
-    return hipGetLastError();
-}
-
-hipError_t qiw_launch_eval_scalar(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm) {
-    if (km <= 4 && sm <= 4 && tm <= 2)
-        hipLaunchKernelGGL((k_eval_scalar_t<4, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
-    else if (km <= 4 && sm <= 8 && tm <= 4)
-        hipLaunchKernelGGL((k_eval_scalar_t<4, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
-    else if (km <= QIW_KMAX && sm <= 8 && tm <= 4)
-        hipLaunchKernelGGL((k_eval_scalar_t<QIW_KMAX, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
-    else
-        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

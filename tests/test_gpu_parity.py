@@ -179,3 +179,31 @@ def test_table_count_other_than_stride(ctx, n_tables):
     for o in range(1, 5):
         assert np.abs(per_cpu[o]).max() > 0
         assert np.abs(per_gpu[o] - per_cpu[o]).max() < RTOL * np.abs(v_cpu).max()
+
+
+def test_plans_reused_and_rebuilt(ctx):
+    """Compiled batches stay on the device between calls while the model structure, the
+    Sobol tables and the run parameters are unchanged, and are rebuilt when any changes:
+    interleaved calls on two models and two sample counts all match the oracle."""
+    models = {nt: (anderson_tables(nt), anderson_tables(nt)) for nt in (1, 3)}
+    for eg, ec in models.values():
+        P, _, _ = O.inchworm(ec, range(0, 4), range(0, 4), 2 ** 6, n_steps_max=6)
+        for e in (eg, ec):
+            e.P, e.P_n_known = P, e.grid.n_tau
+    ref = {(nt, N): O.inchworm_step(models[nt][1], range(0, 5), N, 5)[0] for nt in (1, 3) for N in (2 ** 7, 2 ** 8)}
+    first = {}
+    for nt, N in [(1, 2 ** 8), (3, 2 ** 8), (1, 2 ** 8), (1, 2 ** 7), (3, 2 ** 8), (1, 2 ** 8)]:
+        v, _ = qiw.inchworm_step(models[nt][0], 5, range(0, 5), N)
+        assert rel(v, ref[(nt, N)]) < RTOL
+        if (nt, N) in first:  # a reused plan reproduces the value bit for bit
+            np.testing.assert_array_equal(v, first[(nt, N)])
+        first.setdefault((nt, N), v)
+
+
+def test_correlator_repeated_call(ctx):
+    exp = model.anderson_single_orbital(n_tau=16, with_correlators=True)
+    qiw.inchworm(exp, exp.grid, range(0, 3), range(0, 3), 2 ** 7)
+    g1 = qiw.correlator_2p(exp, exp.grid, range(0, 3), 2 ** 7)
+    g2 = qiw.correlator_2p(exp, exp.grid, range(0, 3), 2 ** 7)
+    for a, b in zip(g1, g2):
+        np.testing.assert_array_equal(a, b)
