@@ -101,12 +101,15 @@ struct QiwGroup {
     int64_t ps_off;        // offset of the group's samples in EvalArgs::per_sample
 };
 
-// A unit of work of one launch: 64 samples of one group at one time configuration.
+// A unit of work of one launch: up to 64 samples of one group at one time configuration,
+// over one chunk of the group's topologies (flat scalar programs; the whole group otherwise).
 struct QiwTask {
     int32_t group;
     int32_t first_sample;
     int32_t tau_index;     // bold: n_w (tau_w = grid[n_w]); corr: index of tau_A; bare: 0
     int32_t out_index;     // row of the per-task partial-sum output
+    int32_t chunk;         // offset of the chunk's program from the group's, in 64-bit words
+    int32_t pad_;
 };
 
 // Interpolation tables of the hybridization functions, all stored as i*Delta (real).

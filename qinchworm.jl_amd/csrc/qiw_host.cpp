@@ -204,6 +204,7 @@ struct Compiled {
     std::vector<uint64_t> prog;
     std::vector<uint32_t> topo;   // per topology: head/tail nibbles of arcs 0-3, of arcs 4-5
     std::vector<double> coefs;    // scalar programs: leaf coefficients other than +-1
+    std::vector<int32_t> chunks{0};  // program offsets (64-bit words) of the topology chunks
     int64_t leaves_per_sample = 0;
     int64_t edges_per_sample = 0;
     int n_top = 0;
@@ -224,6 +225,8 @@ struct Compiler {
     int pstride;  // scalar programs: inner P entry of level L, sector s: (nm - 1 - L) * pstride + s
     int nm;       // scalar programs: positions of the largest order of the instantiation
     std::vector<uint32_t> p32;  // scalar program words
+    int chunk_words = 0;        // scalar programs: start a new chunk at a topology once the
+                                // current one holds this many words (0: one chunk)
 
     // appends a scalar program word; the last word of every 64-word block is NEXT
     void put32(uint32_t w) {
@@ -422,6 +425,13 @@ struct Compiler {
             }
             out.topo.push_back(tw[0]);
             out.topo.push_back(tw[1]);
+            if (fmt == FMT_SCALAR && chunk_words > 0 && ti > 0 &&
+                (int)p32.size() - 2 * out.chunks.back() >= chunk_words) {
+                // close the chunk: END, then END up to the next 64-word block
+                put32((uint32_t)QIW_FL_OP_END << 28);
+                while (p32.size() % QIW_WAVE) p32.push_back((uint32_t)QIW_FL_OP_END << 28);
+                out.chunks.push_back((int32_t)(p32.size() / 2));
+            }
             if (order > 0 && fmt == FMT_SCALAR) put32(((uint32_t)QIW_FL_OP_TOPO << 28) | (uint32_t)ti);
             if (order > 0 && fmt == FMT_BLOCK) out.prog.push_back(topo_word);
             for (s0 = 0; s0 < m.ns; ++s0) {
@@ -664,7 +674,12 @@ std::vector<int> orders_all(const qiw_run *run) {
 }
 
 // Build the groups of a bold step (src/inchworm.jl:422-452) or the bare step (:373-392).
-int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vector<int> &oall, Batch &b) {
+// Words per topology chunk of flat scalar programs: long groups are cut into several tasks
+// per sample block so that no single task sets the length of a launch.
+constexpr int kChunkWords = 1024;
+
+int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vector<int> &oall, Batch &b,
+                     bool split = true) {
     const HostModel &m = c->m;
     int n_o = bare ? run->n_orders_bare : run->n_orders;
     const int32_t *ord = bare ? run->orders_bare : run->orders;
@@ -691,6 +706,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
                 if (run->n_pts_after_max <= 0 || na <= run->n_pts_after_max) nas.push_back(na);
         for (int na : nas) {
             Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1, b.fmt, b.max_order);
+            if (split) comp.chunk_words = kChunkWords;
             Compiled cg;
             if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
             if (cg.n_top == 0) continue;
@@ -703,8 +719,8 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             b.g_first.push_back((int32_t)b.tasks.size());
             int nt = (int)((N + b.spt - 1) / b.spt);
             for (int t = 0; t < nt; ++t)
-                b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size()});
-            b.g_count.push_back(nt);
+                for (int32_t ch : cg.chunks) b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size(), ch, 0});
+            b.g_count.push_back(nt * (int)cg.chunks.size());
             b.g_slot.push_back(slot);
             b.flops_per_launch += (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
             b.leaves_per_launch += N * cg.leaves_per_sample;
@@ -1446,6 +1462,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
                 int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
                 for (int na = lo; na <= hi; ++na) {
                     Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
+                    comp.chunk_words = kChunkWords;
                     Compiled cg;
                     if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
                     if (cg.n_top == 0) continue;
@@ -1464,9 +1481,11 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
                         if (j == 0 && !(gi == 0 && p.first_is_order0)) continue;
                         int64_t N = b.groups[gi].g.order > 0 ? run->N_samples : 1;
                         int nt = (int)((N + b.spt - 1) / b.spt);
+                        const std::vector<int32_t> &chs = b.groups[gi].chunks;
                         rf[(size_t)j * ng + gi] = (int32_t)b.tasks.size();
-                        rcnt[(size_t)j * ng + gi] = nt;
-                        for (int t = 0; t < nt; ++t) b.tasks.push_back({gi, t * b.spt, j, (int32_t)b.tasks.size()});
+                        rcnt[(size_t)j * ng + gi] = nt * (int)chs.size();
+                        for (int t = 0; t < nt; ++t)
+                            for (int32_t ch : chs) b.tasks.push_back({gi, t * b.spt, j, (int32_t)b.tasks.size(), ch, 0});
                         b.flops_per_launch +=
                             (double)N * (2.0 * b.groups[gi].edges_per_sample + 2.0 * b.groups[gi].leaves_per_sample);
                         b.leaves_per_launch += N * b.groups[gi].leaves_per_sample;
@@ -1625,7 +1644,7 @@ int warp_collect(qiw_ctx *c, const qiw_run *run, const std::vector<int> &steps, 
     std::vector<int> oall = orders_all(&r);
     Batch b;
     DevBatch d;
-    if ((rc = build_step_batch(c, &r, false, oall, b))) return rc;
+    if ((rc = build_step_batch(c, &r, false, oall, b, false))) return rc;  // per-sample sums: one chunk
     if (b.fmt == FMT_BLOCK) return fail(c, QIW_E_LIMIT, "the learned warp needs sector blocks <= 6 and orders <= 4");
     if (b.groups.empty()) return fail(c, QIW_E_ARG, "no sampled bold groups to train on");
     if ((rc = upload(c, "wtrain", b, n, d))) return rc;

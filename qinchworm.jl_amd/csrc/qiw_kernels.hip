@@ -371,7 +371,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     const double *coefs = a.coefs + g.coef_off;
     const uint32_t *topo = a.topo + g.topo_off;
     ProgStream32 ps;
-    ps.init(reinterpret_cast<const uint32_t *>(a.prog + g.prog_off), lane);
+    ps.init(reinterpret_cast<const uint32_t *>(a.prog + g.prog_off + task.chunk), lane);
     uint32_t w = ps.next();
     for (;;) {
         // A run of EDGE words: the inner loop reads the arc values and the propagators but
