@@ -1274,18 +1274,30 @@ __device__ double simplex_volume(int d, double edge) {  // _simplex_volume, src/
 
 // Per-group sums of the partial rows: block (group, 64-element chunk); the 4 wavefronts
 // take every 4th row and are combined in wave order, so the sum order is fixed.
+// Sum over the task rows of each group, one block per group (and per 256 elements when
+// bt > 256).  The rows of a group are contiguous, so the block reads them as one
+// contiguous range: thread t covers element t % bt of rows t / bt, t / bt + R, ... with R
+// = 256 / bt rows per pass, and the R partial sums of each element are then added in a
+// fixed order, so the result does not depend on scheduling.
 __global__ __launch_bounds__(256) void k_reduce_rows(const int32_t *__restrict__ first, const int32_t *__restrict__ count,
                                                      const double *__restrict__ partial, int bt,
                                                      double *__restrict__ gsum) {
-    __shared__ double red[4][64];
-    const int g = blockIdx.x, e = blockIdx.y * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+    __shared__ double red[256];
+    const int g = blockIdx.x, t = threadIdx.x;
     const int f0 = first[g], nr = count[g];
+    const int R = bt <= 256 ? 256 / bt : 1;     // rows per pass
+    const int e = bt <= 256 ? t % bt : blockIdx.y * 256 + t;
+    const int r0 = bt <= 256 ? t / bt : 0;
     double s = 0.0;
-    if (e < bt)
-        for (int r = r0; r < nr; r += 4) s += partial[(int64_t)(f0 + r) * bt + e];
-    red[r0][threadIdx.x & 63] = s;
+    if (r0 < R && e < bt)
+        for (int r = r0; r < nr; r += R) s += partial[(int64_t)(f0 + r) * bt + e];
+    red[t] = s;
     __syncthreads();
-    if (r0 == 0 && e < bt) gsum[(int64_t)g * bt + e] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+    if (t < (bt <= 256 ? bt : 256) && e < bt) {
+        double v = 0.0;
+        for (int q = 0; q < R; ++q) v += red[q * (bt <= 256 ? bt : 0) + t];
+        gsum[(int64_t)g * bt + e] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_finalize_step(FinalizeArgs a) {
@@ -1338,9 +1350,12 @@ __global__ __launch_bounds__(256) void k_finalize_step(FinalizeArgs a) {
 // prefactor-weighted traces.  out[c][j] = sum_g (-1)^k J_g / N_g * R_g (real; the value
 // is -i * out / Z).
 
+// One block per tau row: for each group the block sums the diagonal elements of the
+// group's task rows (items spread over the threads, then a fixed-order tree), scaled by
+// the group's sign, simplex volume and 1/N.
 __global__ __launch_bounds__(256) void k_finalize_corr(CorrFinalizeArgs a) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= a.n_rows) return;
+    __shared__ double red[256];
+    const int row = blockIdx.x, t = threadIdx.x;
     const int j = row % a.n_tau;
     const double tau_i = 0.0, tau_w = j * a.h, tau_f = a.beta;
     double tot = 0.0;
@@ -1348,18 +1363,30 @@ __global__ __launch_bounds__(256) void k_finalize_corr(CorrFinalizeArgs a) {
         const int nt = a.row_task_count[row * a.n_groups + gi];
         if (nt == 0) continue;
         const int f0 = a.row_task_first[row * a.n_groups + gi];
-        const QiwGroup g = a.groups[gi];
+        const int64_t items = (int64_t)nt * a.n_diag;
         double s = 0.0;
-        for (int r = 0; r < nt; ++r)
-            for (int e = 0; e < a.n_diag; ++e) s += a.partial[(int64_t)(f0 + r) * a.bt + a.diag_idx[e]];
-        double fac = 1.0;
-        if (g.sampled) {
-            double J = simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
-            fac = ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
+        for (int64_t i = t; i < items; i += 256) {
+            const int r = (int)(i / a.n_diag), e = (int)(i % a.n_diag);
+            s += a.partial[(int64_t)(f0 + r) * a.bt + a.diag_idx[e]];
         }
-        tot += fac * s;
+        red[t] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) red[t] += red[t + w];
+            __syncthreads();
+        }
+        if (t == 0) {
+            const QiwGroup g = a.groups[gi];
+            double fac = 1.0;
+            if (g.sampled) {
+                double J = simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
+                fac = ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
+            }
+            tot += fac * red[0];
+        }
+        __syncthreads();
     }
-    a.out[row] = tot;
+    if (t == 0) a.out[row] = tot;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1459,14 +1486,13 @@ hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks
 }
 
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a) {
-    hipLaunchKernelGGL(k_reduce_rows, dim3(a.n_groups, (a.bt + 63) / 64), dim3(256), 0, st, a.group_task_first,
+    hipLaunchKernelGGL(k_reduce_rows, dim3(a.n_groups, a.bt <= 256 ? 1 : (a.bt + 255) / 256), dim3(256), 0, st, a.group_task_first,
                        a.group_task_count, a.partial, a.bt, a.gsum);
     hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a) {
-    int blocks = (a.n_rows + 255) / 256;
-    hipLaunchKernelGGL(k_finalize_corr, dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_finalize_corr, dim3(a.n_rows), dim3(256), 0, st, a);
     return hipGetLastError();
 }
