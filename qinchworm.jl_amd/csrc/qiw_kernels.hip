@@ -264,10 +264,12 @@ struct FlatShape {
         else if ((NC_) > 4 && c_ == 4) P4[i_] = (x);        \
         else if ((NC_) > 5) P5[i_] = (x);                   \
     } while (0)
-#define QIW_PIN_GET(NC_, e, out)                            \
+// Entry of the EDGE word w (bits 0-7: twice the entry): vector (w >> 5) & 7, element
+// (w >> 1) & 15.
+#define QIW_PIN_GET(NC_, w, out)                            \
     do {                                                    \
-        const int c_ = (e) >> 4, i_ = (e)&15;               \
-        if ((NC_) == 1 || c_ == 0) out = P0[i_];            \
+        const int c_ = ((w) >> 5) & 7, i_ = ((w) >> 1) & 15; \
+        if ((NC_) == 1 || ((NC_) == 2 ? ((w) & 32) == 0 : c_ == 0)) out = P0[i_]; \
         else if ((NC_) == 2 || c_ == 1) out = P1[i_];       \
         else if ((NC_) == 3 || c_ == 2) out = P2[i_];       \
         else if ((NC_) == 4 || c_ == 3) out = P3[i_];       \
@@ -374,11 +376,23 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     ps.init(reinterpret_cast<const uint32_t *>(a.prog + g.prog_off + task.chunk), lane);
     uint32_t w = ps.next();
     for (;;) {
-        // A run of EDGE words: the inner loop reads the arc values and the propagators but
-        // never writes them, so they stay in place across its iterations.
-        while (w < QIW_FL_OP) {
-            // The empty volatile asm statements keep the rare cases as uniform branches: the
-            // compiler would otherwise evaluate them on every word and select the result.
+        // Runs of chain words (EDGE without POP, LEAF or PUSH) take the short inner loop; it
+        // reads the arc values and the propagators but never writes them, so they stay in
+        // place across its iterations.  The empty volatile asm statements keep the rare
+        // cases as uniform branches: the compiler would otherwise evaluate them on every
+        // word and select the result.
+        while ((w & (0xF0000000u | QIW_FL_POP | QIW_FL_LEAF | QIW_FL_PUSH)) == 0) {
+            const uint32_t wn = ps.next();  // the next word, read ahead (the block ends in NEXT)
+            if (w & QIW_FL_TAIL) {
+                asm volatile("");
+                cur *= W[(int)((w >> 8) & (NW - 1))];
+            }
+            double pv;
+            QIW_PIN_GET(NC, w, pv);
+            cur *= pv;
+            w = wn;
+        }
+        if (w < QIW_FL_OP) {  // EDGE with POP, LEAF or PUSH
             const int L = (int)((w >> 16) & 15);
             if (w & QIW_FL_POP) cur = sk[(L - 2) * QIW_WAVE + lane];
             if (w & QIW_FL_TAIL) {
@@ -394,11 +408,12 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
                 }
             } else {
                 double pv;
-                QIW_PIN_GET(NC, (int)((w & 0xFF) >> 1), pv);
+                QIW_PIN_GET(NC, w, pv);
                 cur *= pv;
                 if (w & QIW_FL_PUSH) sk[(L - 1) * QIW_WAVE + lane] = cur;
             }
             w = ps.next();
+            continue;
         }
         const uint32_t op = w >> 28;
         if (op == QIW_FL_OP_NEXT) {
