@@ -53,13 +53,15 @@ static inline int qiw_vec_tm(int tm) { return tm <= 2 ? 2 : 4; }
 //        bits 16-19 : level L
 //        bit  20 TAIL : multiply the arc value
 //        bit  21 LEAF : the only child is the last position: acc += coefficient * product
-//        bit  22 PUSH : after the propagator, save the product in slot L + 1
-//        bit  23 POP  : first restore the product from slot L
+//        bit  22 PUSH : after the propagator, save the product for level L + 1
+//        bit  23 POP  : first restore the product saved for level L (level 2: the root's)
 //        bit  24 UNIT : coefficient +1, or -1 with bit 25 NEG
 //        bits 28-31 : op, 0 for EDGE
-// ROOT   op 4, position 1: product = iP_2[s], s in bits 0-7 (PUSH: save it in slot 2); with
-//        LEAF (two positions only): acc += coefficient (bits 0-7, UNIT, NEG as for EDGE)
+// ROOT   op 4, position 1: product = iP_2[s], s in bits 0-7 (kept for the level-2 children);
+//        with LEAF (two positions only): acc += coefficient (bits 0-7, UNIT, NEG as for EDGE)
 // TOPO   op 1, bits 0-23 : topology index into the group's table of arc head/tail nibbles
+//        (each arc end as its simplex coordinate j: position n - 1 - j for j < d_after,
+//        pos_w - 1 - (j - d_after) for the coordinates before tau_w)
 // FLUSH  op 2, bits 0-7 : s0 (out[s0] += acc * iP_n[s0]; acc = 0)
 // END    op 3
 // NEXT   op 5, the last word of every 64-word block: continue with the next block

@@ -421,7 +421,10 @@ struct Compiler {
                 conf_kind[tail] = N_PAIR_TAIL;
                 conf_arc[head] = conf_arc[tail] = a;
                 topo_word |= ((uint64_t)head << (8 * a)) | ((uint64_t)tail << (8 * a + 4));
-                tw[a / 4] |= (uint32_t)(head | (tail << 4)) << (8 * (a % 4));
+                // flat scalar programs name arc ends by simplex coordinate: j = n - 1 - p
+                // after tau_w, d_after + pos_w - 1 - p before it
+                auto coord = [&](int p) { return p > pos_w ? n - 1 - p : d_after + pos_w - 1 - p; };
+                tw[a / 4] |= (uint32_t)(coord(head) | (coord(tail) << 4)) << (8 * (a % 4));
             }
             out.topo.push_back(tw[0]);
             out.topo.push_back(tw[1]);
@@ -676,7 +679,7 @@ std::vector<int> orders_all(const qiw_run *run) {
 // Build the groups of a bold step (src/inchworm.jl:422-452) or the bare step (:373-392).
 // Words per topology chunk of flat scalar programs: long groups are cut into several tasks
 // per sample block so that no single task sets the length of a launch.
-constexpr int kChunkWords = 1024;
+constexpr int kChunkWords = 512;
 
 int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vector<int> &oall, Batch &b,
                      bool split = true) {

@@ -282,8 +282,8 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     using F = FlatShape<KM, SM>;
     constexpr int NM = F::NM, NC = F::NC;
     constexpr int NW = regvec_size(KM * TM);
-    __shared__ double tl[(NM - 2) * QIW_WAVE];   // time of sampled position p: tl[(p - 2) * 64 + lane]
-    __shared__ double sk[(NM - 2) * QIW_WAVE];   // saved product of level L: sk[(L - 2) * 64 + lane]
+    __shared__ double tl[2 * KM * QIW_WAVE];     // time of simplex coordinate j: tl[j * 64 + lane]
+    __shared__ double sk[(NM - 3) * QIW_WAVE];   // saved product of level L >= 3: sk[(L - 3) * 64 + lane]
     __shared__ double so[SM * QIW_WAVE];         // output of initial sector s: so[s * 64 + lane]
     const int lane = threadIdx.x;
     const QiwTask task = a.tasks[blockIdx.x];
@@ -322,11 +322,13 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
         for (int j = 0; j < 2 * KM; ++j) {
             if (j < g.d_after) {
                 const double v = u[j * N] * dfw + tau_w;
+                tl[j * QIW_WAVE + lane] = v;
 #pragma unroll
                 for (int p = 1; p <= NM; ++p)
                     if (p == n - 1 - j) t[p] = v;
             } else if (j < g.d_after + g.d_before) {
                 const double v = u[j * N] * dwi + tau_i;
+                tl[j * QIW_WAVE + lane] = v;
                 const int pos = g.pos_w - 1 - (j - g.d_after);
 #pragma unroll
                 for (int p = 1; p <= NM; ++p)
@@ -334,8 +336,6 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
             }
         }
     }
-#pragma unroll
-    for (int p = 2; p <= NM - 1; ++p) tl[(p - 2) * QIW_WAVE + lane] = t[p];
 
     // ---- propagator values i*P_s(t_L - t_{L-1}), L = 2..n (ordering clamp) ----
     typename RegVec<double, 16>::type P0 = 0.0, P1 = 0.0, P2 = 0.0, P3 = 0.0, P4 = 0.0, P5 = 0.0;
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     for (int q = 0; q < NW; ++q) W[q] = 0.0;
 #pragma unroll
     for (int s = 0; s < SM; ++s) so[s * QIW_WAVE + lane] = 0.0;
-    double acc = 0.0, cur = 0.0;
+    double acc = 0.0, cur = 0.0, cur0 = 0.0;  // cur0: product of the root (level 2 restores it)
 
     // ---- walk the program: one word per iteration, until END ----
     const double *coefs = a.coefs + g.coef_off;
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
         }
         if (w < QIW_FL_OP) {  // EDGE with POP, LEAF or PUSH
             const int L = (int)((w >> 16) & 15);
-            if (w & QIW_FL_POP) cur = sk[(L - 2) * QIW_WAVE + lane];
+            if (w & QIW_FL_POP) cur = (L == 2) ? cur0 : sk[(L - 3) * QIW_WAVE + lane];
             if (w & QIW_FL_TAIL) {
                 asm volatile("");
                 cur *= W[(int)((w >> 8) & (NW - 1))];
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
                 double pv;
                 QIW_PIN_GET(NC, w, pv);
                 cur *= pv;
-                if (w & QIW_FL_PUSH) sk[(L - 1) * QIW_WAVE + lane] = cur;
+                if (w & QIW_FL_PUSH) sk[(L - 2) * QIW_WAVE + lane] = cur;
             }
             w = ps.next();
             continue;
@@ -425,19 +425,18 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
                 acc += (w & QIW_FL_UNIT) ? ((w & QIW_FL_NEG) ? -1.0 : 1.0) : coefs[w & 0xFF];
             } else {
                 cur = pick(p_first, (int)(w & 0xFF));
-                if (w & QIW_FL_PUSH) sk[lane] = cur;
+                cur0 = cur;
             }
         } else if (op == QIW_FL_OP_TOPO) {
             const uint32_t *tw = topo + 2 * (w & 0xFFFFFF);
             const uint32_t lo = tw[0], hi = tw[1];
             // every slot is rewritten, so the old values die here: arcs beyond the order
-            // read position 2, tables beyond the model's count read its last table (arc
-            // ends are sampled positions 2..n-1)
+            // read coordinate 0, tables beyond the model's count read its last table
 #pragma unroll
             for (int ai = 0; ai < KM; ++ai) {
-                const uint32_t b = ai >= k ? 0x22u : ai < 4 ? (lo >> (8 * ai)) : (hi >> (8 * (ai - 4)));
+                const uint32_t b = ai >= k ? 0u : ai < 4 ? (lo >> (8 * ai)) : (hi >> (8 * (ai - 4)));
                 const int head = (int)(b & 15), tail = (int)((b >> 4) & 15);
-                double z = tl[(tail - 2) * QIW_WAVE + lane] - tl[(head - 2) * QIW_WAVE + lane];
+                double z = tl[tail * QIW_WAVE + lane] - tl[head * QIW_WAVE + lane];
                 z = z > 0.0 ? z : 0.0;
                 const double x = z * inv_h;
                 int j = (int)floor(x);
