@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.join(REPO, "qinchworm.jl_amd"))
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_eval_scalar_c1_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_eval_flat_c1_hbm_traffic.md")
 
 
 def measured_traffic():
@@ -281,7 +281,7 @@ def main():
         "time_to_solution_s": ms_per_step / 1e3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "k_eval_scalar", "avg_launch_ms": avg_launch_ms,
+                     "kernel": "k_eval_flat", "avg_launch_ms": avg_launch_ms,
                      "launches_per_step": launches / args.steps},
     }
     if not args.no_cpu:

@@ -637,6 +637,35 @@ int check_run(qiw_ctx *c, const qiw_run *run) {
     return 0;
 }
 
+// Workgroup i of a launch runs on XCD i % 8 (round-robin dispatch over the 8 XCDs, each
+// with its own L2).  The chunk tasks of one (group, sample block, tau) read the same
+// coordinates and propagator rows, so they are kept on one XCD: bundles of such tasks go
+// round-robin to the XCDs, and each XCD's list is laid out at positions x, x + 8, ...
+constexpr int kXcds = 8;
+void xcd_interleave(std::vector<QiwTask> &tasks) {
+    std::vector<std::vector<QiwTask>> lists(kXcds);
+    int xcd = -1;
+    for (size_t i = 0; i < tasks.size(); ++i) {
+        const QiwTask &t = tasks[i];
+        const bool same = i > 0 && t.group == tasks[i - 1].group && t.first_sample == tasks[i - 1].first_sample &&
+                          t.tau_index == tasks[i - 1].tau_index;
+        if (!same) xcd = (xcd + 1) % kXcds;
+        lists[xcd].push_back(t);
+    }
+    std::vector<QiwTask> out;
+    out.reserve(tasks.size());
+    std::vector<size_t> pos(kXcds, 0);
+    for (bool more = true; more;) {  // one task of each XCD list per round; short lists drop out
+        more = false;
+        for (int x = 0; x < kXcds; ++x)
+            if (pos[x] < lists[x].size()) {
+                out.push_back(lists[x][pos[x]++]);
+                more = true;
+            }
+    }
+    tasks.swap(out);
+}
+
 // Heaviest groups first: workgroups are dispatched in task order, so the long tasks do
 // not form the tail of the launch.  Each task keeps its output row, so the summation
 // order of the results does not change.  Tasks are bucketed by group (linear time).
@@ -656,6 +685,7 @@ void order_tasks(Batch &b) {
     std::vector<QiwTask> out(b.tasks.size());
     for (const QiwTask &t : b.tasks) out[start[t.group]++] = t;
     b.tasks.swap(out);
+    xcd_interleave(b.tasks);
 }
 
 // Samples per task.  The scalar and vector walks map one sample to a lane (64 per task);
