@@ -227,6 +227,7 @@ struct Compiler {
     std::vector<uint32_t> p32;  // scalar program words
     int chunk_words = 0;        // scalar programs: start a new chunk at a topology once the
                                 // current one holds this many words (0: one chunk)
+    int64_t chunk_cost = 0;     // vector programs: the same, in words x columns replayed
 
     // appends a scalar program word; the last word of every 64-word block is NEXT
     void put32(uint32_t w) {
@@ -408,7 +409,9 @@ struct Compiler {
         selected.assign(std::max(order, 1), -1);
         out.n_top = (int)tops.size();
         // vector format: segments per initial sector, collected first
-        std::vector<std::vector<uint64_t>> seg(fmt == FMT_VEC ? m.ns : 0);
+        // vector format: words of every (topology, initial sector), regrouped into chunks
+        std::vector<std::vector<std::vector<uint64_t>>> vtop(fmt == FMT_VEC ? tops.size() : 0,
+                                                             std::vector<std::vector<uint64_t>>(m.ns));
         for (int ti = 0; ti < (int)tops.size(); ++ti) {
             const Topology &top = tops[ti];
             parity = top.parity;
@@ -471,7 +474,7 @@ struct Compiler {
                                        ((uint64_t)trie[0].children.size() << 24) | (op_offset(root_blk) << 32) |
                                        ((uint64_t)m.bt_off[s] << 52));
                     for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], out);
-                    seg[s0].insert(seg[s0].end(), out.prog.begin(), out.prog.end());
+                    vtop[ti][s0] = out.prog;
                     saved.swap(out.prog);
                     continue;
                 }
@@ -481,13 +484,33 @@ struct Compiler {
                 out.prog.push_back(((uint64_t)QIW_OP_FLUSH << 48) | ((uint64_t)s0 << 8));
             }
         }
-        if (fmt == FMT_VEC) {  // every initial sector gets a segment, so every output is written
-            for (int s = 0; s < m.ns; ++s) {
-                out.prog.push_back(((uint64_t)QIW_OP_SEG << 48) | (uint64_t)m.dims[s] | ((uint64_t)s << 8) |
-                                   ((uint64_t)m.bt_off[s] << 52));
-                out.prog.insert(out.prog.end(), seg[s].begin(), seg[s].end());
-                out.prog.push_back((uint64_t)QIW_OP_FLUSH << 48);
+        if (fmt == FMT_VEC) {
+            // Chunks of topologies, each a program of its own: every initial sector gets a
+            // segment in every chunk, so every output element of a task row is written.
+            // A chunk closes once its words, weighted by the columns each segment is
+            // replayed for, reach chunk_cost.
+            auto put_chunk = [&](size_t t0, size_t t1) {
+                for (int s = 0; s < m.ns; ++s) {
+                    out.prog.push_back(((uint64_t)QIW_OP_SEG << 48) | (uint64_t)m.dims[s] | ((uint64_t)s << 8) |
+                                       ((uint64_t)m.bt_off[s] << 52));
+                    for (size_t t = t0; t < t1; ++t) out.prog.insert(out.prog.end(), vtop[t][s].begin(), vtop[t][s].end());
+                    out.prog.push_back((uint64_t)QIW_OP_FLUSH << 48);
+                }
+            };
+            size_t t0 = 0;
+            int64_t cost = 0;
+            for (size_t t = 0; t < vtop.size(); ++t) {
+                if (chunk_cost > 0 && t > t0 && cost >= chunk_cost) {
+                    put_chunk(t0, t);
+                    out.prog.push_back((uint64_t)QIW_OP_END << 48);
+                    while (out.prog.size() % QIW_WAVE) out.prog.push_back((uint64_t)QIW_OP_END << 48);
+                    out.chunks.push_back((int32_t)out.prog.size());
+                    t0 = t;
+                    cost = 0;
+                }
+                for (int s = 0; s < m.ns; ++s) cost += (int64_t)m.dims[s] * (int64_t)vtop[t][s].size();
             }
+            put_chunk(t0, vtop.size());
         }
         if (fmt == FMT_SCALAR) {  // 32-bit words, two to a 64-bit program element
             put32((uint32_t)QIW_FL_OP_END << 28);
@@ -710,6 +733,9 @@ std::vector<int> orders_all(const qiw_run *run) {
 // Words per topology chunk of flat scalar programs: long groups are cut into several tasks
 // per sample block so that no single task sets the length of a launch.
 constexpr int kChunkWords = 512;
+// Column-words (program words x columns each segment is replayed for) per topology chunk of
+// vector programs, for the vector and matrix-core walks.
+constexpr int64_t kChunkCost = 1 << 18;
 
 int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vector<int> &oall, Batch &b,
                      bool split = true) {
@@ -739,7 +765,10 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
                 if (run->n_pts_after_max <= 0 || na <= run->n_pts_after_max) nas.push_back(na);
         for (int na : nas) {
             Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1, b.fmt, b.max_order);
-            if (split) comp.chunk_words = kChunkWords;
+            if (split) {
+                comp.chunk_words = kChunkWords;
+                comp.chunk_cost = kChunkCost;
+            }
             Compiled cg;
             if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
             if (cg.n_top == 0) continue;
@@ -1496,6 +1525,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
                 for (int na = lo; na <= hi; ++na) {
                     Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
                     comp.chunk_words = kChunkWords;
+                    comp.chunk_cost = kChunkCost;
                     Compiled cg;
                     if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
                     if (cg.n_top == 0) continue;

@@ -993,7 +993,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
                                  : 1.0;
     double absum = 0.0;
     double *prow = a.partial + (int64_t)task.out_index * va.bt;
-    st.ps.init(a.prog + g.prog_off, lane);
+    st.ps.init(a.prog + g.prog_off + task.chunk, lane);
     if (g.kind == QIW_G_BARE)
         vec_program<V, MB, KM, TM, true>(st, a, g, t, inv_h, active, wt, absum, prow);
     else
@@ -1182,7 +1182,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
         // ---- walk the program: one segment per initial sector ----
         double mag = 0.0;
         int m0 = 1, off0 = 0;
-        st.ps.init(a.prog + g.prog_off, lane);
+        st.ps.init(a.prog + g.prog_off + task.chunk, lane);
         for (;;) {
             const uint64_t w = st.ps.next();
             const int op = (int)((w >> 48) & 0xF);
@@ -1286,8 +1286,6 @@ __device__ double simplex_volume(int d, double edge) {  // _simplex_volume, src/
     return p;
 }
 
-// Per-group sums of the partial rows: block (group, 64-element chunk); the 4 wavefronts
-// take every 4th row and are combined in wave order, so the sum order is fixed.
 // Sum over the task rows of each group, one block per group (and per 256 elements when
 // bt > 256).  The rows of a group are contiguous, so the block reads them as one
 // contiguous range: thread t covers element t % bt of rows t / bt, t / bt + R, ... with R
