@@ -1286,29 +1286,45 @@ __device__ double simplex_volume(int d, double edge) {  // _simplex_volume, src/
     return p;
 }
 
+// Compensated (Kahan) summation of the qMC sums: the running sum carries the low-order part
+// lost by each addition.
+struct KahanSum {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x) {
+        const double y = x - c, t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
 // Sum over the task rows of each group, one block per group (and per 256 elements when
 // bt > 256).  The rows of a group are contiguous, so the block reads them as one
 // contiguous range: thread t covers element t % bt of rows t / bt, t / bt + R, ... with R
 // = 256 / bt rows per pass, and the R partial sums of each element are then added in a
-// fixed order, so the result does not depend on scheduling.
+// fixed order, so the result does not depend on scheduling.  Both stages are compensated.
 __global__ __launch_bounds__(256) void k_reduce_rows(const int32_t *__restrict__ first, const int32_t *__restrict__ count,
                                                      const double *__restrict__ partial, int bt,
                                                      double *__restrict__ gsum) {
-    __shared__ double red[256];
+    __shared__ double red[256], cmp[256];
     const int g = blockIdx.x, t = threadIdx.x;
     const int f0 = first[g], nr = count[g];
     const int R = bt <= 256 ? 256 / bt : 1;     // rows per pass
     const int e = bt <= 256 ? t % bt : blockIdx.y * 256 + t;
     const int r0 = bt <= 256 ? t / bt : 0;
-    double s = 0.0;
+    KahanSum k;
     if (r0 < R && e < bt)
-        for (int r = r0; r < nr; r += R) s += partial[(int64_t)(f0 + r) * bt + e];
-    red[t] = s;
+        for (int r = r0; r < nr; r += R) k.add(partial[(int64_t)(f0 + r) * bt + e]);
+    red[t] = k.s;
+    cmp[t] = k.c;
     __syncthreads();
     if (t < (bt <= 256 ? bt : 256) && e < bt) {
-        double v = 0.0;
-        for (int q = 0; q < R; ++q) v += red[q * (bt <= 256 ? bt : 0) + t];
-        gsum[(int64_t)g * bt + e] = v;
+        KahanSum v;
+        for (int q = 0; q < R; ++q) {
+            const int i = q * (bt <= 256 ? bt : 0) + t;
+            v.add(red[i]);
+            v.add(-cmp[i]);
+        }
+        gsum[(int64_t)g * bt + e] = v.s;
     }
 }
 
@@ -1376,12 +1392,12 @@ __global__ __launch_bounds__(256) void k_finalize_corr(CorrFinalizeArgs a) {
         if (nt == 0) continue;
         const int f0 = a.row_task_first[row * a.n_groups + gi];
         const int64_t items = (int64_t)nt * a.n_diag;
-        double s = 0.0;
+        KahanSum s;
         for (int64_t i = t; i < items; i += 256) {
             const int r = (int)(i / a.n_diag), e = (int)(i % a.n_diag);
-            s += a.partial[(int64_t)(f0 + r) * a.bt + a.diag_idx[e]];
+            s.add(a.partial[(int64_t)(f0 + r) * a.bt + a.diag_idx[e]]);
         }
-        red[t] = s;
+        red[t] = s.s - s.c;
         __syncthreads();
         for (int w = 128; w > 0; w >>= 1) {
             if (t < w) red[t] += red[t + w];

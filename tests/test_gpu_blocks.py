@@ -77,6 +77,28 @@ def test_block_fp32_chain_products():
     assert rel(kg.P, P_c) > 1e-13  # really float32 (differs from the float64 path)
 
 
+def test_block_fp32_rho_and_g():
+    """QIW_PREC_FP32 errors against the float64 oracle for the density matrix rho (from the
+    normalised P(beta)) and the Green's function G(tau) = -<c(tau) c^dag> on the two-orbital
+    model; bound stated here: 2e-6 relative and 2e-6 max-abs for both."""
+    kg, kc = kanamori2(n_tau=10), kanamori2(n_tau=10)
+    f = kg.ed.fock
+    for e in (kg, kc):
+        e.add_corr_operators((-f.c((0, "up")), f.c_dag((0, "up"))))
+    P_c, _, _ = O.inchworm(kc, range(0, 4), range(0, 4), 2 ** 6)
+    kc.P, kc.P_n_known = P_c, kc.grid.n_tau
+    qiw.inchworm(kg, kg.grid, range(0, 4), range(0, 4), 2 ** 6, precision=qiw.abi.QIW_PREC_FP32)
+    rho_g, rho_c = qiw.density_matrix(kg), qiw.density_matrix(kc)
+    rg, rc = np.concatenate([b.ravel() for b in rho_g]), np.concatenate([b.ravel() for b in rho_c])
+    err_rho, abs_rho = rel(rg, rc), np.abs(rg - rc).max()
+    g_g = qiw.correlator_2p(kg, kg.grid, range(0, 4), 2 ** 6, precision=qiw.abi.QIW_PREC_FP32)[0]
+    g_c = O.correlator_2p(kc, range(0, 4), 2 ** 6)[0]
+    err_g, abs_g = rel(g_g, g_c), np.abs(g_g - g_c).max()
+    print(f"fp32 chain products: rho rel {err_rho:.2e} max-abs {abs_rho:.2e}; G rel {err_g:.2e} max-abs {abs_g:.2e}")
+    assert err_rho < 2e-6 and abs_rho < 2e-6
+    assert err_g < 2e-6 and abs_g < 2e-6
+
+
 def kanamori3(n_tau=5):
     # three-orbital Kanamori model (C3 physics): blocks 1, 6, 15, 20, 15, 6, 1, 12 pairs
     return model.kanamori(3, 3.0, 0.3, 0.0, 20.0, n_tau, 0.4)
