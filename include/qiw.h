@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define QIW_ABI_VERSION 2
+#define QIW_ABI_VERSION 3
 
 /* ---- return codes (every reference @assert on this path maps to one) ---------------- */
 enum {
@@ -201,6 +201,32 @@ typedef struct {
     double flops;               /* algorithmic real FLOPs of the chain products           */
 } qiw_stats;
 int qiw_get_stats(qiw_ctx *ctx, qiw_stats *out);
+
+/* ---- work mapping and its tuner ----------------------------------------------------------
+ * No counterpart in the reference (its work units are fixed by MPI rank ranges).  How
+ * the evaluator's work is cut into workgroups:
+ *   chunk_words       flat scalar programs (1x1 sectors): program words per topology chunk;
+ *                     each (chunk, 64-sample block) is one workgroup.  0: 512.
+ *   chunk_cost        vector programs (blocks <= 32): program words x replayed columns per
+ *                     topology chunk.  0: 2^18.
+ *   samples_per_task  walks that evaluate one sample per wavefront at a time (matrix-core
+ *                     and LDS block walks): samples per workgroup, a power of 2 <= 64.
+ *                     0: the largest that gives >= 4096 workgroups per launch.
+ * The settings change only how work is split and summed (results agree to rounding).
+ * qiw_set_tuning fixes them; qiw_autotune times candidate settings on one bold step of
+ * `run` (tau_w = grid[n_tau/2], the current propagator) and keeps the fastest.  Both last
+ * until the next qiw_set_model with a different model structure. */
+typedef struct {
+    int32_t chunk_words;
+    int32_t samples_per_task;
+    int64_t chunk_cost;
+    int32_t n_tried;    /* out (qiw_autotune): candidate settings timed              */
+    int32_t pad_;
+    double best_ms;     /* out (qiw_autotune): evaluator launch time of the kept one */
+} qiw_tuning;
+int qiw_set_tuning(qiw_ctx *ctx, const qiw_tuning *t);
+int qiw_get_tuning(qiw_ctx *ctx, qiw_tuning *out);
+int qiw_autotune(qiw_ctx *ctx, const qiw_run *run, qiw_tuning *out);
 
 /* ---- learned warp of the unit cube ----------------------------------------------------
  * No counterpart in the reference, which integrates with plain qMC

@@ -98,6 +98,7 @@ struct qiw_ctx {
     qiw_stats stats{};
     qiwi::WarpState *warp = nullptr;  // learned warp, or null
     qiwi::PlanStore *plans = nullptr;  // compiled batches kept between calls
+    qiw_tuning tune{};                 // work mapping (qiw_set_tuning / qiw_autotune); 0 = default
 };
 
 namespace qiwi {

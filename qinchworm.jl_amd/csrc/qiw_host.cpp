@@ -714,8 +714,9 @@ void order_tasks(Batch &b) {
 // Samples per task.  The scalar and vector walks map one sample to a lane (64 per task);
 // the block and matrix-core walks evaluate one sample at a time per wavefront, so their
 // tasks are cut smaller until a launch holds at least 4 tasks per SIMD of the device.
-int samples_per_task(const HostModel &m, int fmt, int64_t total_samples) {
+int samples_per_task(const qiw_ctx *c, const HostModel &m, int fmt, int64_t total_samples) {
     if (fmt == FMT_SCALAR || (fmt == FMT_VEC && m.mmax <= 6)) return QIW_WAVE;
+    if (c->tune.samples_per_task > 0) return c->tune.samples_per_task;
     int spt = QIW_WAVE;
     while (spt > 1 && total_samples / spt < 4096) spt /= 2;
     return spt;
@@ -750,7 +751,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
     }
     b.fmt = program_format(m, max_order, run->precision);
     b.max_order = max_order;
-    b.spt = samples_per_task(m, b.fmt, n_groups * std::max<int64_t>(run->N_samples, 1));
+    b.spt = samples_per_task(c, m, b.fmt, n_groups * std::max<int64_t>(run->N_samples, 1));
     b.precision = run->precision;
     for (int oi = 0; oi < n_o; ++oi) {
         int order = ord[oi];
@@ -766,8 +767,8 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
         for (int na : nas) {
             Compiler comp(m, bare ? QIW_G_BARE : QIW_G_BOLD, order, na, -1, b.fmt, b.max_order);
             if (split) {
-                comp.chunk_words = kChunkWords;
-                comp.chunk_cost = kChunkCost;
+                comp.chunk_words = c->tune.chunk_words > 0 ? c->tune.chunk_words : kChunkWords;
+                comp.chunk_cost = c->tune.chunk_cost > 0 ? c->tune.chunk_cost : kChunkCost;
             }
             Compiled cg;
             if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
@@ -839,6 +840,9 @@ std::string plan_key(const qiw_ctx *c, const qiw_run *run, const std::vector<int
     put(&c->sob_dim, 4);
     put(c->sob_m.data(), 4 * c->sob_m.size());
     put(c->sob_x.data(), 4 * c->sob_x.size());
+    put(&c->tune.chunk_words, 4);
+    put(&c->tune.samples_per_task, 4);
+    put(&c->tune.chunk_cost, 8);
     return k + c->model_fp;
 }
 
@@ -1194,7 +1198,9 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
                 m.op_data.size() < 0xFFFF - 64;
     c->m = m;
     c->has_model = true;
-    c->model_fp = model_fingerprint(m);
+    std::string fp = model_fingerprint(m);
+    if (fp != c->model_fp) c->tune = qiw_tuning{};  // the work mapping was chosen for another model
+    c->model_fp = fp;
     c->vec_staged = false;
     c->mfma_staged = false;
     atomic_ip(c->m, c->ip);
@@ -1524,8 +1530,8 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
                 int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
                 for (int na = lo; na <= hi; ++na) {
                     Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
-                    comp.chunk_words = kChunkWords;
-                    comp.chunk_cost = kChunkCost;
+                    comp.chunk_words = c->tune.chunk_words > 0 ? c->tune.chunk_words : kChunkWords;
+                    comp.chunk_cost = c->tune.chunk_cost > 0 ? c->tune.chunk_cost : kChunkCost;
                     Compiled cg;
                     if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
                     if (cg.n_top == 0) continue;
@@ -1536,7 +1542,7 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
             }
             if (!b.groups.empty()) {
                 int ng = (int)b.groups.size();
-                b.spt = samples_per_task(m, b.fmt, (int64_t)m.ntau * ng * std::max<int64_t>(run->N_samples, 1));
+                b.spt = samples_per_task(c, m, b.fmt, (int64_t)m.ntau * ng * std::max<int64_t>(run->N_samples, 1));
                 std::vector<int32_t> rf((size_t)m.ntau * ng, 0), rcnt((size_t)m.ntau * ng, 0);
                 for (int j = 0; j < m.ntau; ++j)
                     for (int gi = 0; gi < ng; ++gi) {
@@ -1742,3 +1748,130 @@ int warp_collect(qiw_ctx *c, const qiw_run *run, const std::vector<int> &steps, 
 }
 
 }  // namespace qiwi
+
+// ---------------------------------------------------------------------------------------
+// Work mapping: fixed settings and the tuner
+// ---------------------------------------------------------------------------------------
+namespace {
+int check_tuning(qiw_ctx *c, const qiw_tuning &t) {
+    if (t.chunk_words < 0 || (t.chunk_words > 0 && t.chunk_words < 64))
+        return fail(c, QIW_E_ARG, "chunk_words must be 0 or >= 64");
+    if (t.chunk_cost < 0) return fail(c, QIW_E_ARG, "chunk_cost must be >= 0");
+    const int s = t.samples_per_task;
+    if (s < 0 || s > QIW_WAVE || (s & (s - 1))) return fail(c, QIW_E_ARG, "samples_per_task must be 0 or a power of 2 <= 64");
+    return 0;
+}
+
+// Evaluator time (ms, best of 2 after a warm-up launch) of one bold step of `run` at
+// tau_w = grid[n_tau/2] under the context's current work mapping.
+int time_bold_step(qiw_ctx *c, const qiw_run *run, double *ms_out) {
+    const HostModel &m = c->m;
+    int rc = 0;
+    qiw_run r = *run;
+    r.n_orders_bare = 0;
+    std::vector<int> oall(r.orders, r.orders + r.n_orders);
+    Batch b;
+    DevBatch d;
+    if ((rc = build_step_batch(c, &r, false, oall, b))) return rc;
+    if (b.groups.empty()) return fail(c, QIW_E_ARG, "no bold groups to time");
+    if ((rc = upload(c, "tune", b, r.N_samples, d))) return rc;
+    double *dcoef, *eshift;
+    int32_t *dkind;
+    if ((rc = ensure_model_buffers(c, &dcoef, &dkind, &eshift))) return rc;
+    double *partial = (double *)dev(c, "tune.partial", std::max<size_t>(b.tasks.size(), 1) * m.bt * 8, &rc);
+    double *ptab = (double *)dev(c, "ptab", (size_t)m.ntau * m.bt * 8, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
+    EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
+    a.step_w = std::max(1, m.ntau / 2 - 1);
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    double best = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt, b.precision))) return rc;
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0) best = std::min(best, (double)ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = best;
+    return 0;
+}
+
+void free_tag(qiw_ctx *c, const std::string &prefix) {
+    for (auto it = c->bufs.begin(); it != c->bufs.end();) {
+        if (it->first.compare(0, prefix.size(), prefix) == 0) {
+            if (it->second.p) (void)hipFree(it->second.p);
+            it = c->bufs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+}  // namespace
+
+extern "C" int qiw_set_tuning(qiw_ctx *c, const qiw_tuning *t) {
+    if (!c || !t) return QIW_E_ARG;
+    int rc = check_tuning(c, *t);
+    if (rc) return rc;
+    c->tune = *t;
+    c->tune.n_tried = 0;
+    c->tune.best_ms = 0.0;
+    return QIW_OK;
+}
+
+extern "C" int qiw_get_tuning(qiw_ctx *c, qiw_tuning *out) {
+    if (!c || !out) return QIW_E_ARG;
+    *out = c->tune;
+    return QIW_OK;
+}
+
+extern "C" int qiw_autotune(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
+    if (!c) return QIW_E_ARG;
+    int rc = check_run(c, run);
+    if (rc) return rc;
+    if (run->N_samples <= 0 || run->n_orders == 0) return fail(c, QIW_E_ARG, "autotune needs sampled orders");
+    (void)hipSetDevice(c->device);
+    const HostModel &m = c->m;
+    int max_order = 0;
+    for (int i = 0; i < run->n_orders; ++i) max_order = std::max(max_order, (int)run->orders[i]);
+    const int fmt = program_format(m, max_order, run->precision);
+    std::vector<qiw_tuning> cand;
+    if (fmt == FMT_SCALAR) {
+        for (int w : {256, 512, 1024, 2048}) cand.push_back(qiw_tuning{w, 0, 0, 0, 0, 0.0});
+    } else if (fmt == FMT_VEC && m.mmax <= 6) {
+        for (int e = 16; e <= 20; ++e) cand.push_back(qiw_tuning{0, 0, (int64_t)1 << e, 0, 0, 0.0});
+    } else if (fmt == FMT_VEC) {
+        for (int e = 17; e <= 19; ++e)
+            for (int spt : {1, 2, 4, 8, 16}) cand.push_back(qiw_tuning{0, spt, (int64_t)1 << e, 0, 0, 0.0});
+    } else {
+        for (int spt : {1, 2, 4, 8, 16, 32, 64}) cand.push_back(qiw_tuning{0, spt, 0, 0, 0, 0.0});
+    }
+    const qiw_tuning keep = c->tune;
+    qiw_tuning best = keep;
+    double best_ms = 1e300;
+    for (const qiw_tuning &t : cand) {
+        c->tune = t;
+        double ms = 0;
+        if ((rc = time_bold_step(c, run, &ms))) {
+            c->tune = keep;
+            free_tag(c, "tune");
+            return rc;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    free_tag(c, "tune");
+    best.n_tried = (int32_t)cand.size();
+    best.best_ms = best_ms;
+    c->tune = best;
+    if (out) *out = best;
+    return QIW_OK;
+}

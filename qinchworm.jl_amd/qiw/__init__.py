@@ -5,7 +5,7 @@ libqiw.so (include/qiw.h).  Setup (model construction, topologies, Sobol directi
 happens here; all sampling and diagram evaluation runs in the native library on the GPU.
 """
 from . import abi, diagrammatics, model, sobol  # noqa: F401
-from .inchworm import (context, correlator_2p, density_matrix, inchworm, inchworm_step,  # noqa: F401
-                       inchworm_step_bare, normalize, partition_function, stats)
+from .inchworm import (autotune, context, correlator_2p, density_matrix, get_tuning, inchworm,  # noqa: F401
+                       inchworm_step, inchworm_step_bare, normalize, partition_function, set_tuning, stats)
 from .warp import (group_index, warp_apply, warp_enable, warp_init, warp_loss_grad,  # noqa: F401
                    warp_params, warp_train)

@@ -50,6 +50,11 @@ class qiw_warp_config(C.Structure):
                 ("n_train", C.c_int64), ("lr", C.c_double), ("seed", C.c_uint64)]
 
 
+class qiw_tuning(C.Structure):
+    _fields_ = [("chunk_words", C.c_int32), ("samples_per_task", C.c_int32), ("chunk_cost", C.c_int64),
+                ("n_tried", C.c_int32), ("pad_", C.c_int32), ("best_ms", C.c_double)]
+
+
 class qiw_stats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double),
                 ("n_kernel_launches", C.c_int64), ("diagram_evals", C.c_int64),
@@ -128,7 +133,7 @@ EXPORTS = ["qiw_create", "qiw_destroy", "qiw_last_error", "qiw_abi_version", "qi
            "qiw_inchworm_step_bare", "qiw_correlator_2p", "qiw_partition_function",
            "qiw_density_matrix", "qiw_sobol_points", "qiw_simplex_points", "qiw_get_stats",
            "qiw_warp_init", "qiw_warp_params", "qiw_warp_apply", "qiw_warp_loss_grad", "qiw_warp_train",
-           "qiw_warp_enable"]
+           "qiw_warp_enable", "qiw_set_tuning", "qiw_get_tuning", "qiw_autotune"]
 
 _lib = None
 
@@ -169,6 +174,9 @@ def load_library(path=None):
     lib.qiw_warp_loss_grad.argtypes = [vp, C.c_int64, _f64p, _i32p, _i32p, _f64p, _f64p, _f64p, _f32p]
     lib.qiw_warp_train.argtypes = [vp, C.POINTER(qiw_run), _f64p]
     lib.qiw_warp_enable.argtypes = [vp, C.c_int32]
+    lib.qiw_set_tuning.argtypes = [vp, C.POINTER(qiw_tuning)]
+    lib.qiw_get_tuning.argtypes = [vp, C.POINTER(qiw_tuning)]
+    lib.qiw_autotune.argtypes = [vp, C.POINTER(qiw_run), C.POINTER(qiw_tuning)]
     _lib = lib
     return lib
 

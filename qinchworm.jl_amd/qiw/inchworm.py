@@ -142,3 +142,35 @@ def density_matrix(expansion):
     """rho = i P(beta) of a normalized propagator (src/ppgf.jl:729-733)."""
     row = expansion.P[-1]
     return [1j * expansion.block(row, s) for s in range(len(expansion.ed.dims))]
+
+
+def _tuning_dict(t):
+    return {f: getattr(t, f) for f, _ in abi.qiw_tuning._fields_ if f != "pad_"}
+
+
+def set_tuning(chunk_words=0, chunk_cost=0, samples_per_task=0, ctx=None):
+    """Fix the work mapping of the evaluators (qiw_set_tuning); 0 keeps a default."""
+    ctx = ctx or context()
+    t = abi.qiw_tuning(chunk_words=int(chunk_words), chunk_cost=int(chunk_cost),
+                       samples_per_task=int(samples_per_task))
+    ctx.check(ctx.lib.qiw_set_tuning(ctx.h, C.byref(t)))
+
+
+def get_tuning(ctx=None):
+    ctx = ctx or context()
+    t = abi.qiw_tuning()
+    ctx.check(ctx.lib.qiw_get_tuning(ctx.h, C.byref(t)))
+    return _tuning_dict(t)
+
+
+def autotune(expansion, orders, N_samples, precision=abi.QIW_PREC_FP64, seq=None, ctx=None):
+    """Time candidate work mappings on one bold step of the expansion at its current P
+    (qiw_autotune) and keep the fastest for the following calls.  Returns the setting."""
+    ctx = ctx or context()
+    orders = list(orders)
+    _stage(ctx, expansion, max(orders + [0]), seq)
+    _set_P(ctx, expansion)
+    run = abi.RunStruct(orders, [], N_samples, 0, precision)
+    t = abi.qiw_tuning()
+    ctx.check(ctx.lib.qiw_autotune(ctx.h, run.ref(), C.byref(t)))
+    return _tuning_dict(t)

@@ -26,7 +26,7 @@ def test_header_declarations_are_exported():
 
 def test_abi_version_and_no_gpu_error():
     lib = abi.load_library()
-    assert lib.qiw_abi_version() == 2
+    assert lib.qiw_abi_version() == 3
     if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
         h = C.c_void_p()
         assert lib.qiw_create(C.byref(h), 0) == abi.QIW_E_NO_GPU
@@ -39,3 +39,5 @@ def test_struct_layout_matches_header():
     assert abi.qiw_run.N_samples.offset == 32 and C.sizeof(abi.qiw_run) == 48
     assert C.sizeof(abi.qiw_stats) == 48
     assert abi.qiw_warp_config.n_train.offset == 24 and C.sizeof(abi.qiw_warp_config) == 48
+    assert abi.qiw_tuning.chunk_cost.offset == 8 and abi.qiw_tuning.best_ms.offset == 24
+    assert C.sizeof(abi.qiw_tuning) == 32

@@ -150,3 +150,28 @@ def test_block_one_table(precision):
     v_gpu, _ = qiw.inchworm_step(kg, 4, range(0, 4), 2 ** 5, precision=prec)
     v_cpu, _ = O.inchworm_step(kc, range(0, 4), 2 ** 5, 4)
     assert rel(v_gpu, v_cpu) < (RTOL if precision == "fp64" else 2e-6)
+
+
+def test_block_work_mappings():
+    """Vector programs cut into one-topology chunks, and the matrix-core walk at 1 and 16
+    samples per workgroup, keep parity."""
+    kg, kc = kanamori2(), kanamori2()
+    P_c, _, _ = O.inchworm(kc, range(0, 3), range(0, 3), 2 ** 6, n_steps_max=2)
+    kg.P, kg.P_n_known = P_c, kg.grid.n_tau
+    kc.P, kc.P_n_known = P_c, kc.grid.n_tau
+    v_cpu, _ = O.inchworm_step(kc, range(0, 4), 2 ** 5, 3)
+    g3, c3 = kanamori3(n_tau=4), kanamori3(n_tau=4)
+    P3, _, _ = O.inchworm(c3, range(0, 3), range(0, 3), 2 ** 2, n_steps_max=2)
+    for e in (g3, c3):
+        e.P, e.P_n_known = P3, e.grid.n_tau
+    v3_cpu, _ = O.inchworm_step(c3, range(0, 3), 2 ** 3, 2)
+    try:
+        qiw.set_tuning(chunk_cost=256)
+        v_gpu, _ = qiw.inchworm_step(kg, 3, range(0, 4), 2 ** 5)
+        assert rel(v_gpu, v_cpu) < RTOL
+        for spt in (1, 16):
+            qiw.set_tuning(samples_per_task=spt)
+            v3, _ = qiw.inchworm_step(g3, 2, range(0, 3), 2 ** 3, precision=qiw.abi.QIW_PREC_FP32)
+            assert rel(v3, v3_cpu) < 2e-6
+    finally:
+        qiw.set_tuning()

@@ -207,3 +207,27 @@ def test_correlator_repeated_call(ctx):
     g2 = qiw.correlator_2p(exp, exp.grid, range(0, 3), 2 ** 7)
     for a, b in zip(g1, g2):
         np.testing.assert_array_equal(a, b)
+
+
+def test_work_mappings_keep_parity(ctx):
+    """Every work mapping (topology chunk size) gives the oracle's values, and the tuner
+    keeps one of its timed candidates."""
+    exp_g, exp_c = anderson_tables(3), anderson_tables(3)
+    P, _, _ = O.inchworm(exp_c, range(0, 4), range(0, 4), 2 ** 6, n_steps_max=6)
+    for e in (exp_g, exp_c):
+        e.P, e.P_n_known = P, e.grid.n_tau
+    ref, _ = O.inchworm_step(exp_c, range(0, 5), 2 ** 8, 5)
+    try:
+        for cw in (64, 256, 2048):
+            qiw.set_tuning(chunk_words=cw)
+            assert qiw.get_tuning()["chunk_words"] == cw
+            v, _ = qiw.inchworm_step(exp_g, 5, range(0, 5), 2 ** 8)
+            assert rel(v, ref) < RTOL
+        t = qiw.autotune(exp_g, range(0, 5), 2 ** 8)
+        assert t["chunk_words"] in (256, 512, 1024, 2048) and t["n_tried"] == 4 and t["best_ms"] > 0
+        v, _ = qiw.inchworm_step(exp_g, 5, range(0, 5), 2 ** 8)
+        assert rel(v, ref) < RTOL
+        with pytest.raises(qiw.abi.QiwError):
+            qiw.set_tuning(samples_per_task=3)
+    finally:
+        qiw.set_tuning()
