@@ -75,6 +75,18 @@ QiwWarpConfig(; n_layers = 4, n_bins = 8, hidden = 64, iters = 4000, batch = 819
               n_train = 2^14, lr = 2e-3, seed = 1) =
     QiwWarpConfig(n_layers, n_bins, hidden, iters, batch, n_train_steps, n_train, lr, seed)
 
+# Work mapping of the evaluators (qiw_tuning); no counterpart in the reference
+struct QiwTuning
+    chunk_words::Int32
+    samples_per_task::Int32
+    chunk_cost::Int64
+    n_tried::Int32
+    pad_::Int32
+    best_ms::Float64
+end
+QiwTuning(; chunk_words = 0, samples_per_task = 0, chunk_cost = 0) =
+    QiwTuning(chunk_words, samples_per_task, chunk_cost, 0, 0, 0.0)
+
 mutable struct Context
     h::Ptr{Cvoid}
 end
@@ -266,5 +278,40 @@ Return to plain quasi Monte Carlo points.
 """
 disable_warp!(ctx::Context = context()) =
     check(ctx, ccall((:qiw_warp_enable, libqiw), Cint, (Ptr{Cvoid}, Int32), ctx.h, 0))
+
+"""
+    set_tuning!(tuning = QiwTuning(); ctx)
+
+Fix the work mapping of the evaluators (topology chunk size, samples per workgroup;
+0 keeps a default).  Results agree with any setting to rounding.
+"""
+set_tuning!(t::QiwTuning = QiwTuning(); ctx::Context = context()) =
+    check(ctx, ccall((:qiw_set_tuning, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwTuning}), ctx.h, Ref(t)))
+
+"""
+    autotune!(expansion, grid, orders, N_samples; precision = QIW_PREC_FP64, ctx)
+
+Time candidate work mappings on one bold step at the expansion's current `P` (tau_w at
+the middle of the grid) and keep the fastest for the following calls on `ctx`.  Returns
+the kept `QiwTuning` (with the number of candidates and the best launch time).
+"""
+function autotune!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Integer;
+                   precision::Int32 = QIW_PREC_FP64, ctx::Context = context())
+    keep = Any[]
+    orders_v = Int32.(collect(orders))
+    stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, Int32[0]))), keep)
+    n_tau = length(grid)
+    T = ppgf_table(exp, n_tau)
+    der0 = zeros(ComplexF64, size(T, 1))
+    check(ctx, ccall((:qiw_set_ppgf, libqiw), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                     ctx.h, n_tau, T, der0))
+    out = Ref(QiwTuning())
+    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, precision))
+    GC.@preserve keep orders_v begin
+        check(ctx, ccall((:qiw_autotune, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwRun}, Ref{QiwTuning}),
+                         ctx.h, run, out))
+    end
+    return out[]
+end
 
 end # module QInchwormAMD
