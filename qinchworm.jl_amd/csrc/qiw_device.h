@@ -145,7 +145,7 @@ struct EvalArgs {
     int spt;                   // samples per task (64 for the lane-per-sample walks)
 };
 
-// Arguments of the step bookkeeping kernel (k_finalize_step)
+// Arguments of the step reduction and bookkeeping kernel (k_step_reduce_finalize)
 struct FinalizeArgs {
     const QiwGroup *groups;
     const int32_t *group_task_first;   // [n_groups] first out row
@@ -169,6 +169,7 @@ struct FinalizeArgs {
     int normalize;                     // apply normalize!(P, tau_f)
     double *scratch;                   // [n_order_slots][bt]
     double *gsum;                      // [n_groups][bt] per-group sums of the partial rows
+    unsigned *done;                    // blocks of the step launch that have finished (0 between launches)
 };
 
 struct CorrFinalizeArgs {

@@ -1293,7 +1293,9 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     double *gsum = (double *)dev(c, "fin.gsum",
                                  (size_t)std::max(bare_b.groups.size(), bold_b.groups.size()) * m.bt * 8 + 8, &rc);
     int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
+    unsigned *done = (unsigned *)dev(c, "fin.done", 4, &rc);
     if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(done, 0, 4, c->stream));
     std::vector<int32_t> dg;
     for (int s = 0; s < m.ns; ++s)
         for (int i = 0; i < m.dims[s]; ++i) dg.push_back(m.bt_off[s] + i + i * m.dims[s]);
@@ -1315,6 +1317,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     fa.n_order_slots = n_slots;
     fa.scratch = scratch;
     fa.gsum = gsum;
+    fa.done = done;
 
     // every evaluator launch is bracketed by an event pair on the launch stream
     std::vector<hipEvent_t> ev;

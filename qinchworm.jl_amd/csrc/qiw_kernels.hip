@@ -1302,9 +1302,8 @@ struct KahanSum {
 // contiguous range: thread t covers element t % bt of rows t / bt, t / bt + R, ... with R
 // = 256 / bt rows per pass, and the R partial sums of each element are then added in a
 // fixed order, so the result does not depend on scheduling.  Both stages are compensated.
-__global__ __launch_bounds__(256) void k_reduce_rows(const int32_t *__restrict__ first, const int32_t *__restrict__ count,
-                                                     const double *__restrict__ partial, int bt,
-                                                     double *__restrict__ gsum) {
+__device__ void reduce_rows_block(const int32_t *__restrict__ first, const int32_t *__restrict__ count,
+                                  const double *__restrict__ partial, int bt, double *__restrict__ gsum) {
     __shared__ double red[256], cmp[256];
     const int g = blockIdx.x, t = threadIdx.x;
     const int f0 = first[g], nr = count[g];
@@ -1328,7 +1327,10 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const int32_t *__restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void k_finalize_step(FinalizeArgs a) {
+// Step bookkeeping on the per-group sums gsum, run by one block of 256 threads.  gsum is
+// read through a volatile pointer: other blocks of the same launch wrote it.
+__device__ void finalize_step_body(const FinalizeArgs &a) {
+    const volatile double *gsum = a.gsum;
     const int tid = threadIdx.x;
     const int jf = a.jf;
     const double tau_i = 0.0, tau_w = a.tau_w, tau_f = a.tau_f;
@@ -1338,7 +1340,7 @@ __global__ __launch_bounds__(256) void k_finalize_step(FinalizeArgs a) {
     for (int e = tid; e < a.bt; e += blockDim.x) {
         for (int gi = 0; gi < a.n_groups; ++gi) {
             const QiwGroup g = a.groups[gi];
-            const double s = a.gsum[(int64_t)gi * a.bt + e];
+            const double s = gsum[(int64_t)gi * a.bt + e];
             double fac = 1.0;
             if (g.sampled) {
                 double J = a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
@@ -1372,6 +1374,21 @@ __global__ __launch_bounds__(256) void k_finalize_step(FinalizeArgs a) {
         const int j = i / a.bt;
         a.ptab[i] *= exp(-(j * a.h) * lam);
     }
+}
+
+// One launch per step: every block reduces its group's rows; the last block to finish
+// (a counter every block increments after its sums are visible) runs the bookkeeping and
+// resets the counter for the next step.  This saves a launch and its gap per step.
+__global__ __launch_bounds__(256) void k_step_reduce_finalize(FinalizeArgs a) {
+    reduce_rows_block(a.group_task_first, a.group_task_count, a.partial, a.bt, a.gsum);
+    __threadfence();
+    __syncthreads();
+    __shared__ int last;
+    if (threadIdx.x == 0) last = atomicAdd(a.done, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!last) return;
+    if (threadIdx.x == 0) *a.done = 0u;
+    finalize_step_body(a);
 }
 
 // Correlator bookkeeping (src/inchworm.jl:825-889): per (pair, tau) row, sum the groups'
@@ -1514,9 +1531,8 @@ hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks
 }
 
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a) {
-    hipLaunchKernelGGL(k_reduce_rows, dim3(a.n_groups, a.bt <= 256 ? 1 : (a.bt + 255) / 256), dim3(256), 0, st, a.group_task_first,
-                       a.group_task_count, a.partial, a.bt, a.gsum);
-    hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_step_reduce_finalize, dim3(a.n_groups, a.bt <= 256 ? 1 : (a.bt + 255) / 256), dim3(256), 0,
+                       st, a);
     return hipGetLastError();
 }
 
