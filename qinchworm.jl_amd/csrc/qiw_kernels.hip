@@ -281,7 +281,11 @@ template <int KM, int SM, int TM>
 __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     using F = FlatShape<KM, SM>;
     constexpr int NM = F::NM, NC = F::NC;
-    constexpr int NW = regvec_size(KM * TM);
+    // arc values: slots arc * TM + table in one register vector of up to 16 doubles, and a
+    // second one for slots 16..31 when KM * TM > 16 (the order-6 instantiation)
+    constexpr int NW = regvec_size(KM * TM < 16 ? KM * TM : 16);
+    constexpr bool W2 = KM * TM > 16;
+    static_assert(KM * TM <= 32, "arc-value slots beyond 32");
     __shared__ double tl[2 * KM * QIW_WAVE];     // time of simplex coordinate j: tl[j * 64 + lane]
     __shared__ double sk[(NM - 3) * QIW_WAVE];   // saved product of level L >= 3: sk[(L - 3) * 64 + lane]
     __shared__ double so[SM * QIW_WAVE];         // output of initial sector s: so[s * 64 + lane]
@@ -363,8 +367,11 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     }
 
     typename RegVec<double, NW>::type W;
+    typename RegVec<double, 16>::type Wh;  // slots 16..31 (unused unless W2)
 #pragma unroll
     for (int q = 0; q < NW; ++q) W[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Wh[q] = 0.0;
 #pragma unroll
     for (int s = 0; s < SM; ++s) so[s * QIW_WAVE + lane] = 0.0;
     double acc = 0.0, cur = 0.0, cur0 = 0.0;  // cur0: product of the root (level 2 restores it)
@@ -385,7 +392,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
             const uint32_t wn = ps.next();  // the next word, read ahead (the block ends in NEXT)
             if (w & QIW_FL_TAIL) {
                 asm volatile("");
-                cur *= W[(int)((w >> 8) & (NW - 1))];
+                cur *= (W2 && (w & (16u << 8))) ? Wh[(int)((w >> 8) & 15)] : W[(int)((w >> 8) & (NW - 1))];
             }
             double pv;
             QIW_PIN_GET(NC, w, pv);
@@ -397,7 +404,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
             if (w & QIW_FL_POP) cur = (L == 2) ? cur0 : sk[(L - 3) * QIW_WAVE + lane];
             if (w & QIW_FL_TAIL) {
                 asm volatile("");
-                cur *= W[(int)((w >> 8) & (NW - 1))];
+                cur *= (W2 && (w & (16u << 8))) ? Wh[(int)((w >> 8) & 15)] : W[(int)((w >> 8) & (NW - 1))];
             }
             if (w & QIW_FL_LEAF) {
                 if (w & QIW_FL_UNIT) {
@@ -446,7 +453,11 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
                 for (int tb = 0; tb < TM; ++tb) {
                     const int tt = tb < ntab ? tb : (ntab > 0 ? ntab - 1 : 0);
                     const double *c = a.dtab.coef + ((int64_t)tt * (a.n_tau - 1) + j) * 4;
-                    W[ai * TM + tb] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                    const double v = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                    if (ai * TM + tb < 16)
+                        W[(ai * TM + tb) & (NW - 1)] = v;
+                    else
+                        Wh[(ai * TM + tb) & 15] = v;
                 }
             }
 #pragma unroll

@@ -231,3 +231,36 @@ def test_work_mappings_keep_parity(ctx):
             qiw.set_tuning(samples_per_task=3)
     finally:
         qiw.set_tuning()
+
+
+def spinless3(n_tau=8):
+    """Three spinless orbitals with density-density interactions: 8 one-dimensional
+    sectors (one per Fock state), pairs c^dag_i/c_i and their conjugates, 2 tables."""
+    labels = ["a", "b", "c"]
+    fock = model.FockSpace(labels)
+    eps = {"a": -0.3, "b": 0.1, "c": 0.4}
+    H = sum(eps[l] * fock.n(l) for l in labels)
+    H = H + 0.8 * (fock.n("a") @ fock.n("b") + fock.n("b") @ fock.n("c") + fock.n("a") @ fock.n("c"))
+    ed = model.EDCore(H, fock, lambda b: b)
+    grid = model.ImaginaryTimeGrid(5.0, n_tau)
+    delta = 0.25 * model.bethe_gf(1.0, 5.0, grid)
+    pairs = []
+    for l in labels:
+        pairs.append(model.InteractionPair(fock.c_dag(l), fock.c(l), delta))
+        pairs.append(model.InteractionPair(fock.c(l), fock.c_dag(l), model.ph_conj(delta)))
+    return model.Expansion(ed, grid, pairs)
+
+
+@pytest.mark.parametrize("K,N", [(4, 2 ** 6), (5, 2 ** 4)])
+def test_eight_sectors_and_order_five(ctx, K, N):
+    """Flat walk with 8 sectors (orders up to 4) and its order-6 instantiation (order 5)."""
+    exp_g, exp_c = spinless3(), spinless3()
+    P, _, _ = O.inchworm(exp_c, range(0, 3), range(0, 3), 2 ** 4, n_steps_max=3)
+    for e in (exp_g, exp_c):
+        e.P, e.P_n_known = P, e.grid.n_tau
+    v_gpu, per_gpu = qiw.inchworm_step(exp_g, 3, range(0, K + 1), N)
+    v_cpu, per_cpu = O.inchworm_step(exp_c, range(0, K + 1), N, 3)
+    assert rel(v_gpu, v_cpu) < RTOL
+    for o in range(K + 1):
+        assert np.abs(per_gpu[o] - per_cpu[o]).max() < RTOL * np.abs(v_cpu).max()
+    assert np.abs(per_cpu[K]).max() > 0
