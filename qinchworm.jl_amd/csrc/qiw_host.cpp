@@ -791,6 +791,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             b.groups.push_back(std::move(cg));
         }
     }
+    if (b.groups.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 (order, n_pts_after) groups in one step");
     order_tasks(b);
     return 0;
 }

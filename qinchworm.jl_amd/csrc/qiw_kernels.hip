@@ -1339,32 +1339,47 @@ __device__ void reduce_rows_block(const int32_t *__restrict__ first, const int32
 }
 
 // Step bookkeeping on the per-group sums gsum, run by one block of 256 threads.  gsum is
-// read through a volatile pointer: other blocks of the same launch wrote it.
+// read through a volatile pointer: other blocks of the same launch wrote it.  The group
+// prefactors are computed once per group (in parallel), each thread then sums its
+// elements over the groups in group order, per order slot in registers.
+constexpr int kFinMaxGroups = 64;
+constexpr int kFinMaxSlots = QIW_KMAX + 1;
 __device__ void finalize_step_body(const FinalizeArgs &a) {
     const volatile double *gsum = a.gsum;
     const int tid = threadIdx.x;
     const int jf = a.jf;
     const double tau_i = 0.0, tau_w = a.tau_w, tau_f = a.tau_f;
-    for (int i = tid; i < a.n_order_slots * a.bt; i += blockDim.x) a.scratch[i] = 0.0;
+    __shared__ double fac_sh[kFinMaxGroups];
+    __shared__ int slot_sh[kFinMaxGroups];
+    for (int gi = tid; gi < a.n_groups; gi += blockDim.x) {
+        const QiwGroup g = a.groups[gi];
+        double fac = 1.0;
+        if (g.sampled) {
+            double J = a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
+                              : simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
+            fac = ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
+        }
+        fac_sh[gi] = fac;
+        slot_sh[gi] = a.group_order_slot[gi];
+    }
     __syncthreads();
-    // one thread per output element, groups in order (fixed summation order)
     for (int e = tid; e < a.bt; e += blockDim.x) {
+        double acc[kFinMaxSlots];
+#pragma unroll
+        for (int o = 0; o < kFinMaxSlots; ++o) acc[o] = 0.0;
         for (int gi = 0; gi < a.n_groups; ++gi) {
-            const QiwGroup g = a.groups[gi];
-            const double s = gsum[(int64_t)gi * a.bt + e];
-            double fac = 1.0;
-            if (g.sampled) {
-                double J = a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
-                                  : simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
-                fac = ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
-            }
-            a.scratch[a.group_order_slot[gi] * a.bt + e] += fac * s;
+            const double v = fac_sh[gi] * gsum[(int64_t)gi * a.bt + e];
+            const int sl = slot_sh[gi];
+#pragma unroll
+            for (int o = 0; o < kFinMaxSlots; ++o) acc[o] += (o == sl) ? v : 0.0;
         }
         double tot = 0.0;
-        for (int o = 0; o < a.n_order_slots; ++o) {
-            double v = a.scratch[o * a.bt + e];
-            a.porders[((int64_t)o * a.n_tau + jf) * a.bt + e] = v;
-            tot += v;
+#pragma unroll
+        for (int o = 0; o < kFinMaxSlots; ++o) {
+            if (o < a.n_order_slots) {
+                a.porders[((int64_t)o * a.n_tau + jf) * a.bt + e] = acc[o];
+                tot += acc[o];
+            }
         }
         a.ptab[(int64_t)jf * a.bt + e] = tot;
     }
@@ -1380,10 +1395,10 @@ __device__ void finalize_step_body(const FinalizeArgs &a) {
     }
     __syncthreads();
     const double lam = lam_sh;
-    const int rows = a.n_known;
-    for (int i = tid; i < rows * a.bt; i += blockDim.x) {
-        const int j = i / a.bt;
-        a.ptab[i] *= exp(-(j * a.h) * lam);
+    for (int j = tid; j < a.n_known; j += blockDim.x) {  // one exp per row
+        const double f = exp(-(j * a.h) * lam);
+        double *row = a.ptab + (int64_t)j * a.bt;
+        for (int e = 0; e < a.bt; ++e) row[e] *= f;
     }
 }
 
