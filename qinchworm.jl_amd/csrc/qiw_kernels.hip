@@ -1321,9 +1321,28 @@ __device__ void reduce_rows_block(const int32_t *__restrict__ first, const int32
     const int R = bt <= 256 ? 256 / bt : 1;     // rows per pass
     const int e = bt <= 256 ? t % bt : blockIdx.y * 256 + t;
     const int r0 = bt <= 256 ? t / bt : 0;
+    // four independent compensated sums over rows r0, r0 + R, ... (round robin), so four
+    // loads are in flight at a time; combined in a fixed order
+    KahanSum k4[4];
+    if (r0 < R && e < bt) {
+        const double *p = partial + (int64_t)(f0 + r0) * bt + e;
+        const int64_t step = (int64_t)R * bt;
+        int r = r0;
+        for (; r + 3 * R < nr; r += 4 * R, p += 4 * step) {
+            const double x0 = p[0], x1 = p[step], x2 = p[2 * step], x3 = p[3 * step];
+            k4[0].add(x0);
+            k4[1].add(x1);
+            k4[2].add(x2);
+            k4[3].add(x3);
+        }
+        for (int q = 0; r < nr; r += R, p += step, ++q) k4[q].add(p[0]);
+    }
     KahanSum k;
-    if (r0 < R && e < bt)
-        for (int r = r0; r < nr; r += R) k.add(partial[(int64_t)(f0 + r) * bt + e]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        k.add(k4[q].s);
+        k.add(-k4[q].c);
+    }
     red[t] = k.s;
     cmp[t] = k.c;
     __syncthreads();
