@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define QIW_ABI_VERSION 3
+#define QIW_ABI_VERSION 4
 
 /* ---- return codes (every reference @assert on this path maps to one) ---------------- */
 enum {
@@ -98,6 +98,38 @@ typedef struct {
     const qiw_sbm *corr_B;         /* [n_corr] B, placed at 0                              */
 } qiw_model;
 
+/* Randomized qMC, RandomizationParams (src/randomization.jl:46-57) consumed by
+ * mean_std_from_randomization (:86-100): every sampled (order, n_pts_after) group of a
+ * step (a tau row of a correlator) is integrated with up to N_seqs sequences, stopping
+ * after sequence s >= 2 once the largest element of the standard deviation over the
+ * sequences so far is <= target_std; the group contributes the mean and (corrected)
+ * standard deviation over the sequences used.
+ *   scramble = 0  rng = nothing: every sequence is the staged one (qiw_set_sobol), so
+ *                 the mean is its integral and the deviation 0 from the second on.
+ *   scramble = 1  sequence s of a group is the unscrambled Sobol sequence after an
+ *                 LMS + digital-shift scramble (src/scrambled_sobol.jl:102-140) whose
+ *                 bits are drawn from a counter-based generator keyed by seed and
+ *                 (step, order, n_pts_after, s), not from one RNG stream in evaluation
+ *                 order, so that all sequences of a step are evaluated in one launch:
+ *                 bit word w of dimension d is the low 32 bits of
+ *                   mix(mix(seed ^ mix(stream)) + 33 d + w),
+ *                 mix = SplitMix64's output function applied after adding
+ *                 0x9E3779B97F4A7C15, stream = (domain << 60) | (key << 32) |
+ *                 (order << 24) | (n_pts_after << 16) | s, domain 1 for the initial
+ *                 step (key 0), 2 for the bold step at tau_w = grid[key], 3 for the
+ *                 correlator of pair `key` (one set of sequences for all its tau rows).
+ *                 Word 0 is the shift x (bit j = shift bit j); word 1 + p holds row p
+ *                 of the lower-triangular matrix: its bits 31..32-p are the entries
+ *                 (p, 0..p-1), the diagonal is 1.
+ * The GPU evaluates all N_seqs sequences and applies the stopping rule afterwards; the
+ * result is that of the reference's sequential loop over the same sequences. */
+typedef struct {
+    int32_t N_seqs;                /* >= 1                                                */
+    int32_t scramble;              /* 0 or 1 (above)                                      */
+    uint64_t seed;
+    double target_std;
+} qiw_rand;
+
 /* Parameters of one inchworm!/correlator_2p run (keyword arguments of
  * src/inchworm.jl:332-339 and :1075-1081). */
 typedef struct {
@@ -108,6 +140,7 @@ typedef struct {
     int64_t N_samples;             /* qMC points per (order, n_pts_after) group, 0 or 2^n */
     int32_t n_pts_after_max;       /* <= 0 means unrestricted                             */
     int32_t precision;             /* QIW_PREC_*                                          */
+    const qiw_rand *rand_params;   /* NULL: RandomizationParams() (N_seqs = 1, unscrambled) */
 } qiw_run;
 
 typedef struct qiw_ctx qiw_ctx;
@@ -147,8 +180,10 @@ int qiw_get_ppgf(qiw_ctx *ctx, int32_t *n_known_out, double *P, double *der0);
  *   P_out            [n_tau][sum m^2]   expansion.P after the run
  *   P_orders_out     [n_orders_all][n_tau][sum m^2]  per-order contributions, orders_all
  *                    sorted ascending (union of orders and orders_bare)
- *   P_orders_std_out same shape; NaN for sampled orders (one sequence), 0 for order 0,
- *                    as mean_std_from_randomization with N_seqs = 1 returns. */
+ *   P_orders_std_out same shape: the standard deviation over the sequences, summed over
+ *                    the order's groups (real part); NaN for sampled orders with
+ *                    N_seqs = 1 (std of one sample), 0 for order 0 and for rows no step
+ *                    wrote. */
 int qiw_inchworm(qiw_ctx *ctx, const qiw_run *run, double *P_out, double *P_orders_out,
                  double *P_orders_std_out);
 
@@ -168,6 +203,12 @@ int qiw_inchworm_step_bare(qiw_ctx *ctx, const qiw_run *run, double *value_out,
  * (src/inchworm.jl:918-1052, :1075-1084) for every (A, B) pair of the model, using the
  * current expansion.P.  corr_out [n_corr][n_tau] complex128: <A(tau_j) B(0)> / Z. */
 int qiw_correlator_2p(qiw_ctx *ctx, const qiw_run *run, double *corr_out);
+
+/* qiw_correlator_2p_std: correlator_2p(expansion, grid, orders, N_samples, RequestStdDev();
+ * rand_params) (src/inchworm.jl:918-1052): corr_out as above and corr_std_out
+ * [n_corr][n_tau] complex128, the sum over the groups of each row's standard deviation
+ * over the sequences, divided by Z (NaN with N_seqs = 1, 0 at tau = 0). */
+int qiw_correlator_2p_std(qiw_ctx *ctx, const qiw_run *run, double *corr_out, double *corr_std_out);
 
 /* ---- observables (src/ppgf.jl:611-636, :729-733) --------------------------------------
  * partition_function(P) = sum_s tr iP_s(beta); density_matrix after normalize!(P, beta):

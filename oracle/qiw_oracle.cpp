@@ -20,6 +20,7 @@
 //                            initial_ppgf_derivative, density_matrix
 //   src/topology_eval.jl     TopologyEvaluator and _traverse_configuration_tree!
 //   src/inchworm.jl          inchworm_step, inchworm_step_bare, inchworm!, correlator_2p
+//   src/randomization.jl     mean_std_from_randomization               (mean_std)
 //
 // Parity of each part is pinned by the tests in tests/test_oracle_*.py (golden vectors of
 // the reference's test suite, SciPy, closed forms).  Grid-stored Green's functions are
@@ -180,6 +181,52 @@ extern "C" int orc_sobol_points(int D, const uint32_t *m, const uint32_t *shift,
     s.skip(skip, exact != 0);
     for (int64_t i = 0; i < npts; ++i) s.next(out + (size_t)i * D);
     return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// Scramble bits of randomized qMC.  The reference draws the LMS + shift bits of each
+// ScrambledSobolSeq(D; scramble_rng) from one RNG (src/scrambled_sobol.jl:104-109); the
+// ABI (qiw_rand) defines them by a counter-based generator instead: word w of dimension
+// d of a stream is the low half of mix(mix(seed ^ mix(stream)) + 33 d + w), mix being
+// SplitMix64's output step after its increment.  Word 0 is the shift, word 1 + p row p of
+// the lower-triangular matrix (entries (p, q < p) in bits 31 - q, unit diagonal).
+// ------------------------------------------------------------------------------------
+static uint64_t rq_mix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static uint64_t rq_stream(int domain, int64_t key, int order, int n_after, int seq) {
+    return ((uint64_t)domain << 60) | (((uint64_t)key & 0xFFFFFFFull) << 32) | ((uint64_t)order << 24) |
+           ((uint64_t)n_after << 16) | (uint64_t)seq;
+}
+static uint32_t rq_word(uint64_t seed, uint64_t stream, int d, int w) {
+    return (uint32_t)rq_mix(rq_mix(seed ^ rq_mix(stream)) + (uint64_t)(33 * d + w));
+}
+
+// LMS + shift scramble of direction numbers m [D][32] (src/scrambled_sobol.jl:111-140):
+// bit 32 - p (1-based row p) of a scrambled number is the parity of row p AND the plain one.
+extern "C" int orc_rq_scramble(uint64_t seed, uint64_t stream, int D, const uint32_t *m, uint32_t *m_out,
+                               uint32_t *x_out) {
+    for (int d = 0; d < D; ++d) {
+        x_out[d] = rq_word(seed, stream, d, 0);
+        uint32_t rows[32];
+        for (int p = 0; p < 32; ++p) {
+            const uint32_t below = p == 0 ? 0u : (0xFFFFFFFFu << (32 - p));
+            rows[p] = (rq_word(seed, stream, d, 1 + p) & below) | (1u << (31 - p));
+        }
+        for (int j = 0; j < 32; ++j) {
+            uint32_t t2 = 0;
+            for (int p = 0; p < 32; ++p) t2 |= (uint32_t)(__builtin_popcount(rows[p] & m[d * 32 + j]) & 1) << (31 - p);
+            m_out[d * 32 + j] = t2;
+        }
+    }
+    return 0;
+}
+
+extern "C" uint64_t orc_rq_stream(int domain, int64_t key, int order, int n_after, int seq) {
+    return rq_stream(domain, key, order, n_after, seq);
 }
 
 // ------------------------------------------------------------------------------------
@@ -910,6 +957,80 @@ struct SobolSource {
 
 static int g_threads = 1;
 
+// RandomizationParams of a run (src/randomization.jl:46-57)
+struct RandParams {
+    int n_seqs = 1;
+    bool scramble = false;
+    uint64_t seed = 0;
+    double target_std = 0.0;
+};
+static RandParams rand_params(const qiw_run *run) {
+    RandParams r;
+    if (run->rand_params) {
+        r.n_seqs = run->rand_params->N_seqs;
+        r.scramble = run->rand_params->scramble != 0;
+        r.seed = run->rand_params->seed;
+        r.target_std = run->rand_params->target_std;
+    }
+    return r;
+}
+
+// Direction numbers and shift of sequence `seq` of a group: the staged ones (rng =
+// nothing), or the staged direction numbers scrambled with the stream's bits.
+struct SeqStore {
+    std::vector<uint32_t> m, x;
+};
+static SobolSource sequence_source(const RandParams &rq, SobolSource staged, int dim, int domain, int64_t key,
+                                   int order, int n_after, int seq, SeqStore &store) {
+    if (!rq.scramble) return staged;
+    store.m.assign((size_t)dim * 32, 0u);
+    store.x.assign((size_t)std::max(dim, 1), 0u);
+    orc_rq_scramble(rq.seed, rq_stream(domain, key, order, n_after, seq), dim, staged.m, store.m.data(),
+                    store.x.data());
+    return SobolSource{store.m.data(), store.x.data()};
+}
+
+// mean_std_from_randomization (src/randomization.jl:86-100) on flattened samples: sample
+// s = f(s) for s = 0, 1, ..., stopping after s >= 1 once the largest element of the
+// standard deviation is <= target_std.  The mean is sum * (1/n) for sector block matrices
+// (their `/` multiplies by the reciprocal, src/sector_block_matrix.jl:125) and sum / n for
+// scalars; the variance is sum |x - sum/n|^2 / (n - 1) (Statistics.var,
+// src/sector_block_matrix.jl:191-223), NaN for one sample.
+template <class F>
+static void mean_std(F f, const RandParams &rq, bool recip_mean, std::vector<cd> &mean, std::vector<double> &sd) {
+    std::vector<std::vector<cd>> xs;
+    auto stdev = [&](size_t n) {
+        const size_t ne = xs[0].size();
+        std::vector<double> out(ne);
+        for (size_t e = 0; e < ne; ++e) {
+            cd sum = 0;
+            for (size_t s = 0; s < n; ++s) sum += xs[s][e];
+            const cd mu = sum / (double)n;
+            double v = 0;
+            for (size_t s = 0; s < n; ++s) v += std::norm(xs[s][e] - mu);
+            out[e] = std::sqrt(v / (double)(n - 1));
+        }
+        return out;
+    };
+    for (int s = 0; s < rq.n_seqs; ++s) {
+        xs.push_back(f(s));
+        if (s >= 1) {
+            std::vector<double> st = stdev(xs.size());
+            double mx = 0;
+            for (double v : st) mx = std::max(mx, v);
+            if (mx <= rq.target_std) break;
+        }
+    }
+    const size_t n = xs.size(), ne = xs[0].size();
+    mean.assign(ne, 0.0);
+    for (size_t e = 0; e < ne; ++e) {
+        cd sum = 0;
+        for (size_t s = 0; s < n; ++s) sum += xs[s][e];
+        mean[e] = recip_mean ? sum * (1.0 / (double)n) : sum / (double)n;
+    }
+    sd = stdev(n);
+}
+
 static std::vector<CMat> integrate_group(const Model &m, const PPGF *P, bool bold, int order,
                                          const std::map<int, FixedNode> &fixed, const Transform &tr,
                                          const std::vector<Topology> &tops, int64_t N, SobolSource sob,
@@ -994,21 +1115,52 @@ static void bt_add(std::vector<CMat> &a, const std::vector<CMat> &b) {
     for (size_t s = 0; s < a.size(); ++s)
         for (size_t e = 0; e < a[s].a.size(); ++e) a[s].a[e] += b[s].a[e];
 }
+static std::vector<cd> bt_flat(const std::vector<CMat> &v) {
+    std::vector<cd> out;
+    for (auto &b : v) out.insert(out.end(), b.a.begin(), b.a.end());
+    return out;
+}
+
+// One sampled group of a step under randomized qMC: per_order += mean, per_order_std +=
+// deviation (complex with zero imaginary part; NaN + NaN i for one sequence).
+template <class Integrate>
+static void add_randomized(const Model &m, const RandParams &rq, SobolSource staged, int domain, int64_t key,
+                           const TopData &d, Integrate integrate, std::vector<CMat> &acc, std::vector<CMat> &acc_std) {
+    std::vector<cd> cached;
+    auto f = [&](int s) -> std::vector<cd> {
+        if (!rq.scramble && !cached.empty()) return cached;  // every sequence is the staged one
+        SeqStore st;
+        SobolSource src = sequence_source(rq, staged, 2 * d.order, domain, key, d.order, d.n_pts_after, s, st);
+        std::vector<cd> v = bt_flat(integrate(src));
+        if (!rq.scramble) cached = v;
+        return v;
+    };
+    std::vector<cd> mean;
+    std::vector<double> sd;
+    mean_std(f, rq, true, mean, sd);
+    size_t k = 0;
+    for (int s = 0; s < m.ns; ++s)
+        for (size_t e = 0; e < acc[s].a.size(); ++e, ++k) {
+            acc[s].a[e] += mean[k];
+            acc_std[s].a[e] += std::isnan(sd[k]) ? cd(sd[k], sd[k]) : cd(sd[k], 0.0);
+        }
+}
 
 struct StepResult {
     std::vector<CMat> value;
     std::map<int, std::vector<CMat>> per_order, per_order_std;
 };
 
+enum { RQ_BARE = 1, RQ_BOLD = 2, RQ_CORR = 3 };  // stream domains of the scramble bits
+
 static StepResult inchworm_step(const Model &m, const PPGF &P, int jw, int jf, const std::vector<TopData> &td,
-                                int64_t N, SobolSource sob) {  // :123-203
+                                int64_t N, SobolSource sob, const RandParams &rq) {  // :123-203
     double t_i = 0.0, t_w = jw * m.h, t_f = jf * m.h;
     StepResult r;
     for (auto &d : td) {
         r.per_order[d.order] = bt_zero(m);
         r.per_order_std[d.order] = bt_zero(m);
     }
-    const double nan = std::nan("");
     for (auto &d : td) {
         if (d.order == 0) {
             std::map<int, FixedNode> fixed{{1, {{IDENTITY, -1, -1}, t_i}},
@@ -1026,9 +1178,10 @@ static StepResult inchworm_step(const Model &m, const PPGF &P, int jw, int jf, c
                                            {d_before + 2, {{INCH, -1, -1}, t_w}},
                                            {2 * d.order + 3, {{IDENTITY, -1, -1}, t_f}}};
             Transform tr = Transform::dsimplex(d_before, d_after, t_i, t_w, t_f);
-            bt_add(r.per_order[d.order], integrate_group(m, &P, true, d.order, fixed, tr, d.tops, N, sob));
-            for (auto &b : r.per_order_std[d.order])
-                for (auto &v : b.a) v = cd(nan, nan);
+            add_randomized(
+                m, rq, sob, RQ_BOLD, jw, d,
+                [&](SobolSource src) { return integrate_group(m, &P, true, d.order, fixed, tr, d.tops, N, src); },
+                r.per_order[d.order], r.per_order_std[d.order]);
         }
     }
     r.value = bt_zero(m);
@@ -1037,14 +1190,13 @@ static StepResult inchworm_step(const Model &m, const PPGF &P, int jw, int jf, c
 }
 
 static StepResult inchworm_step_bare(const Model &m, const std::vector<TopData> &td, int64_t N,
-                                     SobolSource sob) {  // :228-304
+                                     SobolSource sob, const RandParams &rq) {  // :228-304
     double t_i = 0.0, t_f = m.h;
     StepResult r;
     for (auto &d : td) {
         r.per_order[d.order] = bt_zero(m);
         r.per_order_std[d.order] = bt_zero(m);
     }
-    const double nan = std::nan("");
     for (auto &d : td) {
         if (d.order == 0) {
             std::map<int, FixedNode> fixed{{1, {{IDENTITY, -1, -1}, t_i}}, {2, {{IDENTITY, -1, -1}, t_f}}};
@@ -1058,9 +1210,10 @@ static StepResult inchworm_step_bare(const Model &m, const std::vector<TopData> 
             std::map<int, FixedNode> fixed{{1, {{IDENTITY, -1, -1}, t_i}},
                                            {2 * d.order + 2, {{IDENTITY, -1, -1}, t_f}}};
             Transform tr = Transform::root(2 * d.order, t_i, t_f);
-            bt_add(r.per_order[d.order], integrate_group(m, nullptr, false, d.order, fixed, tr, d.tops, N, sob));
-            for (auto &b : r.per_order_std[d.order])
-                for (auto &v : b.a) v = cd(nan, nan);
+            add_randomized(
+                m, rq, sob, RQ_BARE, 0, d,
+                [&](SobolSource src) { return integrate_group(m, nullptr, false, d.order, fixed, tr, d.tops, N, src); },
+                r.per_order[d.order], r.per_order_std[d.order]);
         }
     }
     r.value = bt_zero(m);
@@ -1180,7 +1333,7 @@ extern "C" int orc_inchworm(const qiw_model *mi, const qiw_run *run, const uint3
     atomic_ppgf(m, P);
     try {
         auto tdb = bare_top_data(run);
-        StepResult r = inchworm_step_bare(m, tdb, run->N_samples, sob);
+        StepResult r = inchworm_step_bare(m, tdb, run->N_samples, sob, rand_params(run));
         set_ppgf(m, P, 1, r.value);
         for (auto &kv : r.per_order) {
             P_orders[kv.first][1] = kv.second;
@@ -1191,7 +1344,7 @@ extern "C" int orc_inchworm(const qiw_model *mi, const qiw_run *run, const uint3
         for (int n = 2; n <= m.ntau - 1; ++n) {  // :475-493
             if (n_steps_max >= 0 && steps >= n_steps_max) break;
             int jw = n - 1, jf = n;
-            StepResult s = inchworm_step(m, P, jw, jf, td, run->N_samples, sob);
+            StepResult s = inchworm_step(m, P, jw, jf, td, run->N_samples, sob, rand_params(run));
             set_ppgf(m, P, jf, s.value);
             normalize_tau(m, P, jf);
             for (auto &kv : s.per_order) {
@@ -1224,7 +1377,8 @@ extern "C" int orc_inchworm_step(const qiw_model *mi, const qiw_run *run, const 
     ppgf_from_table(m, P, n_known, P_tab, der0);
     auto td = bold_top_data(run, false);
     try {
-        StepResult r = inchworm_step(m, P, jw, jw + 1, td, run->N_samples, SobolSource{sob_m, sob_shift});
+        StepResult r =
+            inchworm_step(m, P, jw, jw + 1, td, run->N_samples, SobolSource{sob_m, sob_shift}, rand_params(run));
         if (value_out) bt_store(m, r.value, value_out);
         if (orders_out)
             for (int oi = 0; oi < run->n_orders; ++oi) {
@@ -1245,7 +1399,7 @@ extern "C" int orc_inchworm_step_bare(const qiw_model *mi, const qiw_run *run, c
     if (!load_model(mi, m)) return QIW_E_NOT_SBM;
     auto td = bare_top_data(run);
     try {
-        StepResult r = inchworm_step_bare(m, td, run->N_samples, SobolSource{sob_m, sob_shift});
+        StepResult r = inchworm_step_bare(m, td, run->N_samples, SobolSource{sob_m, sob_shift}, rand_params(run));
         if (value_out) bt_store(m, r.value, value_out);
         if (orders_out)
             for (int oi = 0; oi < run->n_orders_bare; ++oi) bt_store(m, r.per_order[run->orders_bare[oi]], orders_out + (size_t)2 * oi * m.bt_size);
@@ -1256,11 +1410,14 @@ extern "C" int orc_inchworm_step_bare(const qiw_model *mi, const qiw_run *run, c
     return 0;
 }
 
-// correlator_2p at one tau index jA (src/inchworm.jl:805-890), without the final 1/Z.
+// correlator_2p at one tau index jA (src/inchworm.jl:805-890), without the final 1/Z:
+// the value and the sum of the groups' deviations.  One set of scrambled sequences per
+// operator pair serves all its tau rows (stream key = pair index).
 static cd correlator_at(const Model &m, const PPGF &P, int pair_idx, int jA, const std::vector<TopData> &td,
-                        int64_t N, SobolSource sob) {
+                        int64_t N, SobolSource sob, const RandParams &rq, double *result_std) {
     double t_B = 0.0, t_A = jA * m.h, t_f = m.beta;
     cd result = 0;
+    *result_std = 0.0;
     for (auto &d : td) {
         cd contrib = 0;
         if (d.order == 0) {
@@ -1281,9 +1438,24 @@ static cd correlator_at(const Model &m, const PPGF &P, int pair_idx, int jA, con
                                            {d_before + 2, {{OPERATOR, pair_idx, 1}, t_A}},
                                            {2 * d.order + 3, {{IDENTITY, -1, -1}, t_f}}};
             Transform tr = Transform::dsimplex(d_before, d_after, t_B, t_A, t_f);
-            std::vector<CMat> r = integrate_group(m, &P, true, d.order, fixed, tr, d.tops, N, sob);
-            for (int s = 0; s < m.ns; ++s)
-                for (int i = 0; i < m.dims[s]; ++i) contrib += r[s](i, i);
+            std::vector<cd> cached;
+            auto f = [&](int q) -> std::vector<cd> {
+                if (!rq.scramble && !cached.empty()) return cached;  // every sequence is the staged one
+                SeqStore st;
+                SobolSource src =
+                    sequence_source(rq, sob, 2 * d.order, RQ_CORR, pair_idx, d.order, d.n_pts_after, q, st);
+                std::vector<CMat> r = integrate_group(m, &P, true, d.order, fixed, tr, d.tops, N, src);
+                cd tr_sum = 0;
+                for (int s = 0; s < m.ns; ++s)
+                    for (int i = 0; i < m.dims[s]; ++i) tr_sum += r[s](i, i);
+                cached = {tr_sum};
+                return cached;
+            };
+            std::vector<cd> mean;
+            std::vector<double> sd;
+            mean_std(f, rq, false, mean, sd);
+            contrib = mean[0];
+            *result_std += sd[0];
         }
         result += contrib;
     }
@@ -1292,9 +1464,10 @@ static cd correlator_at(const Model &m, const PPGF &P, int pair_idx, int jA, con
 
 // correlator_2p (src/inchworm.jl:918-1052).  tau_first/tau_count restrict the tau
 // indices evaluated (bounded samples); the rest of corr_out is left untouched.
-extern "C" int orc_correlator_2p(const qiw_model *mi, const qiw_run *run, const uint32_t *sob_m,
-                                 const uint32_t *sob_shift, int n_known, const double *P_tab,
-                                 const double *der0, int tau_first, int tau_count, double *corr_out) {
+extern "C" int orc_correlator_2p_std(const qiw_model *mi, const qiw_run *run, const uint32_t *sob_m,
+                                     const uint32_t *sob_shift, int n_known, const double *P_tab,
+                                     const double *der0, int tau_first, int tau_count, double *corr_out,
+                                     double *corr_std_out) {
     Model m;
     if (!load_model(mi, m)) return QIW_E_NOT_SBM;
     PPGF P;
@@ -1308,16 +1481,23 @@ extern "C" int orc_correlator_2p(const qiw_model *mi, const qiw_run *run, const 
         for (int c = 0; c < m.ncorr; ++c) {
             for (int j = tau_first; j < std::min(m.ntau, tau_first + tau_count); ++j) {
                 cd v;
+                double sd = 0.0;
+                const RandParams rq = rand_params(run);
                 if (j == 0) {
                     if (td[0].order != 0) continue;
                     std::vector<TopData> td0(td.begin(), td.begin() + 1);
-                    v = correlator_at(m, P, c, 0, td0, run->N_samples, sob);
+                    v = correlator_at(m, P, c, 0, td0, run->N_samples, sob, rq, &sd);
                 } else {
-                    v = correlator_at(m, P, c, j, td, run->N_samples, sob);
+                    v = correlator_at(m, P, c, j, td, run->N_samples, sob, rq, &sd);
                 }
                 v = v / Z;
+                const cd vs = sd / Z;  // (result, result_std) ./ Z
                 corr_out[2 * ((size_t)c * m.ntau + j)] = v.real();
                 corr_out[2 * ((size_t)c * m.ntau + j) + 1] = v.imag();
+                if (corr_std_out) {
+                    corr_std_out[2 * ((size_t)c * m.ntau + j)] = std::isnan(sd) ? sd : vs.real();
+                    corr_std_out[2 * ((size_t)c * m.ntau + j) + 1] = std::isnan(sd) ? sd : vs.imag();
+                }
             }
         }
     } catch (std::exception &e) {
@@ -1325,6 +1505,13 @@ extern "C" int orc_correlator_2p(const qiw_model *mi, const qiw_run *run, const 
         return QIW_E_OFFDIAG;
     }
     return 0;
+}
+
+extern "C" int orc_correlator_2p(const qiw_model *mi, const qiw_run *run, const uint32_t *sob_m,
+                                 const uint32_t *sob_shift, int n_known, const double *P_tab,
+                                 const double *der0, int tau_first, int tau_count, double *corr_out) {
+    return orc_correlator_2p_std(mi, run, sob_m, sob_shift, n_known, P_tab, der0, tau_first, tau_count, corr_out,
+                                 nullptr);
 }
 
 // TopologyEvaluator at explicit variable times (unit tests of the evaluator).

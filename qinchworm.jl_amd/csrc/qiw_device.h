@@ -99,7 +99,7 @@ struct QiwGroup {
     int32_t n_samples;     // qMC points (1 for order 0: exact evaluation)
     int32_t sampled;       // 0 for order 0
     int32_t weighted;      // row d_before + d_after of the coordinates holds per-sample weights
-    int32_t pad_;
+    int32_t seq;           // randomized qMC: index of the group's sequence (0 otherwise)
     int64_t ps_off;        // offset of the group's samples in EvalArgs::per_sample
 };
 
@@ -170,6 +170,15 @@ struct FinalizeArgs {
     double *scratch;                   // [n_order_slots][bt]
     double *gsum;                      // [n_groups][bt] per-group sums of the partial rows
     unsigned *done;                    // blocks of the step launch that have finished (0 between launches)
+    // randomized qMC (k_step_reduce_finalize_rqmc): the groups above are the sequences of
+    // n_real (order, n_pts_after) groups, those of real group r at real_first[r] ..
+    // real_first[r] + real_count[r] - 1 (one entry when every sequence is the same one)
+    int n_real;
+    const int32_t *real_first;
+    const int32_t *real_count;
+    int n_seqs;                        // N_seqs
+    double target_std;
+    double *porders_std;               // [n_order_slots][n_tau][bt] standard deviations
 };
 
 struct CorrFinalizeArgs {
@@ -185,6 +194,13 @@ struct CorrFinalizeArgs {
     int n_diag;
     double h, beta;
     double *out;                     // [n_rows]
+    // randomized qMC (k_finalize_corr_rqmc): real groups as in FinalizeArgs
+    int n_real;
+    const int32_t *real_first;
+    const int32_t *real_count;
+    int n_seqs;
+    double target_std;
+    double *out_std;                 // [n_rows]
 };
 
 // Arguments of the block-sector evaluator (k_eval_block)
@@ -217,6 +233,28 @@ struct MfmaArgs {
     int bt;
 };
 
+// ---- randomized qMC scrambles -------------------------------------------------------------
+#define QIW_MAX_SEQS 64                 // largest N_seqs
+// SplitMix64 output function after the increment (Steele, Lea, Flood, OOPSLA 2014).
+__host__ __device__ static inline uint64_t qiw_mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// Stream of sequence `seq` of group (order, n_after): domain 1 initial step (key 0),
+// 2 bold step (key = tau_w index), 3 correlator (key = operator pair).
+enum { QIW_RQ_BARE = 1, QIW_RQ_BOLD = 2, QIW_RQ_CORR = 3 };
+__host__ __device__ static inline uint64_t qiw_rq_stream(uint64_t domain, uint64_t key, int order, int n_after,
+                                                         int seq) {
+    return (domain << 60) | ((key & 0xFFFFFFFull) << 32) | ((uint64_t)order << 24) | ((uint64_t)n_after << 16) |
+           (uint64_t)seq;
+}
+// Bit word w of dimension d of a stream: 0 the shift, 1 + p row p of the scramble matrix.
+__host__ __device__ static inline uint32_t qiw_rq_word(uint64_t seed, uint64_t stream, int d, int w) {
+    return (uint32_t)qiw_mix64(qiw_mix64(seed ^ qiw_mix64(stream)) + (uint64_t)(33 * d + w));
+}
+
 // Launch wrappers (qiw_kernels.hip)
 hipError_t qiw_launch_sobol_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int64_t first,
                                    int64_t n, double *out);
@@ -235,3 +273,10 @@ hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, in
 hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, float2 *out32);
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);
+hipError_t qiw_launch_finalize_step_rqmc(hipStream_t st, const FinalizeArgs &a);
+hipError_t qiw_launch_finalize_corr_rqmc(hipStream_t st, const CorrFinalizeArgs &a);
+// Simplex coordinates of every sampled group's sequence for one step: the staged
+// direction numbers m (dimensions 0..dim_max-1) LMS + shift scrambled with the bits of the
+// group's stream (domain, key, order, n_after, seq).
+hipError_t qiw_launch_rqmc_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int max_dim, int64_t n,
+                                  const uint32_t *m, uint64_t seed, int domain, int64_t key, double *ut);

@@ -208,6 +208,7 @@ struct Compiled {
     int64_t leaves_per_sample = 0;
     int64_t edges_per_sample = 0;
     int n_top = 0;
+    int replica_of = -1;  // randomized qMC: another sequence of group `replica_of` (no program of its own)
 };
 
 struct TrieNode {
@@ -546,9 +547,12 @@ int lds_slots(const QiwGroup &g, const HostModel &m, int fmt) {
 // A batch of compiled groups resident on the device
 // ---------------------------------------------------------------------------------------
 struct Batch {
-    std::vector<Compiled> groups;
+    std::vector<Compiled> groups;   // one entry per evaluated sequence of each group
     std::vector<QiwTask> tasks;
     std::vector<int32_t> g_first, g_count, g_slot;  // per group: task rows, order slot
+    std::vector<int32_t> r_first, r_count;  // per (order, n_pts_after) group: its sequences' entries
+    int n_eval = 1;        // sequences evaluated per sampled group
+    bool scrambled = false;  // coordinates are drawn per step (randomized qMC)
     int lds = 0;
     int fmt = FMT_SCALAR;  // program format of every group of the batch
     int spt = QIW_WAVE;    // samples per task
@@ -566,6 +570,7 @@ struct DevBatch {
     double *coefs = nullptr;
     double *ut = nullptr;
     int32_t *g_first = nullptr, *g_count = nullptr, *g_slot = nullptr;
+    int32_t *r_first = nullptr, *r_count = nullptr;
 };
 
 // Warp group index of a bold group (order k <= 4, n_after points after tau_w), else -1.
@@ -588,11 +593,17 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
     int64_t ut_total = 0, ps_total = 0;
     for (auto &cg : b.groups) {
         QiwGroup g = cg.g;
-        // programs start on a 64-word block boundary (the NOP placement assumes it)
-        while (prog.size() % QIW_WAVE) prog.push_back(pad);
-        g.prog_off = (int32_t)prog.size();
-        g.topo_off = (int32_t)topo.size();
-        g.coef_off = (int32_t)coefs.size();
+        if (cg.replica_of >= 0) {  // another sequence of an earlier group: same program
+            g.prog_off = gs[cg.replica_of].prog_off;
+            g.topo_off = gs[cg.replica_of].topo_off;
+            g.coef_off = gs[cg.replica_of].coef_off;
+        } else {
+            // programs start on a 64-word block boundary (the NOP placement assumes it)
+            while (prog.size() % QIW_WAVE) prog.push_back(pad);
+            g.prog_off = (int32_t)prog.size();
+            g.topo_off = (int32_t)topo.size();
+            g.coef_off = (int32_t)coefs.size();
+        }
         g.sampled = g.order > 0;
         g.n_samples = g.sampled ? (int32_t)N : 1;
         g.weighted = (weighted && g.sampled && warp_gid(g) >= 0) ? 1 : 0;
@@ -619,6 +630,8 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
     d.g_first = (int32_t *)dev(c, tag + ".gfirst", b.g_first.size() * 4 + 4, &rc);
     d.g_count = (int32_t *)dev(c, tag + ".gcount", b.g_count.size() * 4 + 4, &rc);
     d.g_slot = (int32_t *)dev(c, tag + ".gslot", b.g_slot.size() * 4 + 4, &rc);
+    d.r_first = (int32_t *)dev(c, tag + ".rfirst", b.r_first.size() * 4 + 4, &rc);
+    d.r_count = (int32_t *)dev(c, tag + ".rcount", b.r_count.size() * 4 + 4, &rc);
     if (rc) return rc;
     HIPCHK(c, hipMemcpy(d.groups, gs.data(), gs.size() * sizeof(QiwGroup), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.prog, prog.data(), prog.size() * 8, hipMemcpyHostToDevice));
@@ -630,7 +643,11 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
         HIPCHK(c, hipMemcpy(d.g_count, b.g_count.data(), b.g_count.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(d.g_slot, b.g_slot.data(), b.g_slot.size() * 4, hipMemcpyHostToDevice));
     }
-    // simplex coordinates of every sampled group (step independent)
+    if (!b.r_first.empty()) {
+        HIPCHK(c, hipMemcpy(d.r_first, b.r_first.data(), b.r_first.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.r_count, b.r_count.data(), b.r_count.size() * 4, hipMemcpyHostToDevice));
+    }
+    // simplex coordinates of every sampled group (step independent unless scrambled per step)
     uint32_t *sm = (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc);
     uint32_t *sx = (uint32_t *)dev(c, "sobol.x", c->sob_x.size() * 4, &rc);
     if (rc) return rc;
@@ -640,9 +657,55 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
         if (!cg.g.sampled) continue;
         int dim = cg.g.d_before + cg.g.d_after;
         if (dim > c->sob_dim) return fail(c, QIW_E_DIM, "Sobol dimension beyond the staged direction numbers");
+        if (b.scrambled) continue;
         HIPCHK(c, qiw_launch_simplex_coords(c->stream, sm, sx, cg.g.d_before, cg.g.d_after, N, d.ut + cg.g.ut_off));
     }
     return 0;
+}
+
+// Randomized qMC parameters of a run (RandomizationParams, src/randomization.jl:46-57).
+struct RandPlan {
+    int n_seqs = 1;
+    bool scrambled = false;
+    uint64_t seed = 0;
+    double target_std = 0.0;
+    int n_eval() const { return scrambled ? n_seqs : 1; }  // sequences evaluated per group
+};
+RandPlan rand_plan(const qiw_run *run) {
+    RandPlan r;
+    if (run && run->rand_params) {
+        r.n_seqs = run->rand_params->N_seqs;
+        r.scrambled = run->rand_params->scramble != 0;
+        r.seed = run->rand_params->seed;
+        r.target_std = run->rand_params->target_std;
+    }
+    return r;
+}
+
+// Appends group cg (compiled) and, for a sampled group of a randomized run, its further
+// sequences as replicas; `add_tasks(gi)` appends the tasks of entry gi.
+template <class AddTasks>
+void push_sequences(Batch &b, Compiled &&cg, int n_eval, AddTasks add_tasks) {
+    const int n = cg.g.order > 0 ? n_eval : 1;
+    const int first = (int)b.groups.size();
+    b.r_first.push_back(first);
+    b.r_count.push_back(n);
+    Compiled rep;
+    if (n > 1) {  // the replicas carry no program
+        rep.g = cg.g;
+        rep.chunks = cg.chunks;
+        rep.leaves_per_sample = cg.leaves_per_sample;
+        rep.edges_per_sample = cg.edges_per_sample;
+        rep.n_top = cg.n_top;
+        rep.replica_of = first;
+    }
+    b.groups.push_back(std::move(cg));
+    add_tasks(first);
+    for (int q = 1; q < n; ++q) {
+        b.groups.push_back(rep);
+        b.groups.back().g.seq = q;
+        add_tasks(first + q);
+    }
 }
 
 int check_run(qiw_ctx *c, const qiw_run *run) {
@@ -657,6 +720,13 @@ int check_run(qiw_ctx *c, const qiw_run *run) {
         if (run->orders_bare[i] < 0 || run->orders_bare[i] > QIW_KMAX)
             return fail(c, QIW_E_LIMIT, "order beyond QIW_KMAX");
     if (c->sob_dim == 0) return fail(c, QIW_E_STATE, "no Sobol direction numbers staged (qiw_set_sobol)");
+    if (run->rand_params) {
+        const qiw_rand &r = *run->rand_params;
+        if (r.N_seqs < 1) return fail(c, QIW_E_ARG, "rand_params.N_seqs must be > 0");
+        if (r.N_seqs > QIW_MAX_SEQS) return fail(c, QIW_E_LIMIT, "rand_params.N_seqs beyond 64");
+        if (r.scramble != 0 && r.scramble != 1) return fail(c, QIW_E_ARG, "rand_params.scramble must be 0 or 1");
+        if (!(r.target_std >= 0.0)) return fail(c, QIW_E_ARG, "rand_params.target_std must be >= 0");
+    }
     return 0;
 }
 
@@ -749,9 +819,12 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
         max_order = std::max(max_order, (int)ord[oi]);
         if (ord[oi] > 0) n_groups += bare ? 1 : 2 * ord[oi] - 1;
     }
+    const RandPlan rq = rand_plan(run);
+    b.n_eval = rq.n_eval();
+    b.scrambled = rq.scrambled;
     b.fmt = program_format(m, max_order, run->precision);
     b.max_order = max_order;
-    b.spt = samples_per_task(c, m, b.fmt, n_groups * std::max<int64_t>(run->N_samples, 1));
+    b.spt = samples_per_task(c, m, b.fmt, n_groups * b.n_eval * std::max<int64_t>(run->N_samples, 1));
     b.precision = run->precision;
     for (int oi = 0; oi < n_o; ++oi) {
         int order = ord[oi];
@@ -778,20 +851,23 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
             int slot = (int)(std::find(oall.begin(), oall.end(), order) - oall.begin());
             int64_t N = order > 0 ? run->N_samples : 1;
-            int gi = (int)b.groups.size();
-            b.g_first.push_back((int32_t)b.tasks.size());
-            int nt = (int)((N + b.spt - 1) / b.spt);
-            for (int t = 0; t < nt; ++t)
-                for (int32_t ch : cg.chunks) b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size(), ch, 0});
-            b.g_count.push_back(nt * (int)cg.chunks.size());
-            b.g_slot.push_back(slot);
-            b.flops_per_launch += (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
-            b.leaves_per_launch += N * cg.leaves_per_sample;
-            b.diagram_evals_per_launch += N * cg.n_top;
-            b.groups.push_back(std::move(cg));
+            const std::vector<int32_t> chunks = cg.chunks;
+            const double flops = (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
+            const int64_t leaves = N * cg.leaves_per_sample, evals = N * cg.n_top;
+            push_sequences(b, std::move(cg), b.n_eval, [&](int gi) {
+                b.g_first.push_back((int32_t)b.tasks.size());
+                int nt = (int)((N + b.spt - 1) / b.spt);
+                for (int t = 0; t < nt; ++t)
+                    for (int32_t ch : chunks) b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size(), ch, 0});
+                b.g_count.push_back(nt * (int)chunks.size());
+                b.g_slot.push_back(slot);
+                b.flops_per_launch += flops;
+                b.leaves_per_launch += leaves;
+                b.diagram_evals_per_launch += evals;
+            });
         }
     }
-    if (b.groups.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 (order, n_pts_after) groups in one step");
+    if (b.r_first.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 (order, n_pts_after) groups in one step");
     order_tasks(b);
     return 0;
 }
@@ -844,6 +920,9 @@ std::string plan_key(const qiw_ctx *c, const qiw_run *run, const std::vector<int
     put(&c->tune.chunk_words, 4);
     put(&c->tune.samples_per_task, 4);
     put(&c->tune.chunk_cost, 8);
+    const RandPlan rq = rand_plan(run);
+    const int rk[2] = {rq.n_eval(), rq.scrambled ? 1 : 0};
+    put(rk, 8);
     return k + c->model_fp;
 }
 
@@ -1270,6 +1349,10 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     int rc = 0;
     Plan none, *bp = &none, *sp = &none;
     const bool want_warp = warp_enabled(c->warp);
+    const RandPlan rq = rand_plan(run);
+    const bool rqmc = rq.n_seqs > 1;  // mean and deviation over several sequences
+    if (want_warp && (rqmc || rq.scrambled))
+        return fail(c, QIW_E_ARG, "the learned warp takes one unscrambled sequence (rand_params)");
     if (do_bare && (rc = step_plan(c, run, oall, true, false, bp))) return rc;
     if (last_w >= first_w && (rc = step_plan(c, run, oall, false, want_warp, sp))) return rc;
     Batch &bare_b = bp->b, &bold_b = sp->b;
@@ -1295,8 +1378,12 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
                                  (size_t)std::max(bare_b.groups.size(), bold_b.groups.size()) * m.bt * 8 + 8, &rc);
     int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
     unsigned *done = (unsigned *)dev(c, "fin.done", 4, &rc);
+    double *pstd = rqmc ? (double *)dev(c, "porders_std", (size_t)n_slots * m.ntau * m.bt * 8, &rc) : nullptr;
+    uint32_t *sobm = rq.scrambled ? (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc) : nullptr;
     if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(done, 0, 4, c->stream));
+    if (pstd) HIPCHK(c, hipMemsetAsync(pstd, 0, (size_t)n_slots * m.ntau * m.bt * 8, c->stream));
+    if (sobm) HIPCHK(c, hipMemcpy(sobm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
     std::vector<int32_t> dg;
     for (int s = 0; s < m.ns; ++s)
         for (int i = 0; i < m.dims[s]; ++i) dg.push_back(m.bt_off[s] + i + i * m.dims[s]);
@@ -1319,6 +1406,21 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     fa.scratch = scratch;
     fa.gsum = gsum;
     fa.done = done;
+    fa.n_seqs = rq.n_seqs;
+    fa.target_std = rq.target_std;
+    fa.porders_std = pstd;
+    auto finalize = [&](const Batch &b, const DevBatch &d) -> hipError_t {
+        fa.n_real = (int)b.r_first.size();
+        fa.real_first = d.r_first;
+        fa.real_count = d.r_count;
+        return rqmc ? qiw_launch_finalize_step_rqmc(c->stream, fa) : qiw_launch_finalize_step(c->stream, fa);
+    };
+    // per-step scrambled sequences: coordinates of every group's sequences for this step
+    auto coords = [&](const Batch &b, const DevBatch &d, int domain, int key) -> hipError_t {
+        if (!b.scrambled) return hipSuccess;
+        return qiw_launch_rqmc_coords(c->stream, d.groups, (int)b.groups.size(), 2 * b.max_order, run->N_samples,
+                                      sobm, rq.seed, domain, key, d.ut);
+    };
 
     // every evaluator launch is bracketed by an event pair on the launch stream
     std::vector<hipEvent_t> ev;
@@ -1333,6 +1435,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     int n_known = c->n_known;
     if (!bare_b.groups.empty()) {
         EvalArgs a = make_eval_args(c, bare_b, bare_d, ptab, eshift, dcoef, dkind, partial);
+        HIPCHK(c, coords(bare_b, bare_d, QIW_RQ_BARE, 0));
         HIPCHK(c, mark());
         if ((rc = launch_eval(c, a, (int)bare_b.tasks.size(), bare_b.fmt, bare_b.precision))) return rc;
         HIPCHK(c, mark());
@@ -1347,7 +1450,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         fa.bare = 1;
         fa.normalize = 0;
         fa.n_known = 0;
-        HIPCHK(c, qiw_launch_finalize_step(c->stream, fa));
+        HIPCHK(c, finalize(bare_b, bare_d));
         ++launches;
         if (m.ppgf_kind == QIW_GF_INC_SPLINE) n_known = 2;
         c->stats.flops += bare_b.flops_per_launch;
@@ -1365,6 +1468,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         for (int jw = first_w; jw <= last_w; ++jw) {
             a.step_w = jw;
             if (warped && (rc = warp_points(c, jw * m.h / m.beta, bold_d.ut))) return rc;
+            HIPCHK(c, coords(bold_b, bold_d, QIW_RQ_BOLD, jw));
             HIPCHK(c, mark());
             if ((rc = launch_eval(c, a, (int)bold_b.tasks.size(), bold_b.fmt, bold_b.precision))) return rc;
             HIPCHK(c, mark());
@@ -1374,7 +1478,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
             if (m.ppgf_kind == QIW_GF_INC_SPLINE && normalize) n_known = std::max(n_known, jw + 2);
             fa.n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? n_known : m.ntau;
             fa.normalize = normalize ? 1 : 0;
-            HIPCHK(c, qiw_launch_finalize_step(c->stream, fa));
+            HIPCHK(c, finalize(bold_b, bold_d));
             ++launches;
             c->stats.flops += bold_b.flops_per_launch;
             c->stats.leaves += bold_b.leaves_per_launch;
@@ -1391,6 +1495,8 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     c->stats.n_kernel_launches += launches;
 
     std::vector<double> ptab_h((size_t)m.ntau * m.bt), por((size_t)n_slots * m.ntau * m.bt), lam_h(m.ntau);
+    std::vector<double> pstd_h(pstd ? por.size() : 0);
+    if (pstd) HIPCHK(c, hipMemcpy(pstd_h.data(), pstd, pstd_h.size() * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(ptab_h.data(), ptab, ptab_h.size() * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(por.data(), porders, por.size() * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(lam_h.data(), lam, lam_h.size() * 8, hipMemcpyDeviceToHost));
@@ -1409,7 +1515,10 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
                 for (int e = 0; e < m.bt; ++e) {
                     size_t i = ((size_t)o * m.ntau + j) * m.bt + e;
                     if (P_orders_out) store_complex_from_ip(por[i], P_orders_out + 2 * i);
-                    if (P_orders_std_out) {
+                    if (P_orders_std_out && pstd) {  // real: sqrt of the variance of complex values
+                        P_orders_std_out[2 * i] = pstd_h[i];
+                        P_orders_std_out[2 * i + 1] = 0.0;
+                    } else if (P_orders_std_out) {
                         bool sampled_here = oall[o] > 0 && run->N_samples > 0 &&
                                             ((j == 1 && in_bare && do_bare) || (j >= first_w + 1 && j <= last_w + 1 && in_bold));
                         P_orders_std_out[2 * i] = sampled_here ? nan : 0.0;
@@ -1495,7 +1604,10 @@ extern "C" int qiw_inchworm_step_bare(qiw_ctx *c, const qiw_run *run, double *va
     return QIW_OK;
 }
 
-extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_out) {
+namespace {
+// correlator_2p(expansion, grid, orders, N_samples, RequestStdDev(); rand_params)
+// (src/inchworm.jl:918-1052); corr_std_out may be NULL.
+int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *corr_std_out) {
     if (!c || !corr_out) return QIW_E_ARG;
     int rc = check_run(c, run);
     if (rc) return rc;
@@ -1503,7 +1615,10 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
     c->stats = qiw_stats{};
     auto t0 = std::chrono::steady_clock::now();
     const HostModel &m = c->m;
+    const RandPlan rq = rand_plan(run);
+    const bool rqmc = rq.n_seqs > 1;
     std::memset(corr_out, 0, sizeof(double) * 2 * (size_t)m.ncorr * m.ntau);
+    if (corr_std_out) std::memset(corr_std_out, 0, sizeof(double) * 2 * (size_t)m.ncorr * m.ntau);
     if (m.ncorr == 0 || run->n_orders == 0) return QIW_OK;
     double Z = 0;  // partition_function(P) = sum_s tr iP_s(beta), src/ppgf.jl:611-617
     for (int s = 0; s < m.ns; ++s)
@@ -1528,6 +1643,8 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
             b.fmt = program_format(m, max_order, run->precision);
             b.max_order = max_order;
             b.precision = run->precision;
+            b.n_eval = rq.n_eval();
+            b.scrambled = rq.scrambled;
             for (int oi = 0; oi < run->n_orders; ++oi) {
                 int order = run->orders[oi];
                 if (order > 0 && run->N_samples <= 0) continue;
@@ -1541,9 +1658,10 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
                     if (cg.n_top == 0) continue;
                     if (b.groups.empty() && order == 0) p.first_is_order0 = true;
                     b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
-                    b.groups.push_back(std::move(cg));
+                    push_sequences(b, std::move(cg), b.n_eval, [](int) {});
                 }
             }
+            if (b.r_first.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 (order, n_pts_after) groups");
             if (!b.groups.empty()) {
                 int ng = (int)b.groups.size();
                 b.spt = samples_per_task(c, m, b.fmt, (int64_t)m.ntau * ng * std::max<int64_t>(run->N_samples, 1));
@@ -1582,7 +1700,14 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
         int32_t *drf = p.rf, *drc = p.rcnt;
         double *dout = (double *)dev(c, "corr.out", (size_t)m.ntau * 8, &rc);
+        double *dout_std = rqmc ? (double *)dev(c, "corr.out_std", (size_t)m.ntau * 8, &rc) : nullptr;
+        uint32_t *sobm = b.scrambled ? (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc) : nullptr;
         if (rc) return rc;
+        if (sobm) {  // one set of scrambled sequences for all tau rows of the pair
+            HIPCHK(c, hipMemcpy(sobm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, ng, 2 * b.max_order, run->N_samples, sobm, rq.seed,
+                                             QIW_RQ_CORR, ci, d.ut));
+        }
         hipEvent_t e0, e1;
         HIPCHK(c, hipEventCreate(&e0));
         HIPCHK(c, hipEventCreate(&e1));
@@ -1608,9 +1733,16 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         fa.h = m.h;
         fa.beta = m.beta;
         fa.out = dout;
-        HIPCHK(c, qiw_launch_finalize_corr(c->stream, fa));
-        std::vector<double> out(m.ntau);
+        fa.n_real = (int)b.r_first.size();
+        fa.real_first = d.r_first;
+        fa.real_count = d.r_count;
+        fa.n_seqs = rq.n_seqs;
+        fa.target_std = rq.target_std;
+        fa.out_std = dout_std;
+        HIPCHK(c, rqmc ? qiw_launch_finalize_corr_rqmc(c->stream, fa) : qiw_launch_finalize_corr(c->stream, fa));
+        std::vector<double> out(m.ntau), out_std(m.ntau, 0.0);
         HIPCHK(c, hipMemcpyAsync(out.data(), dout, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
+        if (dout_std) HIPCHK(c, hipMemcpyAsync(out_std.data(), dout_std, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         float ms = 0;
         HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
@@ -1621,13 +1753,35 @@ extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_ou
         c->stats.flops += b.flops_per_launch;
         c->stats.leaves += b.leaves_per_launch;
         c->stats.diagram_evals += b.diagram_evals_per_launch;
+        bool any_sampled = false;
+        for (auto &cg : b.groups) any_sampled = any_sampled || cg.g.sampled;
         for (int j = 0; j < m.ntau; ++j) {  // value = -i * out / Z
-            corr_out[2 * ((size_t)ci * m.ntau + j)] = 0.0;
-            corr_out[2 * ((size_t)ci * m.ntau + j) + 1] = -out[j] / Z;
+            const size_t i = (size_t)ci * m.ntau + j;
+            corr_out[2 * i] = 0.0;
+            corr_out[2 * i + 1] = -out[j] / Z;
+            if (!corr_std_out) continue;
+            if (rqmc) {  // sum of the groups' deviations / Z (real)
+                corr_std_out[2 * i] = out_std[j] / Z;
+                corr_std_out[2 * i + 1] = 0.0;
+            } else {  // one sequence: the deviation of one sample is NaN; tau = 0 is exact
+                const double v = (j > 0 && any_sampled) ? std::nan("") : 0.0;
+                corr_std_out[2 * i] = v;
+                corr_std_out[2 * i + 1] = v;
+            }
         }
     }
     c->stats.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return QIW_OK;
+}
+}  // namespace
+
+extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_out) {
+    return run_correlator(c, run, corr_out, nullptr);
+}
+
+extern "C" int qiw_correlator_2p_std(qiw_ctx *c, const qiw_run *run, double *corr_out, double *corr_std_out) {
+    if (!corr_std_out) return c ? fail(c, QIW_E_ARG, "null corr_std_out") : QIW_E_ARG;
+    return run_correlator(c, run, corr_out, corr_std_out);
 }
 
 extern "C" int qiw_partition_function(qiw_ctx *c, double *Z_out) {
@@ -1714,6 +1868,7 @@ int warp_collect(qiw_ctx *c, const qiw_run *run, const std::vector<int> &steps, 
     qiw_run r = *run;
     r.N_samples = n;
     r.n_orders_bare = 0;
+    r.rand_params = nullptr;  // training points: the staged sequence
     std::vector<int> oall = orders_all(&r);
     Batch b;
     DevBatch d;
@@ -1773,6 +1928,7 @@ int time_bold_step(qiw_ctx *c, const qiw_run *run, double *ms_out) {
     int rc = 0;
     qiw_run r = *run;
     r.n_orders_bare = 0;
+    r.rand_params = nullptr;  // the work mapping is timed on one sequence
     std::vector<int> oall(r.orders, r.orders + r.n_orders);
     Batch b;
     DevBatch d;

@@ -1363,22 +1363,48 @@ __device__ void reduce_rows_block(const int32_t *__restrict__ first, const int32
 // elements over the groups in group order, per order slot in registers.
 constexpr int kFinMaxGroups = 64;
 constexpr int kFinMaxSlots = QIW_KMAX + 1;
+
+// Prefactor of a group's sample sum: the sign (-1)^k of the order, the volume of the
+// integration domain (RootTransform / DoubleSimplexRootTransform Jacobians) and 1/N.
+__device__ double step_group_factor(const QiwGroup &g, const FinalizeArgs &a) {
+    if (!g.sampled) return 1.0;
+    const double tau_i = 0.0, tau_w = a.tau_w, tau_f = a.tau_f;
+    const double J = a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
+                            : simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
+    return ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
+}
+
+// normalize!(P, tau_f) after row jf was stored (src/ppgf.jl:646-668): lambda from the
+// largest diagonal element of row jf, then every known row j scaled by exp(-tau_j lambda)
+// (one exp per row).
+__device__ void step_normalize(const FinalizeArgs &a) {
+    const int tid = threadIdx.x, jf = a.jf;
+    __threadfence_block();
+    __syncthreads();
+    __shared__ double lam_sh;
+    if (tid == 0) {
+        double pmax = -INFINITY;
+        for (int i = 0; i < a.n_diag; ++i) pmax = fmax(pmax, a.ptab[(int64_t)jf * a.bt + a.diag_idx[i]]);
+        lam_sh = log(pmax) / a.tau_f;
+        a.lambda_out[jf] = lam_sh;
+    }
+    __syncthreads();
+    const double lam = lam_sh;
+    for (int j = tid; j < a.n_known; j += blockDim.x) {
+        const double f = exp(-(j * a.h) * lam);
+        double *row = a.ptab + (int64_t)j * a.bt;
+        for (int e = 0; e < a.bt; ++e) row[e] *= f;
+    }
+}
+
 __device__ void finalize_step_body(const FinalizeArgs &a) {
     const volatile double *gsum = a.gsum;
     const int tid = threadIdx.x;
     const int jf = a.jf;
-    const double tau_i = 0.0, tau_w = a.tau_w, tau_f = a.tau_f;
     __shared__ double fac_sh[kFinMaxGroups];
     __shared__ int slot_sh[kFinMaxGroups];
     for (int gi = tid; gi < a.n_groups; gi += blockDim.x) {
-        const QiwGroup g = a.groups[gi];
-        double fac = 1.0;
-        if (g.sampled) {
-            double J = a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
-                              : simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
-            fac = ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
-        }
-        fac_sh[gi] = fac;
+        fac_sh[gi] = step_group_factor(a.groups[gi], a);
         slot_sh[gi] = a.group_order_slot[gi];
     }
     __syncthreads();
@@ -1402,23 +1428,7 @@ __device__ void finalize_step_body(const FinalizeArgs &a) {
         }
         a.ptab[(int64_t)jf * a.bt + e] = tot;
     }
-    if (!a.normalize) return;
-    __threadfence_block();
-    __syncthreads();
-    __shared__ double lam_sh;
-    if (tid == 0) {
-        double pmax = -INFINITY;
-        for (int i = 0; i < a.n_diag; ++i) pmax = fmax(pmax, a.ptab[(int64_t)jf * a.bt + a.diag_idx[i]]);
-        lam_sh = log(pmax) / tau_f;
-        a.lambda_out[jf] = lam_sh;
-    }
-    __syncthreads();
-    const double lam = lam_sh;
-    for (int j = tid; j < a.n_known; j += blockDim.x) {  // one exp per row
-        const double f = exp(-(j * a.h) * lam);
-        double *row = a.ptab + (int64_t)j * a.bt;
-        for (int e = 0; e < a.bt; ++e) row[e] *= f;
-    }
+    if (a.normalize) step_normalize(a);
 }
 
 // One launch per step: every block reduces its group's rows; the last block to finish
@@ -1477,6 +1487,207 @@ __global__ __launch_bounds__(256) void k_finalize_corr(CorrFinalizeArgs a) {
         __syncthreads();
     }
     if (t == 0) a.out[row] = tot;
+}
+
+// ---------------------------------------------------------------------------------------
+// Randomized qMC (mean_std_from_randomization, src/randomization.jl:86-100)
+// ---------------------------------------------------------------------------------------
+
+// Sample statistics of sequences 0..n-1 of one element, x(s) its value for sequence s:
+// the sum in sequence order, the mean as Statistics.var takes it (sum / n) and the
+// corrected variance sum |x - mean|^2 / (n - 1) (src/sector_block_matrix.jl:191-223).
+// Every operation rounds on its own, as in the reference (no fused multiply-adds: a fused
+// x - mean would give equal samples a deviation of the size of their rounding error).
+template <class X>
+__device__ double seq_sum(X x, int n) {
+#pragma clang fp contract(off)
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += x(i);
+    return s;
+}
+template <class X>
+__device__ double seq_std(X x, int n) {
+#pragma clang fp contract(off)
+    const double mean = seq_sum(x, n) / n;
+    double v = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double d = x(i) - mean;
+        v += d * d;
+    }
+    return sqrt(v / (n - 1));
+}
+
+// Simplex coordinates of one step's randomized-qMC sequences; grid: 256-sample blocks x
+// groups.  The block scrambles the staged direction numbers of its group's dimensions
+// with the bits of the group's stream (LMS + digital shift, src/scrambled_sobol.jl:102-140:
+// row p of the lower-triangular matrix holds entries (p, q < p) in bits 31 - q and 1 on
+// the diagonal; bit 31 - p of a scrambled direction number is the parity of row p AND the
+// plain one), then maps its samples as k_simplex_coords does.
+__global__ __launch_bounds__(256) void k_rqmc_coords(const QiwGroup *__restrict__ groups, const uint32_t *__restrict__ m,
+                                                     int64_t n, uint64_t seed, uint64_t domain, uint64_t key,
+                                                     double *__restrict__ ut) {
+    const QiwGroup g = groups[blockIdx.y];
+    if (!g.sampled) return;
+    const int d = g.d_before + g.d_after;
+    __shared__ uint32_t rows[2 * QIW_KMAX][33];
+    __shared__ uint32_t ms[2 * QIW_KMAX * 32];
+    const uint64_t stream = qiw_rq_stream(domain, key, g.order, g.n_after, g.seq);
+    for (int i = threadIdx.x; i < d * 33; i += blockDim.x) {
+        const int k = i / 33, w = i % 33;
+        uint32_t v = qiw_rq_word(seed, stream, k, w);
+        if (w > 0) {
+            const int p = w - 1;
+            const uint32_t below = p == 0 ? 0u : (0xFFFFFFFFu << (32 - p));
+            v = (v & below) | (1u << (31 - p));
+        }
+        rows[k][w] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < d * 32; i += blockDim.x) {
+        const int k = i / 32, j = i % 32;
+        const uint32_t mkj = m[k * 32 + j];
+        uint32_t v = 0;
+        for (int p = 0; p < 32; ++p) v |= (uint32_t)(__popc(rows[k][1 + p] & mkj) & 1) << (31 - p);
+        ms[k * 32 + j] = v;
+    }
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double x[2 * QIW_KMAX], u[2 * QIW_KMAX];
+    for (int k = 0; k < d; ++k) x[k] = ldexp((double)sobol_word(ms, rows[k][0], k, (uint64_t)r), -32);
+    root_map(x, g.d_after, g.d_before, u);
+    double *out = ut + g.ut_off;
+    for (int k = 0; k < d; ++k) out[(int64_t)k * n + r] = u[k];
+}
+
+// Step bookkeeping with randomized qMC (inchworm_step, src/inchworm.jl:172-195), one
+// block of 256 threads after all per-sequence sums are in gsum.  For each (order,
+// n_pts_after) group, the prefactor-scaled sums of its sequences are the samples; the
+// sequences are taken in order until, from the second on, the largest element of their
+// standard deviation is <= target_std.  The group adds the mean (sum * (1/n), as
+// SectorBlockMatrix / n) and the deviation over the sequences taken to its order's rows.
+// Thread t owns elements t, t + 256, ... throughout, so each element's sums run in group
+// order.
+__device__ void finalize_step_rqmc_body(const FinalizeArgs &a) {
+    const volatile double *gsum = a.gsum;
+    const int tid = threadIdx.x, jf = a.jf, bt = a.bt;
+    __shared__ double red[256];
+    for (int o = 0; o < a.n_order_slots; ++o)
+        for (int e = tid; e < bt; e += blockDim.x) {
+            a.porders[((int64_t)o * a.n_tau + jf) * bt + e] = 0.0;
+            a.porders_std[((int64_t)o * a.n_tau + jf) * bt + e] = 0.0;
+        }
+    for (int r = 0; r < a.n_real; ++r) {
+        const int base = a.real_first[r], cnt = a.real_count[r];
+        const QiwGroup g = a.groups[base];
+        const double fac = step_group_factor(g, a);
+        const int slot = a.group_order_slot[base];
+        const int nmax = g.sampled ? a.n_seqs : 1;
+        int nuse = nmax;
+        for (int n = 2; n <= nmax; ++n) {
+            double loc = 0.0;
+            for (int e = tid; e < bt; e += blockDim.x) {
+                auto x = [&](int s) { return __dmul_rn(fac, gsum[(int64_t)(base + (cnt > 1 ? s : 0)) * bt + e]); };
+                loc = fmax(loc, seq_std(x, n));
+            }
+            red[tid] = loc;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+                __syncthreads();
+            }
+            const bool stop = red[0] <= a.target_std;
+            __syncthreads();
+            if (stop) {
+                nuse = n;
+                break;
+            }
+        }
+        for (int e = tid; e < bt; e += blockDim.x) {
+            auto x = [&](int s) { return __dmul_rn(fac, gsum[(int64_t)(base + (cnt > 1 ? s : 0)) * bt + e]); };
+            const double mean = __dmul_rn(seq_sum(x, nuse), 1.0 / nuse);
+            const double sd = !g.sampled ? 0.0 : (nuse > 1 ? seq_std(x, nuse) : __builtin_nan(""));
+            a.porders[((int64_t)slot * a.n_tau + jf) * bt + e] += mean;  // after the group's mean is rounded
+            a.porders_std[((int64_t)slot * a.n_tau + jf) * bt + e] += sd;
+        }
+    }
+    for (int e = tid; e < bt; e += blockDim.x) {
+        double tot = 0.0;
+        for (int o = 0; o < a.n_order_slots; ++o) tot += a.porders[((int64_t)o * a.n_tau + jf) * bt + e];
+        a.ptab[(int64_t)jf * bt + e] = tot;
+    }
+    if (a.normalize) step_normalize(a);
+}
+
+__global__ __launch_bounds__(256) void k_step_reduce_finalize_rqmc(FinalizeArgs a) {
+    reduce_rows_block(a.group_task_first, a.group_task_count, a.partial, a.bt, a.gsum);
+    __threadfence();
+    __syncthreads();
+    __shared__ int last;
+    if (threadIdx.x == 0) last = atomicAdd(a.done, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!last) return;
+    if (threadIdx.x == 0) *a.done = 0u;
+    finalize_step_rqmc_body(a);
+}
+
+// Correlator bookkeeping with randomized qMC (correlator_2p, src/inchworm.jl:857-885):
+// one block per tau row; per group, the prefactor-scaled trace sums of its sequences are
+// the samples (as in k_finalize_corr), reduced to the mean (sum / n) and the deviation
+// with the same stopping rule; the row's value and deviation are their sums over groups.
+__global__ __launch_bounds__(256) void k_finalize_corr_rqmc(CorrFinalizeArgs a) {
+    __shared__ double red[256];
+    __shared__ double xs[QIW_MAX_SEQS];
+    const int row = blockIdx.x, t = threadIdx.x;
+    const int j = row % a.n_tau;
+    const double tau_i = 0.0, tau_w = j * a.h, tau_f = a.beta;
+    double tot = 0.0, tot_std = 0.0;
+    for (int r = 0; r < a.n_real; ++r) {
+        const int base = a.real_first[r], cnt = a.real_count[r];
+        if (a.row_task_count[row * a.n_groups + base] == 0) continue;  // tau_A = tau_B: order 0 only
+        const QiwGroup g = a.groups[base];
+        double fac = 1.0;
+        if (g.sampled) {
+            double J = simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
+            fac = ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
+        }
+        for (int q = 0; q < cnt; ++q) {
+            const int gi = base + q;
+            const int nt = a.row_task_count[row * a.n_groups + gi];
+            const int f0 = a.row_task_first[row * a.n_groups + gi];
+            const int64_t items = (int64_t)nt * a.n_diag;
+            KahanSum s;
+            for (int64_t i = t; i < items; i += 256) {
+                const int rr = (int)(i / a.n_diag), e = (int)(i % a.n_diag);
+                s.add(a.partial[(int64_t)(f0 + rr) * a.bt + a.diag_idx[e]]);
+            }
+            red[t] = s.s - s.c;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (t < w) red[t] += red[t + w];
+                __syncthreads();
+            }
+            if (t == 0) xs[q] = fac * red[0];
+            __syncthreads();
+        }
+        if (t == 0) {
+            auto x = [&](int s) { return xs[cnt > 1 ? s : 0]; };
+            const int nmax = g.sampled ? a.n_seqs : 1;
+            int nuse = nmax;
+            for (int n = 2; n <= nmax; ++n)
+                if (seq_std(x, n) <= a.target_std) {
+                    nuse = n;
+                    break;
+                }
+            tot += seq_sum(x, nuse) / nuse;
+            tot_std += !g.sampled ? 0.0 : (nuse > 1 ? seq_std(x, nuse) : __builtin_nan(""));
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        a.out[row] = tot;
+        a.out_std[row] = tot_std;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1583,5 +1794,26 @@ hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a) {
 
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a) {
     hipLaunchKernelGGL(k_finalize_corr, dim3(a.n_rows), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_finalize_step_rqmc(hipStream_t st, const FinalizeArgs &a) {
+    if (a.n_seqs < 1 || a.n_seqs > QIW_MAX_SEQS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_step_reduce_finalize_rqmc, dim3(a.n_groups, a.bt <= 256 ? 1 : (a.bt + 255) / 256),
+                       dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_finalize_corr_rqmc(hipStream_t st, const CorrFinalizeArgs &a) {
+    if (a.n_seqs < 1 || a.n_seqs > QIW_MAX_SEQS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_finalize_corr_rqmc, dim3(a.n_rows), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_rqmc_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int max_dim, int64_t n,
+                                  const uint32_t *m, uint64_t seed, int domain, int64_t key, double *ut) {
+    if (max_dim > 2 * QIW_KMAX || n_groups < 1 || n < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rqmc_coords, dim3((unsigned)((n + 255) / 256), (unsigned)n_groups), dim3(256), 0, st,
+                       groups, m, n, seed, (uint64_t)domain, (uint64_t)key, ut);
     return hipGetLastError();
 }

@@ -4,7 +4,8 @@ Host-side Python mirror of QInchworm.jl's inchworm!/correlator_2p API over the C
 libqiw.so (include/qiw.h).  Setup (model construction, topologies, Sobol direction numbers)
 happens here; all sampling and diagram evaluation runs in the native library on the GPU.
 """
-from . import abi, diagrammatics, model, sobol  # noqa: F401
+from . import abi, diagrammatics, model, randomization, sobol  # noqa: F401
+from .randomization import RandomizationParams, RequestStdDev  # noqa: F401
 from .inchworm import (autotune, context, correlator_2p, density_matrix, get_tuning, inchworm,  # noqa: F401
                        inchworm_step, inchworm_step_bare, normalize, partition_function, set_tuning, stats)
 from .warp import (group_index, warp_apply, warp_enable, warp_init, warp_loss_grad,  # noqa: F401

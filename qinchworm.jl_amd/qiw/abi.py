@@ -38,10 +38,14 @@ class qiw_model(C.Structure):
                 ("corr_A", C.POINTER(qiw_sbm)), ("corr_B", C.POINTER(qiw_sbm))]
 
 
+class qiw_rand(C.Structure):
+    _fields_ = [("N_seqs", C.c_int32), ("scramble", C.c_int32), ("seed", C.c_uint64), ("target_std", C.c_double)]
+
+
 class qiw_run(C.Structure):
     _fields_ = [("n_orders", C.c_int32), ("orders", _i32p), ("n_orders_bare", C.c_int32),
                 ("orders_bare", _i32p), ("N_samples", C.c_int64), ("n_pts_after_max", C.c_int32),
-                ("precision", C.c_int32)]
+                ("precision", C.c_int32), ("rand_params", C.POINTER(qiw_rand))]
 
 
 class qiw_warp_config(C.Structure):
@@ -114,14 +118,22 @@ class ModelStruct:
 
 
 class RunStruct:
-    def __init__(self, orders, orders_bare, N_samples, n_pts_after_max=0, precision=QIW_PREC_FP64):
+    """qiw_run; rand: None (RandomizationParams()) or (N_seqs, scramble, seed, target_std)."""
+
+    def __init__(self, orders, orders_bare, N_samples, n_pts_after_max=0, precision=QIW_PREC_FP64, rand=None):
         self.orders = np.ascontiguousarray(list(orders), dtype=np.int32)
         self.orders_bare = np.ascontiguousarray(list(orders_bare), dtype=np.int32)
+        self.rand = None
+        if rand is not None:
+            n_seqs, scramble, seed, target_std = rand
+            self.rand = qiw_rand(N_seqs=int(n_seqs), scramble=int(scramble), seed=int(seed) & (2 ** 64 - 1),
+                                 target_std=float(target_std))
         self.struct = qiw_run(n_orders=len(self.orders), orders=ptr(self.orders, C.c_int32),
                               n_orders_bare=len(self.orders_bare),
                               orders_bare=ptr(self.orders_bare, C.c_int32),
                               N_samples=int(N_samples), n_pts_after_max=int(n_pts_after_max or 0),
-                              precision=int(precision))
+                              precision=int(precision),
+                              rand_params=C.pointer(self.rand) if self.rand is not None else None)
 
     def ref(self):
         return C.byref(self.struct)
@@ -130,7 +142,7 @@ class RunStruct:
 # Exported symbols of libqiw (every declaration of include/qiw.h)
 EXPORTS = ["qiw_create", "qiw_destroy", "qiw_last_error", "qiw_abi_version", "qiw_set_model",
            "qiw_set_sobol", "qiw_set_ppgf", "qiw_get_ppgf", "qiw_inchworm", "qiw_inchworm_step",
-           "qiw_inchworm_step_bare", "qiw_correlator_2p", "qiw_partition_function",
+           "qiw_inchworm_step_bare", "qiw_correlator_2p", "qiw_correlator_2p_std", "qiw_partition_function",
            "qiw_density_matrix", "qiw_sobol_points", "qiw_simplex_points", "qiw_get_stats",
            "qiw_warp_init", "qiw_warp_params", "qiw_warp_apply", "qiw_warp_loss_grad", "qiw_warp_train",
            "qiw_warp_enable", "qiw_set_tuning", "qiw_get_tuning", "qiw_autotune"]
@@ -162,6 +174,7 @@ def load_library(path=None):
     lib.qiw_inchworm_step.argtypes = [vp, C.POINTER(qiw_run), C.c_int32, _f64p, _f64p]
     lib.qiw_inchworm_step_bare.argtypes = [vp, C.POINTER(qiw_run), _f64p, _f64p]
     lib.qiw_correlator_2p.argtypes = [vp, C.POINTER(qiw_run), _f64p]
+    lib.qiw_correlator_2p_std.argtypes = [vp, C.POINTER(qiw_run), _f64p, _f64p]
     lib.qiw_partition_function.argtypes = [vp, _f64p]
     lib.qiw_density_matrix.argtypes = [vp, _f64p]
     lib.qiw_sobol_points.argtypes = [vp, C.c_int32, C.c_int64, C.c_int64, _f64p]

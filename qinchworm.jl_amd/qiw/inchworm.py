@@ -1,10 +1,12 @@
 """Host-side mirror of the reference's drivers (src/inchworm.jl) over the libqiw C ABI.
 
-    inchworm(expansion, grid, orders, orders_bare, N_samples; n_pts_after_max)
+    inchworm(expansion, grid, orders, orders_bare, N_samples; n_pts_after_max, rand_params)
         -> (P_orders, P_orders_std), updates expansion.P          (inchworm!, :332-498)
     inchworm_step(expansion, n_w, orders, N_samples)            (inchworm_step, :123-203)
     inchworm_step_bare(expansion, orders_bare, N_samples)       (inchworm_step_bare, :228-304)
     correlator_2p(expansion, grid, orders, N_samples)           (correlator_2p, :1075-1084)
+    correlator_2p(expansion, grid, orders, N_samples, RequestStdDev(); rand_params)
+        -> (correlators, standard deviations)                    (correlator_2p, :918-1052)
     partition_function(expansion), density_matrix(expansion)    (src/ppgf.jl:611-617, :729-733)
 
 All evaluation runs in libqiw on the GPU; there is no CPU fallback.
@@ -14,6 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
+from .randomization import RandomizationParams, RequestStdDev
 from .sobol import ScrambledSobolSeq
 
 _ctx = None
@@ -62,15 +65,24 @@ def stats(ctx=None):
     return {f: getattr(st, f) for f, _ in abi.qiw_stats._fields_}
 
 
+def _rand(rand_params):
+    if rand_params is None:
+        return None
+    if rand_params.N_seqs <= 0:
+        raise ValueError("rand_params.N_seqs must be > 0")  # @assert, src/inchworm.jl:344
+    return rand_params.struct_args()
+
+
 def inchworm(expansion, grid, orders, orders_bare, N_samples, n_pts_after_max=None,
-             precision=abi.QIW_PREC_FP64, seq=None, ctx=None):
+             precision=abi.QIW_PREC_FP64, seq=None, ctx=None, rand_params=None):
     """inchworm! (src/inchworm.jl:332-498): writes the bold propagator into expansion.P and
     returns the order-resolved contributions and their standard deviations as
-    {order: array (n_tau, sum m^2)}."""
+    {order: array (n_tau, sum m^2)}.  rand_params: RandomizationParams (default: one
+    unscrambled sequence; seq, if given, is the staged sequence)."""
     ctx = ctx or context()
     orders, orders_bare = list(orders), list(orders_bare)
     _stage(ctx, expansion, max(orders + orders_bare + [0]), seq)
-    run = abi.RunStruct(orders, orders_bare, N_samples, n_pts_after_max or 0, precision)
+    run = abi.RunStruct(orders, orders_bare, N_samples, n_pts_after_max or 0, precision, rand=_rand(rand_params))
     oall = sorted(set(orders) | set(orders_bare))
     nt, bt = grid.n_tau, expansion.bt_size
     Po = np.zeros((len(oall), nt, bt), np.complex128)
@@ -83,14 +95,14 @@ def inchworm(expansion, grid, orders, orders_bare, N_samples, n_pts_after_max=No
 
 
 def inchworm_step(expansion, n_w, orders, N_samples, n_pts_after_max=None, seq=None, ctx=None,
-                  precision=abi.QIW_PREC_FP64):
+                  precision=abi.QIW_PREC_FP64, rand_params=None):
     """One regular step (src/inchworm.jl:123-203) with tau_i = grid[0], tau_w = grid[n_w],
     tau_f = grid[n_w + 1] on the current expansion.P.  Returns (value, per-order)."""
     ctx = ctx or context()
     orders = list(orders)
     _stage(ctx, expansion, max(orders + [0]), seq)
     _set_P(ctx, expansion)
-    run = abi.RunStruct(orders, [], N_samples, n_pts_after_max or 0, precision)
+    run = abi.RunStruct(orders, [], N_samples, n_pts_after_max or 0, precision, rand=_rand(rand_params))
     val = np.zeros(expansion.bt_size, np.complex128)
     per = np.zeros((len(orders), expansion.bt_size), np.complex128)
     ctx.check(ctx.lib.qiw_inchworm_step(ctx.h, run.ref(), int(n_w), val.ctypes.data_as(abi._f64p),
@@ -98,12 +110,12 @@ def inchworm_step(expansion, n_w, orders, N_samples, n_pts_after_max=None, seq=N
     return val, per
 
 
-def inchworm_step_bare(expansion, orders_bare, N_samples, seq=None, ctx=None):
+def inchworm_step_bare(expansion, orders_bare, N_samples, seq=None, ctx=None, rand_params=None):
     """The initial step (src/inchworm.jl:228-304) on [grid[0], grid[1]]."""
     ctx = ctx or context()
     orders_bare = list(orders_bare)
     _stage(ctx, expansion, max(orders_bare + [0]), seq)
-    run = abi.RunStruct([], orders_bare, N_samples)
+    run = abi.RunStruct([], orders_bare, N_samples, rand=_rand(rand_params))
     val = np.zeros(expansion.bt_size, np.complex128)
     per = np.zeros((len(orders_bare), expansion.bt_size), np.complex128)
     ctx.check(ctx.lib.qiw_inchworm_step_bare(ctx.h, run.ref(), val.ctypes.data_as(abi._f64p),
@@ -111,17 +123,26 @@ def inchworm_step_bare(expansion, orders_bare, N_samples, seq=None, ctx=None):
     return val, per
 
 
-def correlator_2p(expansion, grid, orders, N_samples, seq=None, ctx=None, precision=abi.QIW_PREC_FP64):
+def correlator_2p(expansion, grid, orders, N_samples, *flag, seq=None, ctx=None, precision=abi.QIW_PREC_FP64,
+                  rand_params=None):
     """correlator_2p (src/inchworm.jl:1075-1084): one array of n_tau complex values
-    <A(tau) B(0)> per (A, B) pair of expansion.corr_operators."""
+    <A(tau) B(0)> per (A, B) pair of expansion.corr_operators.  With RequestStdDev() as
+    fifth argument (src/inchworm.jl:918-1052): (values, standard deviations)."""
+    if len(flag) > 1 or (flag and not isinstance(flag[0], RequestStdDev)):
+        raise TypeError("the only flag argument of correlator_2p is RequestStdDev()")
     ctx = ctx or context()
     orders = list(orders)
     _stage(ctx, expansion, max(orders + [0]), seq)
     _set_P(ctx, expansion)
-    run = abi.RunStruct(orders, [], N_samples, 0, precision)
+    run = abi.RunStruct(orders, [], N_samples, 0, precision, rand=_rand(rand_params))
     out = np.zeros((len(expansion.corr_operators), grid.n_tau), np.complex128)
-    ctx.check(ctx.lib.qiw_correlator_2p(ctx.h, run.ref(), out.ctypes.data_as(abi._f64p)))
-    return [out[i] for i in range(out.shape[0])]
+    if not flag:
+        ctx.check(ctx.lib.qiw_correlator_2p(ctx.h, run.ref(), out.ctypes.data_as(abi._f64p)))
+        return [out[i] for i in range(out.shape[0])]
+    std = np.zeros_like(out)
+    ctx.check(ctx.lib.qiw_correlator_2p_std(ctx.h, run.ref(), out.ctypes.data_as(abi._f64p),
+                                            std.ctypes.data_as(abi._f64p)))
+    return [out[i] for i in range(out.shape[0])], [std[i] for i in range(std.shape[0])]
 
 
 def partition_function(expansion):

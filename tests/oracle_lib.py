@@ -38,6 +38,11 @@ def lib():
                                              f64p, f64p]
         L.orc_correlator_2p.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p,
                                         C.c_int, f64p, f64p, C.c_int, C.c_int, f64p]
+        L.orc_correlator_2p_std.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p,
+                                            C.c_int, f64p, f64p, C.c_int, C.c_int, f64p, f64p]
+        L.orc_rq_scramble.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u32p, u32p, u32p]
+        L.orc_rq_stream.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int]
+        L.orc_rq_stream.restype = C.c_uint64
         L.orc_eval_topologies.argtypes = [C.POINTER(abi.qiw_model), C.c_int, C.c_int, C.c_int, C.c_double,
                                           C.c_double, C.c_double, C.c_int, i32p, i32p, f64p, f64p]
         L.orc_gf_eval.argtypes = [C.c_int, C.c_int, C.c_double, f64p, C.c_int, f64p, C.c_int, C.c_int,
@@ -76,9 +81,9 @@ def sobol_tables(max_dim):
     return m, np.zeros(max_dim, dtype=np.uint32)
 
 
-def inchworm(exp, orders, orders_bare, N, n_pts_after_max=0, n_steps_max=-1, sobol=None):
+def inchworm(exp, orders, orders_bare, N, n_pts_after_max=0, n_steps_max=-1, sobol=None, rand=None):
     ms = exp.to_struct()
-    run = abi.RunStruct(orders, orders_bare, N, n_pts_after_max)
+    run = abi.RunStruct(orders, orders_bare, N, n_pts_after_max, rand=rand)
     max_order = max(list(orders) + list(orders_bare) + [1])
     m, x0 = sobol if sobol is not None else sobol_tables(2 * max_order)
     n_tau, bt = exp.grid.n_tau, exp.bt_size
@@ -91,9 +96,9 @@ def inchworm(exp, orders, orders_bare, N, n_pts_after_max=0, n_steps_max=-1, sob
     return P, {o: Po[i] for i, o in enumerate(oall)}, {o: Ps[i] for i, o in enumerate(oall)}
 
 
-def inchworm_step(exp, orders, N, jw, P=None, n_known=None, der0=None, sobol=None, n_pts_after_max=0):
+def inchworm_step(exp, orders, N, jw, P=None, n_known=None, der0=None, sobol=None, n_pts_after_max=0, rand=None):
     ms = exp.to_struct()
-    run = abi.RunStruct(orders, [], N, n_pts_after_max)
+    run = abi.RunStruct(orders, [], N, n_pts_after_max, rand=rand)
     m, x0 = sobol if sobol is not None else sobol_tables(2 * max(list(orders) + [1]))
     P = np.ascontiguousarray(exp.P if P is None else P, dtype=np.complex128)
     der0 = np.ascontiguousarray(exp.P_der0 if der0 is None else der0, dtype=np.complex128)
@@ -117,15 +122,31 @@ def inchworm_step_bare(exp, orders_bare, N, sobol=None):
     return val, per
 
 
-def correlator_2p(exp, orders, N, P=None, n_known=None, der0=None, tau_first=0, tau_count=-1, sobol=None):
+def correlator_2p(exp, orders, N, P=None, n_known=None, der0=None, tau_first=0, tau_count=-1, sobol=None,
+                  rand=None, with_std=False):
     ms = exp.to_struct()
-    run = abi.RunStruct(orders, [], N)
+    run = abi.RunStruct(orders, [], N, rand=rand)
     m, x0 = sobol if sobol is not None else sobol_tables(2 * max(list(orders) + [1]))
     P = np.ascontiguousarray(exp.P if P is None else P, dtype=np.complex128)
     der0 = np.ascontiguousarray(exp.P_der0 if der0 is None else der0, dtype=np.complex128)
     n_known = exp.P_n_known if n_known is None else n_known
     out = np.zeros((len(exp.corr_operators), exp.grid.n_tau), np.complex128)
-    check(lib().orc_correlator_2p(ms.ref(), run.ref(), _p(m, C.c_uint32), _p(x0, C.c_uint32), int(n_known),
-                                  _p(P, C.c_double), _p(der0, C.c_double), int(tau_first), int(tau_count),
-                                  _p(out, C.c_double)))
-    return out
+    std = np.zeros_like(out)
+    check(lib().orc_correlator_2p_std(ms.ref(), run.ref(), _p(m, C.c_uint32), _p(x0, C.c_uint32), int(n_known),
+                                      _p(P, C.c_double), _p(der0, C.c_double), int(tau_first), int(tau_count),
+                                      _p(out, C.c_double), _p(std, C.c_double)))
+    return (out, std) if with_std else out
+
+
+def rq_scramble(seed, stream, m):
+    """Direction numbers m [D][32] scrambled with the bits of (seed, stream), and the shift."""
+    m = np.ascontiguousarray(m, dtype=np.uint32)
+    D = m.shape[0]
+    mo = np.zeros_like(m)
+    xo = np.zeros(D, np.uint32)
+    lib().orc_rq_scramble(int(seed), int(stream), D, _p(m, C.c_uint32), _p(mo, C.c_uint32), _p(xo, C.c_uint32))
+    return mo, xo
+
+
+def rq_stream(domain, key, order, n_after, seq):
+    return int(lib().orc_rq_stream(int(domain), int(key), int(order), int(n_after), int(seq)))
