@@ -203,6 +203,41 @@ def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10
             "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
 
 
+def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678):
+    """The reference's randomized-qMC study (bench/rqmc/rqmc.jl:37-117) on one GPU: the
+    Bethe single-level model, inchworm! orders 0:order (bare and bold) and then
+    correlator_2p(RequestStdDev) orders 0:order-1, N samples per sequence, N_seqs scrambled
+    sequences (target_std 0: all of them).  Reports the wall time of both calls, diagram
+    evaluations per second, and the largest deviation of G from the exact lattice value
+    beside the largest estimated standard deviation of G."""
+    import numpy as np
+    import qiw
+    from qiw import model
+    N = 2 ** log2n
+
+    def run():
+        exp = model.bethe_single_level(n_tau)
+        rp = qiw.RandomizationParams(seed, n_seqs, 0.0)
+        t0 = time.perf_counter()
+        _, Ps = qiw.inchworm(exp, exp.grid, range(0, order + 1), range(0, order + 1), N, rand_params=rp)
+        s1 = qiw.stats()
+        g, gs = qiw.correlator_2p(exp, exp.grid, range(0, order), N, qiw.RequestStdDev(), rand_params=rp)
+        s2 = qiw.stats()
+        return exp, Ps, g, gs, time.perf_counter() - t0, s1, s2
+
+    run()  # plans and buffers
+    exp, Ps, g, gs, dt, s1, s2 = run()
+    evals = s1["diagram_evals"] + s2["diagram_evals"]
+    return {"workload": f"bench/rqmc study: Bethe single level, beta=10, n_tau={n_tau}, inchworm! orders 0:{order}, "
+                        f"correlator_2p(RequestStdDev) orders 0:{order - 1}, N=2^{log2n} per sequence, "
+                        f"N_seqs={n_seqs} scrambled, float64",
+            "wall_s": dt, "diagram_evals_per_s": evals / dt,
+            "evaluator_ms": s1["kernel_ms"] + s2["kernel_ms"],
+            "max_P_std": float(max(np.abs(v).max() for v in Ps.values())),
+            "max_G_std": float(np.abs(gs[0]).max()),
+            "max_G_error_vs_exact": float(np.abs(g[0] - exp.corr_exact).max())}
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: 2 x mib MiB of HBM traffic per launch."""
     import torch
@@ -222,6 +257,7 @@ def main():
     ap.add_argument("--log2n", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the two- and three-orbital step measurements")
+    ap.add_argument("--no-rqmc", action="store_true", help="skip the randomized-qMC study line")
     ap.add_argument("--warp-study", action="store_true",
                     help="only run the learned-warp error-versus-samples study and print its JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -289,6 +325,8 @@ def main():
     if not args.no_c2:
         line["two_orbital_step"] = two_orbital(with_cpu=not args.no_cpu)
         line["three_orbital_step"] = three_orbital(with_cpu=not args.no_cpu)
+    if not args.no_rqmc:
+        line["rqmc_study"] = rqmc_study()
     print(json.dumps(line), flush=True)
 
 

@@ -14,7 +14,7 @@ using KeldyshED
 using QInchworm.expansion: Expansion
 using QInchworm.spline_gf: SplineInterpolatedGF, IncSplineImaginaryTimeGF
 using QInchworm.scrambled_sobol: ScrambledSobolSeq
-using QInchworm.randomization: RandomizationParams
+using QInchworm.randomization: RandomizationParams, RequestStdDev
 
 const libqiw = get(ENV, "QIW_LIBRARY", joinpath(@__DIR__, "..", "lib", "libqiw.so"))
 
@@ -49,6 +49,19 @@ struct QiwModel
     corr_B::Ptr{QiwSbm}
 end
 
+# Randomized qMC (qiw_rand): the scramble bits of every sequence come from libqiw's
+# counter-based generator keyed by `seed`; one seed is drawn from rand_params.rng per call.
+struct QiwRand
+    N_seqs::Int32
+    scramble::Int32
+    seed::UInt64
+    target_std::Float64
+end
+
+QiwRand(rp::RandomizationParams) =
+    QiwRand(Int32(rp.N_seqs), rp.rng === nothing ? Int32(0) : Int32(1),
+            rp.rng === nothing ? UInt64(0) : rand(rp.rng, UInt64), rp.target_std)
+
 struct QiwRun
     n_orders::Int32
     orders::Ptr{Int32}
@@ -57,6 +70,7 @@ struct QiwRun
     N_samples::Int64
     n_pts_after_max::Int32
     precision::Int32
+    rand_params::Ptr{QiwRand}   # C_NULL: RandomizationParams()
 end
 
 # Learned warp of the unit cube (qiw_warp_config); no counterpart in the reference
@@ -185,16 +199,20 @@ function store_ppgf!(exp::Expansion, T, grid)
 end
 
 """
-    inchworm!(expansion, grid, orders, orders_bare, N_samples; n_pts_after_max)
+    inchworm!(expansion, grid, orders, orders_bare, N_samples; n_pts_after_max, rand_params)
 
 Drop-in for QInchworm.inchworm! on one MI355X.  Returns (P_orders, P_orders_std) as
 Dict{Int, Matrix{ComplexF64}} in BT layout (sum m^2 x n_tau) and writes expansion.P.
+With rand_params.N_seqs > 1 every group is integrated with up to N_seqs scrambled
+sequences (one seed drawn from rand_params.rng per call) and P_orders_std holds the
+standard deviations.
 """
 function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_bare, N_samples::Int64;
                    n_pts_after_max::Int64 = typemax(Int64), rand_params::RandomizationParams = RandomizationParams(),
                    precision::Int32 = QIW_PREC_FP64, ctx::Context = context())
     @assert N_samples == 0 || ispow2(N_samples)
-    @assert rand_params.N_seqs == 1 && rand_params.rng === nothing "randomized qMC is not offloaded"
+    @assert rand_params.N_seqs > 0
+    rq = Ref(QiwRand(rand_params))
     keep = Any[]
     orders_v = Int32.(collect(orders)); bare_v = Int32.(collect(orders_bare))
     stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, bare_v, Int32[0]))), keep)
@@ -204,9 +222,10 @@ function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_ba
     P = zeros(ComplexF64, bt, n_tau)
     Po = zeros(ComplexF64, bt, n_tau, length(oall))
     Ps = zeros(ComplexF64, bt, n_tau, length(oall))
-    run = Ref(QiwRun(length(orders_v), pointer(orders_v), length(bare_v), pointer(bare_v), N_samples,
-                     n_pts_after_max == typemax(Int64) ? 0 : Int32(n_pts_after_max), precision))
-    GC.@preserve keep orders_v bare_v begin
+    GC.@preserve keep orders_v bare_v rq begin
+        run = Ref(QiwRun(length(orders_v), pointer(orders_v), length(bare_v), pointer(bare_v), N_samples,
+                         n_pts_after_max == typemax(Int64) ? 0 : Int32(n_pts_after_max), precision,
+                         Base.unsafe_convert(Ptr{QiwRand}, rq)))
         check(ctx, ccall((:qiw_inchworm, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                          ctx.h, run, P, Po, Ps))
@@ -217,14 +236,28 @@ function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_ba
 end
 
 """
-    correlator_2p(expansion, grid, orders, N_samples)
+    correlator_2p(expansion, grid, orders, N_samples; rand_params)
+    correlator_2p(expansion, grid, orders, N_samples, RequestStdDev(); rand_params)
 
-Drop-in for QInchworm.correlator_2p on one MI355X: one Vector{ComplexF64} of n_tau values
-<A(tau) B(0)> per pair in expansion.corr_operators.
+Drop-ins for QInchworm.correlator_2p on one MI355X (src/inchworm.jl:1075-1084 and
+:918-1052): one Vector{ComplexF64} of n_tau values <A(tau) B(0)> per pair in
+expansion.corr_operators; the RequestStdDev method also returns their standard deviations.
 """
 function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64;
-                       ctx::Context = context())
+                       rand_params::RandomizationParams = RandomizationParams(), ctx::Context = context())
+    return correlator_2p_impl(exp, grid, orders, N_samples, rand_params, false, ctx)[1]
+end
+
+function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64, ::RequestStdDev;
+                       rand_params::RandomizationParams = RandomizationParams(), ctx::Context = context())
+    return correlator_2p_impl(exp, grid, orders, N_samples, rand_params, true, ctx)
+end
+
+function correlator_2p_impl(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64,
+                            rand_params::RandomizationParams, with_std::Bool, ctx::Context)
     @assert N_samples == 0 || ispow2(N_samples)
+    @assert rand_params.N_seqs > 0
+    rq = Ref(QiwRand(rand_params))
     keep = Any[]
     orders_v = Int32.(collect(orders))
     stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, Int32[0]))), keep)
@@ -234,12 +267,14 @@ function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_sam
     check(ctx, ccall((:qiw_set_ppgf, libqiw), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}),
                      ctx.h, n_tau, T, der0))
     out = zeros(ComplexF64, n_tau, length(exp.corr_operators))
-    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, 0))
-    GC.@preserve keep orders_v begin
-        check(ctx, ccall((:qiw_correlator_2p, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}),
-                         ctx.h, run, out))
+    out_std = zeros(ComplexF64, n_tau, length(exp.corr_operators))
+    GC.@preserve keep orders_v rq begin
+        run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, 0,
+                         Base.unsafe_convert(Ptr{QiwRand}, rq)))
+        check(ctx, ccall((:qiw_correlator_2p_std, libqiw), Cint,
+                         (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}), ctx.h, run, out, out_std))
     end
-    return [out[:, c] for c in 1:size(out, 2)]
+    return ([out[:, c] for c in 1:size(out, 2)], [out_std[:, c] for c in 1:size(out_std, 2)])
 end
 
 """
@@ -262,7 +297,7 @@ function train_warp!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders; config:
     cfg = Ref(config)
     check(ctx, ccall((:qiw_warp_init, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwWarpConfig}), ctx.h, cfg))
     loss = zeros(Float64, max(config.iters, 1))
-    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, 0, 0, QIW_PREC_FP64))
+    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, 0, 0, QIW_PREC_FP64, C_NULL))
     GC.@preserve keep orders_v begin
         check(ctx, ccall((:qiw_warp_train, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwRun}, Ptr{Float64}),
                          ctx.h, run, loss))
@@ -306,7 +341,7 @@ function autotune!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples
     check(ctx, ccall((:qiw_set_ppgf, libqiw), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}),
                      ctx.h, n_tau, T, der0))
     out = Ref(QiwTuning())
-    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, precision))
+    run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, precision, C_NULL))
     GC.@preserve keep orders_v begin
         check(ctx, ccall((:qiw_autotune, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwRun}, Ref{QiwTuning}),
                          ctx.h, run, out))

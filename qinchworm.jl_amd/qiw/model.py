@@ -321,6 +321,27 @@ def kanamori(n_orb, U, J, eps_shift, beta, n_tau, V, D=2.0, interpolate=False):
     return Expansion(ed, grid, pairs, interpolate_ppgf=interpolate)
 
 
+def bethe_single_level(n_tau, beta=10.0, t_bethe=2.0, mu=0.0):
+    """The model of the reference's randomized-qMC study (bench/rqmc/rqmc.jl:37-63): one
+    spinless level, H = -mu n (mu = 0), semi-elliptic bath kd.bethe_dos(t = t_bethe / 2)
+    with V = t_bethe / 2, Delta = V^2 G_bethe on the grid, pairs c^dag/c and their
+    particle-hole conjugates, correlator (c, c^dag).  The exact impurity Green's function
+    is the lattice one, G = -Delta / V^2 (the study's gf_ref), so the correlator's exact
+    value is corr_exact = Delta / V^2."""
+    fock = FockSpace(["0"])
+    H = -mu * fock.n("0")
+    ed = EDCore(H, fock, lambda b: fock.occupation(b, "0"))
+    grid = ImaginaryTimeGrid(beta, n_tau)
+    V = 0.5 * t_bethe
+    delta = V ** 2 * bethe_gf(t_bethe / 2.0, beta, grid)
+    pairs = [InteractionPair(fock.c_dag("0"), fock.c("0"), delta),
+             InteractionPair(fock.c("0"), fock.c_dag("0"), ph_conj(delta))]
+    exp = Expansion(ed, grid, pairs, [(fock.c("0"), fock.c_dag("0"))])
+    exp.fock = fock
+    exp.corr_exact = delta.values / V ** 2
+    return exp
+
+
 def single_state_golden_model(sector_order=("occupied", "empty")):
     """The model of test/inchworm.jl:53-67, :180-212: H = -mu n (mu = 0.1), one bath level
     eps = 0.1 (kd.DeltaDOS) with V = -0.1, beta = 10, n_tau = 20, cubic-spline Delta and

@@ -140,3 +140,11 @@ def test_rand_params_rejected():
     run = abi.RunStruct([0, 1], [], 2 ** 4, rand=(2, 1, 1, 0.0))
     out = np.zeros(2 * 8 * 4)
     assert ctx.lib.qiw_correlator_2p_std(ctx.h, run.ref(), out.ctypes.data_as(abi._f64p), None) == abi.QIW_E_ARG
+    # the learned warp takes one unscrambled sequence
+    qiw.warp_init(seed=5)
+    qiw.warp_enable(True)
+    try:
+        run = abi.RunStruct([0, 1], [0, 1], 2 ** 4, rand=(2, 1, 1, 0.0))
+        assert ctx.lib.qiw_inchworm(ctx.h, run.ref(), None, None, None) == abi.QIW_E_ARG
+    finally:
+        qiw.warp_enable(False)
