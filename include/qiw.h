@@ -117,7 +117,8 @@ typedef struct {
  *                 0x9E3779B97F4A7C15, stream = (domain << 60) | (key << 32) |
  *                 (order << 24) | (n_pts_after << 16) | s, domain 1 for the initial
  *                 step (key 0), 2 for the bold step at tau_w = grid[key], 3 for the
- *                 correlator of pair `key` (one set of sequences for all its tau rows).
+ *                 correlator of pair `key` (one set of sequences for all its tau rows),
+ *                 4 for the differential step at tau_f_prev = grid[key] (n_pts_after 1).
  *                 Word 0 is the shift x (bit j = shift bit j); word 1 + p holds row p
  *                 of the lower-triangular matrix: its bits 31..32-p are the entries
  *                 (p, 0..p-1), the diagonal is 1.
@@ -209,6 +210,23 @@ int qiw_correlator_2p(qiw_ctx *ctx, const qiw_run *run, double *corr_out);
  * [n_corr][n_tau] complex128, the sum over the groups of each row's standard deviation
  * over the sequences, divided by Z (NaN with N_seqs = 1, 0 at tau = 0). */
 int qiw_correlator_2p_std(qiw_ctx *ctx, const qiw_run *run, double *corr_out, double *corr_std_out);
+
+/* qiw_diff_inchworm: diff_inchworm!(expansion, grid, orders, N_samples; rand_params)
+ * (src/inchworm.jl:666-778), the differential formulation: n_tau - 1 steps from the staged
+ * atomic propagator, each evaluating the self-energy diagrams of run->orders (order 1
+ * exact, higher orders by qMC over the 2k - 2 inner times, src/inchworm.jl:528-601) at
+ * (tau_f_prev, tau_i), then P(tau_f) = P(tau_f_prev) + h (int Σ P - H P(tau_f_prev)) with
+ * the trapezoid rule and H = E + lambda (:603-635).  run->orders_bare is ignored.
+ * Outputs (caller-owned, may be NULL):
+ *   P_out                [n_tau][sum m^2]   expansion.P after the run
+ *   Sigma_orders_out     [n_orders][n_tau][sum m^2] per-order self-energy, orders as in
+ *                        run->orders, stored at tau_f of each step as the reference does
+ *   Sigma_orders_std_out same shape: standard deviations (real part); NaN for orders >= 2
+ *                        with N_seqs = 1, 0 for order 1.
+ * The self-energy container follows the propagator's kind: with QIW_GF_INC_SPLINE it is
+ * incremental and step a's value lands one grid row later, as in the reference. */
+int qiw_diff_inchworm(qiw_ctx *ctx, const qiw_run *run, double *P_out, double *Sigma_orders_out,
+                      double *Sigma_orders_std_out);
 
 /* ---- observables (src/ppgf.jl:611-636, :729-733) --------------------------------------
  * partition_function(P) = sum_s tr iP_s(beta); density_matrix after normalize!(P, beta):

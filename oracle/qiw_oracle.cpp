@@ -21,6 +21,7 @@
 //   src/topology_eval.jl     TopologyEvaluator and _traverse_configuration_tree!
 //   src/inchworm.jl          inchworm_step, inchworm_step_bare, inchworm!, correlator_2p
 //   src/randomization.jl     mean_std_from_randomization               (mean_std)
+//   src/inchworm.jl          diff_inchworm_step!, diff_inchworm!      (diff_step, orc_diff_inchworm)
 //
 // Parity of each part is pinned by the tests in tests/test_oracle_*.py (golden vectors of
 // the reference's test suite, SciPy, closed forms).  Grid-stored Green's functions are
@@ -1034,8 +1035,8 @@ static void mean_std(F f, const RandParams &rq, bool recip_mean, std::vector<cd>
 static std::vector<CMat> integrate_group(const Model &m, const PPGF *P, bool bold, int order,
                                          const std::map<int, FixedNode> &fixed, const Transform &tr,
                                          const std::vector<Topology> &tops, int64_t N, SobolSource sob,
-                                         int64_t sample_first = 0, int64_t sample_count = -1) {
-    int d = 2 * order;
+                                         int64_t sample_first = 0, int64_t sample_count = -1, int n_var = -1) {
+    int d = n_var >= 0 ? n_var : 2 * order;  // variable times (self-energy diagrams: 2k - 2)
     if (sample_count < 0) sample_count = N;
     double J = tr.jacobian();
     cd dir = 1.0;
@@ -1151,7 +1152,7 @@ struct StepResult {
     std::map<int, std::vector<CMat>> per_order, per_order_std;
 };
 
-enum { RQ_BARE = 1, RQ_BOLD = 2, RQ_CORR = 3 };  // stream domains of the scramble bits
+enum { RQ_BARE = 1, RQ_BOLD = 2, RQ_CORR = 3, RQ_DIFF = 4 };  // stream domains of the scramble bits
 
 static StepResult inchworm_step(const Model &m, const PPGF &P, int jw, int jf, const std::vector<TopData> &td,
                                 int64_t N, SobolSource sob, const RandParams &rq) {  // :123-203
@@ -1512,6 +1513,141 @@ extern "C" int orc_correlator_2p(const qiw_model *mi, const qiw_run *run, const 
                                  const double *der0, int tau_first, int tau_count, double *corr_out) {
     return orc_correlator_2p_std(mi, run, sob_m, sob_shift, n_known, P_tab, der0, tau_first, tau_count, corr_out,
                                  nullptr);
+}
+
+// ------------------------------------------------------------------------------------
+// Differential inchworm -- src/inchworm.jl:528-778
+// ------------------------------------------------------------------------------------
+static std::vector<TopData> diff_top_data(const qiw_run *run) {  // :711-727
+    std::vector<TopData> td;
+    for (int oi = 0; oi < run->n_orders; ++oi) {
+        const int order = run->orders[oi];
+        if (order == 0) continue;  // no 0-th order self-energy
+        auto tops = topologies_at_order(order, 1, false);
+        if (!tops.empty()) td.push_back({order, 1, tops});
+    }
+    return td;
+}
+
+// The self-energy container Σ = zero(expansion.P).  set_ppgf! on a grid-stored Σ writes
+// the row of tau_f_prev; on an IncSplineImaginaryTimeGF it appends (extend!, src/spline_gf.jl
+// :392-402), so the k-th value lands on row k: one row later than tau_f_prev.  Σ(t, t')
+// of a grid point pair reads the row of t - t' (time-translation-invariant storage).
+struct SigmaTable {
+    std::vector<std::vector<CMat>> tab;  // [n_tau][ns]
+    int len = 1;                         // values held by the incremental container
+    void store(const Model &m, int a, const std::vector<CMat> &v) {
+        if (m.ppgf_kind == QIW_GF_INC_SPLINE) tab[len++] = v;
+        else tab[a] = v;
+    }
+};
+
+// One differential step (diff_inchworm_step!, :528-636) from tau_f_prev = grid[a] to
+// grid[a + 1]: the self-energy at (tau_f_prev, tau_i) order by order, then
+// P(tau_f) = P_prev + h (-i sum_j w_j Σ(tau_f_prev - tau_j) P(tau_j) - H P_prev) with the
+// trapezoid weights w of kd.integrate over grid[0..a] and H = diag(E + lambda).  kd.integrate
+// integrates over contour time, dt = -i dtau on the imaginary branch; the reference's golden
+// /diff_inchworm/P pins both that factor and the row shift of the incremental Σ container
+// (without either, the run misses it by 1e-3 or more).
+static std::vector<CMat> diff_step(const Model &m, const PPGF &P, SigmaTable &S, int a, const std::vector<TopData> &td,
+                                   int64_t N, SobolSource sob, const RandParams &rq,
+                                   std::map<int, std::vector<CMat>> &per_order,
+                                   std::map<int, std::vector<CMat>> &per_order_std) {
+    const double t_i = 0.0, t_fp = a * m.h;
+    per_order.clear();
+    per_order_std.clear();
+    for (auto &d : td) {
+        per_order[d.order] = bt_zero(m);
+        per_order_std[d.order] = bt_zero(m);
+    }
+    for (auto &d : td) {
+        if (d.order == 1) {  // exact: no variable times
+            std::map<int, FixedNode> fixed{{1, {{PAIR, -1, -1}, t_i}}, {2, {{PAIR, -1, -1}, t_fp}}};
+            Evaluator ev;
+            ev.build(m, &P, 1, true, fixed);
+            per_order[1] = ev(d.tops, nullptr);
+            g_diagram_evals += (int64_t)d.tops.size();
+            g_leaves += ev.leaves;
+            continue;
+        }
+        if (N <= 0) continue;
+        const int dv = 2 * d.order - 2;
+        std::map<int, FixedNode> fixed{{1, {{PAIR, -1, -1}, t_i}}, {dv + 2, {{PAIR, -1, -1}, t_fp}}};
+        Transform tr = Transform::root(dv, t_i, t_fp);
+        add_randomized(
+            m, rq, sob, RQ_DIFF, a, d,
+            [&](SobolSource src) { return integrate_group(m, &P, true, d.order, fixed, tr, d.tops, N, src, 0, -1, dv); },
+            per_order[d.order], per_order_std[d.order]);
+    }
+    std::vector<CMat> sig = bt_zero(m);
+    for (auto &kv : per_order) bt_add(sig, kv.second);
+    S.store(m, a, sig);
+    std::vector<CMat> out = bt_zero(m);
+    for (int s = 0; s < m.ns; ++s) {
+        const int dm = m.dims[s];
+        const CMat &Pp = P.tab[a][s];
+        CMat conv(dm, dm), tmp(dm, dm);
+        for (int j = 0; j <= a && a > 0; ++j) {  // kd.integrate: trapezoid over grid[0..a]
+            const double w = (j == 0 || j == a) ? 0.5 * m.h : m.h;
+            matmul(tmp, S.tab[a - j][s], P.tab[j][s]);
+            for (int e = 0; e < dm * dm; ++e) conv.a[e] += w * tmp.a[e];
+        }
+        for (int r = 0; r < dm; ++r)
+            for (int c = 0; c < dm; ++c) {
+                const cd rhs = -I * conv(r, c) - (m.E[s][r] + m.lambda0) * Pp(r, c);
+                out[s](r, c) = Pp(r, c) + m.h * rhs;
+            }
+    }
+    return out;
+}
+
+// diff_inchworm! (:666-778).  n_steps_max < 0: all n_tau - 1 steps.  Sigma_orders_out
+// [n_orders][n_tau][bt] for run->orders (order 0: zero), written at tau_f of each step as
+// the reference's set_ppgf!(Σ_orders[order], tau_i, tau_f, ...) does.
+extern "C" int orc_diff_inchworm(const qiw_model *mi, const qiw_run *run, const uint32_t *sob_m,
+                                 const uint32_t *sob_shift, int n_steps_max, double *P_out, double *Sigma_orders_out,
+                                 double *Sigma_orders_std_out) {
+    Model m;
+    if (!load_model(mi, m)) return QIW_E_NOT_SBM;
+    if (!(run->N_samples == 0 || (run->N_samples & (run->N_samples - 1)) == 0)) {
+        set_err("N_samples must be 0 or a power of 2");
+        return QIW_E_NOT_POW2;
+    }
+    SobolSource sob{sob_m, sob_shift};
+    const RandParams rq = rand_params(run);
+    PPGF P;
+    atomic_ppgf(m, P);
+    SigmaTable S;
+    S.tab.assign(m.ntau, bt_zero(m));
+    auto td = diff_top_data(run);
+    const int no = run->n_orders;
+    std::vector<std::vector<std::vector<CMat>>> so(no, std::vector<std::vector<CMat>>(m.ntau, bt_zero(m)));
+    std::vector<std::vector<std::vector<CMat>>> ss = so;
+    try {
+        std::map<int, std::vector<CMat>> per, per_std;
+        for (int a = 0; a + 1 < m.ntau; ++a) {  // :752-774, tau_f_prev = grid[a], tau_f = grid[a + 1]
+            if (n_steps_max >= 0 && a >= n_steps_max) break;
+            std::vector<CMat> Pn = diff_step(m, P, S, a, td, run->N_samples, sob, rq, per, per_std);
+            set_ppgf(m, P, a + 1, Pn);
+            for (int oi = 0; oi < no; ++oi) {
+                auto it = per.find(run->orders[oi]);
+                if (it == per.end()) continue;
+                so[oi][a + 1] = it->second;
+                ss[oi][a + 1] = per_std[run->orders[oi]];
+            }
+        }
+    } catch (std::exception &e) {
+        set_err("A block off-diagonal contribution to a pseudo-particle propagator detected");
+        return QIW_E_OFFDIAG;
+    }
+    if (P_out) ppgf_to_table(m, P, P_out, nullptr);
+    for (int oi = 0; oi < no; ++oi)
+        for (int j = 0; j < m.ntau; ++j) {
+            if (Sigma_orders_out) bt_store(m, so[oi][j], Sigma_orders_out + (size_t)2 * (oi * m.ntau + j) * m.bt_size);
+            if (Sigma_orders_std_out)
+                bt_store(m, ss[oi][j], Sigma_orders_std_out + (size_t)2 * (oi * m.ntau + j) * m.bt_size);
+        }
+    return 0;
 }
 
 // TopologyEvaluator at explicit variable times (unit tests of the evaluator).

@@ -80,8 +80,14 @@ enum { QIW_FL_OP_TOPO = 1, QIW_FL_OP_FLUSH = 2, QIW_FL_OP_END = 3, QIW_FL_OP_ROO
 static inline int qiw_scalar_sm(int km, int sm, int tm) { return qiw_scalar_tm(km, sm, tm) == 2 ? 4 : 8; }
 static inline int qiw_scalar_km(int km, int sm, int tm) { return (km <= 4 && sm <= 8 && tm <= 4) ? 4 : QIW_KMAX; }
 
-// Kinds of integration group
-enum { QIW_G_BOLD = 0, QIW_G_BARE = 1, QIW_G_CORR = 2 };
+// Kinds of integration group (below).  A group is exact (no qMC sum, no standard
+// deviation) at order 0 and for the order-1 self-energy, which has no variable times.
+// SIGMA: a self-energy diagram of diff_inchworm_step! (src/inchworm.jl:557-601) laid out
+// as a bold group whose 2k pair positions all lie before tau_w = tau_f_prev, with tau_f =
+// tau_w: the first and last pair positions are pinned to tau_f_prev and tau_i by their
+// coordinates (1 and 0), so the three identity segments are P(0) = -i and the chain is the
+// self-energy's.
+enum { QIW_G_BOLD = 0, QIW_G_BARE = 1, QIW_G_CORR = 2, QIW_G_SIGMA = 3 };
 
 // One (order, n_pts_after) group of diagrams, compiled.
 struct QiwGroup {
@@ -102,6 +108,10 @@ struct QiwGroup {
     int32_t seq;           // randomized qMC: index of the group's sequence (0 otherwise)
     int64_t ps_off;        // offset of the group's samples in EvalArgs::per_sample
 };
+
+__host__ __device__ static inline bool qiw_group_exact(const QiwGroup &g) {
+    return !g.sampled || (g.kind == QIW_G_SIGMA && g.order == 1);
+}
 
 // A unit of work of one launch: up to 64 samples of one group at one time configuration,
 // over one chunk of the group's topologies (flat scalar programs; the whole group otherwise).
@@ -157,7 +167,8 @@ struct FinalizeArgs {
     int bt;                            // outputs per row (sum m^2)
     const int32_t *diag_idx;           // [n_diag] BT indices of diagonal elements
     int n_diag;
-    double *ptab;                      // [n_tau][bt] i*P
+    double *ptab;                      // [n_tau][bt] i*P (differential step: the self-energy table)
+    int jt;                            // row of ptab receiving the sum over orders (jf for inchworm steps)
     double *porders;                   // [n_order_slots][n_tau][bt] raw i*P contributions
     double *lambda_out;                // [n_tau] per-step normalization constants
     int n_tau;
@@ -179,6 +190,14 @@ struct FinalizeArgs {
     int n_seqs;                        // N_seqs
     double target_std;
     double *porders_std;               // [n_order_slots][n_tau][bt] standard deviations
+    // differential step (k_diff_reduce_finalize): ptab is the table of sigma (Σ = i sigma,
+    // rows by time difference), porders its order-resolved rows; the Euler step writes row
+    // a + 1 of P from rows 0..a
+    double *P;                         // [n_tau][bt] i*P
+    const double *hdiag;               // [bt] E + lambda on the diagonal positions
+    const int32_t *dims, *bt_off;      // [ns]
+    int ns;
+    int a;                             // index of tau_f_prev
 };
 
 struct CorrFinalizeArgs {
@@ -243,8 +262,9 @@ __host__ __device__ static inline uint64_t qiw_mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 // Stream of sequence `seq` of group (order, n_after): domain 1 initial step (key 0),
-// 2 bold step (key = tau_w index), 3 correlator (key = operator pair).
-enum { QIW_RQ_BARE = 1, QIW_RQ_BOLD = 2, QIW_RQ_CORR = 3 };
+// 2 bold step (key = tau_w index), 3 correlator (key = operator pair), 4 differential step
+// (key = tau_f_prev index, n_after = 1).
+enum { QIW_RQ_BARE = 1, QIW_RQ_BOLD = 2, QIW_RQ_CORR = 3, QIW_RQ_DIFF = 4 };
 __host__ __device__ static inline uint64_t qiw_rq_stream(uint64_t domain, uint64_t key, int order, int n_after,
                                                          int seq) {
     return (domain << 60) | ((key & 0xFFFFFFFull) << 32) | ((uint64_t)order << 24) | ((uint64_t)n_after << 16) |
@@ -274,6 +294,11 @@ hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);
 hipError_t qiw_launch_finalize_step_rqmc(hipStream_t st, const FinalizeArgs &a);
+hipError_t qiw_launch_finalize_diff(hipStream_t st, const FinalizeArgs &a, int rqmc);
+// Coordinates of a self-energy group: rows 0 and 2k-1 are 1 and 0 (tau_f_prev, tau_i),
+// rows 1..2k-2 the RootTransform of Sobol dimensions 0..2k-3.
+hipError_t qiw_launch_sigma_coords(hipStream_t st, const uint32_t *m, const uint32_t *x0, int order, int64_t n,
+                                   double *out);
 hipError_t qiw_launch_finalize_corr_rqmc(hipStream_t st, const CorrFinalizeArgs &a);
 // Simplex coordinates of every sampled group's sequence for one step: the staged
 // direction numbers m (dimensions 0..dim_max-1) LMS + shift scrambled with the bits of the

@@ -390,7 +390,8 @@ struct Compiler {
 
     bool compile(Compiled &out) {
         n = (kind == QIW_G_BARE) ? 2 * order + 2 : 2 * order + 3;
-        int d_after = (kind == QIW_G_BARE) ? 2 * order : n_after;
+        // self-energy groups: all 2k pair positions before tau_w (= tau_f_prev = tau_f)
+        int d_after = (kind == QIW_G_BARE) ? 2 * order : (kind == QIW_G_SIGMA) ? 0 : n_after;
         int d_before = 2 * order - d_after;
         pos_w = (kind == QIW_G_BARE) ? 0 : d_before + 2;
         conf_kind.assign(n + 2, N_IDENTITY);
@@ -404,9 +405,10 @@ struct Compiler {
             bool fixed = (pos == 1 || pos == n || pos == pos_w);
             if (!fixed) top_to_conf.push_back(pos);
         }
-        std::vector<Topology> tops = (kind == QIW_G_BARE)
-                                         ? topologies_at_order(order, -1, false)
-                                         : topologies_at_order(order, n_after, kind == QIW_G_CORR);
+        // self-energy: get_topologies_at_order(order, 1) (src/inchworm.jl:715)
+        std::vector<Topology> tops = (kind == QIW_G_BARE)    ? topologies_at_order(order, -1, false)
+                                     : (kind == QIW_G_SIGMA) ? topologies_at_order(order, 1, false)
+                                                             : topologies_at_order(order, n_after, kind == QIW_G_CORR);
         selected.assign(std::max(order, 1), -1);
         out.n_top = (int)tops.size();
         // vector format: segments per initial sector, collected first
@@ -605,12 +607,12 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
             g.coef_off = (int32_t)coefs.size();
         }
         g.sampled = g.order > 0;
-        g.n_samples = g.sampled ? (int32_t)N : 1;
+        g.n_samples = (g.sampled && !qiw_group_exact(g)) ? (int32_t)N : 1;
         g.weighted = (weighted && g.sampled && warp_gid(g) >= 0) ? 1 : 0;
         g.ps_off = ps_total;
         ps_total += g.n_samples;
         g.ut_off = ut_total;
-        if (g.sampled) ut_total += (int64_t)(g.d_before + g.d_after + g.weighted) * N;
+        if (g.sampled) ut_total += (int64_t)(g.d_before + g.d_after + g.weighted) * g.n_samples;
         prog.insert(prog.end(), cg.prog.begin(), cg.prog.end());
         topo.insert(topo.end(), cg.topo.begin(), cg.topo.end());
         coefs.insert(coefs.end(), cg.coefs.begin(), cg.coefs.end());
@@ -658,7 +660,10 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
         int dim = cg.g.d_before + cg.g.d_after;
         if (dim > c->sob_dim) return fail(c, QIW_E_DIM, "Sobol dimension beyond the staged direction numbers");
         if (b.scrambled) continue;
-        HIPCHK(c, qiw_launch_simplex_coords(c->stream, sm, sx, cg.g.d_before, cg.g.d_after, N, d.ut + cg.g.ut_off));
+        if (cg.g.kind == QIW_G_SIGMA)
+            HIPCHK(c, qiw_launch_sigma_coords(c->stream, sm, sx, cg.g.order, cg.g.n_samples, d.ut + cg.g.ut_off));
+        else
+            HIPCHK(c, qiw_launch_simplex_coords(c->stream, sm, sx, cg.g.d_before, cg.g.d_after, N, d.ut + cg.g.ut_off));
     }
     return 0;
 }
@@ -686,7 +691,8 @@ RandPlan rand_plan(const qiw_run *run) {
 // sequences as replicas; `add_tasks(gi)` appends the tasks of entry gi.
 template <class AddTasks>
 void push_sequences(Batch &b, Compiled &&cg, int n_eval, AddTasks add_tasks) {
-    const int n = cg.g.order > 0 ? n_eval : 1;
+    const bool exact = cg.g.order == 0 || (cg.g.kind == QIW_G_SIGMA && cg.g.order == 1);
+    const int n = exact ? 1 : n_eval;
     const int first = (int)b.groups.size();
     b.r_first.push_back(first);
     b.r_count.push_back(n);
@@ -1444,7 +1450,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         fa.group_task_count = bare_d.g_count;
         fa.group_order_slot = bare_d.g_slot;
         fa.n_groups = (int)bare_b.groups.size();
-        fa.jf = 1;
+        fa.jf = fa.jt = 1;
         fa.tau_w = 0.0;
         fa.tau_f = m.h;
         fa.bare = 1;
@@ -1472,7 +1478,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
             HIPCHK(c, mark());
             if ((rc = launch_eval(c, a, (int)bold_b.tasks.size(), bold_b.fmt, bold_b.precision))) return rc;
             HIPCHK(c, mark());
-            fa.jf = jw + 1;
+            fa.jf = fa.jt = jw + 1;
             fa.tau_w = jw * m.h;
             fa.tau_f = (jw + 1) * m.h;
             if (m.ppgf_kind == QIW_GF_INC_SPLINE && normalize) n_known = std::max(n_known, jw + 2);
@@ -1562,6 +1568,229 @@ extern "C" int qiw_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, doubl
             for (int s = 0; s < c->m.ns; ++s)
                 for (int i = 0; i < c->m.dims[s]; ++i) c->der0[c->m.bt_off[s] + i + i * c->m.dims[s]] -= c->lambdas[j];
     }
+    if (P_out) qiw_get_ppgf(c, nullptr, P_out, nullptr);
+    return QIW_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Differential inchworm (src/inchworm.jl:528-778)
+// ---------------------------------------------------------------------------------------
+namespace {
+
+// Self-energy groups of run->orders (order 0 has none; order 1 is exact), slot = index in
+// run->orders.
+int build_diff_batch(qiw_ctx *c, const qiw_run *run, Batch &b) {
+    const HostModel &m = c->m;
+    int max_order = 0;
+    int64_t n_groups = 0;
+    for (int oi = 0; oi < run->n_orders; ++oi) {
+        max_order = std::max(max_order, (int)run->orders[oi]);
+        n_groups += run->orders[oi] > 0;
+    }
+    const RandPlan rq = rand_plan(run);
+    b.n_eval = rq.n_eval();
+    b.scrambled = rq.scrambled;
+    b.fmt = program_format(m, max_order, run->precision);
+    b.max_order = max_order;
+    b.spt = samples_per_task(c, m, b.fmt, n_groups * b.n_eval * std::max<int64_t>(run->N_samples, 1));
+    b.precision = run->precision;
+    for (int oi = 0; oi < run->n_orders; ++oi) {
+        const int order = run->orders[oi];
+        if (order == 0 || (order > 1 && run->N_samples <= 0)) continue;  // :557, :567
+        Compiler comp(m, QIW_G_SIGMA, order, 1, -1, b.fmt, b.max_order);
+        comp.chunk_words = c->tune.chunk_words > 0 ? c->tune.chunk_words : kChunkWords;
+        comp.chunk_cost = c->tune.chunk_cost > 0 ? c->tune.chunk_cost : kChunkCost;
+        Compiled cg;
+        if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
+        if (cg.n_top == 0) continue;
+        if (cg.g.n_wslots > 255 || m.ns > 255) return fail(c, QIW_E_LIMIT, "too many slots");
+        b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
+        const int64_t N = order == 1 ? 1 : run->N_samples;
+        const std::vector<int32_t> chunks = cg.chunks;
+        const double flops = (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
+        const int64_t leaves = N * cg.leaves_per_sample, evals = N * cg.n_top;
+        push_sequences(b, std::move(cg), b.n_eval, [&](int gi) {
+            b.g_first.push_back((int32_t)b.tasks.size());
+            const int nt = (int)((N + b.spt - 1) / b.spt);
+            for (int t = 0; t < nt; ++t)
+                for (int32_t ch : chunks) b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size(), ch, 0});
+            b.g_count.push_back(nt * (int)chunks.size());
+            b.g_slot.push_back(oi);
+            b.flops_per_launch += flops;
+            b.leaves_per_launch += leaves;
+            b.diagram_evals_per_launch += evals;
+        });
+    }
+    if (b.r_first.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 self-energy groups");
+    order_tasks(b);
+    return 0;
+}
+
+// diff_inchworm!: n_tau - 1 steps from the atomic propagator, each one evaluator launch
+// over the self-energy groups at tau_f_prev = grid[a] and one k_diff_reduce_finalize
+// (self-energy row, Σ * P convolution, Euler step into row a + 1).
+int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
+    const HostModel &m = c->m;
+    auto t0 = std::chrono::steady_clock::now();
+    const RandPlan rq = rand_plan(run);
+    const bool rqmc = rq.n_seqs > 1;
+    const std::vector<int> oslots(run->orders, run->orders + run->n_orders);
+    const int n_slots = std::max(run->n_orders, 1);
+    int rc = 0;
+    Plan &p = plan_slot(c, "diff");
+    const std::string key = plan_key(c, run, oslots, "diff");
+    if (p.key != key) {
+        p = Plan{};
+        if ((rc = build_diff_batch(c, run, p.b))) return rc;
+        if (!p.b.groups.empty() && (rc = upload(c, "diff", p.b, run->N_samples, p.d))) return rc;
+        p.key = key;
+    }
+    const Batch &b = p.b;
+    const DevBatch &d = p.d;
+    // the reference sums the orders' self-energies (src/inchworm.jl:606), which needs one
+    if (b.groups.empty()) return fail(c, QIW_E_ARG, "diff_inchworm needs a self-energy order >= 1");
+    double *dcoef, *eshift;
+    int32_t *dkind;
+    if ((rc = ensure_model_buffers(c, &dcoef, &dkind, &eshift))) return rc;
+    const size_t tab = (size_t)m.ntau * m.bt;
+    double *partial = (double *)dev(c, "partial", std::max<size_t>(b.tasks.size(), 1) * m.bt * 8, &rc);
+    double *ptab = (double *)dev(c, "ptab", tab * 8, &rc);
+    double *sig = (double *)dev(c, "diff.sigma", tab * 8, &rc);
+    double *porders = (double *)dev(c, "porders", (size_t)n_slots * tab * 8, &rc);
+    double *pstd = rqmc ? (double *)dev(c, "porders_std", (size_t)n_slots * tab * 8, &rc) : nullptr;
+    double *scratch = (double *)dev(c, "fin.scratch", (size_t)n_slots * m.bt * 8, &rc);
+    double *gsum = (double *)dev(c, "fin.gsum", std::max<size_t>(b.groups.size(), 1) * m.bt * 8, &rc);
+    double *hdiag = (double *)dev(c, "diff.hdiag", (size_t)m.bt * 8, &rc);
+    int32_t *ddims = (int32_t *)dev(c, "diff.dims", (size_t)m.ns * 4, &rc);
+    int32_t *dbtoff = (int32_t *)dev(c, "diff.btoff", (size_t)m.ns * 4, &rc);
+    unsigned *done = (unsigned *)dev(c, "fin.done", 4, &rc);
+    uint32_t *sobm = rq.scrambled ? (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc) : nullptr;
+    if (rc) return rc;
+    std::vector<double> hd((size_t)m.bt, 0.0);
+    for (int s = 0; s < m.ns; ++s)
+        for (int i = 0; i < m.dims[s]; ++i) hd[m.bt_off[s] + i + i * m.dims[s]] = m.E[s][i] + m.lambda0;
+    std::vector<int32_t> d32(m.dims.begin(), m.dims.end()), o32(m.bt_off.begin(), m.bt_off.end());
+    HIPCHK(c, hipMemcpy(hdiag, hd.data(), hd.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(ddims, d32.data(), (size_t)m.ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dbtoff, o32.data(), (size_t)m.ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemsetAsync(done, 0, 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(sig, 0, tab * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(porders, 0, (size_t)n_slots * tab * 8, c->stream));
+    if (pstd) HIPCHK(c, hipMemsetAsync(pstd, 0, (size_t)n_slots * tab * 8, c->stream));
+    if (sobm) HIPCHK(c, hipMemcpy(sobm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
+
+    FinalizeArgs fa{};
+    fa.groups = d.groups;
+    fa.group_task_first = d.g_first;
+    fa.group_task_count = d.g_count;
+    fa.group_order_slot = d.g_slot;
+    fa.n_groups = (int)b.groups.size();
+    fa.n_order_slots = n_slots;
+    fa.partial = partial;
+    fa.bt = m.bt;
+    fa.ptab = sig;  // the self-energy table
+    fa.porders = porders;
+    fa.n_tau = m.ntau;
+    fa.h = m.h;
+    fa.bare = 0;
+    fa.normalize = 0;
+    fa.scratch = scratch;
+    fa.gsum = gsum;
+    fa.done = done;
+    fa.n_real = (int)b.r_first.size();
+    fa.real_first = d.r_first;
+    fa.real_count = d.r_count;
+    fa.n_seqs = rq.n_seqs;
+    fa.target_std = rq.target_std;
+    fa.porders_std = pstd;
+    fa.P = ptab;
+    fa.hdiag = hdiag;
+    fa.dims = ddims;
+    fa.bt_off = dbtoff;
+    fa.ns = m.ns;
+    // set_ppgf! on Σ: an incremental container appends (row a + 1 at step a), a grid one
+    // writes the row of tau_f_prev (row a)
+    const int sig_shift = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? 1 : 0;
+
+    std::vector<hipEvent_t> ev;
+    auto mark = [&](void) -> hipError_t {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        ev.push_back(e);
+        return hipEventRecord(e, c->stream);
+    };
+    EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
+    int64_t launches = 0;
+    for (int st = 0; st + 1 < m.ntau; ++st) {
+        fa.a = st;
+        fa.jf = st + 1;  // Σ_orders at tau_f (src/inchworm.jl:770-773)
+        fa.jt = st + sig_shift;
+        fa.tau_w = fa.tau_f = st * m.h;
+        a.step_w = st;
+        if (b.scrambled)
+            HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, (int)b.groups.size(), 2 * b.max_order,
+                                             run->N_samples > 0 ? run->N_samples : 1, sobm, rq.seed, QIW_RQ_DIFF, st,
+                                             d.ut));
+        HIPCHK(c, mark());
+        if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt, b.precision))) return rc;
+        HIPCHK(c, mark());
+        HIPCHK(c, qiw_launch_finalize_diff(c->stream, fa, rqmc ? 1 : 0));
+        ++launches;
+        c->stats.flops += b.flops_per_launch;
+        c->stats.leaves += b.leaves_per_launch;
+        c->stats.diagram_evals += b.diagram_evals_per_launch;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        c->stats.kernel_ms += ms;
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    c->stats.n_kernel_launches += launches;
+    std::vector<double> P_h(tab), por((size_t)n_slots * tab), pstd_h(pstd ? por.size() : 0);
+    HIPCHK(c, hipMemcpy(P_h.data(), ptab, tab * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(por.data(), porders, por.size() * 8, hipMemcpyDeviceToHost));
+    if (pstd) HIPCHK(c, hipMemcpy(pstd_h.data(), pstd, pstd_h.size() * 8, hipMemcpyDeviceToHost));
+    c->ip = P_h;
+    c->n_known = m.ntau;
+    std::vector<char> sampled(n_slots, 0);  // orders with a qMC group (std NaN for one sequence)
+    for (size_t gi = 0; gi < b.groups.size(); ++gi)
+        if (!qiw_group_exact(b.groups[gi].g)) sampled[b.g_slot[gi]] = 1;
+    const double nan = std::nan("");
+    for (int o = 0; o < run->n_orders; ++o)
+        for (int j = 0; j < m.ntau; ++j)
+            for (int e = 0; e < m.bt; ++e) {
+                const size_t i = ((size_t)o * m.ntau + j) * m.bt + e;
+                if (S_out) {  // Σ = i sigma
+                    S_out[2 * i] = 0.0;
+                    S_out[2 * i + 1] = por[i];
+                }
+                if (S_std_out) {
+                    const double v = pstd ? pstd_h[i] : (sampled[o] && j >= 1 ? nan : 0.0);
+                    S_std_out[2 * i] = v;
+                    S_std_out[2 * i + 1] = pstd ? 0.0 : v;
+                }
+            }
+    c->stats.wall_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+}  // namespace
+
+extern "C" int qiw_diff_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, double *Sigma_orders_out,
+                                 double *Sigma_orders_std_out) {
+    if (!c) return QIW_E_ARG;
+    int rc = check_run(c, run);
+    if (rc) return rc;
+    (void)hipSetDevice(c->device);
+    c->stats = qiw_stats{};
+    atomic_ip(c->m, c->ip);  // diff_inchworm! starts from the staged atomic propagator
+    for (int s = 0; s < c->m.ns; ++s)
+        for (int i = 0; i < c->m.dims[s]; ++i)
+            c->der0[c->m.bt_off[s] + i + i * c->m.dims[s]] = cd(0.0, c->m.E[s][i] + c->m.lambda0);
+    if ((rc = run_diff(c, run, Sigma_orders_out, Sigma_orders_std_out))) return rc;
     if (P_out) qiw_get_ppgf(c, nullptr, P_out, nullptr);
     return QIW_OK;
 }

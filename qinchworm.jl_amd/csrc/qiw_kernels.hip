@@ -65,6 +65,28 @@ __global__ void k_simplex_coords(const uint32_t *__restrict__ m, const uint32_t 
     for (int k = 0; k < d; ++k) out[(int64_t)k * n + r] = u[k];
 }
 
+// Self-energy groups (RootTransform of the 2k - 2 variable times on [tau_i, tau_f_prev],
+// src/inchworm.jl:573-577): row 0 = 1 and row 2k - 1 = 0 pin the first and last pair
+// positions to tau_f_prev and tau_i; rows 1..2k-2 hold the descending root-map coordinates.
+__device__ __forceinline__ void sigma_row_values(const double *x, int order, double *u) {
+    const int dv = 2 * order - 2;
+    double r[2 * QIW_KMAX];
+    root_map(x, 0, dv, r);
+    u[0] = 1.0;
+    for (int k = 0; k < dv; ++k) u[1 + k] = r[k];
+    u[dv + 1] = 0.0;
+}
+
+__global__ void k_sigma_coords(const uint32_t *__restrict__ m, const uint32_t *__restrict__ x0, int order, int64_t n,
+                               double *__restrict__ out) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double x[2 * QIW_KMAX], u[2 * QIW_KMAX];
+    for (int k = 0; k < 2 * order - 2; ++k) x[k] = ldexp((double)sobol_word(m, x0[k], k, (uint64_t)r), -32);
+    sigma_row_values(x, order, u);
+    for (int k = 0; k < 2 * order; ++k) out[(int64_t)k * n + r] = u[k];
+}
+
 // Points of the unit cube for the learned warp: Sobol coordinates 0..dim-1 of sample r in
 // row r (stride `stride`, unused coordinates 0).
 __global__ void k_sobol_cube(const uint32_t *__restrict__ m, const uint32_t *__restrict__ x0, int dim, int stride,
@@ -303,6 +325,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     } else if (g.kind == QIW_G_BOLD) {
         tau_w = a.step_w * a.h;
         tau_f = (a.step_w + 1) * a.h;
+    } else if (g.kind == QIW_G_SIGMA) {  // tau_f_prev = grid[step_w]
+        tau_w = a.step_w * a.h;
+        tau_f = tau_w;
     } else {
         tau_w = task.tau_index * a.h;
         tau_f = a.beta;
@@ -530,6 +555,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
     } else if (g.kind == QIW_G_BOLD) {
         tau_w = a.step_w * a.h;
         tau_f = (a.step_w + 1) * a.h;
+    } else if (g.kind == QIW_G_SIGMA) {  // tau_f_prev = grid[step_w]
+        tau_w = a.step_w * a.h;
+        tau_f = tau_w;
     } else {
         tau_w = task.tau_index * a.h;
         tau_f = a.beta;
@@ -935,6 +963,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
     } else if (g.kind == QIW_G_BOLD) {
         tau_w = a.step_w * a.h;
         tau_f = (a.step_w + 1) * a.h;
+    } else if (g.kind == QIW_G_SIGMA) {  // tau_f_prev = grid[step_w]
+        tau_w = a.step_w * a.h;
+        tau_f = tau_w;
     } else {
         tau_w = task.tau_index * a.h;
         tau_f = a.beta;
@@ -1137,6 +1168,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
     } else if (g.kind == QIW_G_BOLD) {
         tau_w = a.step_w * a.h;
         tau_f = (a.step_w + 1) * a.h;
+    } else if (g.kind == QIW_G_SIGMA) {  // tau_f_prev = grid[step_w]
+        tau_w = a.step_w * a.h;
+        tau_f = tau_w;
     } else {
         tau_w = task.tau_index * a.h;
         tau_f = a.beta;
@@ -1369,8 +1403,11 @@ constexpr int kFinMaxSlots = QIW_KMAX + 1;
 __device__ double step_group_factor(const QiwGroup &g, const FinalizeArgs &a) {
     if (!g.sampled) return 1.0;
     const double tau_i = 0.0, tau_w = a.tau_w, tau_f = a.tau_f;
-    const double J = a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
-                            : simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
+    // self-energy: RootTransform of 2k - 2 times on [tau_i, tau_f_prev = tau_w]; the group
+    // sum times this factor is sigma with Σ = i sigma
+    const double J = g.kind == QIW_G_SIGMA ? simplex_volume(2 * g.order - 2, tau_w - tau_i)
+                     : a.bare ? simplex_volume(g.d_after, tau_f - tau_i)
+                              : simplex_volume(g.d_before, tau_w - tau_i) * simplex_volume(g.d_after, tau_f - tau_w);
     return ((g.order & 1) ? -1.0 : 1.0) * J / (double)g.n_samples;
 }
 
@@ -1426,7 +1463,7 @@ __device__ void finalize_step_body(const FinalizeArgs &a) {
                 tot += acc[o];
             }
         }
-        a.ptab[(int64_t)jf * a.bt + e] = tot;
+        a.ptab[(int64_t)a.jt * a.bt + e] = tot;
     }
     if (a.normalize) step_normalize(a);
 }
@@ -1528,10 +1565,12 @@ __global__ __launch_bounds__(256) void k_rqmc_coords(const QiwGroup *__restrict_
                                                      double *__restrict__ ut) {
     const QiwGroup g = groups[blockIdx.y];
     if (!g.sampled) return;
-    const int d = g.d_before + g.d_after;
+    const bool sigma = g.kind == QIW_G_SIGMA;
+    const int d = sigma ? 2 * g.order - 2 : g.d_before + g.d_after;  // Sobol dimensions used
     __shared__ uint32_t rows[2 * QIW_KMAX][33];
     __shared__ uint32_t ms[2 * QIW_KMAX * 32];
-    const uint64_t stream = qiw_rq_stream(domain, key, g.order, g.n_after, g.seq);
+    // self-energy groups are keyed like the reference's topology data, n_pts_after = 1
+    const uint64_t stream = qiw_rq_stream(domain, key, g.order, sigma ? 1 : g.n_after, g.seq);
     for (int i = threadIdx.x; i < d * 33; i += blockDim.x) {
         const int k = i / 33, w = i % 33;
         uint32_t v = qiw_rq_word(seed, stream, k, w);
@@ -1552,12 +1591,18 @@ __global__ __launch_bounds__(256) void k_rqmc_coords(const QiwGroup *__restrict_
     }
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    const int64_t ng = g.n_samples;  // <= n
+    if (r >= ng) return;
     double x[2 * QIW_KMAX], u[2 * QIW_KMAX];
     for (int k = 0; k < d; ++k) x[k] = ldexp((double)sobol_word(ms, rows[k][0], k, (uint64_t)r), -32);
-    root_map(x, g.d_after, g.d_before, u);
     double *out = ut + g.ut_off;
-    for (int k = 0; k < d; ++k) out[(int64_t)k * n + r] = u[k];
+    if (sigma) {
+        sigma_row_values(x, g.order, u);
+        for (int k = 0; k < 2 * g.order; ++k) out[(int64_t)k * ng + r] = u[k];
+        return;
+    }
+    root_map(x, g.d_after, g.d_before, u);
+    for (int k = 0; k < d; ++k) out[(int64_t)k * ng + r] = u[k];
 }
 
 // Step bookkeeping with randomized qMC (inchworm_step, src/inchworm.jl:172-195), one
@@ -1582,7 +1627,8 @@ __device__ void finalize_step_rqmc_body(const FinalizeArgs &a) {
         const QiwGroup g = a.groups[base];
         const double fac = step_group_factor(g, a);
         const int slot = a.group_order_slot[base];
-        const int nmax = g.sampled ? a.n_seqs : 1;
+        const bool exact = qiw_group_exact(g);
+        const int nmax = exact ? 1 : a.n_seqs;
         int nuse = nmax;
         for (int n = 2; n <= nmax; ++n) {
             double loc = 0.0;
@@ -1606,7 +1652,7 @@ __device__ void finalize_step_rqmc_body(const FinalizeArgs &a) {
         for (int e = tid; e < bt; e += blockDim.x) {
             auto x = [&](int s) { return __dmul_rn(fac, gsum[(int64_t)(base + (cnt > 1 ? s : 0)) * bt + e]); };
             const double mean = __dmul_rn(seq_sum(x, nuse), 1.0 / nuse);
-            const double sd = !g.sampled ? 0.0 : (nuse > 1 ? seq_std(x, nuse) : __builtin_nan(""));
+            const double sd = exact ? 0.0 : (nuse > 1 ? seq_std(x, nuse) : __builtin_nan(""));
             a.porders[((int64_t)slot * a.n_tau + jf) * bt + e] += mean;  // after the group's mean is rounded
             a.porders_std[((int64_t)slot * a.n_tau + jf) * bt + e] += sd;
         }
@@ -1614,7 +1660,7 @@ __device__ void finalize_step_rqmc_body(const FinalizeArgs &a) {
     for (int e = tid; e < bt; e += blockDim.x) {
         double tot = 0.0;
         for (int o = 0; o < a.n_order_slots; ++o) tot += a.porders[((int64_t)o * a.n_tau + jf) * bt + e];
-        a.ptab[(int64_t)jf * bt + e] = tot;
+        a.ptab[(int64_t)a.jt * bt + e] = tot;
     }
     if (a.normalize) step_normalize(a);
 }
@@ -1629,6 +1675,52 @@ __global__ __launch_bounds__(256) void k_step_reduce_finalize_rqmc(FinalizeArgs 
     if (!last) return;
     if (threadIdx.x == 0) *a.done = 0u;
     finalize_step_rqmc_body(a);
+}
+
+// Euler step of diff_inchworm_step! (src/inchworm.jl:603-635) after the self-energy row
+// is stored: with p = i P and Σ = i sigma (real),
+//   p(tau_{a+1}) = p(tau_a) + h (sum_j w_j sigma(tau_a - tau_j) p(tau_j) - H p(tau_a)),
+// w the trapezoid weights of kd.integrate over grid[0..a] (the contour measure -i dtau
+// turns Σ P into sigma p), H = diag(E + lambda).  Per element of each sector block.
+__device__ void diff_euler(const FinalizeArgs &a) {
+    __threadfence_block();
+    __syncthreads();
+    const int A = a.a, bt = a.bt;
+    for (int e = threadIdx.x; e < bt; e += blockDim.x) {
+        int s = 0;
+        while (s + 1 < a.ns && a.bt_off[s + 1] <= e) ++s;
+        const int m = a.dims[s], o = a.bt_off[s], r = (e - o) % m, c = (e - o) / m;
+        double conv = 0.0;
+        for (int j = 0; j <= A && A > 0; ++j) {
+            const double w = (j == 0 || j == A) ? 0.5 * a.h : a.h;
+            const double *S = a.ptab + (int64_t)(A - j) * bt + o;
+            const double *Pj = a.P + (int64_t)j * bt + o;
+            double t = 0.0;
+            for (int l = 0; l < m; ++l) t += S[r + l * m] * Pj[l + c * m];
+            conv += w * t;
+        }
+        const double pp = a.P[(int64_t)A * bt + e];
+        a.P[(int64_t)(A + 1) * bt + e] = pp + a.h * (conv - a.hdiag[o + r * (m + 1)] * pp);
+    }
+}
+
+// One launch per differential step: per-group row sums, then (last workgroup) the
+// self-energy bookkeeping as for an inchworm step (ptab = sigma table, no normalization)
+// and the Euler step.
+__global__ __launch_bounds__(256) void k_diff_reduce_finalize(FinalizeArgs a, int rqmc) {
+    reduce_rows_block(a.group_task_first, a.group_task_count, a.partial, a.bt, a.gsum);
+    __threadfence();
+    __syncthreads();
+    __shared__ int last;
+    if (threadIdx.x == 0) last = atomicAdd(a.done, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!last) return;
+    if (threadIdx.x == 0) *a.done = 0u;
+    if (rqmc)
+        finalize_step_rqmc_body(a);
+    else
+        finalize_step_body(a);
+    diff_euler(a);
 }
 
 // Correlator bookkeeping with randomized qMC (correlator_2p, src/inchworm.jl:857-885):
@@ -1672,7 +1764,7 @@ __global__ __launch_bounds__(256) void k_finalize_corr_rqmc(CorrFinalizeArgs a) 
         }
         if (t == 0) {
             auto x = [&](int s) { return xs[cnt > 1 ? s : 0]; };
-            const int nmax = g.sampled ? a.n_seqs : 1;
+            const int nmax = qiw_group_exact(g) ? 1 : a.n_seqs;
             int nuse = nmax;
             for (int n = 2; n <= nmax; ++n)
                 if (seq_std(x, n) <= a.target_std) {
@@ -1680,7 +1772,7 @@ __global__ __launch_bounds__(256) void k_finalize_corr_rqmc(CorrFinalizeArgs a) 
                     break;
                 }
             tot += seq_sum(x, nuse) / nuse;
-            tot_std += !g.sampled ? 0.0 : (nuse > 1 ? seq_std(x, nuse) : __builtin_nan(""));
+            tot_std += qiw_group_exact(g) ? 0.0 : (nuse > 1 ? seq_std(x, nuse) : __builtin_nan(""));
         }
         __syncthreads();
     }
@@ -1801,6 +1893,20 @@ hipError_t qiw_launch_finalize_step_rqmc(hipStream_t st, const FinalizeArgs &a) 
     if (a.n_seqs < 1 || a.n_seqs > QIW_MAX_SEQS) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_step_reduce_finalize_rqmc, dim3(a.n_groups, a.bt <= 256 ? 1 : (a.bt + 255) / 256),
                        dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_finalize_diff(hipStream_t st, const FinalizeArgs &a, int rqmc) {
+    if (a.n_seqs < 1 || a.n_seqs > QIW_MAX_SEQS || a.normalize) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_diff_reduce_finalize, dim3(a.n_groups, a.bt <= 256 ? 1 : (a.bt + 255) / 256), dim3(256), 0,
+                       st, a, rqmc);
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_sigma_coords(hipStream_t st, const uint32_t *m, const uint32_t *x0, int order, int64_t n,
+                                   double *out) {
+    if (order < 1 || order > QIW_KMAX || n < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sigma_coords, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m, x0, order, n, out);
     return hipGetLastError();
 }
 

@@ -7,6 +7,8 @@
     correlator_2p(expansion, grid, orders, N_samples)           (correlator_2p, :1075-1084)
     correlator_2p(expansion, grid, orders, N_samples, RequestStdDev(); rand_params)
         -> (correlators, standard deviations)                    (correlator_2p, :918-1052)
+    diff_inchworm(expansion, grid, orders, N_samples; rand_params)
+        -> (Σ_orders, Σ_orders_std), updates expansion.P          (diff_inchworm!, :666-778)
     partition_function(expansion), density_matrix(expansion)    (src/ppgf.jl:611-617, :729-733)
 
 All evaluation runs in libqiw on the GPU; there is no CPU fallback.
@@ -121,6 +123,25 @@ def inchworm_step_bare(expansion, orders_bare, N_samples, seq=None, ctx=None, ra
     ctx.check(ctx.lib.qiw_inchworm_step_bare(ctx.h, run.ref(), val.ctypes.data_as(abi._f64p),
                                              per.ctypes.data_as(abi._f64p)))
     return val, per
+
+
+def diff_inchworm(expansion, grid, orders, N_samples, precision=abi.QIW_PREC_FP64, seq=None, ctx=None,
+                  rand_params=None):
+    """diff_inchworm! (src/inchworm.jl:666-778): the differential formulation.  Writes the
+    propagator into expansion.P and returns the order-resolved self-energy and its
+    standard deviations as {order: array (n_tau, sum m^2)}."""
+    ctx = ctx or context()
+    orders = list(orders)
+    _stage(ctx, expansion, max(orders + [0]), seq)
+    run = abi.RunStruct(orders, [], N_samples, 0, precision, rand=_rand(rand_params))
+    nt, bt = grid.n_tau, expansion.bt_size
+    So = np.zeros((len(orders), nt, bt), np.complex128)
+    Ss = np.zeros_like(So)
+    P = np.zeros((nt, bt), np.complex128)
+    ctx.check(ctx.lib.qiw_diff_inchworm(ctx.h, run.ref(), P.ctypes.data_as(abi._f64p), So.ctypes.data_as(abi._f64p),
+                                        Ss.ctypes.data_as(abi._f64p)))
+    _get_P(ctx, expansion)
+    return {o: So[i] for i, o in enumerate(orders)}, {o: Ss[i] for i, o in enumerate(orders)}
 
 
 def correlator_2p(expansion, grid, orders, N_samples, *flag, seq=None, ctx=None, precision=abi.QIW_PREC_FP64,

@@ -40,6 +40,8 @@ def lib():
                                         C.c_int, f64p, f64p, C.c_int, C.c_int, f64p]
         L.orc_correlator_2p_std.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p,
                                             C.c_int, f64p, f64p, C.c_int, C.c_int, f64p, f64p]
+        L.orc_diff_inchworm.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p, C.c_int,
+                                        f64p, f64p, f64p]
         L.orc_rq_scramble.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u32p, u32p, u32p]
         L.orc_rq_stream.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int]
         L.orc_rq_stream.restype = C.c_uint64
@@ -150,3 +152,17 @@ def rq_scramble(seed, stream, m):
 
 def rq_stream(domain, key, order, n_after, seq):
     return int(lib().orc_rq_stream(int(domain), int(key), int(order), int(n_after), int(seq)))
+
+
+def diff_inchworm(exp, orders, N, n_steps_max=-1, sobol=None, rand=None):
+    """diff_inchworm! on the oracle: (P table, {order: Σ contributions}, {order: std})."""
+    ms = exp.to_struct()
+    run = abi.RunStruct(orders, [], N, rand=rand)
+    m, x0 = sobol if sobol is not None else sobol_tables(2 * max(list(orders) + [1]))
+    n_tau, bt = exp.grid.n_tau, exp.bt_size
+    P = np.zeros((n_tau, bt), np.complex128)
+    So = np.zeros((len(list(orders)), n_tau, bt), np.complex128)
+    Ss = np.zeros_like(So)
+    check(lib().orc_diff_inchworm(ms.ref(), run.ref(), _p(m, C.c_uint32), _p(x0, C.c_uint32), int(n_steps_max),
+                                  _p(P, C.c_double), _p(So, C.c_double), _p(Ss, C.c_double)))
+    return P, {o: So[i] for i, o in enumerate(orders)}, {o: Ss[i] for i, o in enumerate(orders)}
