@@ -238,6 +238,37 @@ def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678):
             "max_G_error_vs_exact": float(np.abs(g[0] - exp.corr_exact).max())}
 
 
+def diff_study(log2n=14, with_cpu=True, cpu_steps=4):
+    """diff_inchworm! (the differential formulation) on the C1 model: orders 0:4, N=2^14,
+    all n_tau - 1 = 199 steps on one GPU; the float64 oracle times the first `cpu_steps`
+    steps of the same run on the host cores."""
+    import qiw
+    N = 2 ** log2n
+    exp = c1_expansion()
+    qiw.diff_inchworm(exp, exp.grid, range(0, 5), N)  # plans and buffers
+    exp = c1_expansion()
+    t0 = time.perf_counter()
+    qiw.diff_inchworm(exp, exp.grid, range(0, 5), N)
+    dt = time.perf_counter() - t0
+    st = qiw.stats()
+    out = {"workload": f"diff_inchworm! on the C1 model (single-orbital Anderson, beta=10, n_tau=200), orders 0:4, "
+                       f"N=2^{log2n}, float64, 199 steps",
+           "gpu_s": dt, "gpu_evals_per_s": st["diagram_evals"] / dt, "evaluator_ms": st["kernel_ms"]}
+    if with_cpu:
+        import oracle_lib as O
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        O.reset_counters()
+        t0 = time.perf_counter()
+        O.diff_inchworm(c1_expansion(), range(0, 5), N, n_steps_max=cpu_steps)
+        dc = time.perf_counter() - t0
+        _, evals, _ = O.counters()
+        out.update({"cpu_evals_per_s": evals / dc, "cpu_cores": threads,
+                    "cpu_sample": f"first {cpu_steps} steps, {evals} diagram evals in {dc:.2f} s, float64 oracle"})
+        out["speedup"] = out["gpu_evals_per_s"] / out["cpu_evals_per_s"]
+    return out
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: 2 x mib MiB of HBM traffic per launch."""
     import torch
@@ -327,6 +358,7 @@ def main():
         line["three_orbital_step"] = three_orbital(with_cpu=not args.no_cpu)
     if not args.no_rqmc:
         line["rqmc_study"] = rqmc_study()
+        line["diff_inchworm"] = diff_study(with_cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -236,6 +236,39 @@ function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_ba
 end
 
 """
+    diff_inchworm!(expansion, grid, orders, N_samples; rand_params)
+
+Drop-in for QInchworm.diff_inchworm! (src/inchworm.jl:666-778) on one MI355X.  Returns
+(Σ_orders, Σ_orders_std) as Dict{Int, Matrix{ComplexF64}} in BT layout (sum m^2 x n_tau)
+and writes expansion.P.
+"""
+function diff_inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64;
+                        rand_params::RandomizationParams = RandomizationParams(),
+                        precision::Int32 = QIW_PREC_FP64, ctx::Context = context())
+    @assert N_samples == 0 || ispow2(N_samples)
+    @assert rand_params.N_seqs > 0
+    rq = Ref(QiwRand(rand_params))
+    keep = Any[]
+    orders_v = Int32.(collect(orders))
+    stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, Int32[0]))), keep)
+    n_tau = length(grid)
+    bt = sum(length(s)^2 for s in exp.ed.subspaces)
+    P = zeros(ComplexF64, bt, n_tau)
+    So = zeros(ComplexF64, bt, n_tau, length(orders_v))
+    Ss = zeros(ComplexF64, bt, n_tau, length(orders_v))
+    GC.@preserve keep orders_v rq begin
+        run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, precision,
+                         Base.unsafe_convert(Ptr{QiwRand}, rq)))
+        check(ctx, ccall((:qiw_diff_inchworm, libqiw), Cint,
+                         (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                         ctx.h, run, P, So, Ss))
+    end
+    store_ppgf!(exp, P, grid)
+    return (Dict(Int(o) => So[:, :, i] for (i, o) in enumerate(orders_v)),
+            Dict(Int(o) => Ss[:, :, i] for (i, o) in enumerate(orders_v)))
+end
+
+"""
     correlator_2p(expansion, grid, orders, N_samples; rand_params)
     correlator_2p(expansion, grid, orders, N_samples, RequestStdDev(); rand_params)
 
