@@ -50,14 +50,19 @@ class FockSpace:
 # ------------------------------------------------------------------------------------
 class EDCore:
     """Diagonalizes H inside the sectors given by `sector_key(basis_state)`; sectors are
-    ordered by sorted key.  Operators are represented by their blocks between eigenbases
+    ordered by sorted key.  With sector_key None the sectors are autopartition(H, fock,
+    symmetry_breakers).  Operators are represented by their blocks between eigenbases
     (the role of KeldyshED's EDCore / operator_blocks)."""
 
-    def __init__(self, H, fock, sector_key):
+    def __init__(self, H, fock, sector_key=None, symmetry_breakers=()):
         self.fock = fock
         keys = {}
-        for b in range(fock.dim):
-            keys.setdefault(sector_key(b), []).append(b)
+        if sector_key is None:
+            for s, states in enumerate(autopartition(H, fock, symmetry_breakers)):
+                keys[s] = states
+        else:
+            for b in range(fock.dim):
+                keys.setdefault(sector_key(b), []).append(b)
         self.sector_keys = sorted(keys)
         self.subspaces = [keys[k] for k in self.sector_keys]
         sec_of = np.empty(fock.dim, dtype=int)
@@ -99,6 +104,56 @@ class EDCore:
         w = [np.exp(-beta * (e - emin)) for e in self.energies]
         z = sum(x.sum() for x in w)
         return [np.diag(x / z) for x in w]
+
+
+def autopartition(H, fock, symmetry_breakers=(), tol=1e-13):
+    """The finest partition of the Fock basis into invariant subspaces, the role of
+    KeldyshED's autopartition behind EDCore(H, soi; symmetry_breakers)
+    (src/expansion.jl:283).  Phase 1: basis states coupled by H or by a symmetry breaker
+    share a subspace (union-find).  Phase 2: every c_i and c^dag_i must map each subspace
+    into a single subspace (one block per column of its SectorBlockMatrix): the target
+    subspaces of one source are merged, for both operators of every mode, until nothing
+    changes.  Returns the subspaces as sorted lists of basis states, ordered by their
+    smallest state."""
+    parent = list(range(fock.dim))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    def union(x, y):
+        x, y = find(x), find(y)
+        if x != y:
+            parent[max(x, y)] = min(x, y)
+            return True
+        return False
+
+    for M in [H] + list(symmetry_breakers):
+        rows, cols = np.nonzero(np.abs(M) > tol)
+        for r, c in zip(rows, cols):
+            union(int(r), int(c))
+    ops = []
+    for label in fock.labels:
+        C = fock.c(label)
+        ops += [C, C.T.copy()]
+    changed = True
+    while changed:
+        changed = False
+        for C in ops:
+            targets = {}
+            rows, cols = np.nonzero(np.abs(C) > tol)
+            for r, c in zip(rows, cols):
+                targets.setdefault(find(int(c)), set()).add(find(int(r)))
+            for tset in targets.values():
+                tl = sorted(tset)
+                for t in tl[1:]:
+                    changed |= union(tl[0], t)
+    groups = {}
+    for b in range(fock.dim):
+        groups.setdefault(find(b), []).append(b)
+    return sorted(groups.values(), key=lambda g: g[0])
 
 
 # ------------------------------------------------------------------------------------
@@ -206,6 +261,43 @@ class Expansion:
     @property
     def ppgf_kind(self):
         return abi.QIW_GF_INC_SPLINE if self.interpolate_ppgf else abi.QIW_GF_GRID_LINEAR
+
+    @classmethod
+    def from_hamiltonian(cls, hamiltonian, fock, grid, hybridization, nn_interaction=None, corr_operators=(),
+                         interpolate_ppgf=False, tol=0.0):
+        """The higher-level constructor Expansion(hamiltonian, soi, grid; hybridization,
+        nn_interaction, corr_operators, interpolate_ppgf) (src/expansion.jl:222-291).
+        hybridization: complex array [n_modes][n_modes][n_tau] of Delta_ij(tau_k), modes in
+        fock.labels order; nn_interaction: real array of U_ij(tau_k) of the same shape.
+        For each non-zero Delta_ij: the pairs (c^dag_i, c_j, Delta_ij) and (c_j, c^dag_i,
+        Delta_ij(beta - tau)), and c^dag_i c_j, c_j c^dag_i as symmetry breakers of the
+        autopartition; for each non-zero U_ij the pair (n_i, n_j, U_ij)."""
+        labels = fock.labels
+        hyb = np.asarray(hybridization, dtype=np.complex128)
+        assert hyb.shape == (len(labels), len(labels), grid.n_tau)
+        pairs, breakers = [], []
+        for i, li in enumerate(labels):
+            for j, lj in enumerate(labels):
+                vals = hyb[i, j]
+                if np.all(np.abs(vals) <= tol):
+                    continue
+                delta = ScalarGF(vals)
+                cd_i, c_j = fock.c_dag(li), fock.c(lj)
+                pairs.append(InteractionPair(cd_i, c_j, delta))
+                pairs.append(InteractionPair(c_j, cd_i, ph_conj(delta)))
+                breakers += [cd_i @ c_j, c_j @ cd_i]
+        if nn_interaction is not None:
+            U = np.asarray(nn_interaction, dtype=np.float64)
+            assert U.shape == (len(labels), len(labels), grid.n_tau)
+            for i, li in enumerate(labels):
+                for j, lj in enumerate(labels):
+                    if np.all(np.abs(U[i, j]) <= tol):
+                        continue
+                    pairs.append(InteractionPair(fock.n(li), fock.n(lj), ScalarGF(U[i, j].astype(np.complex128))))
+        ed = EDCore(hamiltonian, fock, symmetry_breakers=breakers)
+        exp = cls(ed, grid, pairs, corr_operators, interpolate_ppgf=interpolate_ppgf)
+        exp.fock = fock
+        return exp
 
     def add_corr_operators(self, ops):
         """add_corr_operators! (src/expansion.jl:303-310)."""

@@ -175,3 +175,42 @@ def test_block_work_mappings():
             assert rel(v3, v3_cpu) < 2e-6
     finally:
         qiw.set_tuning()
+
+
+def test_autopartitioned_model_from_hamiltonian():
+    """Two-orbital Kanamori model built by Expansion(H, soi, grid; hybridization)
+    (src/expansion.jl:222-291): the autopartition gives blocks finer than the particle
+    number; GPU against the oracle for a short inchworm!."""
+    from test_model_construction import kanamori_H
+    print("kanamori2 autopartitioned n_tau=6 orders 0:2 N=2^5", flush=True)
+
+    def build():
+        f, H, labels = kanamori_H()
+        grid = model.ImaginaryTimeGrid(10.0, 6)
+        d = 0.25 * model.bethe_gf(1.0, 10.0, grid).values
+        hyb = np.zeros((len(labels), len(labels), grid.n_tau), complex)
+        for i in range(len(labels)):
+            hyb[i, i] = d
+        return model.Expansion.from_hamiltonian(H, f, grid, hyb)
+
+    kg, kc = build(), build()
+    assert len(kg.ed.dims) > 5
+    P_c, _, _ = O.inchworm(kc, range(0, 3), range(0, 3), 2 ** 5)
+    qiw.inchworm(kg, kg.grid, range(0, 3), range(0, 3), 2 ** 5)
+    assert rel(kg.P, P_c) < RTOL
+
+
+def test_real_nn_interaction_is_rejected():
+    """A real-valued nn-interaction U (the reference's ImaginaryTimeGF{Float64}) makes i*U
+    imaginary, which the real-arithmetic kernels do not take: QIW_E_ARG, no evaluation."""
+    from test_model_construction import anderson_H
+    f = model.FockSpace(["up", "dn"])
+    grid = model.ImaginaryTimeGrid(10.0, 8)
+    hyb = np.zeros((2, 2, 8), complex)
+    hyb[0, 0] = hyb[1, 1] = 0.25 * model.bethe_gf(1.0, 10.0, grid).values
+    U = np.zeros((2, 2, 8))
+    U[0, 1] = 0.3 * np.cosh(grid.tau - 5.0) / np.cosh(5.0)
+    e = model.Expansion.from_hamiltonian(anderson_H(f), f, grid, hyb, nn_interaction=U)
+    with pytest.raises(qiw.abi.QiwError) as err:
+        qiw.inchworm(e, e.grid, range(0, 2), range(0, 2), 2 ** 4)
+    assert err.value.code == qiw.abi.QIW_E_ARG
