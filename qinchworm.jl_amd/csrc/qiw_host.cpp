@@ -809,7 +809,7 @@ std::vector<int> orders_all(const qiw_run *run) {
 // Build the groups of a bold step (src/inchworm.jl:422-452) or the bare step (:373-392).
 // Words per topology chunk of flat scalar programs: long groups are cut into several tasks
 // per sample block so that no single task sets the length of a launch.
-constexpr int kChunkWords = 512;
+constexpr int kChunkWords = 640;
 // Column-words (program words x columns each segment is replayed for) per topology chunk of
 // vector programs, for the vector and matrix-core walks.
 constexpr int64_t kChunkCost = 1 << 18;
@@ -1423,7 +1423,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     };
     // per-step scrambled sequences: coordinates of every group's sequences for this step
     auto coords = [&](const Batch &b, const DevBatch &d, int domain, int key) -> hipError_t {
-        if (!b.scrambled) return hipSuccess;
+        if (!b.scrambled || run->N_samples <= 0) return hipSuccess;  // N = 0: order 0 only
         return qiw_launch_rqmc_coords(c->stream, d.groups, (int)b.groups.size(), 2 * b.max_order, run->N_samples,
                                       sobm, rq.seed, domain, key, d.ut);
     };
@@ -1932,7 +1932,7 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         double *dout_std = rqmc ? (double *)dev(c, "corr.out_std", (size_t)m.ntau * 8, &rc) : nullptr;
         uint32_t *sobm = b.scrambled ? (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc) : nullptr;
         if (rc) return rc;
-        if (sobm) {  // one set of scrambled sequences for all tau rows of the pair
+        if (sobm && run->N_samples > 0) {  // one set of scrambled sequences for all tau rows of the pair
             HIPCHK(c, hipMemcpy(sobm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
             HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, ng, 2 * b.max_order, run->N_samples, sobm, rq.seed,
                                              QIW_RQ_CORR, ci, d.ut));
@@ -2232,7 +2232,7 @@ extern "C" int qiw_autotune(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
     const int fmt = program_format(m, max_order, run->precision);
     std::vector<qiw_tuning> cand;
     if (fmt == FMT_SCALAR) {
-        for (int w : {256, 512, 1024, 2048}) cand.push_back(qiw_tuning{w, 0, 0, 0, 0, 0.0});
+        for (int w : {256, 384, 512, 640, 768, 1024, 2048}) cand.push_back(qiw_tuning{w, 0, 0, 0, 0, 0.0});
     } else if (fmt == FMT_VEC && m.mmax <= 6) {
         for (int e = 16; e <= 20; ++e) cand.push_back(qiw_tuning{0, 0, (int64_t)1 << e, 0, 0, 0.0});
     } else if (fmt == FMT_VEC) {

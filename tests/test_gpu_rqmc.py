@@ -148,3 +148,19 @@ def test_rand_params_rejected():
         assert ctx.lib.qiw_inchworm(ctx.h, run.ref(), None, None, None) == abi.QIW_E_ARG
     finally:
         qiw.warp_enable(False)
+
+
+def test_no_samples_with_scrambled_sequences():
+    """N_samples = 0: only the exact order 0 contributes (src/inchworm.jl:158), with or
+    without randomization."""
+    e1, e2 = anderson(8), anderson(8)
+    Po1, Ps1 = qiw.inchworm(e1, e1.grid, range(0, 3), range(0, 3), 0)
+    Po2, Ps2 = qiw.inchworm(e2, e2.grid, range(0, 3), range(0, 3), 0,
+                            rand_params=qiw.RandomizationParams(SEED, 3, 0.0))
+    np.testing.assert_array_equal(e1.P, e2.P)
+    assert np.all(Po2[1] == 0) and np.all(Ps2[1] == 0)
+    ec = model.anderson_single_orbital(n_tau=8, with_correlators=True)
+    ec.P, ec.P_n_known = e1.P, ec.grid.n_tau
+    g, gs = qiw.correlator_2p(ec, ec.grid, range(0, 3), 0, qiw.RequestStdDev(),
+                              rand_params=qiw.RandomizationParams(SEED, 3, 0.0))
+    assert np.all(np.isfinite(g[0])) and np.all(gs[0] == 0)
