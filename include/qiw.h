@@ -252,7 +252,9 @@ int qiw_simplex_points(qiw_ctx *ctx, int32_t d_before, int32_t d_after, double t
 typedef struct {
     double wall_s;              /* host wall time of the call                             */
     double kernel_ms;           /* summed device time of the evaluator launches (HIP events
-                                   on the launch stream around each launch)               */
+                                   on the launch stream around each launch); a bold-step
+                                   loop replayed from its hipGraph counts as a whole, its
+                                   bookkeeping kernels included (qiw_set_graphs)          */
     int64_t n_kernel_launches;  /* evaluator launches                                     */
     int64_t diagram_evals;      /* (topology, sample) pairs evaluated                     */
     int64_t leaves;             /* (topology, sample, initial sector, pair assignment)
@@ -284,6 +286,12 @@ typedef struct {
     double best_ms;     /* out (qiw_autotune): evaluator launch time of the kept one */
 } qiw_tuning;
 int qiw_set_tuning(qiw_ctx *ctx, const qiw_tuning *t);
+/* qiw_set_graphs: no counterpart in the reference.  With on = 1 (the default) the
+ * bold-step loop of qiw_inchworm / qiw_inchworm_step on the flat scalar walk (all sectors
+ * 1x1, no learned warp) is captured once in a hipGraph and replayed while every launch
+ * argument is the same, so a repeated run issues one graph launch instead of two kernel
+ * launches per step; results are identical.  on = 0 launches every kernel from the host. */
+int qiw_set_graphs(qiw_ctx *ctx, int32_t on);
 int qiw_get_tuning(qiw_ctx *ctx, qiw_tuning *out);
 int qiw_autotune(qiw_ctx *ctx, const qiw_run *run, qiw_tuning *out);
 

@@ -59,6 +59,9 @@ struct DevBuf {
 struct PlanStore;  // compiled batches resident on the device, by buffer tag (qiw_host.cpp)
 void plans_destroy(PlanStore *p);
 
+struct StepGraph;  // the bold-step loop of an inchworm run captured in a hipGraph (qiw_host.cpp)
+void step_graph_destroy(StepGraph *g);
+
 struct WarpState;  // learned warp of the unit cube (qiw_warp.cpp)
 void warp_destroy(WarpState *w);
 bool warp_enabled(const WarpState *w);
@@ -99,6 +102,8 @@ struct qiw_ctx {
     qiwi::WarpState *warp = nullptr;  // learned warp, or null
     qiwi::PlanStore *plans = nullptr;  // compiled batches kept between calls
     qiw_tuning tune{};                 // work mapping (qiw_set_tuning / qiw_autotune); 0 = default
+    bool graphs = true;                // replay the bold-step loop from a hipGraph (qiw_set_graphs)
+    qiwi::StepGraph *graph = nullptr;  // the captured loop and the launch arguments it holds
 };
 
 namespace qiwi {

@@ -898,6 +898,29 @@ struct PlanStore {
     std::map<std::string, Plan> by_tag;
 };
 void plans_destroy(PlanStore *p) { delete p; }
+
+// The bold-step loop of an inchworm run (per step: coordinates when scrambled, the
+// evaluator between two timing events, the bookkeeping kernel) captured once and replayed
+// while every launch argument is the same: `key` holds their bytes.
+struct StepGraph {
+    std::string key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;  // two around each replay
+    void clear() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : ev) (void)hipEventDestroy(e);
+        exec = nullptr;
+        graph = nullptr;
+        ev.clear();
+        key.clear();
+    }
+};
+void step_graph_destroy(StepGraph *g) {
+    if (g) g->clear();
+    delete g;
+}
 }  // namespace qiwi
 
 namespace {
@@ -1178,6 +1201,7 @@ extern "C" void qiw_destroy(qiw_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     warp_destroy(c->warp);
     plans_destroy(c->plans);
+    step_graph_destroy(c->graph);
     for (auto &kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1463,6 +1487,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         c->stats.leaves += bare_b.leaves_per_launch;
         c->stats.diagram_evals += bare_b.diagram_evals_per_launch;
     }
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> graph_ev;  // timing events of a replayed loop
     if (!bold_b.groups.empty()) {
         EvalArgs a = make_eval_args(c, bold_b, bold_d, ptab, eshift, dcoef, dkind, partial);
         fa.groups = bold_d.groups;
@@ -1471,7 +1496,96 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         fa.group_order_slot = bold_d.g_slot;
         fa.n_groups = (int)bold_b.groups.size();
         fa.bare = 0;
-        for (int jw = first_w; jw <= last_w; ++jw) {
+        // arguments of every step, as the eager loop below would launch them
+        auto step_args = [&](int jw) {
+            a.step_w = jw;
+            fa.jf = fa.jt = jw + 1;
+            fa.tau_w = jw * m.h;
+            fa.tau_f = (jw + 1) * m.h;
+            if (m.ppgf_kind == QIW_GF_INC_SPLINE && normalize) n_known = std::max(n_known, jw + 2);
+            fa.n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? n_known : m.ntau;
+            fa.normalize = normalize ? 1 : 0;
+        };
+        // flat scalar walk without the warp: the whole loop is one hipGraph, replayed while
+        // the launch arguments are unchanged (the other walks stage data from the host per
+        // launch and stay eager)
+        bool graph = c->graphs && bold_b.fmt == FMT_SCALAR && !warped;
+        const int n_known_start = n_known;
+        if (graph) {
+            if (!c->graph) c->graph = new StepGraph;
+            StepGraph &G = *c->graph;
+            const int n_known0 = n_known;
+            std::string key;
+            auto put = [&](const void *p, size_t n) { key.append((const char *)p, n); };
+            const int n_tasks = (int)bold_b.tasks.size();
+            put(&n_tasks, 4);
+            put(&bold_b.max_order, 4);
+            put(&rq.seed, 8);
+            const uint32_t *sm = sobm;
+            put(&sm, sizeof(sm));
+            const int scr = bold_b.scrambled ? 1 : 0;
+            put(&scr, 4);
+            put(&run->N_samples, 8);
+            put(&m.ns, 4);
+            put(&m.ntab, 4);
+            fa.n_real = (int)bold_b.r_first.size();  // as finalize() sets them for this batch
+            fa.real_first = bold_d.r_first;
+            fa.real_count = bold_d.r_count;
+            for (int jw = first_w; jw <= last_w; ++jw) {
+                step_args(jw);
+                put(&a, sizeof(a));
+                put(&fa, sizeof(fa));
+            }
+            if (G.key != key || !G.exec) {
+                G.clear();
+                n_known = n_known0;
+                for (int i = 0; i < 2; ++i) {
+                    hipEvent_t e;
+                    HIPCHK(c, hipEventCreate(&e));
+                    G.ev.push_back(e);
+                }
+                HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                hipError_t err = hipSuccess;
+                for (int jw = first_w; jw <= last_w && err == hipSuccess; ++jw) {
+                    step_args(jw);
+                    if ((err = coords(bold_b, bold_d, QIW_RQ_BOLD, jw)) != hipSuccess) break;
+                    if ((err = qiw_launch_eval_flat(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab)) != hipSuccess) break;
+                    err = finalize(bold_b, bold_d);
+                }
+                hipGraph_t gr = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
+                G.graph = gr;
+                if (err == hipSuccess) err = e2;
+                if (err == hipSuccess) err = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
+                if (err != hipSuccess) {  // capture not available: the eager loop from now on
+                    G.clear();
+                    (void)hipGetLastError();
+                    c->graphs = false;
+                    graph = false;
+                    n_known = n_known0;
+                } else {
+                    G.key = key;
+                }
+            }
+        }
+        if (graph) {
+            // events recorded inside a captured graph carry no timing here: the replay is timed
+            // as a whole (evaluators, bookkeeping and the gaps between them)
+            StepGraph &G = *c->graph;
+            HIPCHK(c, hipEventRecord(G.ev[0], c->stream));
+            HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
+            HIPCHK(c, hipEventRecord(G.ev[1], c->stream));
+            graph_ev.push_back({G.ev[0], G.ev[1]});
+            n_known = n_known_start;
+            for (int jw = first_w; jw <= last_w; ++jw) {  // host-side state of the loop
+                step_args(jw);
+                ++launches;
+                c->stats.flops += bold_b.flops_per_launch;
+                c->stats.leaves += bold_b.leaves_per_launch;
+                c->stats.diagram_evals += bold_b.diagram_evals_per_launch;
+            }
+        }
+        for (int jw = first_w; jw <= last_w && !graph; ++jw) {
             a.step_w = jw;
             if (warped && (rc = warp_points(c, jw * m.h / m.beta, bold_d.ut))) return rc;
             HIPCHK(c, coords(bold_b, bold_d, QIW_RQ_BOLD, jw));
@@ -1495,6 +1609,11 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     for (size_t i = 0; i + 1 < ev.size(); i += 2) {
         float ms = 0;
         HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        c->stats.kernel_ms += ms;
+    }
+    for (auto &pr : graph_ev) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, pr.first, pr.second));
         c->stats.kernel_ms += ms;
     }
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -2203,6 +2322,12 @@ void free_tag(qiw_ctx *c, const std::string &prefix) {
     }
 }
 }  // namespace
+
+extern "C" int qiw_set_graphs(qiw_ctx *c, int32_t on) {
+    if (!c) return QIW_E_ARG;
+    c->graphs = on != 0;
+    return QIW_OK;
+}
 
 extern "C" int qiw_set_tuning(qiw_ctx *c, const qiw_tuning *t) {
     if (!c || !t) return QIW_E_ARG;

@@ -146,7 +146,7 @@ EXPORTS = ["qiw_create", "qiw_destroy", "qiw_last_error", "qiw_abi_version", "qi
            "qiw_partition_function",
            "qiw_density_matrix", "qiw_sobol_points", "qiw_simplex_points", "qiw_get_stats",
            "qiw_warp_init", "qiw_warp_params", "qiw_warp_apply", "qiw_warp_loss_grad", "qiw_warp_train",
-           "qiw_warp_enable", "qiw_set_tuning", "qiw_get_tuning", "qiw_autotune"]
+           "qiw_warp_enable", "qiw_set_tuning", "qiw_get_tuning", "qiw_autotune", "qiw_set_graphs"]
 
 _lib = None
 
@@ -192,6 +192,7 @@ def load_library(path=None):
     lib.qiw_set_tuning.argtypes = [vp, C.POINTER(qiw_tuning)]
     lib.qiw_get_tuning.argtypes = [vp, C.POINTER(qiw_tuning)]
     lib.qiw_autotune.argtypes = [vp, C.POINTER(qiw_run), C.POINTER(qiw_tuning)]
+    lib.qiw_set_graphs.argtypes = [vp, C.c_int32]
     _lib = lib
     return lib
 

@@ -190,6 +190,12 @@ def _tuning_dict(t):
     return {f: getattr(t, f) for f, _ in abi.qiw_tuning._fields_ if f != "pad_"}
 
 
+def set_graphs(on=True, ctx=None):
+    """Replay the bold-step loop from a hipGraph (default on; results are identical)."""
+    ctx = ctx or context()
+    ctx.check(ctx.lib.qiw_set_graphs(ctx.h, 1 if on else 0))
+
+
 def set_tuning(chunk_words=0, chunk_cost=0, samples_per_task=0, ctx=None):
     """Fix the work mapping of the evaluators (qiw_set_tuning); 0 keeps a default."""
     ctx = ctx or context()

@@ -264,3 +264,36 @@ def test_eight_sectors_and_order_five(ctx, K, N):
     for o in range(K + 1):
         assert np.abs(per_gpu[o] - per_cpu[o]).max() < RTOL * np.abs(v_cpu).max()
     assert np.abs(per_cpu[K]).max() > 0
+
+
+def test_step_graph_replays_match_eager():
+    """The bold-step loop replayed from its hipGraph gives the eager run's results bit for
+    bit, on every replay, with other GPU work (a correlator, Sobol points) and host
+    synchronisation between the replays; a change of the run (another N) is captured anew."""
+    print("step graph: anderson n_tau=24 orders 0:3 N=2^8, 3 replays", flush=True)
+    ctx = qiw.context()
+    try:
+        qiw.set_graphs(False)
+        e = model.anderson_single_orbital(n_tau=24, with_correlators=True)
+        Po_e, _ = qiw.inchworm(e, e.grid, range(0, 4), range(0, 4), 2 ** 8)
+        P_e = e.P.copy()
+        s_e = qiw.stats()
+        qiw.set_graphs(True)
+        for rep in range(3):
+            g = model.anderson_single_orbital(n_tau=24, with_correlators=True)
+            Po_g, _ = qiw.inchworm(g, g.grid, range(0, 4), range(0, 4), 2 ** 8)
+            np.testing.assert_array_equal(g.P, P_e)
+            for o in Po_e:
+                np.testing.assert_array_equal(Po_g[o], Po_e[o])
+            s_g = qiw.stats()
+            assert s_g["n_kernel_launches"] == s_e["n_kernel_launches"] and s_g["kernel_ms"] > 0
+            qiw.correlator_2p(g, g.grid, range(0, 3), 2 ** 6)  # eager work between replays
+            sobol_on_gpu(ctx, ScrambledSobolSeq(4), 0, 64)
+        h = model.anderson_single_orbital(n_tau=24, with_correlators=True)
+        qiw.inchworm(h, h.grid, range(0, 4), range(0, 4), 2 ** 7)
+        hc = model.anderson_single_orbital(n_tau=24, with_correlators=True)
+        qiw.set_graphs(False)
+        qiw.inchworm(hc, hc.grid, range(0, 4), range(0, 4), 2 ** 7)
+        np.testing.assert_array_equal(h.P, hc.P)
+    finally:
+        qiw.set_graphs(True)
