@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the two- and three-orbital step measurements")
     ap.add_argument("--no-rqmc", action="store_true", help="skip the randomized-qMC study line")
+    ap.add_argument("--block-models-only", action="store_true",
+                    help="only the two- and three-orbital steps (vector and matrix-core walks), one JSON line")
     ap.add_argument("--warp-study", action="store_true",
                     help="only run the learned-warp error-versus-samples study and print its JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -300,6 +302,10 @@ def main():
         return
     if args.warp_study:
         print(json.dumps(warp_study()), flush=True)
+        return
+    if args.block_models_only:
+        print(json.dumps({"two_orbital_step": two_orbital(with_cpu=not args.no_cpu),
+                          "three_orbital_step": three_orbital(with_cpu=not args.no_cpu)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "

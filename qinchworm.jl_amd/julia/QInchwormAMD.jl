@@ -357,6 +357,15 @@ set_tuning!(t::QiwTuning = QiwTuning(); ctx::Context = context()) =
     check(ctx, ccall((:qiw_set_tuning, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwTuning}), ctx.h, Ref(t)))
 
 """
+    set_graphs!(on::Bool; ctx)
+
+Replay the bold-step loop of `inchworm!` from a hipGraph (the default; results are
+identical) or launch every kernel from the host (`qiw_set_graphs`).
+"""
+set_graphs!(on::Bool; ctx::Context = context()) =
+    check(ctx, ccall((:qiw_set_graphs, libqiw), Cint, (Ptr{Cvoid}, Int32), ctx.h, Int32(on)))
+
+"""
     autotune!(expansion, grid, orders, N_samples; precision = QIW_PREC_FP64, ctx)
 
 Time candidate work mappings on one bold step at the expansion's current `P` (tau_w at
