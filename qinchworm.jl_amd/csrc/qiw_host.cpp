@@ -1025,7 +1025,7 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
             c->mfma_staged = true;
         }
         const int KM = a.max_order <= 3 ? 3 : 4, NM = 2 * KM + 3;
-        const size_t lds = ((((size_t)(NM - 1) * m.bt + 3) & ~(size_t)3) + 1024) * 4;
+        const size_t lds = ((((size_t)(NM - 1) * m.bt + 3) & ~(size_t)3) + 32 * 34) * 4;  // + scratch, kScrLd 34
         if (lds > 160 * 1024) return fail(c, QIW_E_LIMIT, "propagator blocks exceed the 160 KiB of LDS of a CU");
         MfmaArgs mf{};
         mf.e = a;

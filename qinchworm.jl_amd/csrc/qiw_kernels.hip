@@ -1059,6 +1059,14 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
 // float32 products (the QIW_PREC_FP32 mode); sums over samples in float64.
 // ---------------------------------------------------------------------------------------
 typedef float mf4 __attribute__((ext_vector_type(4)));
+typedef float mf2 __attribute__((ext_vector_type(2)));
+
+// Leading dimension of the 32-column scratch.  34 = 2 (mod 32) spreads both accesses over
+// the 32 LDS banks: a B-operand read (lane (q, c) at column 16 tj + c, row 4 kc + q) hits
+// bank 2c + q, distinct within each half wave, and a tile store, two 8-byte writes per lane,
+// covers banks 2c, 2c + 1 in each 16-lane group.  (A stride of 32 put all 16 columns of a
+// read in one bank: 8 conflict cycles per access, 45 % of the walk's time.)
+constexpr int kScrLd = 34;
 
 template <int TB>
 struct MTile {
@@ -1071,7 +1079,7 @@ struct MState {
     double acc[TB][TB][4];                                  // leaf sums, accumulator layout
     ProgStream ps;
     const float *__restrict__ pl;   // i*P blocks of the sample in LDS: [(L - 2) * bt + BT offset]
-    float *__restrict__ scr;        // 32 x 32 column-major scratch
+    float *__restrict__ scr;        // 32-column column-major scratch, leading dimension kScrLd
     const float *__restrict__ ops;  // operator blocks, column-major
     int bt;
 };
@@ -1084,7 +1092,11 @@ __device__ __forceinline__ MTile<TB> mmul(const float *__restrict__ A, int lda, 
 #pragma unroll
     for (int ti = 0; ti < TB; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < TB; ++tj) *(mf4 *)&scr[(16 * tj + c) * 32 + 16 * ti + 4 * q] = X.t[ti][tj];
+        for (int tj = 0; tj < TB; ++tj) {
+            float *p = &scr[(16 * tj + c) * kScrLd + 16 * ti + 4 * q];
+            *(mf2 *)p = mf2{X.t[ti][tj][0], X.t[ti][tj][1]};
+            *(mf2 *)(p + 2) = mf2{X.t[ti][tj][2], X.t[ti][tj][3]};
+        }
     MTile<TB> C;
 #pragma unroll
     for (int ti = 0; ti < TB; ++ti)
@@ -1103,7 +1115,7 @@ __device__ __forceinline__ MTile<TB> mmul(const float *__restrict__ A, int lda, 
 #pragma unroll
                     for (int tj = 0; tj < TB; ++tj) {
                         if (tj < nt) {
-                            const float b = scr[(16 * tj + c) * 32 + k];
+                            const float b = scr[(16 * tj + c) * kScrLd + k];
                             C.t[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, C.t[ti][tj], 0, 0, 0);
                         }
                     }
@@ -1144,7 +1156,7 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, TB> &st, int n, const MTile
 template <int KM, int TM, int TB>
 __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
     constexpr int NM = 2 * KM + 3;
-    extern __shared__ float mlds[];  // [(NM - 1) * bt] i*P blocks, then the 32 x 32 scratch
+    extern __shared__ float mlds[];  // [(NM - 1) * bt] i*P blocks, then the 32 x kScrLd scratch
     const EvalArgs &a = ma.e;
     const int lane = threadIdx.x, q = lane >> 4, cl = lane & 15;
     const QiwTask task = a.tasks[blockIdx.x];
