@@ -150,12 +150,12 @@ def three_orbital(log2n=12, cpu_n=16, with_cpu=True):
     return out
 
 
-def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10, 17)), steps=(10, 25, 45),
+def warp_study(n_tau=50, orders=(0, 1, 2), ref_log2n=20, log2ns=tuple(range(10, 17)), steps=(10, 25, 45),
                iters=4000, n_train=2 ** 14):
     """Error versus samples of the learned warp against plain qMC on the two-orbital model
     (C2 physics at n_tau=50, orders 0:2).  The propagator comes from a plain float64 run;
-    the flow is trained on |integrand| samples of that run; for three bold steps the judge is
-    the float64 step at 2^judge_log2n plain points (the GPU float64 path, equal to the
+    the flow is trained on |integrand| samples of that run; for three bold steps the reference is
+    the float64 step at 2^ref_log2n plain points (the GPU float64 path, equal to the
     oracle to 1e-10 in the parity tests)."""
     import numpy as np
     import qiw
@@ -172,12 +172,12 @@ def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10
     for si, jw in enumerate(steps):
         k.P = P.copy()
         qiw.warp_enable(False)
-        judge, _ = qiw.inchworm_step(k, jw, orders, 2 ** judge_log2n)
+        ref, _ = qiw.inchworm_step(k, jw, orders, 2 ** ref_log2n)
         for ni, ln in enumerate(log2ns):
             for name, on in (("plain", False), ("warped", True)):
                 qiw.warp_enable(on)
                 v, _ = qiw.inchworm_step(k, jw, orders, 2 ** ln)
-                err[name][si, ni] = np.linalg.norm(v - judge) / np.linalg.norm(judge)
+                err[name][si, ni] = np.linalg.norm(v - ref) / np.linalg.norm(ref)
     qiw.warp_enable(False)
     e_plain, e_warp = err["plain"].mean(axis=0), err["warped"].mean(axis=0)
     # plain samples needed to reach the warped error, by log-log interpolation
@@ -196,7 +196,7 @@ def warp_study(n_tau=50, orders=(0, 1, 2), judge_log2n=20, log2ns=tuple(range(10
         x = log2ns[j - 1] + (lp[j - 1] - target) / (lp[j - 1] - lp[j])
         gains.append(float(2 ** x / 2 ** ln))
     return {"workload": f"two-orbital Kanamori, n_tau={n_tau}, orders {orders[0]}:{orders[-1]}, bold steps "
-                        f"{list(steps)}; judge: plain float64 at 2^{judge_log2n}",
+                        f"{list(steps)}; reference: plain float64 at 2^{ref_log2n}",
             "N": [2 ** ln for ln in log2ns], "rel_err_plain": e_plain.tolist(), "rel_err_warped": e_warp.tolist(),
             "sample_gain": gains, "train_iters": iters, "train_samples_per_step_group": n_train,
             "train_s": train_s,

@@ -120,7 +120,7 @@ def test_training_lowers_loss_and_keeps_estimate():
 
 def test_trained_warp_gains_over_plain():
     """What the trained network gains: at 2^10 points per group, the warped estimate of two
-    bold steps of the two-orbital model is closer to a 2^18-point plain judge than the plain
+    bold steps of the two-orbital model is closer to a 2^18-point plain reference than the plain
     estimate at the same cost (measured gain about 10x in error)."""
     k = kanamori2(n_tau=20)
     orders = [0, 1, 2]
@@ -131,14 +131,14 @@ def test_trained_warp_gains_over_plain():
     e_plain, e_warp = [], []
     for jw in (5, 15):
         k.P = P.copy()
-        judge, _ = qiw.inchworm_step(k, jw, orders, 2 ** 18)
+        ref, _ = qiw.inchworm_step(k, jw, orders, 2 ** 18)
         v_plain, _ = qiw.inchworm_step(k, jw, orders, 2 ** 10)
         qiw.warp_enable(True)
         try:
             v_warp, _ = qiw.inchworm_step(k, jw, orders, 2 ** 10)
         finally:
             qiw.warp_enable(False)
-        e_plain.append(np.linalg.norm(v_plain - judge) / np.linalg.norm(judge))
-        e_warp.append(np.linalg.norm(v_warp - judge) / np.linalg.norm(judge))
+        e_plain.append(np.linalg.norm(v_plain - ref) / np.linalg.norm(ref))
+        e_warp.append(np.linalg.norm(v_warp - ref) / np.linalg.norm(ref))
     print(f"rel. error at 2^10: plain {np.mean(e_plain):.2e}, warped {np.mean(e_warp):.2e}")
     assert np.mean(e_warp) < 0.5 * np.mean(e_plain)
