@@ -1978,6 +1978,24 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
     if (rc) return rc;
     HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
     std::vector<int> oall(run->orders, run->orders + run->n_orders);
+    std::vector<int32_t> dg;  // BT indices of the diagonal elements (the trace)
+    for (int s = 0; s < m.ns; ++s)
+        for (int i = 0; i < m.dims[s]; ++i) dg.push_back(m.bt_off[s] + i + i * m.dims[s]);
+    int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
+    uint32_t *sobm = rq.scrambled ? (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc) : nullptr;
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(diag, dg.data(), dg.size() * 4, hipMemcpyHostToDevice));
+    if (sobm) HIPCHK(c, hipMemcpy(sobm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
+    // the pairs are queued on the stream back to back; their results are read after one
+    // synchronisation
+    struct PairOut {
+        int ci;
+        std::vector<double> out, out_std;
+        hipEvent_t e0, e1;
+        bool any_sampled;
+    };
+    std::vector<PairOut> pending;
+    pending.reserve(m.ncorr);
     for (int ci = 0; ci < m.ncorr; ++ci) {
         // groups (src/inchworm.jl:957-981): every n_pts_after of every order
         const std::string tag = "corr" + std::to_string(ci);
@@ -2044,19 +2062,24 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         DevBatch &d = p.d;
         if (b.groups.empty()) continue;
         int ng = (int)b.groups.size();
-        double *partial = (double *)dev(c, "corr.partial", b.tasks.size() * m.bt * 8, &rc);
-        int32_t *diag = (int32_t *)dev(c, "diag", (size_t)m.bt * 4, &rc);
+        // the device buffers are reused by the next pair in stream order (its evaluator runs
+        // after this pair's bookkeeping and copies); the host results stay per pair
+        const size_t partial_bytes = b.tasks.size() * m.bt * 8;
+        if (c->bufs["corr.partial"].n < partial_bytes)  // the buffer grows: let queued pairs finish with it
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        double *partial = (double *)dev(c, "corr.partial", partial_bytes, &rc);
         int32_t *drf = p.rf, *drc = p.rcnt;
         double *dout = (double *)dev(c, "corr.out", (size_t)m.ntau * 8, &rc);
         double *dout_std = rqmc ? (double *)dev(c, "corr.out_std", (size_t)m.ntau * 8, &rc) : nullptr;
-        uint32_t *sobm = b.scrambled ? (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc) : nullptr;
         if (rc) return rc;
-        if (sobm && run->N_samples > 0) {  // one set of scrambled sequences for all tau rows of the pair
-            HIPCHK(c, hipMemcpy(sobm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
+        if (sobm && run->N_samples > 0)  // one set of scrambled sequences for all tau rows of the pair
             HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, ng, 2 * b.max_order, run->N_samples, sobm, rq.seed,
                                              QIW_RQ_CORR, ci, d.ut));
-        }
-        hipEvent_t e0, e1;
+        pending.push_back(PairOut{ci, std::vector<double>(m.ntau), std::vector<double>(m.ntau, 0.0), nullptr, nullptr,
+                                  false});
+        PairOut &po = pending.back();
+        for (auto &cg : b.groups) po.any_sampled = po.any_sampled || cg.g.sampled;
+        hipEvent_t &e0 = po.e0, &e1 = po.e1;
         HIPCHK(c, hipEventCreate(&e0));
         HIPCHK(c, hipEventCreate(&e1));
         HIPCHK(c, hipEventRecord(e0, c->stream));
@@ -2070,10 +2093,6 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         fa.n_groups = ng;
         fa.n_rows = m.ntau;
         fa.n_tau = m.ntau;
-        std::vector<int32_t> dg;
-        for (int s = 0; s < m.ns; ++s)
-            for (int i = 0; i < m.dims[s]; ++i) dg.push_back(m.bt_off[s] + i + i * m.dims[s]);
-        HIPCHK(c, hipMemcpy(diag, dg.data(), dg.size() * 4, hipMemcpyHostToDevice));
         fa.partial = partial;
         fa.bt = m.bt;
         fa.diag_idx = diag;
@@ -2088,21 +2107,24 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         fa.target_std = rq.target_std;
         fa.out_std = dout_std;
         HIPCHK(c, rqmc ? qiw_launch_finalize_corr_rqmc(c->stream, fa) : qiw_launch_finalize_corr(c->stream, fa));
-        std::vector<double> out(m.ntau), out_std(m.ntau, 0.0);
-        HIPCHK(c, hipMemcpyAsync(out.data(), dout, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
-        if (dout_std) HIPCHK(c, hipMemcpyAsync(out_std.data(), dout_std, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        c->stats.kernel_ms += ms;
+        HIPCHK(c, hipMemcpyAsync(po.out.data(), dout, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
+        if (dout_std)
+            HIPCHK(c, hipMemcpyAsync(po.out_std.data(), dout_std, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
         c->stats.n_kernel_launches += 1;
         c->stats.flops += b.flops_per_launch;
         c->stats.leaves += b.leaves_per_launch;
         c->stats.diagram_evals += b.diagram_evals_per_launch;
-        bool any_sampled = false;
-        for (auto &cg : b.groups) any_sampled = any_sampled || cg.g.sampled;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (PairOut &po : pending) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, po.e0, po.e1));
+        (void)hipEventDestroy(po.e0);
+        (void)hipEventDestroy(po.e1);
+        c->stats.kernel_ms += ms;
+        const int ci = po.ci;
+        const std::vector<double> &out = po.out, &out_std = po.out_std;
+        const bool any_sampled = po.any_sampled;
         for (int j = 0; j < m.ntau; ++j) {  // value = -i * out / Z
             const size_t i = (size_t)ci * m.ntau + j;
             corr_out[2 * i] = 0.0;
