@@ -104,6 +104,7 @@ struct qiw_ctx {
     qiw_tuning tune{};                 // work mapping (qiw_set_tuning / qiw_autotune); 0 = default
     bool graphs = true;                // replay the bold-step loop from a hipGraph (qiw_set_graphs)
     qiwi::StepGraph *graph = nullptr;  // the captured loop and the launch arguments it holds
+    std::vector<hipEvent_t> evpool;    // timing events of the eager launches, reused
 };
 
 namespace qiwi {

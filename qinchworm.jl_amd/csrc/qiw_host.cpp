@@ -1202,6 +1202,7 @@ extern "C" void qiw_destroy(qiw_ctx *c) {
     warp_destroy(c->warp);
     plans_destroy(c->plans);
     step_graph_destroy(c->graph);
+    for (auto e : c->evpool) (void)hipEventDestroy(e);
     for (auto &kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1453,13 +1454,16 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     };
 
     // every evaluator launch is bracketed by an event pair on the launch stream
-    std::vector<hipEvent_t> ev;
+    // timing events from the context's pool (created once, reused by every call)
+    size_t n_ev = 0;
     auto mark = [&](void) -> hipError_t {
-        hipEvent_t e;
-        hipError_t r = hipEventCreate(&e);
-        if (r != hipSuccess) return r;
-        ev.push_back(e);
-        return hipEventRecord(e, c->stream);
+        if (n_ev == c->evpool.size()) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            c->evpool.push_back(e);
+        }
+        return hipEventRecord(c->evpool[n_ev++], c->stream);
     };
     int64_t launches = 0;
     int n_known = c->n_known;
@@ -1606,9 +1610,9 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+    for (size_t i = 0; i + 1 < n_ev; i += 2) {
         float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->evpool[i], c->evpool[i + 1]));
         c->stats.kernel_ms += ms;
     }
     for (auto &pr : graph_ev) {
@@ -1616,7 +1620,6 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         HIPCHK(c, hipEventElapsedTime(&ms, pr.first, pr.second));
         c->stats.kernel_ms += ms;
     }
-    for (auto e : ev) (void)hipEventDestroy(e);
     c->stats.n_kernel_launches += launches;
 
     std::vector<double> ptab_h((size_t)m.ntau * m.bt), por((size_t)n_slots * m.ntau * m.bt), lam_h(m.ntau);
@@ -1832,13 +1835,16 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
     // writes the row of tau_f_prev (row a)
     const int sig_shift = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? 1 : 0;
 
-    std::vector<hipEvent_t> ev;
+    // timing events from the context's pool (created once, reused by every call)
+    size_t n_ev = 0;
     auto mark = [&](void) -> hipError_t {
-        hipEvent_t e;
-        hipError_t r = hipEventCreate(&e);
-        if (r != hipSuccess) return r;
-        ev.push_back(e);
-        return hipEventRecord(e, c->stream);
+        if (n_ev == c->evpool.size()) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            c->evpool.push_back(e);
+        }
+        return hipEventRecord(c->evpool[n_ev++], c->stream);
     };
     EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
     int64_t launches = 0;
@@ -1862,12 +1868,11 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
         c->stats.diagram_evals += b.diagram_evals_per_launch;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+    for (size_t i = 0; i + 1 < n_ev; i += 2) {
         float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->evpool[i], c->evpool[i + 1]));
         c->stats.kernel_ms += ms;
     }
-    for (auto e : ev) (void)hipEventDestroy(e);
     c->stats.n_kernel_launches += launches;
     std::vector<double> P_h(tab), por((size_t)n_slots * tab), pstd_h(pstd ? por.size() : 0);
     HIPCHK(c, hipMemcpy(P_h.data(), ptab, tab * 8, hipMemcpyDeviceToHost));
