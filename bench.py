@@ -203,7 +203,7 @@ def warp_study(n_tau=50, orders=(0, 1, 2), ref_log2n=20, log2ns=tuple(range(10, 
             "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
 
 
-def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678):
+def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678, with_cpu=True, cpu_steps=2):
     """The reference's randomized-qMC study (bench/rqmc/rqmc.jl:37-117) on one GPU: the
     Bethe single-level model, inchworm! orders 0:order (bare and bold) and then
     correlator_2p(RequestStdDev) orders 0:order-1, N samples per sequence, N_seqs scrambled
@@ -228,6 +228,21 @@ def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678):
     run()  # plans and buffers
     exp, Ps, g, gs, dt, s1, s2 = run()
     evals = s1["diagram_evals"] + s2["diagram_evals"]
+    cpu = {}
+    if with_cpu:  # the oracle on the bare step and the first bold steps of the same inchworm!
+        import oracle_lib as O
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        O.reset_counters()
+        t0 = time.perf_counter()
+        O.inchworm(model.bethe_single_level(n_tau), range(0, order + 1), range(0, order + 1), N,
+                   n_steps_max=cpu_steps, rand=(n_seqs, 1, seed, 0.0))
+        dc = time.perf_counter() - t0
+        _, ce, _ = O.counters()
+        cpu = {"cpu_evals_per_s": ce / dc, "cpu_cores": threads,
+               "cpu_sample": f"inchworm! bare step + {cpu_steps} bold steps, {ce} diagram evals in {dc:.2f} s, "
+                             "float64 oracle, same sequences",
+               "speedup": (evals / dt) / (ce / dc)}
     return {"workload": f"bench/rqmc study: Bethe single level, beta=10, n_tau={n_tau}, inchworm! orders 0:{order}, "
                         f"correlator_2p(RequestStdDev) orders 0:{order - 1}, N=2^{log2n} per sequence, "
                         f"N_seqs={n_seqs} scrambled, float64",
@@ -235,7 +250,7 @@ def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678):
             "evaluator_ms": s1["kernel_ms"] + s2["kernel_ms"],
             "max_P_std": float(max(np.abs(v).max() for v in Ps.values())),
             "max_G_std": float(np.abs(gs[0]).max()),
-            "max_G_error_vs_exact": float(np.abs(g[0] - exp.corr_exact).max())}
+            "max_G_error_vs_exact": float(np.abs(g[0] - exp.corr_exact).max()), **cpu}
 
 
 def diff_study(log2n=14, with_cpu=True, cpu_steps=4):
@@ -381,7 +396,7 @@ def main():
         line["two_orbital_step"] = two_orbital(with_cpu=not args.no_cpu)
         line["three_orbital_step"] = three_orbital(with_cpu=not args.no_cpu)
     if not args.no_rqmc:
-        line["rqmc_study"] = rqmc_study()
+        line["rqmc_study"] = rqmc_study(with_cpu=not args.no_cpu)
         line["diff_inchworm"] = diff_study(with_cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
