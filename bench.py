@@ -11,6 +11,8 @@ sample) integrand evaluation summed over initial sectors and pair assignments), 
 roofline of the evaluator kernel and the CPU float64 oracle timed on a bounded sample.
 """
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -53,9 +55,10 @@ def two_orbital(log2n=14, cpu_n=16, with_cpu=True):
     out = {"workload": f"two-orbital Kanamori (C2 model), one bold step tau_w=grid[100], orders 0:4, N=2^{log2n}"}
     vals = {}
     for name, prec in (("fp64", abi.QIW_PREC_FP64), ("fp32", abi.QIW_PREC_FP32)):
-        t0 = time.perf_counter()
-        v, _ = qiw.inchworm_step(k, 100, range(0, 5), N, precision=prec)
-        dt = time.perf_counter() - t0
+        with no_gc():
+            t0 = time.perf_counter()
+            v, _ = qiw.inchworm_step(k, 100, range(0, 5), N, precision=prec)
+            dt = time.perf_counter() - t0
         st = qiw.stats()
         vals[name] = v
         out[f"gpu_{name}_s"] = dt
@@ -77,6 +80,22 @@ def two_orbital(log2n=14, cpu_n=16, with_cpu=True):
         out["speedup_fp64"] = out["gpu_fp64_evals_per_s"] / out["cpu_evals_per_s"]
         out["speedup_fp32"] = out["gpu_fp32_evals_per_s"] / out["cpu_evals_per_s"]
     return out
+
+
+
+@contextlib.contextmanager
+def no_gc():
+    """A timed region without Python's cyclic collector: collect first, then keep it off, as
+    timeit does (a full collection of the interpreter's objects takes tens of milliseconds
+    and would otherwise land inside whichever step happens to cross its threshold)."""
+    gc.collect()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
 
 
 def c1_expansion():
@@ -126,9 +145,10 @@ def three_orbital(log2n=12, cpu_n=16, with_cpu=True):
     N = 2 ** log2n
     out = {"workload": f"three-orbital Kanamori (C3 model), one bold step tau_w=grid[200], orders 0:3, "
                        f"N=2^{log2n}, float32 chain products on MFMA"}
-    t0 = time.perf_counter()
-    qiw.inchworm_step(k, 200, range(0, 4), N, precision=abi.QIW_PREC_FP32)
-    dt = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        qiw.inchworm_step(k, 200, range(0, 4), N, precision=abi.QIW_PREC_FP32)
+        dt = time.perf_counter() - t0
     st = qiw.stats()
     out["gpu_fp32_s"] = dt
     out["gpu_fp32_evals_per_s"] = st["diagram_evals"] / dt
@@ -226,7 +246,8 @@ def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678, with_cpu=T
         return exp, Ps, g, gs, time.perf_counter() - t0, s1, s2
 
     run()  # plans and buffers
-    exp, Ps, g, gs, dt, s1, s2 = run()
+    with no_gc():
+        exp, Ps, g, gs, dt, s1, s2 = run()
     evals = s1["diagram_evals"] + s2["diagram_evals"]
     cpu = {}
     if with_cpu:  # the oracle on the bare step and the first bold steps of the same inchworm!
@@ -262,9 +283,10 @@ def diff_study(log2n=14, with_cpu=True, cpu_steps=4):
     exp = c1_expansion()
     qiw.diff_inchworm(exp, exp.grid, range(0, 5), N)  # plans and buffers
     exp = c1_expansion()
-    t0 = time.perf_counter()
-    qiw.diff_inchworm(exp, exp.grid, range(0, 5), N)
-    dt = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        qiw.diff_inchworm(exp, exp.grid, range(0, 5), N)
+        dt = time.perf_counter() - t0
     st = qiw.stats()
     out = {"workload": f"diff_inchworm! on the C1 model (single-orbital Anderson, beta=10, n_tau=200), orders 0:4, "
                        f"N=2^{log2n}, float64, 199 steps",
@@ -345,15 +367,16 @@ def main():
             one_step(exp, N)
         sync()
         acc = {"evals": 0.0, "flops": 0.0, "kernel_ms": 0.0, "launches": 0}
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            s1, s2 = one_step(exp, N)
-            acc["evals"] += s1["diagram_evals"] + s2["diagram_evals"]
-            acc["flops"] += s1["flops"] + s2["flops"]
-            acc["kernel_ms"] += s1["kernel_ms"] + s2["kernel_ms"]
-            acc["launches"] += s1["n_kernel_launches"] + s2["n_kernel_launches"]
-        sync()
-        acc["dt"] = time.perf_counter() - t0
+        with no_gc():
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                s1, s2 = one_step(exp, N)
+                acc["evals"] += s1["diagram_evals"] + s2["diagram_evals"]
+                acc["flops"] += s1["flops"] + s2["flops"]
+                acc["kernel_ms"] += s1["kernel_ms"] + s2["kernel_ms"]
+                acc["launches"] += s1["n_kernel_launches"] + s2["n_kernel_launches"]
+            sync()
+            acc["dt"] = time.perf_counter() - t0
         return acc
 
     g = timed(True)
