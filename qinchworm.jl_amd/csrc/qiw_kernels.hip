@@ -199,6 +199,10 @@ template <typename T>
 struct RegVec<T, 16> {
     typedef T type __attribute__((ext_vector_type(16)));
 };
+template <typename T>
+struct RegVec<T, 32> {
+    typedef T type __attribute__((ext_vector_type(32)));
+};
 constexpr int regvec_size(int n) { return n <= 8 ? 8 : 16; }
 
 // v[i] for a uniform i as a v_cndmask chain over registers.  The empty asm makes each
@@ -269,6 +273,8 @@ struct FlatShape {
     static constexpr int NH = (NM - 3) * SM;   // inner entries (NM-1-L)*SM + s, L = 3..NM-1
     static constexpr int NC = (NH + 15) / 16;  // register vectors of 16 entries
     static_assert(NC >= 1 && NC <= 6, "inner propagator entries beyond 6 register vectors");
+    // at most 32 entries: low and high words in one pair of 32-dword vectors
+    static constexpr bool SPLIT = NH <= 32;
 };
 
 // The inner propagator entries of one lane live in up to six register vectors of 16 doubles
@@ -276,10 +282,20 @@ struct FlatShape {
 // then selects between register values and never becomes a dynamically indexed (scratch)
 // address.  Entry e is element e % 16 of vector e / 16 (e uniform; constant in unrolled
 // loops).  NC_ is the number of vectors in use.
+//
+// Up to 32 entries (SPLIT) are held instead as two vectors of 32 dwords, the low words of
+// the entries in PL and their high words in PH: one uniform index then reads both halves of
+// an entry from the same element of the two vectors (two indexed moves), where two vectors
+// of 16 doubles need a read from each and a select.  Wider shapes keep the double vectors:
+// the dword pairs cost them registers (and occupancy) that the select does not.
 #define QIW_PIN_SET(NC_, e, x)                              \
     do {                                                    \
         const int c_ = (e) >> 4, i_ = (e)&15;               \
-        if (c_ == 0) P0[i_] = (x);                          \
+        if (F::SPLIT) {                                     \
+            const unsigned long long b_ = (unsigned long long)__double_as_longlong(x); \
+            PL[(e)&31] = (uint32_t)b_;                      \
+            PH[(e)&31] = (uint32_t)(b_ >> 32);              \
+        } else if (c_ == 0) P0[i_] = (x);                   \
         else if ((NC_) > 1 && c_ == 1) P1[i_] = (x);        \
         else if ((NC_) > 2 && c_ == 2) P2[i_] = (x);        \
         else if ((NC_) > 3 && c_ == 3) P3[i_] = (x);        \
@@ -291,7 +307,11 @@ struct FlatShape {
 #define QIW_PIN_GET(NC_, w, out)                            \
     do {                                                    \
         const int c_ = ((w) >> 5) & 7, i_ = ((w) >> 1) & 15; \
-        if ((NC_) == 1 || ((NC_) == 2 ? ((w) & 32) == 0 : c_ == 0)) out = P0[i_]; \
+        if (F::SPLIT) {                                     \
+            const int j_ = ((w) >> 1) & 31;                 \
+            const uint32_t lo_ = PL[j_], hi_ = PH[j_];      \
+            out = __longlong_as_double((long long)(((unsigned long long)hi_ << 32) | lo_)); \
+        } else if ((NC_) == 1 || ((NC_) == 2 ? ((w) & 32) == 0 : c_ == 0)) out = P0[i_]; \
         else if ((NC_) == 2 || c_ == 1) out = P1[i_];       \
         else if ((NC_) == 3 || c_ == 2) out = P2[i_];       \
         else if ((NC_) == 4 || c_ == 3) out = P3[i_];       \
@@ -368,6 +388,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
 
     // ---- propagator values i*P_s(t_L - t_{L-1}), L = 2..n (ordering clamp) ----
     typename RegVec<double, 16>::type P0 = 0.0, P1 = 0.0, P2 = 0.0, P3 = 0.0, P4 = 0.0, P5 = 0.0;
+    typename RegVec<uint32_t, 32>::type PL = 0u, PH = 0u;  // SPLIT shapes
     double p_first[SM], p_last[SM];  // iP_s of segment 2 and of segment n
 #pragma unroll
     for (int L = 2; L <= NM; ++L) {
