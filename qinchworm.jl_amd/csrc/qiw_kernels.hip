@@ -453,18 +453,21 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
                 cur *= (W2 && (w & (16u << 8))) ? Wh[(int)((w >> 8) & 15)] : W[(int)((w >> 8) & (NW - 1))];
             }
             if (w & QIW_FL_LEAF) {
-                if (w & QIW_FL_UNIT) {
-                    acc += (w & QIW_FL_NEG) ? -cur : cur;
-                } else {
+                // acc += c * product with c = +-1 (UNIT, a uniform select) or the group's
+                // coefficient; one fused multiply-add either way (exact for c = +-1)
+                double c = (w & QIW_FL_NEG) ? -1.0 : 1.0;
+                if (!(w & QIW_FL_UNIT)) {
                     asm volatile("");
-                    acc += coefs[w & 0xFF] * cur;
+                    c = coefs[w & 0xFF];
                 }
-            } else {
-                double pv;
-                QIW_PIN_GET(NC, w, pv);
-                cur *= pv;
-                if (w & QIW_FL_PUSH) sk[(L - 2) * QIW_WAVE + lane] = cur;
+                acc = fma(c, cur, acc);
+                w = ps.next();
+                continue;
             }
+            double pv;
+            QIW_PIN_GET(NC, w, pv);
+            cur *= pv;
+            if (w & QIW_FL_PUSH) sk[(L - 2) * QIW_WAVE + lane] = cur;
             w = ps.next();
             continue;
         }
