@@ -447,7 +447,11 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
         }
         if (w < QIW_FL_OP) {  // EDGE with POP, LEAF or PUSH
             const int L = (int)((w >> 16) & 15);
-            if (w & QIW_FL_POP) cur = (L == 2) ? cur0 : sk[(L - 3) * QIW_WAVE + lane];
+            if (w & QIW_FL_POP) {  // level 2 restores the root's product, deeper levels a saved row
+                asm volatile("");
+                const double saved = sk[(L > 2 ? L - 3 : 0) * QIW_WAVE + lane];
+                cur = (L == 2) ? cur0 : saved;
+            }
             if (w & QIW_FL_TAIL) {
                 asm volatile("");
                 cur *= (W2 && (w & (16u << 8))) ? Wh[(int)((w >> 8) & 15)] : W[(int)((w >> 8) & (NW - 1))];
