@@ -809,7 +809,7 @@ std::vector<int> orders_all(const qiw_run *run) {
 // Build the groups of a bold step (src/inchworm.jl:422-452) or the bare step (:373-392).
 // Words per topology chunk of flat scalar programs: long groups are cut into several tasks
 // per sample block so that no single task sets the length of a launch.
-constexpr int kChunkWords = 640;
+constexpr int kChunkWords = 896;
 // Column-words (program words x columns each segment is replayed for) per topology chunk of
 // vector programs, for the vector and matrix-core walks.
 constexpr int64_t kChunkCost = 1 << 18;
@@ -2384,7 +2384,7 @@ extern "C" int qiw_autotune(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
     const int fmt = program_format(m, max_order, run->precision);
     std::vector<qiw_tuning> cand;
     if (fmt == FMT_SCALAR) {
-        for (int w : {256, 384, 512, 640, 768, 1024, 2048}) cand.push_back(qiw_tuning{w, 0, 0, 0, 0, 0.0});
+        for (int w : {256, 384, 512, 640, 896, 1024, 2048}) cand.push_back(qiw_tuning{w, 0, 0, 0, 0, 0.0});
     } else if (fmt == FMT_VEC && m.mmax <= 6) {
         for (int e = 16; e <= 20; ++e) cand.push_back(qiw_tuning{0, 0, (int64_t)1 << e, 0, 0, 0.0});
     } else if (fmt == FMT_VEC) {
