@@ -267,7 +267,8 @@ int qiw_get_stats(qiw_ctx *ctx, qiw_stats *out);
  * No counterpart in the reference (its work units are fixed by MPI rank ranges).  How
  * the evaluator's work is cut into workgroups:
  *   chunk_words       flat scalar programs (1x1 sectors): program words per topology chunk;
- *                     each (chunk, 64-sample block) is one workgroup.  0: 896.
+ *                     each (chunk, 64-sample block) is one workgroup.  0: 896
+ *                     (640 for the self-energy batches of qiw_diff_inchworm).
  *   chunk_cost        vector programs (blocks <= 32): program words x replayed columns per
  *                     topology chunk.  0: 2^18.
  *   samples_per_task  walks that evaluate one sample per wavefront at a time (matrix-core
