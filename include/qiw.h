@@ -47,8 +47,10 @@ enum {
 
 /* ---- interpolation kinds of time-dependent quantities ------------------------------- */
 enum {
-    QIW_GF_GRID_LINEAR = 0,  /* Keldysh ImaginaryTimeGF stored on the grid, linear in the
-                                time difference                                          */
+    QIW_GF_GRID_LINEAR = 0,  /* Keldysh ImaginaryTimeGF stored on the grid, evaluated at
+                                (t1, t2) as Keldysh's interpolate! does: bilinear over the
+                                grid cell of (t1, t2) with G[i1, i2] = g(i1 - i2), linear
+                                in t1 - t2 inside a diagonal cell                        */
     QIW_GF_CUBIC_SPLINE = 1, /* SplineInterpolatedGF: natural cubic spline through the
                                 grid values (src/spline_gf.jl:194-242)                   */
     QIW_GF_INC_SPLINE = 2    /* IncSplineImaginaryTimeGF: incremental quadratic spline

@@ -486,15 +486,39 @@ struct IncSpline {
     }
 };
 
-// Linear interpolation in the time difference over the uniform grid.  Stand-in for
-// Keldysh.jl's ImaginaryTimeGF evaluation (parity unpinned).
-static inline cd grid_linear(const cd *vals, int n, double h, double z) {
-    double x = z / h;
-    int j = (int)std::floor(x);
-    if (j < 0) j = 0;
-    if (j > n - 2) j = n - 2;
-    double f = x - j;
-    return (1.0 - f) * vals[j] + f * vals[j + 1];
+// A grid-stored imaginary-time Green's function evaluated at two contour times t1 >= t2:
+// Keldysh.jl's interpolate! on a grid container, behind kd.interpolate!(x, P_s, t_f, t_i)
+// (src/topology_eval.jl:368) and behind the call of an ImaginaryTimeGF pair propagator
+// (src/topology_eval.jl:414).  The container stores g by index difference, G[i1, i2] =
+// g(i1 - i2); the value is the bilinear interpolation over the grid cell (i1, i2) that holds
+// (t1, t2), and inside a diagonal cell (i1 == i2) the interpolation over its lower triangle,
+// which is linear in t1 - t2.  With a, b the fractions of t1, t2 in their intervals and
+// d = i1 - i2 both read
+//     g(d) + a (1 - b) (g(d+1) - g(d)) - (1 - a) b (g(d) - g(d-1)),   g(-1) = 2 g(0) - g(1).
+// Pinned by the reference's golden /inchworm/P and /inchworm/g (test/inchworm.jl:180-212,
+// a grid P) and /values of test/topology_eval.h5 (test/topology_eval.jl:37-155, a grid P
+// and grid Delta), both reproduced to round-off; linear interpolation in t1 - t2 alone
+// misses them by 2e-8 and 3e-4.
+struct GridCell {
+    int d;          // i1 - i2
+    double wp, wm;  // a (1 - b), (1 - a) b
+};
+static inline GridCell grid_cell(double t1, double t2, double h, int n) {
+    double x1 = t1 / h, x2 = t2 / h;
+    int i1 = (int)std::floor(x1), i2 = (int)std::floor(x2);
+    if (i1 < 0) i1 = 0;
+    if (i2 < 0) i2 = 0;
+    if (i1 > n - 2) i1 = n - 2;
+    if (i2 > n - 2) i2 = n - 2;
+    const double a = x1 - i1, b = x2 - i2;
+    return GridCell{i1 - i2, a * (1.0 - b), (1.0 - a) * b};
+}
+// g at cell c from grid values vals(j), j = 0..n-1
+template <class G>
+static inline cd grid_bilinear(const GridCell &c, G vals) {
+    const cd g0 = vals(c.d), gp = vals(c.d + 1);
+    const cd dm = c.d > 0 ? g0 - vals(c.d - 1) : gp - g0;
+    return g0 + c.wp * (gp - g0) - c.wm * dm;
 }
 
 // A scalar propagator of an InteractionPair: g(t_f, t_i)
@@ -513,9 +537,16 @@ struct ScalarGF {
         for (int j = 0; j < n; ++j) vals[j] = cd(v[2 * j], v[2 * j + 1]);
         if (kind == QIW_GF_CUBIC_SPLINE) spl.build(vals, beta);
     }
-    cd at(double z) const { return kind == QIW_GF_CUBIC_SPLINE ? spl.eval(z) : grid_linear(vals.data(), (int)vals.size(), h, z); }
-    cd operator()(double ref1, double ref2) const {  // src/spline_gf.jl:210-219
-        return ref1 >= ref2 ? at(ref1 - ref2) : double(xi) * at(beta + ref1 - ref2);
+    // SplineInterpolatedGF (src/spline_gf.jl:210-219): the spline in ref1 - ref2, or
+    // xi * spline(beta + ref1 - ref2) when ref1 < ref2.  A grid ImaginaryTimeGF: the cell
+    // interpolation above (the evaluator calls it with ref1 >= ref2 only, after its ordering
+    // clamp, src/topology_eval.jl:408-411).
+    cd operator()(double ref1, double ref2) const {
+        if (kind == QIW_GF_CUBIC_SPLINE)
+            return ref1 >= ref2 ? spl.eval(ref1 - ref2) : double(xi) * spl.eval(beta + ref1 - ref2);
+        if (!(ref1 >= ref2)) return cd(0);
+        const int n = (int)vals.size();
+        return grid_bilinear(grid_cell(ref1, ref2, h, n), [&](int j) { return vals[j]; });
     }
 };
 
@@ -635,12 +666,11 @@ static void atomic_ppgf(const Model &m, PPGF &P) {  // src/ppgf.jl:144-207, :710
     }
 }
 
-// Bold P lookup, kd.interpolate!(x, P_s, t_f, t_i) (src/topology_eval.jl:368).
-// For both container kinds the value is the linear interpolation, in the time difference,
-// of the grid values P_s(tau_j).  For IncSplineImaginaryTimeGF this is what reproduces the
-// reference's golden first bold step to 1e-16 (the quadratic IncrementalSpline of
-// src/utility.jl:147-157 misses it by 1e-6): Keldysh's generic interpolate! works on the
-// wrapped grid function, the incremental splines are bookkeeping only.
+// Bold P lookup, kd.interpolate!(x, P_s, t_f, t_i) (src/topology_eval.jl:368): the grid
+// cell interpolation of the stored values P_s(tau_j) (grid_bilinear) for both container
+// kinds.  For IncSplineImaginaryTimeGF Keldysh's generic interpolate! works on the wrapped
+// grid function, the incremental splines are bookkeeping only (the quadratic
+// IncrementalSpline of src/utility.jl:147-157 misses the golden first bold step by 1e-6).
 static void ppgf_lookup(const Model &m, const PPGF &P, int s, double tf, double ti, CMat &out) {
     int d = m.dims[s];
     out.r = out.c = d;
@@ -649,14 +679,8 @@ static void ppgf_lookup(const Model &m, const PPGF &P, int s, double tf, double 
         for (auto &v : out.a) v = cd(0);
         return;
     }
-    double z = tf - ti;
-    double x = z / m.h;
-    int j = (int)std::floor(x);
-    if (j < 0) j = 0;
-    if (j > m.ntau - 2) j = m.ntau - 2;
-    double f = x - j;
-    const CMat &a = P.tab[j][s], &b = P.tab[j + 1][s];
-    for (int e = 0; e < d * d; ++e) out.a[e] = (1.0 - f) * a.a[e] + f * b.a[e];
+    const GridCell c = grid_cell(tf, ti, m.h, m.ntau);
+    for (int e = 0; e < d * d; ++e) out.a[e] = grid_bilinear(c, [&](int j) { return P.tab[j][s].a[e]; });
 }
 
 // Exact atomic P0 (src/exact_atomic_ppgf.jl:109-118)

@@ -5,6 +5,8 @@
 
 #include <complex>
 #include <map>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -111,6 +113,21 @@ namespace qiwi {
 
 // Records msg as the context's last error and returns code.
 int fail(qiw_ctx *c, int code, const std::string &msg);
+
+// Runs the body of a C entry point: a C++ exception (host memory exhausted, an index out of
+// range) becomes a QIW_E_* code and a message instead of crossing the C ABI.
+template <class F>
+int guarded(qiw_ctx *c, F &&body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return c ? fail(c, QIW_E_LIMIT, "host memory exhausted") : QIW_E_LIMIT;
+    } catch (const std::exception &e) {
+        return c ? fail(c, QIW_E_ARG, std::string("internal error: ") + e.what()) : QIW_E_ARG;
+    } catch (...) {
+        return c ? fail(c, QIW_E_ARG, "internal error") : QIW_E_ARG;
+    }
+}
 
 // Device buffer `name` of at least `bytes` bytes, kept by the context between calls.
 void *dev(qiw_ctx *c, const std::string &name, size_t bytes, int *rc);

@@ -129,8 +129,12 @@ struct QiwDeltaTables {
     int32_t n_tables;
     int32_t n_tau;
     double h, beta;
+    int32_t grid_mask;     // bit t set: table t is a grid container (else a natural cubic spline)
     const int32_t *kind;   // [n_tables] QIW_GF_GRID_LINEAR / QIW_GF_CUBIC_SPLINE
-    const double *coef;    // [n_tables][n_tau-1][4] local cubic (linear: c2 = c3 = 0)
+    // [n_tables][n_tau-1][4]: a spline table holds the cubic (c0..c3) of interval j in the
+    // time difference; a grid table holds, for index d, (g(d), g(d+1) - g(d), g(d) - g(d-1), 0)
+    // with g(-1) = 2 g(0) - g(1) (the grid cell interpolation of the kernels' grid_cell)
+    const double *coef;
 };
 
 // Arguments of the diagram evaluators (k_eval_flat_t; the other walks embed them)
@@ -235,8 +239,9 @@ struct BlockArgs {
 // Arguments of the vector evaluator (k_eval_vec_t)
 struct VecArgs {
     EvalArgs e;
-    const double2 *pt;         // [n_tau-1][bt] (value, slope) of the i*P table (bold groups)
-    const float2 *pt32;        // the same in float32
+    const double2 *pt;         // [n_tau-1][bt] (value, forward difference) of the i*P table (bold groups)
+    const double *ptm;         // [n_tau-1][bt] backward differences (row 0: the forward one)
+    const float4 *pt32;        // (value, forward, backward, 0) in float32
     const double *ediag;       // [bt] E + lambda on the diagonal positions (bare groups)
     const double *ops;         // operator blocks, column-major, padded by 64 entries
     const float *ops32;        // the same in float32
@@ -290,7 +295,8 @@ hipError_t qiw_launch_eval_flat(hipStream_t st, const EvalArgs &a, int n_tasks, 
 hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks, size_t lds_bytes);
 hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes);
 hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm, int fp32);
-hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, float2 *out32);
+hipError_t qiw_launch_ptab_cells(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, double *outm,
+                                 float4 *out32);
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);
 hipError_t qiw_launch_finalize_step_rqmc(hipStream_t st, const FinalizeArgs &a);

@@ -562,7 +562,17 @@ struct Batch {
     int max_order = 0;  // largest order of the batch: selects the kernel instantiation
     double flops_per_launch = 0;  // algorithmic FLOPs of one launch
     int64_t leaves_per_launch = 0, diagram_evals_per_launch = 0;
+    bool overflow = false;  // the launch would exceed kMaxTasks workgroups
 };
+
+// Workgroups of one launch: task rows and grid sizes are 32-bit and every task is a
+// 24-byte record on the host and the device, so a launch holds at most 2^28 tasks
+// (e.g. N = 2^30 samples of 16 groups in 64-sample tasks is already 2^28).
+constexpr int64_t kMaxTasks = (int64_t)1 << 28;
+inline bool tasks_fit(Batch &b, int64_t more) {
+    if ((int64_t)b.tasks.size() + more > kMaxTasks) b.overflow = true;
+    return !b.overflow;
+}
 
 struct DevBatch {
     QiwGroup *groups = nullptr;
@@ -866,7 +876,8 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             const int64_t leaves = N * cg.leaves_per_sample, evals = N * cg.n_top;
             push_sequences(b, std::move(cg), b.n_eval, [&](int gi) {
                 b.g_first.push_back((int32_t)b.tasks.size());
-                int nt = (int)((N + b.spt - 1) / b.spt);
+                const int64_t nt64 = (N + b.spt - 1) / b.spt;
+                const int nt = tasks_fit(b, nt64 * (int64_t)chunks.size()) ? (int)nt64 : 0;
                 for (int t = 0; t < nt; ++t)
                     for (int32_t ch : chunks) b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size(), ch, 0});
                 b.g_count.push_back(nt * (int)chunks.size());
@@ -878,6 +889,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
         }
     }
     if (b.r_first.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 (order, n_pts_after) groups in one step");
+    if (b.overflow) return fail(c, QIW_E_LIMIT, "more than 2^28 workgroups in one launch (lower N_samples)");
     order_tasks(b);
     return 0;
 }
@@ -1048,15 +1060,17 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
         double *ops = (double *)dev(c, "vec.ops", (m.op_data.size() + pad) * 8, &rc);
         double *ediag = (double *)dev(c, "vec.ediag", ((size_t)m.bt + 2 * pad) * 8, &rc);
         double2 *pt = (double2 *)dev(c, "vec.pt", ((size_t)(m.ntau - 1) * m.bt + pad) * 16, &rc);
+        double *ptm = (double *)dev(c, "vec.ptm", ((size_t)(m.ntau - 1) * m.bt + pad) * 8, &rc);
         float *ops32 = (float *)dev(c, "vec.ops32", (m.op_data.size() + pad) * 4, &rc);
-        float2 *pt32 = (float2 *)dev(c, "vec.pt32", ((size_t)(m.ntau - 1) * m.bt + pad) * 8, &rc);
+        float4 *pt32 = (float4 *)dev(c, "vec.pt32", ((size_t)(m.ntau - 1) * m.bt + pad) * 16, &rc);
         if (rc) return rc;
         if (!c->vec_staged) {  // once per model
             std::vector<double> o(m.op_data), ed((size_t)m.bt + 2 * pad, 0.0);
             o.resize(m.op_data.size() + pad, 0.0);
             std::vector<float> o32(o.begin(), o.end());
             HIPCHK(c, hipMemcpy(ops32, o32.data(), o32.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemset(pt32, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 8));
+            HIPCHK(c, hipMemset(pt32, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 16));
+            HIPCHK(c, hipMemset(ptm, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 8));
             for (int s = 0; s < m.ns; ++s)
                 for (int i = 0; i < m.dims[s]; ++i) ed[m.bt_off[s] + i + i * m.dims[s]] = m.E[s][i] + m.lambda0;
             HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1065,10 +1079,11 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
             HIPCHK(c, hipMemset(pt, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 16));
             c->vec_staged = true;
         }
-        HIPCHK(c, qiw_launch_ptab_pairs(c->stream, a.ptab, m.ntau, m.bt, pt, pt32));
+        HIPCHK(c, qiw_launch_ptab_cells(c->stream, a.ptab, m.ntau, m.bt, pt, ptm, pt32));
         VecArgs v{};
         v.e = a;
         v.pt = pt;
+        v.ptm = ptm;
         v.pt32 = pt32;
         v.ediag = ediag;
         v.ops = ops;
@@ -1121,6 +1136,9 @@ EvalArgs make_eval_args(qiw_ctx *c, const Batch &bt, const DevBatch &d, double *
     a.dtab.h = m.h;
     a.dtab.beta = m.beta;
     a.dtab.kind = dkind;
+    a.dtab.grid_mask = 0;
+    for (int t = 0; t < m.ntab && t < 31; ++t)
+        if (m.dkind[t] != QIW_GF_CUBIC_SPLINE) a.dtab.grid_mask |= 1 << t;
     a.dtab.coef = dcoef;
     a.ns = m.ns;
     a.n_tau = m.ntau;
@@ -1153,7 +1171,7 @@ void store_complex_from_ip(double ipv, double *out) {  // P = -i * (i P)
 // =======================================================================================
 extern "C" int qiw_abi_version(void) { return QIW_ABI_VERSION; }
 
-extern "C" int qiw_create(qiw_ctx **out, int device) {
+static int qiw_create_impl(qiw_ctx **out, int device) {
     if (!out) return QIW_E_ARG;
     *out = nullptr;
     int n = 0;
@@ -1199,7 +1217,7 @@ std::string model_fingerprint(const HostModel &m) {
 }
 }  // namespace
 
-extern "C" void qiw_destroy(qiw_ctx *c) {
+static void qiw_destroy_impl(qiw_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1215,7 +1233,7 @@ extern "C" void qiw_destroy(qiw_ctx *c) {
 
 extern "C" const char *qiw_last_error(const qiw_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
+static int qiw_set_model_impl(qiw_ctx *c, const qiw_model *mi) {
     if (!c || !mi) return QIW_E_ARG;
     if (mi->n_tau < 3 || mi->n_sectors < 1 || !(mi->beta > 0)) return fail(c, QIW_E_ARG, "bad grid or sectors");
     HostModel m;
@@ -1272,16 +1290,18 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
         m.pair_table[p] = found;
     }
     m.ntab = (int)tabs.size();
+    if (m.ntab > 31) return fail(c, QIW_E_LIMIT, "more than 31 distinct hybridization tables");
     m.dcoef.assign((size_t)std::max(m.ntab, 1) * (m.ntau - 1) * 4, 0.0);
     m.dkind = tab_kind;
     for (int t = 0; t < m.ntab; ++t) {
         double *cc = m.dcoef.data() + (size_t)t * (m.ntau - 1) * 4;
         if (tab_kind[t] == QIW_GF_CUBIC_SPLINE) {
             natural_spline_coef(tabs[t], m.h, cc);
-        } else {
+        } else {  // grid container: value, forward and backward difference per index
             for (int j = 0; j < m.ntau - 1; ++j) {
                 cc[4 * j] = tabs[t][j];
                 cc[4 * j + 1] = tabs[t][j + 1] - tabs[t][j];
+                cc[4 * j + 2] = j > 0 ? tabs[t][j] - tabs[t][j - 1] : cc[4 * j + 1];
             }
         }
     }
@@ -1327,7 +1347,7 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
     return QIW_OK;
 }
 
-extern "C" int qiw_set_sobol(qiw_ctx *c, int32_t dim, const uint32_t *m, const uint32_t *shift) {
+static int qiw_set_sobol_impl(qiw_ctx *c, int32_t dim, const uint32_t *m, const uint32_t *shift) {
     if (!c || !m || dim < 1 || dim > 2 * QIW_KMAX) return c ? fail(c, QIW_E_DIM, "Sobol dimension out of range") : QIW_E_ARG;
     c->sob_dim = dim;
     c->sob_m.assign(m, m + (size_t)dim * 32);
@@ -1336,7 +1356,7 @@ extern "C" int qiw_set_sobol(qiw_ctx *c, int32_t dim, const uint32_t *m, const u
     return QIW_OK;
 }
 
-extern "C" int qiw_set_ppgf(qiw_ctx *c, int32_t n_known, const double *P, const double *der0) {
+static int qiw_set_ppgf_impl(qiw_ctx *c, int32_t n_known, const double *P, const double *der0) {
     if (!c || !P) return QIW_E_ARG;
     if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
     const HostModel &m = c->m;
@@ -1352,7 +1372,7 @@ extern "C" int qiw_set_ppgf(qiw_ctx *c, int32_t n_known, const double *P, const 
     return QIW_OK;
 }
 
-extern "C" int qiw_get_ppgf(qiw_ctx *c, int32_t *n_known_out, double *P, double *der0) {
+static int qiw_get_ppgf_impl(qiw_ctx *c, int32_t *n_known_out, double *P, double *der0) {
     if (!c) return QIW_E_ARG;
     if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
     const HostModel &m = c->m;
@@ -1673,7 +1693,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
 
 }  // namespace
 
-extern "C" int qiw_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, double *P_orders_out,
+static int qiw_inchworm_impl(qiw_ctx *c, const qiw_run *run, double *P_out, double *P_orders_out,
                             double *P_orders_std_out) {
     if (!c) return QIW_E_ARG;
     int rc = check_run(c, run);
@@ -1737,7 +1757,8 @@ int build_diff_batch(qiw_ctx *c, const qiw_run *run, Batch &b) {
         const int64_t leaves = N * cg.leaves_per_sample, evals = N * cg.n_top;
         push_sequences(b, std::move(cg), b.n_eval, [&](int gi) {
             b.g_first.push_back((int32_t)b.tasks.size());
-            const int nt = (int)((N + b.spt - 1) / b.spt);
+            const int64_t nt64 = (N + b.spt - 1) / b.spt;
+            const int nt = tasks_fit(b, nt64 * (int64_t)chunks.size()) ? (int)nt64 : 0;
             for (int t = 0; t < nt; ++t)
                 for (int32_t ch : chunks) b.tasks.push_back({gi, t * b.spt, 0, (int32_t)b.tasks.size(), ch, 0});
             b.g_count.push_back(nt * (int)chunks.size());
@@ -1748,6 +1769,7 @@ int build_diff_batch(qiw_ctx *c, const qiw_run *run, Batch &b) {
         });
     }
     if (b.r_first.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 self-energy groups");
+    if (b.overflow) return fail(c, QIW_E_LIMIT, "more than 2^28 workgroups in one launch (lower N_samples)");
     order_tasks(b);
     return 0;
 }
@@ -1907,7 +1929,7 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
 }
 }  // namespace
 
-extern "C" int qiw_diff_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, double *Sigma_orders_out,
+static int qiw_diff_inchworm_impl(qiw_ctx *c, const qiw_run *run, double *P_out, double *Sigma_orders_out,
                                  double *Sigma_orders_std_out) {
     if (!c) return QIW_E_ARG;
     int rc = check_run(c, run);
@@ -1923,7 +1945,7 @@ extern "C" int qiw_diff_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, 
     return QIW_OK;
 }
 
-extern "C" int qiw_inchworm_step(qiw_ctx *c, const qiw_run *run, int32_t n_w, double *value_out, double *orders_out) {
+static int qiw_inchworm_step_impl(qiw_ctx *c, const qiw_run *run, int32_t n_w, double *value_out, double *orders_out) {
     if (!c) return QIW_E_ARG;
     int rc = check_run(c, run);
     if (rc) return rc;
@@ -1941,7 +1963,7 @@ extern "C" int qiw_inchworm_step(qiw_ctx *c, const qiw_run *run, int32_t n_w, do
     return QIW_OK;
 }
 
-extern "C" int qiw_inchworm_step_bare(qiw_ctx *c, const qiw_run *run, double *value_out, double *orders_out) {
+static int qiw_inchworm_step_bare_impl(qiw_ctx *c, const qiw_run *run, double *value_out, double *orders_out) {
     if (!c) return QIW_E_ARG;
     int rc = check_run(c, run);
     if (rc) return rc;
@@ -2046,8 +2068,11 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
                         // tau_A = tau_B: only the leading order-0 group (src/inchworm.jl:1014-1024)
                         if (j == 0 && !(gi == 0 && p.first_is_order0)) continue;
                         int64_t N = b.groups[gi].g.order > 0 ? run->N_samples : 1;
-                        int nt = (int)((N + b.spt - 1) / b.spt);
                         const std::vector<int32_t> &chs = b.groups[gi].chunks;
+                        const int64_t nt64 = (N + b.spt - 1) / b.spt;
+                        if (!tasks_fit(b, nt64 * (int64_t)chs.size()))
+                            return fail(c, QIW_E_LIMIT, "more than 2^28 workgroups in one launch (lower N_samples)");
+                        const int nt = (int)nt64;
                         rf[(size_t)j * ng + gi] = (int32_t)b.tasks.size();
                         rcnt[(size_t)j * ng + gi] = nt * (int)chs.size();
                         for (int t = 0; t < nt; ++t)
@@ -2154,16 +2179,16 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
 }
 }  // namespace
 
-extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_out) {
+static int qiw_correlator_2p_impl(qiw_ctx *c, const qiw_run *run, double *corr_out) {
     return run_correlator(c, run, corr_out, nullptr);
 }
 
-extern "C" int qiw_correlator_2p_std(qiw_ctx *c, const qiw_run *run, double *corr_out, double *corr_std_out) {
+static int qiw_correlator_2p_std_impl(qiw_ctx *c, const qiw_run *run, double *corr_out, double *corr_std_out) {
     if (!corr_std_out) return c ? fail(c, QIW_E_ARG, "null corr_std_out") : QIW_E_ARG;
     return run_correlator(c, run, corr_out, corr_std_out);
 }
 
-extern "C" int qiw_partition_function(qiw_ctx *c, double *Z_out) {
+static int qiw_partition_function_impl(qiw_ctx *c, double *Z_out) {
     if (!c || !Z_out) return QIW_E_ARG;
     if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
     const HostModel &m = c->m;
@@ -2175,7 +2200,7 @@ extern "C" int qiw_partition_function(qiw_ctx *c, double *Z_out) {
     return QIW_OK;
 }
 
-extern "C" int qiw_density_matrix(qiw_ctx *c, double *rho_out) {
+static int qiw_density_matrix_impl(qiw_ctx *c, double *rho_out) {
     if (!c || !rho_out) return QIW_E_ARG;
     double Z[2];
     int rc = qiw_partition_function(c, Z);
@@ -2191,7 +2216,7 @@ extern "C" int qiw_density_matrix(qiw_ctx *c, double *rho_out) {
     return QIW_OK;
 }
 
-extern "C" int qiw_sobol_points(qiw_ctx *c, int32_t dim, int64_t first, int64_t n, double *out) {
+static int qiw_sobol_points_impl(qiw_ctx *c, int32_t dim, int64_t first, int64_t n, double *out) {
     if (!c || !out || n < 0 || first < 0) return QIW_E_ARG;
     if (dim < 1 || dim > c->sob_dim) return fail(c, QIW_E_DIM, "Sobol dimension beyond the staged direction numbers");
     (void)hipSetDevice(c->device);
@@ -2209,7 +2234,7 @@ extern "C" int qiw_sobol_points(qiw_ctx *c, int32_t dim, int64_t first, int64_t 
     return QIW_OK;
 }
 
-extern "C" int qiw_simplex_points(qiw_ctx *c, int32_t d_before, int32_t d_after, double t_i, double t_w, double t_f,
+static int qiw_simplex_points_impl(qiw_ctx *c, int32_t d_before, int32_t d_after, double t_i, double t_w, double t_f,
                                   int64_t n, double *out) {
     if (!c || !out || n < 0 || d_before < 0 || d_after < 0) return QIW_E_ARG;
     int d = d_before + d_after;
@@ -2229,7 +2254,7 @@ extern "C" int qiw_simplex_points(qiw_ctx *c, int32_t d_before, int32_t d_after,
     return QIW_OK;
 }
 
-extern "C" int qiw_get_stats(qiw_ctx *c, qiw_stats *out) {
+static int qiw_get_stats_impl(qiw_ctx *c, qiw_stats *out) {
     if (!c || !out) return QIW_E_ARG;
     *out = c->stats;
     return QIW_OK;
@@ -2354,13 +2379,13 @@ void free_tag(qiw_ctx *c, const std::string &prefix) {
 }
 }  // namespace
 
-extern "C" int qiw_set_graphs(qiw_ctx *c, int32_t on) {
+static int qiw_set_graphs_impl(qiw_ctx *c, int32_t on) {
     if (!c) return QIW_E_ARG;
     c->graphs = on != 0;
     return QIW_OK;
 }
 
-extern "C" int qiw_set_tuning(qiw_ctx *c, const qiw_tuning *t) {
+static int qiw_set_tuning_impl(qiw_ctx *c, const qiw_tuning *t) {
     if (!c || !t) return QIW_E_ARG;
     int rc = check_tuning(c, *t);
     if (rc) return rc;
@@ -2370,13 +2395,13 @@ extern "C" int qiw_set_tuning(qiw_ctx *c, const qiw_tuning *t) {
     return QIW_OK;
 }
 
-extern "C" int qiw_get_tuning(qiw_ctx *c, qiw_tuning *out) {
+static int qiw_get_tuning_impl(qiw_ctx *c, qiw_tuning *out) {
     if (!c || !out) return QIW_E_ARG;
     *out = c->tune;
     return QIW_OK;
 }
 
-extern "C" int qiw_autotune(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
+static int qiw_autotune_impl(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
     if (!c) return QIW_E_ARG;
     int rc = check_run(c, run);
     if (rc) return rc;
@@ -2419,4 +2444,95 @@ extern "C" int qiw_autotune(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
     c->tune = best;
     if (out) *out = best;
     return QIW_OK;
+}
+
+// ---- C ABI: every entry point runs its body through guarded() ----
+extern "C" int qiw_create(qiw_ctx **out, int device) {
+    return guarded(nullptr, [&] { return qiw_create_impl(out, device); });
+}
+
+extern "C" void qiw_destroy(qiw_ctx *c) {
+    guarded(nullptr, [&] {
+        qiw_destroy_impl(c);
+        return 0;
+    });
+}
+
+extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
+    return guarded(c, [&] { return qiw_set_model_impl(c, mi); });
+}
+
+extern "C" int qiw_set_sobol(qiw_ctx *c, int32_t dim, const uint32_t *m, const uint32_t *shift) {
+    return guarded(c, [&] { return qiw_set_sobol_impl(c, dim, m, shift); });
+}
+
+extern "C" int qiw_set_ppgf(qiw_ctx *c, int32_t n_known, const double *P, const double *der0) {
+    return guarded(c, [&] { return qiw_set_ppgf_impl(c, n_known, P, der0); });
+}
+
+extern "C" int qiw_get_ppgf(qiw_ctx *c, int32_t *n_known_out, double *P, double *der0) {
+    return guarded(c, [&] { return qiw_get_ppgf_impl(c, n_known_out, P, der0); });
+}
+
+extern "C" int qiw_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, double *P_orders_out,
+                            double *P_orders_std_out) {
+    return guarded(c, [&] { return qiw_inchworm_impl(c, run, P_out, P_orders_out, P_orders_std_out); });
+}
+
+extern "C" int qiw_diff_inchworm(qiw_ctx *c, const qiw_run *run, double *P_out, double *Sigma_orders_out,
+                                 double *Sigma_orders_std_out) {
+    return guarded(c, [&] { return qiw_diff_inchworm_impl(c, run, P_out, Sigma_orders_out, Sigma_orders_std_out); });
+}
+
+extern "C" int qiw_inchworm_step(qiw_ctx *c, const qiw_run *run, int32_t n_w, double *value_out, double *orders_out) {
+    return guarded(c, [&] { return qiw_inchworm_step_impl(c, run, n_w, value_out, orders_out); });
+}
+
+extern "C" int qiw_inchworm_step_bare(qiw_ctx *c, const qiw_run *run, double *value_out, double *orders_out) {
+    return guarded(c, [&] { return qiw_inchworm_step_bare_impl(c, run, value_out, orders_out); });
+}
+
+extern "C" int qiw_correlator_2p(qiw_ctx *c, const qiw_run *run, double *corr_out) {
+    return guarded(c, [&] { return qiw_correlator_2p_impl(c, run, corr_out); });
+}
+
+extern "C" int qiw_correlator_2p_std(qiw_ctx *c, const qiw_run *run, double *corr_out, double *corr_std_out) {
+    return guarded(c, [&] { return qiw_correlator_2p_std_impl(c, run, corr_out, corr_std_out); });
+}
+
+extern "C" int qiw_partition_function(qiw_ctx *c, double *Z_out) {
+    return guarded(c, [&] { return qiw_partition_function_impl(c, Z_out); });
+}
+
+extern "C" int qiw_density_matrix(qiw_ctx *c, double *rho_out) {
+    return guarded(c, [&] { return qiw_density_matrix_impl(c, rho_out); });
+}
+
+extern "C" int qiw_sobol_points(qiw_ctx *c, int32_t dim, int64_t first, int64_t n, double *out) {
+    return guarded(c, [&] { return qiw_sobol_points_impl(c, dim, first, n, out); });
+}
+
+extern "C" int qiw_simplex_points(qiw_ctx *c, int32_t d_before, int32_t d_after, double t_i, double t_w, double t_f,
+                                  int64_t n, double *out) {
+    return guarded(c, [&] { return qiw_simplex_points_impl(c, d_before, d_after, t_i, t_w, t_f, n, out); });
+}
+
+extern "C" int qiw_get_stats(qiw_ctx *c, qiw_stats *out) {
+    return guarded(c, [&] { return qiw_get_stats_impl(c, out); });
+}
+
+extern "C" int qiw_set_graphs(qiw_ctx *c, int32_t on) {
+    return guarded(c, [&] { return qiw_set_graphs_impl(c, on); });
+}
+
+extern "C" int qiw_set_tuning(qiw_ctx *c, const qiw_tuning *t) {
+    return guarded(c, [&] { return qiw_set_tuning_impl(c, t); });
+}
+
+extern "C" int qiw_get_tuning(qiw_ctx *c, qiw_tuning *out) {
+    return guarded(c, [&] { return qiw_get_tuning_impl(c, out); });
+}
+
+extern "C" int qiw_autotune(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
+    return guarded(c, [&] { return qiw_autotune_impl(c, run, out); });
 }

@@ -132,26 +132,62 @@ __global__ void k_simplex_points(const uint32_t *__restrict__ m, const uint32_t 
 // ---------------------------------------------------------------------------------------
 // Propagator lookups
 // ---------------------------------------------------------------------------------------
-// i*P_s(dt) by linear interpolation of the grid table in the time difference
-// (the evaluation of expansion.P behind kd.interpolate!, src/topology_eval.jl:368).
-__device__ __forceinline__ double ptab_lookup(const double *__restrict__ T, int ld, int s, int ntau, double inv_h,
-                                              double dt) {
-    double x = dt * inv_h;
-    int j = (int)floor(x);
-    j = j < 0 ? 0 : (j > ntau - 2 ? ntau - 2 : j);
-    double f = x - (double)j;
-    return (1.0 - f) * T[j * ld + s] + f * T[(j + 1) * ld + s];
+// A grid-stored function of the time difference (the bold i*P table, a grid-stored i*Delta)
+// at two contour times t1 >= t2, as Keldysh.jl's interpolate! evaluates a grid container
+// (kd.interpolate! on expansion.P, src/topology_eval.jl:368; the call of an ImaginaryTimeGF
+// pair propagator, :414): bilinear over the grid cell (i1, i2) holding (t1, t2), with the
+// stored G[i1, i2] = g(i1 - i2), and linear in t1 - t2 inside a diagonal cell (i1 == i2).
+// With a, b the fractions of t1, t2 in their grid intervals and d = i1 - i2 >= 0 both are
+//     g(d) + a (1 - b) (g(d+1) - g(d)) - (1 - a) b (g(d) - g(d-1)),   g(-1) = 2 g(0) - g(1),
+// i.e. row d plus wp times the forward and minus wm times the backward difference.
+struct GridCell {
+    int d;
+    double wp, wm;  // a (1 - b), (1 - a) b
+};
+__device__ __forceinline__ GridCell grid_cell(double t1, double t2, double inv_h, int n_tau) {
+    t1 = t1 > t2 ? t1 : t2;  // ordering clamp (src/topology_eval.jl:362-365, :408-411)
+    const double x1 = t1 * inv_h, x2 = t2 * inv_h;
+    int i1 = (int)floor(x1), i2 = (int)floor(x2);
+    i1 = i1 < 0 ? 0 : (i1 > n_tau - 2 ? n_tau - 2 : i1);
+    i2 = i2 < 0 ? 0 : (i2 > n_tau - 2 ? n_tau - 2 : i2);
+    const double a = x1 - (double)i1, b = x2 - (double)i2;
+    return GridCell{i1 - i2, a * (1.0 - b), (1.0 - a) * b};
+}
+// Value at cell c from g(d - 1) (any finite value when d = 0), g(d) and g(d + 1).
+__device__ __forceinline__ double cell_value(const GridCell &c, double gm, double g0, double gp) {
+    const double dm = c.d > 0 ? g0 - gm : gp - g0;
+    return g0 + c.wp * (gp - g0) - c.wm * dm;
 }
 
-// i*Delta_tab(dt) from per-interval cubic coefficients (natural cubic spline,
-// src/spline_gf.jl:194-219, or linear with c2 = c3 = 0).
-__device__ __forceinline__ double delta_lookup(const QiwDeltaTables &dt, int tab, double z) {
-    double x = z / dt.h;
+// Hybridization tables (QiwDeltaTables): per arc, the spline interval j and fraction u of
+// the time difference, and the grid cell; a table is evaluated by its kind (grid_mask).
+struct ArcPos {
+    int j;
+    double u;
+    GridCell c;
+};
+__device__ __forceinline__ ArcPos arc_pos(double t_tail, double t_head, double inv_h, int n_tau) {
+    ArcPos p;
+    double z = t_tail - t_head;
+    z = z > 0.0 ? z : 0.0;
+    const double x = z * inv_h;
     int j = (int)floor(x);
-    j = j < 0 ? 0 : (j > dt.n_tau - 2 ? dt.n_tau - 2 : j);
-    double t = x - (double)j;
-    const double *c = dt.coef + ((int64_t)tab * (dt.n_tau - 1) + j) * 4;
-    return c[0] + t * (c[1] + t * (c[2] + t * c[3]));
+    j = j < 0 ? 0 : (j > n_tau - 2 ? n_tau - 2 : j);
+    p.j = j;
+    p.u = x - (double)j;
+    p.c = grid_cell(t_tail, t_head, inv_h, n_tau);
+    return p;
+}
+// i*Delta of table tab at arc position p: a natural cubic spline in the time difference
+// (SplineInterpolatedGF, src/spline_gf.jl:194-219), coefficients (c0..c3) per interval, or
+// a grid container, (g(d), g(d+1) - g(d), g(d) - g(d-1), 0) per index d.
+__device__ __forceinline__ double delta_value(const QiwDeltaTables &dt, int tab, const ArcPos &p) {
+    if ((dt.grid_mask >> tab) & 1) {
+        const double *c = dt.coef + ((int64_t)tab * (dt.n_tau - 1) + p.c.d) * 4;
+        return c[0] + p.c.wp * c[1] - p.c.wm * c[2];
+    }
+    const double *c = dt.coef + ((int64_t)tab * (dt.n_tau - 1) + p.j) * 4;
+    return c[0] + p.u * (c[1] + p.u * (c[2] + p.u * c[3]));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -395,16 +431,13 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
         if (L <= n) {
             double dt = t[L] - t[L - 1];
             dt = dt > 0.0 ? dt : 0.0;
-            const double x = dt * inv_h;
-            int j = (int)floor(x);
-            j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-            const double f = x - (double)j;
-            const double *r0 = a.ptab + (int64_t)j * ns, *r1 = r0 + ns;
+            const GridCell gc = grid_cell(t[L], t[L - 1], inv_h, a.n_tau);
+            const double *r0 = a.ptab + (int64_t)gc.d * ns, *r1 = r0 + ns, *rm = gc.d > 0 ? r0 - ns : r0;
 #pragma unroll
             for (int s = 0; s < SM; ++s) {
                 double v = 0.0;
                 if (s < ns)
-                    v = (g.kind == QIW_G_BARE) ? exp(-dt * a.e_shift[s]) : (1.0 - f) * r0[s] + f * r1[s];
+                    v = (g.kind == QIW_G_BARE) ? exp(-dt * a.e_shift[s]) : cell_value(gc, rm[s], r0[s], r1[s]);
                 if (L == 2) p_first[s] = v;
                 if (L == n) p_last[s] = v;
                 if (L >= 3 && L <= NM - 1) QIW_PIN_SET(NC, (NM - 1 - L) * SM + s, v);
@@ -496,17 +529,11 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
             for (int ai = 0; ai < KM; ++ai) {
                 const uint32_t b = ai >= k ? 0u : ai < 4 ? (lo >> (8 * ai)) : (hi >> (8 * (ai - 4)));
                 const int head = (int)(b & 15), tail = (int)((b >> 4) & 15);
-                double z = tl[tail * QIW_WAVE + lane] - tl[head * QIW_WAVE + lane];
-                z = z > 0.0 ? z : 0.0;
-                const double x = z * inv_h;
-                int j = (int)floor(x);
-                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-                const double u = x - (double)j;
+                const ArcPos ap = arc_pos(tl[tail * QIW_WAVE + lane], tl[head * QIW_WAVE + lane], inv_h, a.n_tau);
 #pragma unroll
                 for (int tb = 0; tb < TM; ++tb) {
                     const int tt = tb < ntab ? tb : (ntab > 0 ? ntab - 1 : 0);
-                    const double *c = a.dtab.coef + ((int64_t)tt * (a.n_tau - 1) + j) * 4;
-                    const double v = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                    const double v = delta_value(a.dtab, tt, ap);
                     if (ai * TM + tb < 16)
                         W[(ai * TM + tb) & (NW - 1)] = v;
                     else
@@ -633,12 +660,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
                         o += m;
                     }
                 } else {
-                    const double x = dt * inv_h;
-                    int j = (int)floor(x);
-                    j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-                    const double f = x - (double)j;
-                    const double *r0 = a.ptab + (int64_t)j * bt, *r1 = r0 + bt;
-                    for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = (1.0 - f) * r0[e] + f * r1[e];
+                    const GridCell gc = grid_cell(t[L], t[L - 1], inv_h, a.n_tau);
+                    const double *r0 = a.ptab + (int64_t)gc.d * bt, *r1 = r0 + bt, *rm = gc.d > 0 ? r0 - bt : r0;
+                    for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = cell_value(gc, rm[e], r0[e], r1[e]);
                 }
             }
         }
@@ -712,14 +736,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
                         th = (p == head) ? t[p] : th;
                         tt = (p == tail) ? t[p] : tt;
                     }
-                    double z = tt - th;
-                    z = z > 0.0 ? z : 0.0;
-                    const double x = z * inv_h;
-                    int j = (int)floor(x);
-                    j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-                    const double u = x - (double)j;
-                    const double *c = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
-                    wsl[lane] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+                    wsl[lane] = delta_value(a.dtab, tb, arc_pos(tt, th, inv_h, a.n_tau));
                 }
                 __syncthreads();
             } else if (op == QIW_OP_FLUSH) {
@@ -747,49 +764,65 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
 // iP_n[s0] is the same for every leaf of a column, so it is applied once, to the column
 // sum, when the column ends.
 //
-// Propagator lookups read (value, slope) pairs of the grid table, so one load and one FMA
-// give the linearly interpolated i*P entry at the lane's own time difference.
+// Propagator lookups read the grid table as cells: row d of a segment's grid cell with its
+// forward and backward differences (see grid_cell), so one entry is one load (two in
+// float64) and two FMAs at the lane's own cell.
 // Vector entries beyond the current block size hold finite values that are never read.
 //
 // Value type V: float64, or float32 for the chain products (QIW_PREC_FP32: propagator
-// pairs, operator blocks, arc values and the per-level vectors in float32; the column
+// cells, operator blocks, arc values and the per-level vectors in float32; the column
 // sums, the last propagator and everything after them in float64).
 // ---------------------------------------------------------------------------------------
 
-// (T[j][e], T[j+1][e] - T[j][e]) for every grid interval j < n_tau - 1 and BT element e,
-// in float64 and float32.
-__global__ void k_ptab_pairs(const double *__restrict__ T, int n_tau, int bt, double2 *__restrict__ out,
-                             float2 *__restrict__ out32) {
+// For every index j < n_tau - 1 and BT element e: T[j][e], the forward difference
+// T[j+1][e] - T[j][e] and the backward difference T[j][e] - T[j-1][e] (for j = 0 the
+// forward one, the cell value's convention g(-1) = 2 g(0) - g(1)); float64 as (value,
+// forward) pairs plus the backward differences, float32 as (value, forward, backward, 0).
+__global__ void k_ptab_cells(const double *__restrict__ T, int n_tau, int bt, double2 *__restrict__ out,
+                             double *__restrict__ outm, float4 *__restrict__ out32) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= (int64_t)(n_tau - 1) * bt) return;
     const double v0 = T[r], v1 = T[r + bt];
+    const double dm = r >= bt ? v0 - T[r - bt] : v1 - v0;
     out[r] = make_double2(v0, v1 - v0);
-    out32[r] = make_float2((float)v0, (float)(v1 - v0));
+    outm[r] = dm;
+    out32[r] = make_float4((float)v0, (float)(v1 - v0), (float)dm, 0.f);
 }
 
+// Cell tables of the two value types: entry i at weights (wp, wm) = row + wp forward -
+// wm backward.
 template <typename V>
-struct VecT;
+struct CellTab;
 template <>
-struct VecT<double> {
-    typedef double2 P2;
+struct CellTab<double> {
+    const double2 *__restrict__ p;
+    const double *__restrict__ m;
+    __device__ __forceinline__ double at(int64_t i, double wp, double wm) const {
+        const double2 e = p[i];
+        return fma(wp, e.y, fma(-wm, m[i], e.x));
+    }
 };
 template <>
-struct VecT<float> {
-    typedef float2 P2;
+struct CellTab<float> {
+    const float4 *__restrict__ p;
+    __device__ __forceinline__ float at(int64_t i, float wp, float wm) const {
+        const float4 e = p[i];
+        return fma(wp, e.y, fma(-wm, e.z, e.x));
+    }
 };
 
 template <typename V, int MB, int KM, int TM>
 struct VecState {
     static constexpr int NM = 2 * KM + 3;
-    typedef typename VecT<V>::P2 P2;
-    int row[NM - 1];      // BT offset of the grid interval of segment L (bold)
-    V f[NM - 1];          // interpolation fraction (bold)
+    int row[NM - 1];      // BT offset of the grid index d of segment L's cell (bold)
+    V wp[NM - 1];         // cell weights of segment L (bold)
+    V wm[NM - 1];
     double dtb[NM - 1];   // time difference (bare)
-    V W[KM * TM];         // i*Delta_tab(t_tail - t_head) per arc and table
+    V W[KM * TM];         // i*Delta_tab of the arcs, per arc and table
     double acc[MB];       // column of the current initial sector
     ProgStream ps;
-    const P2 *__restrict__ pt;
-    const double2 *__restrict__ pt64;
+    CellTab<V> pt;
+    CellTab<double> pt64;
     const double *__restrict__ ediag;
     const V *__restrict__ ops;
     int bt;
@@ -804,8 +837,8 @@ __device__ __forceinline__ void prop_apply(const VecState<V, MB, KM, TM> &st, co
 #pragma unroll
         for (int i = 0; i < MB; ++i) y[i] = (i < m) ? (V)exp(-dt * st.ediag[off + i * (m + 1)]) * x[i] : (V)0;
     } else {
-        const typename VecT<V>::P2 *__restrict__ p = st.pt + st.row[L - 2] + off;
-        const V f = st.f[L - 2];
+        const int64_t base = st.row[L - 2] + off;
+        const V wp = st.wp[L - 2], wm = st.wm[L - 2];
 #pragma unroll
         for (int i = 0; i < MB; ++i) y[i] = (V)0;
 #pragma unroll
@@ -813,10 +846,7 @@ __device__ __forceinline__ void prop_apply(const VecState<V, MB, KM, TM> &st, co
             if (q < m) {
                 const V xq = x[q];
 #pragma unroll
-                for (int i = 0; i < MB; ++i) {
-                    const auto e = p[q * m + i];
-                    y[i] = fma(fma(f, e.y, e.x), xq, y[i]);
-                }
+                for (int i = 0; i < MB; ++i) y[i] = fma(st.pt.at(base + q * m + i, wp, wm), xq, y[i]);
             }
         }
     }
@@ -827,17 +857,15 @@ template <typename V, int MB, int KM, int TM, bool BARE>
 __device__ __forceinline__ void last_prop(const VecState<V, MB, KM, TM> &st, const double (&t)[2 * KM + 4], int n,
                                           const double (&x)[MB], int m, int off, double inv_h, int n_tau,
                                           double (&y)[MB]) {
-    double dt = pick(t, n) - pick(t, n - 1);
-    dt = dt > 0.0 ? dt : 0.0;
+    const double tn = pick(t, n), tp = pick(t, n - 1);
     if constexpr (BARE) {
+        double dt = tn - tp;
+        dt = dt > 0.0 ? dt : 0.0;
 #pragma unroll
         for (int i = 0; i < MB; ++i) y[i] = (i < m) ? exp(-dt * st.ediag[off + i * (m + 1)]) * x[i] : 0.0;
     } else {
-        const double xx = dt * inv_h;
-        int j = (int)floor(xx);
-        j = j < 0 ? 0 : (j > n_tau - 2 ? n_tau - 2 : j);
-        const double f = xx - (double)j;
-        const double2 *__restrict__ p = st.pt64 + (int64_t)j * st.bt + off;
+        const GridCell gc = grid_cell(tn, tp, inv_h, n_tau);
+        const int64_t base = (int64_t)gc.d * st.bt + off;
 #pragma unroll
         for (int i = 0; i < MB; ++i) y[i] = 0.0;
 #pragma unroll
@@ -845,10 +873,7 @@ __device__ __forceinline__ void last_prop(const VecState<V, MB, KM, TM> &st, con
             if (q < m) {
                 const double xq = x[q];
 #pragma unroll
-                for (int i = 0; i < MB; ++i) {
-                    const double2 e = p[q * m + i];
-                    y[i] = fma(fma(f, e.y, e.x), xq, y[i]);
-                }
+                for (int i = 0; i < MB; ++i) y[i] = fma(st.pt64.at(base + q * m + i, gc.wp, gc.wm), xq, y[i]);
             }
         }
     }
@@ -932,19 +957,10 @@ __device__ __forceinline__ void vec_program(VecState<V, MB, KM, TM> &st, const E
                     th = (p == head) ? t[p] : th;
                     tt = (p == tail) ? t[p] : tt;
                 }
-                double z = tt - th;
-                z = z > 0.0 ? z : 0.0;
-                const double xx = z * inv_h;
-                int j = (int)floor(xx);
-                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-                const double u = xx - (double)j;
+                const ArcPos ap = arc_pos(tt, th, inv_h, a.n_tau);
 #pragma unroll
-                for (int tb = 0; tb < TM; ++tb) {
-                    if (ai < k && tb < ntab) {
-                        const double *c = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
-                        st.W[ai * TM + tb] = (V)(c[0] + u * (c[1] + u * (c[2] + u * c[3])));
-                    }
-                }
+                for (int tb = 0; tb < TM; ++tb)
+                    if (ai < k && tb < ntab) st.W[ai * TM + tb] = (V)delta_value(a.dtab, tb, ap);
             }
         } else if (op == QIW_OP_SEG) {  // initial sector s0: m0 columns, BT offset
             s_m0 = (int)(w & 0x3F);
@@ -1028,13 +1044,15 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
 
     VecState<V, MB, KM, TM> st;
     if constexpr (sizeof(V) == 8) {
-        st.pt = (const typename VecT<V>::P2 *)va.pt;
+        st.pt.p = va.pt;
+        st.pt.m = va.ptm;
         st.ops = (const V *)va.ops;
     } else {
-        st.pt = (const typename VecT<V>::P2 *)va.pt32;
+        st.pt.p = va.pt32;
         st.ops = (const V *)va.ops32;
     }
-    st.pt64 = va.pt;
+    st.pt64.p = va.pt;
+    st.pt64.m = va.ptm;
     st.ediag = va.ediag;
     st.bt = va.bt;
 #pragma unroll
@@ -1043,14 +1061,14 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
         dt = dt > 0.0 ? dt : 0.0;
         st.dtb[L - 2] = dt;
         if (g.kind == QIW_G_BARE) {
-            st.f[L - 2] = (V)0;
+            st.wp[L - 2] = (V)0;
+            st.wm[L - 2] = (V)0;
             st.row[L - 2] = 0;
         } else {
-            const double x = dt * inv_h;
-            int j = (int)floor(x);
-            j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-            st.f[L - 2] = (V)(x - (double)j);
-            st.row[L - 2] = j * va.bt;
+            const GridCell gc = grid_cell(t[L], t[L - 1], inv_h, a.n_tau);
+            st.wp[L - 2] = (V)gc.wp;
+            st.wm[L - 2] = (V)gc.wm;
+            st.row[L - 2] = gc.d * va.bt;
         }
     }
 #pragma unroll
@@ -1255,12 +1273,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
             if (g.kind == QIW_G_BARE) {
                 for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = ma.emask[e] ? (float)exp(-dt * ma.ediag[e]) : 0.f;
             } else {
-                const double x = dt * inv_h;
-                int j = (int)floor(x);
-                j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-                const double f = x - (double)j;
-                const double *r0 = a.ptab + (int64_t)j * bt, *r1 = r0 + bt;
-                for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = (float)((1.0 - f) * r0[e] + f * r1[e]);
+                const GridCell gc = grid_cell(pick(t, L), pick(t, L - 1), inv_h, a.n_tau);
+                const double *r0 = a.ptab + (int64_t)gc.d * bt, *r1 = r0 + bt, *rm = gc.d > 0 ? r0 - bt : r0;
+                for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = (float)cell_value(gc, rm[e], r0[e], r1[e]);
             }
         }
         __syncthreads();
@@ -1300,19 +1315,10 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
                         th = (p == head) ? t[p] : th;
                         tt = (p == tail) ? t[p] : tt;
                     }
-                    double z = tt - th;
-                    z = z > 0.0 ? z : 0.0;
-                    const double xx = z * inv_h;
-                    int j = (int)floor(xx);
-                    j = j < 0 ? 0 : (j > a.n_tau - 2 ? a.n_tau - 2 : j);
-                    const double u = xx - (double)j;
+                    const ArcPos ap = arc_pos(tt, th, inv_h, a.n_tau);
 #pragma unroll
-                    for (int tb = 0; tb < TM; ++tb) {
-                        if (ai < k && tb < ntab) {
-                            const double *cc = a.dtab.coef + ((int64_t)tb * (a.n_tau - 1) + j) * 4;
-                            st.W[ai * TM + tb] = cc[0] + u * (cc[1] + u * (cc[2] + u * cc[3]));
-                        }
-                    }
+                    for (int tb = 0; tb < TM; ++tb)
+                        if (ai < k && tb < ntab) st.W[ai * TM + tb] = delta_value(a.dtab, tb, ap);
                 }
             } else if (op == QIW_OP_SEG) {
                 m0 = (int)(w & 0x3F);
@@ -1895,9 +1901,11 @@ hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, in
     return hipGetLastError();
 }
 
-hipError_t qiw_launch_ptab_pairs(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, float2 *out32) {
+hipError_t qiw_launch_ptab_cells(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, double *outm,
+                                 float4 *out32) {
     int64_t n = (int64_t)(n_tau - 1) * bt;
-    hipLaunchKernelGGL(k_ptab_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n_tau, bt, out, out32);
+    hipLaunchKernelGGL(k_ptab_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n_tau, bt, out, outm,
+                       out32);
     return hipGetLastError();
 }
 

@@ -275,7 +275,7 @@ using namespace qiwi;
 // ---------------------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------------------
-extern "C" int qiw_warp_init(qiw_ctx *c, const qiw_warp_config *cfg) {
+static int qiw_warp_init_impl(qiw_ctx *c, const qiw_warp_config *cfg) {
     if (!c || !cfg) return QIW_E_ARG;
     if (cfg->n_layers < 1 || cfg->n_layers > 16 || cfg->n_bins < 1 || cfg->n_bins > QIW_FLOW_MAX_BINS ||
         cfg->hidden < 1 || cfg->hidden > QIW_FLOW_MAX_HIDDEN)
@@ -321,7 +321,7 @@ extern "C" int qiw_warp_init(qiw_ctx *c, const qiw_warp_config *cfg) {
     return QIW_OK;
 }
 
-extern "C" int qiw_warp_params(qiw_ctx *c, int64_t *n_out, float *get, const float *set) {
+static int qiw_warp_params_impl(qiw_ctx *c, int64_t *n_out, float *get, const float *set) {
     int rc = need_warp(c);
     if (rc) return rc;
     WarpState &w = *c->warp;
@@ -331,7 +331,7 @@ extern "C" int qiw_warp_params(qiw_ctx *c, int64_t *n_out, float *get, const flo
     return QIW_OK;
 }
 
-extern "C" int qiw_warp_apply(qiw_ctx *c, int32_t inv, int64_t n, const double *pts, const int32_t *order,
+static int qiw_warp_apply_impl(qiw_ctx *c, int32_t inv, int64_t n, const double *pts, const int32_t *order,
                               const int32_t *group, const double *step, double *out, double *logdet) {
     int rc = need_warp(c);
     if (rc) return rc;
@@ -348,7 +348,7 @@ extern "C" int qiw_warp_apply(qiw_ctx *c, int32_t inv, int64_t n, const double *
     return QIW_OK;
 }
 
-extern "C" int qiw_warp_loss_grad(qiw_ctx *c, int64_t n, const double *z, const int32_t *order,
+static int qiw_warp_loss_grad_impl(qiw_ctx *c, int64_t n, const double *z, const int32_t *order,
                                   const int32_t *group, const double *step, const double *coef, double *loss,
                                   float *grad) {
     int rc = need_warp(c);
@@ -366,7 +366,7 @@ extern "C" int qiw_warp_loss_grad(qiw_ctx *c, int64_t n, const double *z, const 
     return QIW_OK;
 }
 
-extern "C" int qiw_warp_train(qiw_ctx *c, const qiw_run *run, double *loss_history) {
+static int qiw_warp_train_impl(qiw_ctx *c, const qiw_run *run, double *loss_history) {
     int rc = need_warp(c);
     if (rc) return rc;
     if (!run) return fail(c, QIW_E_ARG, "null run");
@@ -472,9 +472,37 @@ extern "C" int qiw_warp_train(qiw_ctx *c, const qiw_run *run, double *loss_histo
     return QIW_OK;
 }
 
-extern "C" int qiw_warp_enable(qiw_ctx *c, int32_t on) {
+static int qiw_warp_enable_impl(qiw_ctx *c, int32_t on) {
     int rc = need_warp(c);
     if (rc) return rc;
     c->warp->enabled = on != 0;
     return QIW_OK;
+}
+
+// ---- C ABI: every entry point runs its body through guarded() ----
+extern "C" int qiw_warp_init(qiw_ctx *c, const qiw_warp_config *cfg) {
+    return guarded(c, [&] { return qiw_warp_init_impl(c, cfg); });
+}
+
+extern "C" int qiw_warp_params(qiw_ctx *c, int64_t *n_out, float *get, const float *set) {
+    return guarded(c, [&] { return qiw_warp_params_impl(c, n_out, get, set); });
+}
+
+extern "C" int qiw_warp_apply(qiw_ctx *c, int32_t inv, int64_t n, const double *pts, const int32_t *order,
+                              const int32_t *group, const double *step, double *out, double *logdet) {
+    return guarded(c, [&] { return qiw_warp_apply_impl(c, inv, n, pts, order, group, step, out, logdet); });
+}
+
+extern "C" int qiw_warp_loss_grad(qiw_ctx *c, int64_t n, const double *z, const int32_t *order,
+                                  const int32_t *group, const double *step, const double *coef, double *loss,
+                                  float *grad) {
+    return guarded(c, [&] { return qiw_warp_loss_grad_impl(c, n, z, order, group, step, coef, loss, grad); });
+}
+
+extern "C" int qiw_warp_train(qiw_ctx *c, const qiw_run *run, double *loss_history) {
+    return guarded(c, [&] { return qiw_warp_train_impl(c, run, loss_history); });
+}
+
+extern "C" int qiw_warp_enable(qiw_ctx *c, int32_t on) {
+    return guarded(c, [&] { return qiw_warp_enable_impl(c, on); });
 }

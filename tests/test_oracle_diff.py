@@ -27,6 +27,7 @@ def test_diff_inchworm_golden():
     for s, name in enumerate(["/diff_inchworm/P/1", "/diff_inchworm/P/2"]):
         g = fx(name)
         assert np.linalg.norm(P[:, s] - g) <= np.sqrt(np.finfo(float).eps) * np.linalg.norm(g)
+        assert np.linalg.norm(P[:, s] - g) <= 1e-14 * np.linalg.norm(g)  # bit-identical here
     assert np.all(P.real == 0.0)
     # the self-energy is imaginary; order 1 is exact (no deviation), order 2 and 3 sampled
     for o in (1, 2, 3):

@@ -1,8 +1,10 @@
 """Oracle inchworm!/correlator_2p against the reference's golden vectors
 (test/inchworm.jl:180-212 via test/inchworm.h5) and closed forms.
 
-The first two steps are reproduced to round-off; the remaining residual of the full run
-(P about 2e-8, g about 1e-6 relative) is stated in the asserts below."""
+The reference compares with isapprox (rtol = sqrt(eps), norm-wise per array); the oracle
+reproduces both P sectors and g to round-off (about 1e-15).  The bold P lookups follow
+the grid-cell interpolation of Keldysh's interpolate!: linear interpolation in the time
+difference alone misses /inchworm/P by 2e-8 and /inchworm/g by 7e-7."""
 import json
 import os
 
@@ -51,11 +53,15 @@ def test_gate_discriminates_delta_interpolation():
     assert abs((val[1] / val[0]).real - (P2[1] / P1[1]).real) > 1e-8
 
 
+RTOL = np.sqrt(np.finfo(float).eps)  # Julia's isapprox default
+
+
 def test_full_inchworm_and_correlator():
     exp = golden_model()
     P, Po, Ps = O.inchworm(exp, range(0, 4), range(0, 3), 2 ** 8)
-    rel = np.linalg.norm(np.r_[P[:, 0] - P1, P[:, 1] - P2]) / np.linalg.norm(np.r_[P1, P2])
-    assert rel < 3e-8
+    for got, ref in ((P[:, 0], P1), (P[:, 1], P2)):
+        assert np.linalg.norm(got - ref) <= RTOL * np.linalg.norm(ref)
+        assert np.linalg.norm(got - ref) <= 1e-14 * np.linalg.norm(ref)  # measured 4e-16
     assert np.all(P.real == 0.0)
     # order-resolved contributions are the raw (unnormalized) step results
     bare, _ = O.inchworm_step_bare(exp, [0, 1, 2], 2 ** 8)
@@ -64,8 +70,8 @@ def test_full_inchworm_and_correlator():
     exp.P, exp.P_n_known = P, exp.grid.n_tau
     exp.add_corr_operators((exp.fock.c("0"), exp.fock.c_dag("0")))
     g = -O.correlator_2p(exp, range(0, 4), 2 ** 8)[0]
-    assert abs(g[0] - G[0]) < 5e-8 and abs(g[-1] - G[-1]) < 5e-8
-    assert np.linalg.norm(g - G) / np.linalg.norm(G) < 2e-6
+    assert np.linalg.norm(g - G) <= RTOL * np.linalg.norm(G)
+    assert np.linalg.norm(g - G) <= 1e-14 * np.linalg.norm(G)  # measured 8e-16
 
 
 def test_order_zero_is_atomic():
