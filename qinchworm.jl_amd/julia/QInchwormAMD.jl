@@ -12,7 +12,10 @@ module QInchwormAMD
 using Keldysh; kd = Keldysh
 using KeldyshED
 using QInchworm.expansion: Expansion
-using QInchworm.spline_gf: SplineInterpolatedGF, IncSplineImaginaryTimeGF
+using QInchworm.spline_gf: SplineInterpolatedGF, IncSplineImaginaryTimeGF, make_inc_interpolant
+using QInchworm.utility: extend!
+using QInchworm.ppgf: set_ppgf!
+using QInchworm.sector_block_matrix: SectorBlockMatrix
 using QInchworm.scrambled_sobol: ScrambledSobolSeq
 using QInchworm.randomization: RandomizationParams, RequestStdDev
 
@@ -186,26 +189,60 @@ function ppgf_table(exp::Expansion, n_tau)
     return T
 end
 
-function store_ppgf!(exp::Expansion, T, grid)
-    for (j, τ) in enumerate(grid)
-        off = 0
-        for P_s in exp.P
-            GF = P_s isa IncSplineImaginaryTimeGF ? P_s.GF : P_s
-            n = size(GF.mat.data, 1)
+# expansion.P after a run, read back from the context: the grid values of every sector and,
+# for IncSplineImaginaryTimeGF, the incremental interpolants as the reference leaves them
+# (normalize! rebuilds them from the derivative at 0 over the known rows,
+# src/ppgf.jl:677-695).
+function load_ppgf!(ctx::Context, exp::Expansion, grid)
+    n_tau = length(grid)
+    bt = sum(length(s)^2 for s in exp.ed.subspaces)
+    T = zeros(ComplexF64, bt, n_tau)
+    der0 = zeros(ComplexF64, bt)
+    n_known = Ref{Int32}(0)
+    check(ctx, ccall((:qiw_get_ppgf, libqiw), Cint, (Ptr{Cvoid}, Ref{Int32}, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                     ctx.h, n_known, T, der0))
+    off = 0
+    for P_s in exp.P
+        GF = P_s isa IncSplineImaginaryTimeGF ? P_s.GF : P_s
+        n = size(GF.mat.data, 1)
+        for (j, τ) in enumerate(grid)
             GF[τ, grid[1]] = reshape(T[off+1:off+n*n, j], n, n)
-            off += n * n
         end
+        if P_s isa IncSplineImaginaryTimeGF
+            d0 = reshape(der0[off+1:off+n*n], n, n)
+            for k in 1:n, l in 1:n
+                P_s.interpolants[k, l] = make_inc_interpolant(GF, k, l, d0[k, l])
+                for j in 2:n_known[]
+                    extend!(P_s.interpolants[k, l], GF.mat.data[k, l, j])
+                end
+            end
+        end
+        off += n * n
     end
+end
+
+# Row j of a BT-layout table as a SectorBlockMatrix (block-diagonal, sector s -> s)
+function sbm_row(exp::Expansion, T, j)::SectorBlockMatrix
+    out = SectorBlockMatrix()
+    off = 0
+    for (s, sub) in enumerate(exp.ed.subspaces)
+        n = length(sub)
+        out[s] = (s, reshape(T[off+1:off+n*n, j], n, n))
+        off += n * n
+    end
+    return out
 end
 
 """
     inchworm!(expansion, grid, orders, orders_bare, N_samples; n_pts_after_max, rand_params)
 
-Drop-in for QInchworm.inchworm! on one MI355X.  Returns (P_orders, P_orders_std) as
-Dict{Int, Matrix{ComplexF64}} in BT layout (sum m^2 x n_tau) and writes expansion.P.
-With rand_params.N_seqs > 1 every group is integrated with up to N_seqs scrambled
-sequences (one seed drawn from rand_params.rng per call) and P_orders_std holds the
-standard deviations.
+Drop-in for QInchworm.inchworm! on one MI355X.  Writes expansion.P and returns
+(P_orders, P_orders_std) as the reference does (src/inchworm.jl:369-370, :413-416,
+:487-491): Dict(order => kd.zero(expansion.P)) filled by set_ppgf! with the order's
+contribution of the initial step (orders_bare) and of every regular step (orders), in
+step order.  With rand_params.N_seqs > 1 every group is integrated with up to N_seqs
+scrambled sequences (one seed drawn from rand_params.rng per call) and P_orders_std holds
+the standard deviations.
 """
 function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_bare, N_samples::Int64;
                    n_pts_after_max::Int64 = typemax(Int64), rand_params::RandomizationParams = RandomizationParams(),
@@ -230,17 +267,28 @@ function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_ba
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                          ctx.h, run, P, Po, Ps))
     end
-    store_ppgf!(exp, P, grid)
-    return (Dict(Int(o) => Po[:, :, i] for (i, o) in enumerate(oall)),
-            Dict(Int(o) => Ps[:, :, i] for (i, o) in enumerate(oall)))
+    load_ppgf!(ctx, exp, grid)
+    P_orders = Dict(Int(o) => kd.zero(exp.P) for o in oall)
+    P_orders_std = Dict(Int(o) => kd.zero(exp.P) for o in oall)
+    slot = Dict(Int(o) => i for (i, o) in enumerate(oall))
+    for o in unique(bare_v)          # initial step: grid[1] -> grid[2]
+        set_ppgf!(P_orders[o], grid[1], grid[2], sbm_row(exp, view(Po, :, :, slot[o]), 2))
+        set_ppgf!(P_orders_std[o], grid[1], grid[2], sbm_row(exp, view(Ps, :, :, slot[o]), 2))
+    end
+    for n in 2:n_tau-1, o in unique(orders_v)   # regular steps: grid[1] -> grid[n + 1]
+        set_ppgf!(P_orders[o], grid[1], grid[n + 1], sbm_row(exp, view(Po, :, :, slot[o]), n + 1))
+        set_ppgf!(P_orders_std[o], grid[1], grid[n + 1], sbm_row(exp, view(Ps, :, :, slot[o]), n + 1))
+    end
+    return P_orders, P_orders_std
 end
 
 """
     diff_inchworm!(expansion, grid, orders, N_samples; rand_params)
 
-Drop-in for QInchworm.diff_inchworm! (src/inchworm.jl:666-778) on one MI355X.  Returns
-(Σ_orders, Σ_orders_std) as Dict{Int, Matrix{ComplexF64}} in BT layout (sum m^2 x n_tau)
-and writes expansion.P.
+Drop-in for QInchworm.diff_inchworm! (src/inchworm.jl:666-778) on one MI355X.  Writes
+expansion.P and returns the order-resolved self-energy and its standard deviations as
+Dict{Int, Matrix{ComplexF64}} in BT layout (sum m^2 x n_tau); the reference returns
+nothing, so a caller of the reference's signature ignores them.
 """
 function diff_inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64;
                         rand_params::RandomizationParams = RandomizationParams(),
@@ -263,7 +311,7 @@ function diff_inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_sa
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                          ctx.h, run, P, So, Ss))
     end
-    store_ppgf!(exp, P, grid)
+    load_ppgf!(ctx, exp, grid)
     return (Dict(Int(o) => So[:, :, i] for (i, o) in enumerate(orders_v)),
             Dict(Int(o) => Ss[:, :, i] for (i, o) in enumerate(orders_v)))
 end
@@ -273,11 +321,13 @@ end
     correlator_2p(expansion, grid, orders, N_samples, RequestStdDev(); rand_params)
 
 Drop-ins for QInchworm.correlator_2p on one MI355X (src/inchworm.jl:1075-1084 and
-:918-1052): one Vector{ComplexF64} of n_tau values <A(tau) B(0)> per pair in
-expansion.corr_operators; the RequestStdDev method also returns their standard deviations.
+:918-1052): a Vector{kd.ImaginaryTimeGF{ComplexF64, true}} with <A(tau) B(0)> for every
+pair in expansion.corr_operators, fermionic when A and B both are (:996-1004); the
+RequestStdDev method returns the tuple (values, standard deviations).
 """
 function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64;
-                       rand_params::RandomizationParams = RandomizationParams(), ctx::Context = context())
+                       rand_params::RandomizationParams = RandomizationParams(),
+                       ctx::Context = context())::Vector{kd.ImaginaryTimeGF{ComplexF64, true}}
     return correlator_2p_impl(exp, grid, orders, N_samples, rand_params, false, ctx)[1]
 end
 
@@ -307,7 +357,23 @@ function correlator_2p_impl(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, 
         check(ctx, ccall((:qiw_correlator_2p_std, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}), ctx.h, run, out, out_std))
     end
-    return ([out[:, c] for c in 1:size(out, 2)], [out_std[:, c] for c in 1:size(out_std, 2)])
+    corr_list = kd.ImaginaryTimeGF{ComplexF64, true}[]
+    corr_std_list = kd.ImaginaryTimeGF{ComplexF64, true}[]
+    is_fermion(expr) = isodd(length(first(keys(expr.monomials))))
+    for (c, (A, B)) in enumerate(exp.corr_operators)
+        @assert length(A) == 1 "Operator A must be a single monomial in C/C^+"
+        @assert length(B) == 1 "Operator B must be a single monomial in C/C^+"
+        ξ = (is_fermion(A) && is_fermion(B)) ? kd.fermionic : kd.bosonic
+        g = kd.ImaginaryTimeGF(grid, 1, ξ, true)
+        g_std = kd.ImaginaryTimeGF(grid, 1, ξ, true)
+        for (n, τ) in enumerate(grid)
+            g[τ, grid[1]] = out[n, c]
+            g_std[τ, grid[1]] = out_std[n, c]
+        end
+        push!(corr_list, g)
+        push!(corr_std_list, g_std)
+    end
+    return (corr_list, corr_std_list)
 end
 
 """
