@@ -197,6 +197,21 @@ int qiw_inchworm(qiw_ctx *ctx, const qiw_run *run, double *P_out, double *P_orde
 int qiw_inchworm_step(qiw_ctx *ctx, const qiw_run *run, int32_t n_w, double *value_out,
                       double *orders_out);
 
+/* qiw_inchworm_step_noalloc: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): qiw_inchworm_step with the same arguments and results that calls no
+ * hipMalloc/hipFree.  It needs what a previous qiw_inchworm_step with the same run, model
+ * structure and Sobol tables left resident on the context (compiled programs, simplex
+ * coordinates, model tables; QIW_E_STATE otherwise) and takes its per-call buffers
+ * (per-task partial sums, the propagator table, the bookkeeping sums and counter) from a
+ * static device array of QIW_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends
+ * (QIW_E_LIMIT for a step that needs more).  Every call on one GPU shares that array and
+ * clears what it needs cleared: calls that overlap on one GPU (several contexts or host
+ * threads) race.  The learned warp is not available here (QIW_E_ARG). */
+#define QIW_NOALLOC_SCRATCH_BYTES (64u << 20)
+int qiw_inchworm_step_noalloc(qiw_ctx *ctx, const qiw_run *run, int32_t n_w, double *value_out,
+                              double *orders_out);
+
 /* qiw_inchworm_step_bare: the initial step inchworm_step_bare (src/inchworm.jl:228-304)
  * between grid[0] and grid[1], orders from run->orders_bare. */
 int qiw_inchworm_step_bare(qiw_ctx *ctx, const qiw_run *run, double *value_out,

@@ -17,6 +17,9 @@ namespace qiwi {
 
 typedef std::complex<double> cd;
 
+// Bytes of the static scratch array of qiw_inchworm_step_noalloc (one per GPU)
+constexpr size_t kNoAllocScratchBytes = QIW_NOALLOC_SCRATCH_BYTES;
+
 // ---------------------------------------------------------------------------------------
 // Model in the representation the kernels use: i*P, i*Delta and the operator elements are
 // real for every model the kernels accept.
@@ -107,6 +110,12 @@ struct qiw_ctx {
     bool graphs = true;                // replay the bold-step loop from a hipGraph (qiw_set_graphs)
     qiwi::StepGraph *graph = nullptr;  // the captured loop and the launch arguments it holds
     std::vector<hipEvent_t> evpool;    // timing events of the eager launches, reused
+    // qiw_inchworm_step_noalloc: this device's static scratch array (its device address),
+    // the bytes handed out during the current call, and the flag that makes dev() serve
+    // the per-call buffers from it and allocate nothing
+    void *scratch = nullptr;
+    size_t scratch_used = 0;
+    bool no_alloc = false;
 };
 
 namespace qiwi {

@@ -281,6 +281,7 @@ __host__ __device__ static inline uint32_t qiw_rq_word(uint64_t seed, uint64_t s
 }
 
 // Launch wrappers (qiw_kernels.hip)
+hipError_t qiw_launch_zero(hipStream_t st, void *p, size_t bytes);  // bytes: a multiple of 4
 hipError_t qiw_launch_sobol_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int64_t first,
                                    int64_t n, double *out);
 hipError_t qiw_launch_simplex_coords(hipStream_t st, const uint32_t *m, const uint32_t *x0, int d_before,

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,7 +100,37 @@ int fail(qiw_ctx *c, int code, const std::string &msg) {
     return code;
 }
 
+// Buffers that one step call fills and reads back (per-task partial sums, the i*P table
+// and its order slots, the bookkeeping's sums and counter): a no-alloc call takes them from
+// the device's static scratch array.
+static bool per_call_buffer(const std::string &name) {
+    static const char *const names[] = {"partial", "ptab",     "porders",  "lambda",      "fin.scratch",
+                                        "fin.gsum", "diag",     "fin.done", "porders_std", "sobol.m"};
+    for (const char *n : names)
+        if (name == n) return true;
+    return false;
+}
+
 void *dev(qiw_ctx *c, const std::string &name, size_t bytes, int *rc) {
+    if (c->no_alloc) {
+        if (per_call_buffer(name)) {
+            const size_t off = (c->scratch_used + 255) & ~(size_t)255;
+            if (off + bytes > kNoAllocScratchBytes) {
+                *rc = fail(c, QIW_E_LIMIT, "the step needs more than the " + std::to_string(kNoAllocScratchBytes >> 20) +
+                                               " MiB scratch array of qiw_inchworm_step_noalloc (" + name + ")");
+                return nullptr;
+            }
+            c->scratch_used = off + bytes;
+            return (char *)c->scratch + off;
+        }
+        DevBuf &b = c->bufs[name];
+        if (b.n < bytes) {
+            *rc = fail(c, QIW_E_STATE, "device buffer " + name +
+                                           " is not resident: call qiw_inchworm_step once with the same run first");
+            return nullptr;
+        }
+        return b.p;
+    }
     DevBuf &b = c->bufs[name];
     if (b.n < bytes) {
         if (b.p) (void)hipFree(b.p);
@@ -912,6 +943,8 @@ struct Plan {
 };
 struct PlanStore {
     std::map<std::string, Plan> by_tag;
+    std::map<std::string, uint64_t> last_use;  // step plans: use counter of each buffer tag
+    uint64_t clock = 0;
 };
 void plans_destroy(PlanStore *p) { delete p; }
 
@@ -971,14 +1004,39 @@ std::string plan_key(const qiw_ctx *c, const qiw_run *run, const std::vector<int
     return k + c->model_fp;
 }
 
-// The plan of a bare or bold step batch (tag "bare" / "bold"), built and uploaded unless
-// the resident one matches.
+// Step plans kept per kind (bare / bold), so that runs alternating between settings (two
+// sample counts, the tuner's candidates) reuse them; the least recently used is replaced.
+constexpr int kStepPlanSlots = 2;
+
+// The plan of a bare or bold step batch (buffer tags "bare", "bare.1", "bold", "bold.1"),
+// built and uploaded unless a resident one matches.
 int step_plan(qiw_ctx *c, const qiw_run *run, const std::vector<int> &oall, bool bare, bool warp, Plan *&out) {
-    const std::string tag = bare ? "bare" : "bold";
+    const std::string kind = bare ? "bare" : "bold";
+    const std::string key = plan_key(c, run, oall, kind);
+    if (!c->plans) c->plans = new PlanStore;
+    PlanStore &ps = *c->plans;
+    std::string tag = kind;
+    uint64_t oldest = UINT64_MAX;
+    for (int i = 0; i < kStepPlanSlots; ++i) {
+        const std::string t = i == 0 ? kind : kind + "." + std::to_string(i);
+        auto it = ps.by_tag.find(t);
+        if (!warp && it != ps.by_tag.end() && it->second.key == key) {
+            tag = t;
+            oldest = 0;
+            break;
+        }
+        const uint64_t u = it == ps.by_tag.end() ? 0 : ps.last_use[t];
+        if (u < oldest) {
+            oldest = u;
+            tag = t;
+        }
+    }
     Plan &p = plan_slot(c, tag);
     out = &p;
-    const std::string key = plan_key(c, run, oall, tag);
+    ps.last_use[tag] = ++ps.clock;
     if (!warp && p.key == key) return 0;
+    if (c->no_alloc)
+        return fail(c, QIW_E_STATE, "no resident plan for this run: call qiw_inchworm_step once with the same run first");
     p = Plan{};
     int rc = build_step_batch(c, run, bare, oall, p.b);
     if (rc) return rc;
@@ -1167,6 +1225,33 @@ void store_complex_from_ip(double ipv, double *out) {  // P = -i * (i P)
 }  // namespace
 
 // =======================================================================================
+// Scratch of qiw_inchworm_step_noalloc: one static array per GPU, held until the process
+// ends.  Host code never uses the array's name as an address: at qiw_create, once per
+// device and with that device current, hipGetSymbolAddress gives the device address, which
+// every later call on that device uses.
+// =======================================================================================
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `device`'s scratch array (that device is current).
+int scratch_address(int device, void **out) {
+    if (device < 0 || device >= kMaxDevices) return QIW_E_LIMIT;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return QIW_E_DEVICE;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return QIW_OK;
+}
+}  // namespace
+
+// =======================================================================================
 // C ABI
 // =======================================================================================
 extern "C" int qiw_abi_version(void) { return QIW_ABI_VERSION; }
@@ -1181,6 +1266,12 @@ static int qiw_create_impl(qiw_ctx **out, int device) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
         delete c;
         return QIW_E_DEVICE;
+    }
+    const int rc = scratch_address(device, &c->scratch);
+    if (rc != QIW_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
     }
     *out = c;
     return QIW_OK;
@@ -1436,16 +1527,16 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     double *pstd = rqmc ? (double *)dev(c, "porders_std", (size_t)n_slots * m.ntau * m.bt * 8, &rc) : nullptr;
     uint32_t *sobm = rq.scrambled ? (uint32_t *)dev(c, "sobol.m", c->sob_m.size() * 4, &rc) : nullptr;
     if (rc) return rc;
-    HIPCHK(c, hipMemsetAsync(done, 0, 4, c->stream));
-    if (pstd) HIPCHK(c, hipMemsetAsync(pstd, 0, (size_t)n_slots * m.ntau * m.bt * 8, c->stream));
+    HIPCHK(c, qiw_launch_zero(c->stream, done, 4));
+    if (pstd) HIPCHK(c, qiw_launch_zero(c->stream, pstd, (size_t)n_slots * m.ntau * m.bt * 8));
     if (sobm) HIPCHK(c, hipMemcpy(sobm, c->sob_m.data(), c->sob_m.size() * 4, hipMemcpyHostToDevice));
     std::vector<int32_t> dg;
     for (int s = 0; s < m.ns; ++s)
         for (int i = 0; i < m.dims[s]; ++i) dg.push_back(m.bt_off[s] + i + i * m.dims[s]);
     HIPCHK(c, hipMemcpy(diag, dg.data(), dg.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(ptab, c->ip.data(), c->ip.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemsetAsync(porders, 0, (size_t)n_slots * m.ntau * m.bt * 8, c->stream));
-    HIPCHK(c, hipMemsetAsync(lam, 0, (size_t)m.ntau * 8, c->stream));
+    HIPCHK(c, qiw_launch_zero(c->stream, porders, (size_t)n_slots * m.ntau * m.bt * 8));
+    HIPCHK(c, qiw_launch_zero(c->stream, lam, (size_t)m.ntau * 8));
 
     FinalizeArgs fa{};
     fa.partial = partial;
@@ -1481,6 +1572,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     // timing events from the context's pool (created once, reused by every call)
     size_t n_ev = 0;
     auto mark = [&](void) -> hipError_t {
+        if (c->no_alloc) return hipSuccess;  // creates no events: no timing of a no-alloc call
         if (n_ev == c->evpool.size()) {
             hipEvent_t e;
             hipError_t r = hipEventCreate(&e);
@@ -1537,7 +1629,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         // flat scalar walk without the warp: the whole loop is one hipGraph, replayed while
         // the launch arguments are unchanged (the other walks stage data from the host per
         // launch and stay eager)
-        bool graph = c->graphs && bold_b.fmt == FMT_SCALAR && !warped;
+        bool graph = c->graphs && bold_b.fmt == FMT_SCALAR && !warped && !c->no_alloc;
         const int n_known_start = n_known;
         if (graph) {
             if (!c->graph) c->graph = new StepGraph;
@@ -1864,6 +1956,7 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
     // timing events from the context's pool (created once, reused by every call)
     size_t n_ev = 0;
     auto mark = [&](void) -> hipError_t {
+        if (c->no_alloc) return hipSuccess;  // creates no events: no timing of a no-alloc call
         if (n_ev == c->evpool.size()) {
             hipEvent_t e;
             hipError_t r = hipEventCreate(&e);
@@ -1961,6 +2054,23 @@ static int qiw_inchworm_step_impl(qiw_ctx *c, const qiw_run *run, int32_t n_w, d
         for (size_t i = 0; i < per.size() && i < (size_t)run->n_orders * c->m.bt; ++i)
             store_complex_from_ip(per[i], orders_out + 2 * i);
     return QIW_OK;
+}
+
+// The same step with nothing allocated: the plan, model tables and walk buffers of a
+// previous qiw_inchworm_step with the same run stay in use, the per-call buffers come from
+// the device's static scratch array, and no timing events or graphs are made.
+static int qiw_inchworm_step_noalloc_impl(qiw_ctx *c, const qiw_run *run, int32_t n_w, double *value_out,
+                                          double *orders_out) {
+    if (!c) return QIW_E_ARG;
+    if (!c->scratch) return fail(c, QIW_E_STATE, "no scratch array on this device");
+    if (warp_enabled(c->warp)) return fail(c, QIW_E_ARG, "the learned warp is not available without allocation");
+    struct Mode {
+        qiw_ctx *c;
+        ~Mode() { c->no_alloc = false; }
+    } mode{c};
+    c->no_alloc = true;
+    c->scratch_used = 0;
+    return qiw_inchworm_step_impl(c, run, n_w, value_out, orders_out);
 }
 
 static int qiw_inchworm_step_bare_impl(qiw_ctx *c, const qiw_run *run, double *value_out, double *orders_out) {
@@ -2535,4 +2645,9 @@ extern "C" int qiw_get_tuning(qiw_ctx *c, qiw_tuning *out) {
 
 extern "C" int qiw_autotune(qiw_ctx *c, const qiw_run *run, qiw_tuning *out) {
     return guarded(c, [&] { return qiw_autotune_impl(c, run, out); });
+}
+
+extern "C" int qiw_inchworm_step_noalloc(qiw_ctx *c, const qiw_run *run, int32_t n_w, double *value_out,
+                                         double *orders_out) {
+    return guarded(c, [&] { return qiw_inchworm_step_noalloc_impl(c, run, n_w, value_out, orders_out); });
 }

@@ -1828,9 +1828,22 @@ __global__ __launch_bounds__(256) void k_finalize_corr_rqmc(CorrFinalizeArgs a) 
     }
 }
 
+// Zero n 32-bit words (clears counters and output tables with a kernel instead of a memset
+// node, so the stream work stays capturable).
+__global__ void k_zero_words(uint32_t *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 // ---------------------------------------------------------------------------------------
 // Host-callable launch wrappers (plain C++ signatures for qiw_host.cpp)
 // ---------------------------------------------------------------------------------------
+hipError_t qiw_launch_zero(hipStream_t st, void *p, size_t bytes) {
+    const int64_t n = (int64_t)(bytes / 4);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (uint32_t *)p, n);
+    return hipGetLastError();
+}
 hipError_t qiw_launch_sobol_points(hipStream_t st, const uint32_t *m, const uint32_t *x0, int dim, int64_t first,
                                    int64_t n, double *out) {
     int blocks = (int)((n + 255) / 256);

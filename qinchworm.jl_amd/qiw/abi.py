@@ -142,7 +142,7 @@ class RunStruct:
 # Exported symbols of libqiw (every declaration of include/qiw.h)
 EXPORTS = ["qiw_create", "qiw_destroy", "qiw_last_error", "qiw_abi_version", "qiw_set_model",
            "qiw_set_sobol", "qiw_set_ppgf", "qiw_get_ppgf", "qiw_inchworm", "qiw_inchworm_step",
-           "qiw_inchworm_step_bare", "qiw_correlator_2p", "qiw_correlator_2p_std", "qiw_diff_inchworm",
+           "qiw_inchworm_step_bare", "qiw_inchworm_step_noalloc", "qiw_correlator_2p", "qiw_correlator_2p_std", "qiw_diff_inchworm",
            "qiw_partition_function",
            "qiw_density_matrix", "qiw_sobol_points", "qiw_simplex_points", "qiw_get_stats",
            "qiw_warp_init", "qiw_warp_params", "qiw_warp_apply", "qiw_warp_loss_grad", "qiw_warp_train",
@@ -174,6 +174,7 @@ def load_library(path=None):
     lib.qiw_inchworm.argtypes = [vp, C.POINTER(qiw_run), _f64p, _f64p, _f64p]
     lib.qiw_inchworm_step.argtypes = [vp, C.POINTER(qiw_run), C.c_int32, _f64p, _f64p]
     lib.qiw_inchworm_step_bare.argtypes = [vp, C.POINTER(qiw_run), _f64p, _f64p]
+    lib.qiw_inchworm_step_noalloc.argtypes = [vp, C.POINTER(qiw_run), C.c_int32, _f64p, _f64p]
     lib.qiw_correlator_2p.argtypes = [vp, C.POINTER(qiw_run), _f64p]
     lib.qiw_correlator_2p_std.argtypes = [vp, C.POINTER(qiw_run), _f64p, _f64p]
     lib.qiw_diff_inchworm.argtypes = [vp, C.POINTER(qiw_run), _f64p, _f64p, _f64p]
