@@ -238,6 +238,10 @@ struct Compiled {
     std::vector<int32_t> chunks{0};  // program offsets (64-bit words) of the topology chunks
     int64_t leaves_per_sample = 0;
     int64_t edges_per_sample = 0;
+    // block programs: real FLOPs of the chain products one sample costs (per column of the
+    // initial sector: i*P times the partial product once per node with children, the
+    // operator block per edge, the leaf accumulation)
+    double flops_per_sample = 0;
     int n_top = 0;
     int replica_of = -1;  // randomized qMC: another sequence of group `replica_of` (no program of its own)
 };
@@ -354,6 +358,12 @@ struct Compiler {
         }
         if (t.tail) w |= QIW_F_TAIL;
         if (t.leaf) w |= QIW_F_LEAF;
+        {
+            const double m0 = m.dims[s0], mi = m.dims[t.s_in], mo = m.dims[t.s_out];
+            if (t.blk != 0xFFFF) out.flops_per_sample += 2.0 * mo * mi * m0;
+            if (t.leaf) out.flops_per_sample += 2.0 * mo * m0;
+            if (!t.children.empty() && t.level + 1 < n) out.flops_per_sample += 2.0 * mo * mo * m0;
+        }
         const bool unit = t.leaf && fmt != FMT_BLOCK && std::fabs(t.coef) == 1.0;
         if (unit) w |= QIW_FV_UNIT | (t.coef < 0 ? QIW_FV_NEG : 0);
         out.edges_per_sample++;
@@ -444,7 +454,8 @@ struct Compiler {
         out.n_top = (int)tops.size();
         // vector format: segments per initial sector, collected first
         // vector format: words of every (topology, initial sector), regrouped into chunks
-        std::vector<std::vector<std::vector<uint64_t>>> vtop(fmt == FMT_VEC ? tops.size() : 0,
+        const bool vec_like = (fmt == FMT_VEC);
+        std::vector<std::vector<std::vector<uint64_t>>> vtop(vec_like ? tops.size() : 0,
                                                              std::vector<std::vector<uint64_t>>(m.ns));
         for (int ti = 0; ti < (int)tops.size(); ++ti) {
             const Topology &top = tops[ti];
@@ -507,6 +518,8 @@ struct Compiler {
                     out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | (uint64_t)m.dims[s] |
                                        ((uint64_t)trie[0].children.size() << 24) | (op_offset(root_blk) << 32) |
                                        ((uint64_t)m.bt_off[s] << 52));
+                    // the root's i*P (level 2) for its children
+                    if (n > 2) out.flops_per_sample += 2.0 * m.dims[s] * m.dims[s] * m.dims[s0];
                     for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], out);
                     vtop[ti][s0] = out.prog;
                     saved.swap(out.prog);
@@ -518,7 +531,7 @@ struct Compiler {
                 out.prog.push_back(((uint64_t)QIW_OP_FLUSH << 48) | ((uint64_t)s0 << 8));
             }
         }
-        if (fmt == FMT_VEC) {
+        if (vec_like) {
             // Chunks of topologies, each a program of its own: every initial sector gets a
             // segment in every chunk, so every output element of a task row is written.
             // A chunk closes once its words, weighted by the columns each segment is
@@ -566,6 +579,13 @@ struct Compiler {
         return true;
     }
 };
+
+// Algorithmic real FLOPs of one sample of a compiled group: the scalar walk's multiply per
+// trie edge and leaf, or the block programs' chain products.
+double sample_flops(const Compiled &cg) {
+    if (cg.flops_per_sample > 0) return cg.flops_per_sample;
+    return 2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample;
+}
 
 // LDS doubles of one evaluator wavefront.  Scalar programs keep their state in registers;
 // block programs hold per-sample state shared by the lanes (iP slots, W slots, stack, X,
@@ -743,6 +763,7 @@ void push_sequences(Batch &b, Compiled &&cg, int n_eval, AddTasks add_tasks) {
         rep.chunks = cg.chunks;
         rep.leaves_per_sample = cg.leaves_per_sample;
         rep.edges_per_sample = cg.edges_per_sample;
+        rep.flops_per_sample = cg.flops_per_sample;
         rep.n_top = cg.n_top;
         rep.replica_of = first;
     }
@@ -903,7 +924,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
             int slot = (int)(std::find(oall.begin(), oall.end(), order) - oall.begin());
             int64_t N = order > 0 ? run->N_samples : 1;
             const std::vector<int32_t> chunks = cg.chunks;
-            const double flops = (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
+            const double flops = (double)N * sample_flops(cg);
             const int64_t leaves = N * cg.leaves_per_sample, evals = N * cg.n_top;
             push_sequences(b, std::move(cg), b.n_eval, [&](int gi) {
                 b.g_first.push_back((int32_t)b.tasks.size());
@@ -1845,7 +1866,7 @@ int build_diff_batch(qiw_ctx *c, const qiw_run *run, Batch &b) {
         b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
         const int64_t N = order == 1 ? 1 : run->N_samples;
         const std::vector<int32_t> chunks = cg.chunks;
-        const double flops = (double)N * (2.0 * cg.edges_per_sample + 2.0 * cg.leaves_per_sample);
+        const double flops = (double)N * sample_flops(cg);
         const int64_t leaves = N * cg.leaves_per_sample, evals = N * cg.n_top;
         push_sequences(b, std::move(cg), b.n_eval, [&](int gi) {
             b.g_first.push_back((int32_t)b.tasks.size());
@@ -2187,8 +2208,7 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
                         rcnt[(size_t)j * ng + gi] = nt * (int)chs.size();
                         for (int t = 0; t < nt; ++t)
                             for (int32_t ch : chs) b.tasks.push_back({gi, t * b.spt, j, (int32_t)b.tasks.size(), ch, 0});
-                        b.flops_per_launch +=
-                            (double)N * (2.0 * b.groups[gi].edges_per_sample + 2.0 * b.groups[gi].leaves_per_sample);
+                        b.flops_per_launch += (double)N * sample_flops(b.groups[gi]);
                         b.leaves_per_launch += N * b.groups[gi].leaves_per_sample;
                         b.diagram_evals_per_launch += N * b.groups[gi].n_top;
                     }

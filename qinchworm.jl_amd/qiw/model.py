@@ -208,14 +208,14 @@ def delta_dos_gf(eps, beta, grid, weight=1.0):
     return ScalarGF(-1j * weight * _fermi_weighted_exp([eps], beta, grid.tau)[0])
 
 
-def bethe_gf(t, beta, grid, n_quad=400):
-    """Semi-elliptic density of states of half-bandwidth 2t (kd.bethe_dos(t=t)),
-    G(tau) = -i int d eps rho(eps) (1 - n_F(eps)) e^{-eps tau}, by Gauss-Legendre on
-    eps = 2t cos(theta), where rho d eps = (2/pi) sin^2(theta) d theta."""
+def bethe_gf(t, beta, grid, n_quad=400, eps=0.0):
+    """Semi-elliptic density of states of half-bandwidth 2t centred at eps
+    (kd.bethe_dos(t=t, ϵ=eps)), G(tau) = -i int d w rho(w) (1 - n_F(w)) e^{-w tau}, by
+    Gauss-Legendre on w = eps + 2t cos(theta), where rho d w = (2/pi) sin^2(theta) d theta."""
     x, w = np.polynomial.legendre.leggauss(n_quad)
     theta = 0.5 * np.pi * (x + 1.0)
     wt = 0.5 * np.pi * w * (2.0 / np.pi) * np.sin(theta) ** 2
-    eps = 2.0 * t * np.cos(theta)
+    eps = eps + 2.0 * t * np.cos(theta)
     vals = (wt[:, None] * _fermi_weighted_exp(eps, beta, grid.tau)).sum(axis=0)
     return ScalarGF(-1j * vals)
 
@@ -371,6 +371,35 @@ def anderson_single_orbital(beta=10.0, n_tau=64, eps=0.1, U=1.0, V=0.5, D=2.0,
                 (fock.n("up"), fock.n("dn")),
                 (fock.c_dag("up") @ fock.c("dn"), fock.c_dag("dn") @ fock.c("up"))]
     return Expansion(ed, grid, pairs, corr, interpolate_ppgf=interpolate)
+
+
+def bethe_two_orbitals(n_tau, mu_bethe=0.25, beta=10.0, V=0.5, t_bethe=0.5):
+    """The model of test/bethe.jl:60-88 and test/bethe_gf.jl:62-90: two non-interacting
+    orbitals (H = -mu (n_1 + n_2), mu = 0) each coupled to a semi-elliptic bath of half-
+    bandwidth 2 t_bethe centred at mu_bethe, Delta = V^2 G_bethe I(2), built by the
+    higher-level Expansion constructor (src/expansion.jl:222-291)."""
+    fock = FockSpace([1, 2])
+    H = 0.0 * (fock.n(1) + fock.n(2))
+    grid = ImaginaryTimeGrid(beta, n_tau)
+    g = bethe_gf(t_bethe, beta, grid, eps=mu_bethe).values
+    hyb = np.zeros((2, 2, n_tau), np.complex128)
+    hyb[0, 0] = hyb[1, 1] = V ** 2 * g
+    return Expansion.from_hamiltonian(H, fock, grid, hyb)
+
+
+def fock_density_matrix(expansion, P):
+    """The density matrix in the Fock basis (KeldyshED full_hs_matrix(tofockbasis(...)))
+    from i P(beta) normalised to unit trace; P is the (n_tau, sum m^2) table."""
+    ed = expansion.ed
+    rho = np.zeros((ed.fock.dim, ed.fock.dim), np.complex128)
+    blocks, off = [], 0
+    for s, m in enumerate(ed.dims):
+        blocks.append((1j * P[-1, off:off + m * m]).reshape(m, m, order="F"))
+        off += m * m
+    z = sum(np.trace(b) for b in blocks)
+    for st, U, b in zip(ed.subspaces, ed.U, blocks):
+        rho[np.ix_(st, st)] = U @ (b / z) @ U.T.conj()
+    return rho
 
 
 def kanamori(n_orb, U, J, eps_shift, beta, n_tau, V, D=2.0, interpolate=False):

@@ -2,8 +2,9 @@
 shipped with the package.
 
 Sources (read in this container only; nothing here runs on the GPU box):
-  * /root/reference/test/inchworm.h5, topology_eval.h5  -- the reference's own golden
-    vectors (test/inchworm.jl:180-212, test/topology_eval.jl:37-155), dumped with h5dump;
+  * /root/reference/test/inchworm.h5, topology_eval.h5, bethe.h5  -- the reference's own
+    golden vectors (test/inchworm.jl:180-212, test/topology_eval.jl:37-155) and DLR results
+    (test/bethe.jl, test/bethe_gf.jl), dumped with h5dump;
   * SciPy's scipy.stats.qmc.Sobol -- the library the reference's Sobol tests were written
     against (test/scrambled_sobol.jl:47-93, with the mock RNG of :97-117);
   * SciPy's Joe-Kuo direction numbers (new-joe-kuo-6.21201), the table behind Sobol.jl's
@@ -61,6 +62,17 @@ def topology_eval_fixture():
     return fx
 
 
+def bethe_fixture():
+    """Density matrices and Green's functions of the DLR reference results in
+    test/bethe.h5 (test/bethe.jl:48-58, test/bethe_gf.jl:47-58)."""
+    path = os.path.join(REF, "test", "bethe.h5")
+    fx = {"source": "test/bethe.h5 (test/bethe.jl, test/bethe_gf.jl)"}
+    for name in ["/rho/exact", "/rho/NCA", "/rho/OCA", "/rho/TCA", "/g/NCA", "/g/OCA", "/g/TCA"]:
+        shape, flat = h5_dataset(path, name)
+        fx[name] = {"shape": shape, "data": flat}
+    return fx
+
+
 def sobol_fixture():
     from scipy.stats import qmc
     import scipy._lib._util as util
@@ -105,6 +117,7 @@ def main():
 
     dump(inchworm_fixture(), os.path.join(HERE, "inchworm_h5.json"))
     dump(topology_eval_fixture(), os.path.join(HERE, "topology_eval_h5.json"))
+    dump(bethe_fixture(), os.path.join(HERE, "bethe_h5.json"))
     dump(sobol_fixture(), os.path.join(HERE, "sobol_scipy.json"))
     dump(joe_kuo_table(), os.path.join(REPO, "qinchworm.jl_amd", "qiw", "data", "sobol_joe_kuo.json"))
 
