@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define QIW_ABI_VERSION 4
+#define QIW_ABI_VERSION 5
 
 /* ---- return codes (every reference @assert on this path maps to one) ---------------- */
 enum {
@@ -124,11 +124,30 @@ typedef struct {
  *                 Word 0 is the shift x (bit j = shift bit j); word 1 + p holds row p
  *                 of the lower-triangular matrix: its bits 31..32-p are the entries
  *                 (p, 0..p-1), the diagonal is 1.
+ *   scramble = 2  rng given, target_std = 0: the sequences are the caller's own, staged
+ *                 with qiw_set_scrambles -- the ScrambledSobolSeq(2k; scramble_rng = rng)
+ *                 objects the reference constructs (src/randomization.jl:94), drawn by the
+ *                 caller in the reference's order, so the points are the reference's bit for
+ *                 bit whatever the RNG.  Every call consumes them from the first on, N_seqs
+ *                 per sampled group (order > 0, N_samples > 0; the exact order-1
+ *                 self-energy group of diff_inchworm! draws none), groups in the reference's
+ *                 order (orders as given, n_pts_after ascending, groups without topologies
+ *                 skipped):
+ *                   qiw_inchworm      the initial step's groups (orders_bare), then every
+ *                                     bold step's (orders), step by step;
+ *                   qiw_inchworm_step(_bare)   the one step's groups;
+ *                   qiw_correlator_2p(_std)    per operator pair, per tau index 1..n_tau-1,
+ *                                     the groups (tau = 0 is exact);
+ *                   qiw_diff_inchworm every step's self-energy groups, step by step.
+ *                 A group whose integral vanishes identically takes min(2, N_seqs), where
+ *                 the reference's stopping rule ends (qiw_scramble_dims lists the exact
+ *                 dimensions).  A call with too few staged sequences, or with fewer
+ *                 dimensions than 2k, returns QIW_E_ARG before it evaluates anything.
  * The GPU evaluates all N_seqs sequences and applies the stopping rule afterwards; the
  * result is that of the reference's sequential loop over the same sequences. */
 typedef struct {
     int32_t N_seqs;                /* >= 1                                                */
-    int32_t scramble;              /* 0 or 1 (above)                                      */
+    int32_t scramble;              /* 0, 1 or 2 (above)                                   */
     uint64_t seed;
     double target_std;
 } qiw_rand;
@@ -168,6 +187,33 @@ int qiw_set_model(qiw_ctx *ctx, const qiw_model *model);
  * scrambled, already LMS-scrambled), shift[d] is the initial point x (NULL = zeros,
  * the unscrambled sequence).  Points of an order-k group use dimensions 0..2k-1. */
 int qiw_set_sobol(qiw_ctx *ctx, int32_t dim, const uint32_t *m, const uint32_t *shift);
+
+/* qiw_set_scrambles: the caller-drawn sequences of qiw_rand.scramble = 2, replacing the
+ * ScrambledSobolSeq(D; scramble_rng = rng) constructions of mean_std_from_randomization
+ * (src/randomization.jl:94, src/scrambled_sobol.jl:66-143): sequence i has the direction
+ * numbers m[i][d][0..31] and the initial point shift[i][d] (NULL: zeros) of dimensions
+ * d = 0 .. stride-1 (1 <= stride <= 12; a group of order k reads dimensions 0 .. 2k-1 of
+ * its sequences).  n = 0 clears them.  They stay staged for later calls. */
+int qiw_set_scrambles(qiw_ctx *ctx, int64_t n, int32_t stride, const uint32_t *m, const uint32_t *shift);
+
+/* qiw_scramble_dims: the dimensions D = 2k of the sequences a call with `run` consumes
+ * under scramble = 2, in the order it consumes them -- the order in which the reference
+ * constructs ScrambledSobolSeq(D; scramble_rng = rng) objects, so a caller draws exactly
+ * these from its RNG and stages them with qiw_set_scrambles.  N_seqs per sampled group,
+ * except min(2, N_seqs) for a group whose integral vanishes identically (no contributing
+ * diagram, or a simplex of zero volume: the correlator row tau_A = beta, the first step of
+ * diff_inchworm!), where the reference's stopping rule ends after the second sequence.  A
+ * call whose groups stop elsewhere (values identical across sequences for another reason)
+ * returns QIW_E_ARG after evaluating.  Writes up to cap entries and returns the count, or
+ * a negative error code.  No counterpart in the reference (its loops draw as they go). */
+enum {
+    QIW_CALL_INCHWORM = 0,       /* qiw_inchworm                                          */
+    QIW_CALL_STEP = 1,           /* qiw_inchworm_step                                     */
+    QIW_CALL_STEP_BARE = 2,      /* qiw_inchworm_step_bare                                */
+    QIW_CALL_CORRELATOR = 3,     /* qiw_correlator_2p, qiw_correlator_2p_std              */
+    QIW_CALL_DIFF = 4            /* qiw_diff_inchworm                                     */
+};
+int64_t qiw_scramble_dims(qiw_ctx *ctx, const qiw_run *run, int32_t call, int64_t cap, int32_t *dims);
 
 /* qiw_set_ppgf: overwrite the bold propagator state (expansion.P) with a table in BT
  * layout over the grid.  n_known grid points are valid.  der0 (BT layout, one block) is

@@ -986,6 +986,7 @@ static int g_threads = 1;
 struct RandParams {
     int n_seqs = 1;
     bool scramble = false;
+    bool given = false;  // caller-drawn sequences
     uint64_t seed = 0;
     double target_std = 0.0;
 };
@@ -994,19 +995,59 @@ static RandParams rand_params(const qiw_run *run) {
     if (run->rand_params) {
         r.n_seqs = run->rand_params->N_seqs;
         r.scramble = run->rand_params->scramble != 0;
+        r.given = run->rand_params->scramble == 2;
         r.seed = run->rand_params->seed;
         r.target_std = run->rand_params->target_std;
     }
     return r;
 }
 
+// Caller-drawn sequences (qiw_rand.scramble = 2): the states (m, x) of the
+// ScrambledSobolSeq objects the reference's loops construct with the caller's RNG
+// (src/randomization.jl:94), consumed in construction order -- here, the order in which the
+// oracle, following the reference's loops, asks for them.  Sequence i holds `stride` rows
+// of which the consuming group uses the first 2k.
+// The (domain, key, order, n_pts_after, s) of every consumed sequence is logged (the
+// consumption order, for tests).
+static struct {
+    std::vector<uint32_t> m, x;
+    int stride = 0;
+    int64_t n = 0, next = 0;
+    std::vector<int64_t> log;
+} g_given;
+
+extern "C" int orc_set_scrambles(int64_t n, int stride, const uint32_t *m, const uint32_t *x) {
+    g_given.n = n;
+    g_given.stride = stride;
+    g_given.next = 0;
+    g_given.log.clear();
+    g_given.m.assign(m, m + (size_t)n * stride * 32);
+    g_given.x.assign(x, x + (size_t)n * stride);
+    return 0;
+}
+extern "C" int64_t orc_scrambles_used(void) { return g_given.next; }
+extern "C" int64_t orc_scrambles_log(int64_t cap, int64_t *out) {
+    for (size_t i = 0; i < g_given.log.size() && (int64_t)i < 5 * cap; ++i) out[i] = g_given.log[i];
+    return (int64_t)g_given.log.size() / 5;
+}
+
 // Direction numbers and shift of sequence `seq` of a group: the staged ones (rng =
-// nothing), or the staged direction numbers scrambled with the stream's bits.
+// nothing), the staged direction numbers scrambled with the stream's bits, or the caller's
+// next sequence.
 struct SeqStore {
     std::vector<uint32_t> m, x;
 };
 static SobolSource sequence_source(const RandParams &rq, SobolSource staged, int dim, int domain, int64_t key,
                                    int order, int n_after, int seq, SeqStore &store) {
+    if (rq.given) {
+        if (g_given.next >= g_given.n || dim > g_given.stride)
+            throw std::runtime_error("not enough caller-drawn sequences");
+        const int64_t i = g_given.next++;
+        for (int64_t v : {(int64_t)domain, key, (int64_t)order, (int64_t)n_after, (int64_t)seq}) g_given.log.push_back(v);
+        store.m.assign(g_given.m.begin() + i * g_given.stride * 32, g_given.m.begin() + i * g_given.stride * 32 + (size_t)dim * 32);
+        store.x.assign(g_given.x.begin() + i * g_given.stride, g_given.x.begin() + i * g_given.stride + std::max(dim, 1));
+        return SobolSource{store.m.data(), store.x.data()};
+    }
     if (!rq.scramble) return staged;
     store.m.assign((size_t)dim * 32, 0u);
     store.x.assign((size_t)std::max(dim, 1), 0u);

@@ -96,6 +96,9 @@ struct qiw_ctx {
     // Sobol state
     int sob_dim = 0;
     std::vector<uint32_t> sob_m, sob_x;
+    // caller-drawn sequences (qiw_set_scrambles), on the device as buffers given.m / given.x
+    int64_t given_n = 0;
+    int given_stride = 0;
     // propagator state (host mirror of the device table), i*P in BT layout
     std::vector<double> ip;  // [ntau][bt]
     int n_known = 1;
@@ -126,7 +129,7 @@ int fail(qiw_ctx *c, int code, const std::string &msg);
 // Runs the body of a C entry point: a C++ exception (host memory exhausted, an index out of
 // range) becomes a QIW_E_* code and a message instead of crossing the C ABI.
 template <class F>
-int guarded(qiw_ctx *c, F &&body) {
+auto guarded(qiw_ctx *c, F &&body) -> decltype(body()) {
     try {
         return body();
     } catch (const std::bad_alloc &) {

@@ -145,6 +145,7 @@ struct EvalArgs {
     const uint32_t *topo;      // per topology two words of arc head/tail nibbles
     const double *coefs;
     const double *ut;          // simplex coordinates of all groups
+    int64_t ut_row_stride;     // coordinates of task row tau_index at ut + tau_index * stride (0: shared)
     const double *ptab;        // [n_tau][ns] i*P on the grid (bold groups)
     const double *e_shift;     // [ns] E_s + lambda (bare groups, exact atomic P0)
     QiwDeltaTables dtab;
@@ -194,6 +195,7 @@ struct FinalizeArgs {
     int n_seqs;                        // N_seqs
     double target_std;
     double *porders_std;               // [n_order_slots][n_tau][bt] standard deviations
+    int32_t *nuse_out;                 // optional [n_tau][n_real]: sequences used, row jf
     // differential step (k_diff_reduce_finalize): ptab is the table of sigma (Σ = i sigma,
     // rows by time difference), porders its order-resolved rows; the Euler step writes row
     // a + 1 of P from rows 0..a
@@ -224,6 +226,7 @@ struct CorrFinalizeArgs {
     int n_seqs;
     double target_std;
     double *out_std;                 // [n_rows]
+    int32_t *nuse_out;               // optional [n_rows][n_real]: sequences used
 };
 
 // Arguments of the block-sector evaluator (k_eval_block)
@@ -310,5 +313,10 @@ hipError_t qiw_launch_finalize_corr_rqmc(hipStream_t st, const CorrFinalizeArgs 
 // Simplex coordinates of every sampled group's sequence for one step: the staged
 // direction numbers m (dimensions 0..dim_max-1) LMS + shift scrambled with the bits of the
 // group's stream (domain, key, order, n_after, seq).
+// coordinates from caller-drawn sequences: row r = 0..n_rows-1 of group g (at ut + (row0 +
+// r) * row_stride) maps sequence idx[r * n_groups + g] (< 0: none)
+hipError_t qiw_launch_given_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int64_t n, const uint32_t *gm,
+                                   const uint32_t *gx, int stride, const int64_t *idx, int n_rows, int row0,
+                                   int64_t row_stride, double *ut);
 hipError_t qiw_launch_rqmc_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int max_dim, int64_t n,
                                   const uint32_t *m, uint64_t seed, int domain, int64_t key, double *ut);

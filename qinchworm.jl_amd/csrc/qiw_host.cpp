@@ -606,6 +606,8 @@ struct Batch {
     std::vector<int32_t> r_first, r_count;  // per (order, n_pts_after) group: its sequences' entries
     int n_eval = 1;        // sequences evaluated per sampled group
     bool scrambled = false;  // coordinates are drawn per step (randomized qMC)
+    bool given = false;      // ... from the caller's sequences
+    int64_t ut_total = 0;    // doubles of simplex coordinates of one launch (one tau row)
     int lds = 0;
     int fmt = FMT_SCALAR;  // program format of every group of the batch
     int spt = QIW_WAVE;    // samples per task
@@ -620,6 +622,8 @@ struct Batch {
 // 24-byte record on the host and the device, so a launch holds at most 2^28 tasks
 // (e.g. N = 2^30 samples of 16 groups in 64-sample tasks is already 2^28).
 constexpr int64_t kMaxTasks = (int64_t)1 << 28;
+// Per-row coordinates of a correlator run on caller-drawn sequences stay below this.
+constexpr double kMaxGivenCoordBytes = 64.0 * (1ull << 30);
 inline bool tasks_fit(Batch &b, int64_t more) {
     if ((int64_t)b.tasks.size() + more > kMaxTasks) b.overflow = true;
     return !b.overflow;
@@ -682,6 +686,7 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
     }
     // two blocks of END words after the last program: the evaluator prefetches one block ahead
     prog.insert(prog.end(), 2 * QIW_WAVE, pad);
+    b.ut_total = ut_total;
     if (topo.empty()) topo.push_back(0);
     if (coefs.empty()) coefs.push_back(0.0);
     d.groups = (QiwGroup *)dev(c, tag + ".groups", gs.size() * sizeof(QiwGroup), &rc);
@@ -733,6 +738,7 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
 struct RandPlan {
     int n_seqs = 1;
     bool scrambled = false;
+    bool given = false;  // the sequences are the caller's (qiw_set_scrambles)
     uint64_t seed = 0;
     double target_std = 0.0;
     int n_eval() const { return scrambled ? n_seqs : 1; }  // sequences evaluated per group
@@ -742,10 +748,81 @@ RandPlan rand_plan(const qiw_run *run) {
     if (run && run->rand_params) {
         r.n_seqs = run->rand_params->N_seqs;
         r.scrambled = run->rand_params->scramble != 0;
+        r.given = run->rand_params->scramble == 2;
         r.seed = run->rand_params->seed;
         r.target_std = run->rand_params->target_std;
     }
     return r;
+}
+
+// Caller-drawn sequences of one launch row (a step, a correlator tau row): the sequences
+// the reference constructs for the batch's groups, in group order (src/randomization.jl:
+// 93-99): N_seqs per sampled group, except that a group whose integral vanishes
+// identically -- no leaves in its programs, or a simplex of zero volume (`flat`) -- takes
+// min(2, N_seqs), since with target_std = 0 the reference stops after the second sequence,
+// whose deviation is exactly 0.  idx[g] is the sequence of entry g (-1: exact group; unused
+// replicas repeat their group's first), used[r] the sequences real group r takes (1 for an
+// exact one, as the bookkeeping counts it).  Returns the row's total.
+// (With idx and used null, only the dimensions 2k of the row's sequences are appended to
+// dims.)
+template <class Flat>
+int64_t given_row(const Batch &b, int n_seqs, int64_t base, Flat flat, int64_t *idx, int32_t *used,
+                  std::vector<int32_t> *dims = nullptr) {
+    int64_t n = 0;
+    for (size_t r = 0; r < b.r_first.size(); ++r) {
+        const int first = b.r_first[r], cnt = b.r_count[r];
+        const Compiled &cg = b.groups[first];
+        if (cg.g.order == 0 || (cg.g.kind == QIW_G_SIGMA && cg.g.order == 1)) {  // exact
+            for (int q = 0; q < cnt && idx; ++q) idx[first + q] = -1;
+            if (used) used[r] = 1;
+            continue;
+        }
+        const int u = (cg.leaves_per_sample == 0 || flat(cg.g)) ? std::min(2, n_seqs) : n_seqs;
+        for (int q = 0; q < cnt && idx; ++q) idx[first + q] = base + n + (q < u ? q : 0);
+        if (used) used[r] = u;
+        if (dims) dims->insert(dims->end(), u, 2 * cg.g.order);
+        n += u;
+    }
+    return n;
+}
+
+// The reference's stopping rule ended real groups where given_row assumed it would: a
+// group that given_row takes to vary (or to vanish) but whose sequences gave identical (or
+// different) values would have made the reference consume other sequences from then on.
+int check_given_use(qiw_ctx *c, const Batch &b, const std::vector<int32_t> &expect, const int32_t *dev_nuse,
+                    const std::vector<int> &rows) {
+    const size_t nr = b.r_first.size();
+    if (!dev_nuse || nr == 0 || rows.empty()) return 0;
+    const int max_row = *std::max_element(rows.begin(), rows.end());
+    std::vector<int32_t> got((size_t)(max_row + 1) * nr);
+    HIPCHK(c, hipMemcpy(got.data(), dev_nuse, got.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < rows.size(); ++i)
+        for (size_t r = 0; r < nr; ++r)
+            if (got[(size_t)rows[i] * nr + r] != expect[i * nr + r]) {
+                const QiwGroup &g = b.groups[b.r_first[r]].g;
+                return fail(c, QIW_E_ARG,
+                            "caller-drawn sequences: the group of order " + std::to_string(g.order) + ", n_pts_after " +
+                                std::to_string(g.n_after) + " stopped after " + std::to_string(got[(size_t)rows[i] * nr + r]) +
+                                " sequences, not " + std::to_string(expect[i * nr + r]) +
+                                ": its values are identical across sequences without a structural zero, so the "
+                                "reference would consume the caller's sequences differently");
+            }
+    return 0;
+}
+
+// Caller-drawn sequences for a call that consumes `need` of them, for groups up to order
+// max_order: checks the staged ones and returns their device copies.
+int given_buffers(qiw_ctx *c, int max_order, int64_t need, const uint32_t **gm, const uint32_t **gx) {
+    if (need > c->given_n)
+        return fail(c, QIW_E_ARG, "the call consumes " + std::to_string(need) + " caller-drawn sequences, " +
+                                      std::to_string(c->given_n) + " are staged (qiw_set_scrambles)");
+    if (need > 0 && c->given_stride < 2 * max_order)
+        return fail(c, QIW_E_ARG, "caller-drawn sequences have fewer dimensions than 2 * order");
+    int rc = 0;
+    const size_t n = (size_t)std::max<int64_t>(c->given_n, 1) * std::max(c->given_stride, 1);
+    *gm = (const uint32_t *)dev(c, "given.m", n * 32 * 4, &rc);
+    *gx = (const uint32_t *)dev(c, "given.x", n * 4, &rc);
+    return rc;
 }
 
 // Appends group cg (compiled) and, for a sampled group of a randomized run, its further
@@ -792,7 +869,10 @@ int check_run(qiw_ctx *c, const qiw_run *run) {
         const qiw_rand &r = *run->rand_params;
         if (r.N_seqs < 1) return fail(c, QIW_E_ARG, "rand_params.N_seqs must be > 0");
         if (r.N_seqs > QIW_MAX_SEQS) return fail(c, QIW_E_LIMIT, "rand_params.N_seqs beyond 64");
-        if (r.scramble != 0 && r.scramble != 1) return fail(c, QIW_E_ARG, "rand_params.scramble must be 0 or 1");
+        if (r.scramble < 0 || r.scramble > 2) return fail(c, QIW_E_ARG, "rand_params.scramble must be 0, 1 or 2");
+        // the reference draws sequence s + 1 of a group only if it does not stop after s
+        if (r.scramble == 2 && r.target_std != 0.0)
+            return fail(c, QIW_E_ARG, "caller-drawn sequences (scramble = 2) need target_std = 0");
         if (!(r.target_std >= 0.0)) return fail(c, QIW_E_ARG, "rand_params.target_std must be >= 0");
     }
     return 0;
@@ -894,6 +974,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
     const RandPlan rq = rand_plan(run);
     b.n_eval = rq.n_eval();
     b.scrambled = rq.scrambled;
+    b.given = rq.given;
     b.fmt = program_format(m, max_order, run->precision);
     b.max_order = max_order;
     b.spt = samples_per_task(c, m, b.fmt, n_groups * b.n_eval * std::max<int64_t>(run->N_samples, 1));
@@ -1020,7 +1101,7 @@ std::string plan_key(const qiw_ctx *c, const qiw_run *run, const std::vector<int
     put(&c->tune.samples_per_task, 4);
     put(&c->tune.chunk_cost, 8);
     const RandPlan rq = rand_plan(run);
-    const int rk[2] = {rq.n_eval(), rq.scrambled ? 1 : 0};
+    const int rk[2] = {rq.n_eval(), rq.given ? 2 : rq.scrambled ? 1 : 0};
     put(rk, 8);
     return k + c->model_fp;
 }
@@ -1468,6 +1549,34 @@ static int qiw_set_sobol_impl(qiw_ctx *c, int32_t dim, const uint32_t *m, const 
     return QIW_OK;
 }
 
+static int qiw_set_scrambles_impl(qiw_ctx *c, int64_t n, int32_t stride, const uint32_t *m, const uint32_t *shift) {
+    if (!c) return QIW_E_ARG;
+    if (n < 0 || (n > 0 && (!m || stride < 1 || stride > 2 * QIW_KMAX)))
+        return fail(c, QIW_E_ARG, "qiw_set_scrambles: n >= 0 sequences of 1 .. 12 dimensions");
+    if (n > ((int64_t)1 << 40) / (32 * std::max(stride, 1)))
+        return fail(c, QIW_E_LIMIT, "qiw_set_scrambles: too many sequences");
+    (void)hipSetDevice(c->device);
+    c->given_n = 0;
+    c->given_stride = 0;
+    if (n == 0) return QIW_OK;
+    const size_t rows = (size_t)n * stride;
+    int rc = 0;
+    uint32_t *gm = (uint32_t *)dev(c, "given.m", rows * 32 * 4, &rc);
+    uint32_t *gx = (uint32_t *)dev(c, "given.x", rows * 4, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // earlier calls may still read the buffers
+    HIPCHK(c, hipMemcpy(gm, m, rows * 32 * 4, hipMemcpyHostToDevice));
+    if (shift) {
+        HIPCHK(c, hipMemcpy(gx, shift, rows * 4, hipMemcpyHostToDevice));
+    } else {
+        const std::vector<uint32_t> zero(rows, 0u);
+        HIPCHK(c, hipMemcpy(gx, zero.data(), rows * 4, hipMemcpyHostToDevice));
+    }
+    c->given_n = n;
+    c->given_stride = stride;
+    return QIW_OK;
+}
+
 static int qiw_set_ppgf_impl(qiw_ctx *c, int32_t n_known, const double *P, const double *der0) {
     if (!c || !P) return QIW_E_ARG;
     if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
@@ -1524,6 +1633,42 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     if (last_w >= first_w && (rc = step_plan(c, run, oall, false, want_warp, sp))) return rc;
     Batch &bare_b = bp->b, &bold_b = sp->b;
     DevBatch &bare_d = bp->d, &bold_d = sp->d;
+    // caller-drawn sequences: the initial step's, then every bold step's in step order
+    const uint32_t *gm = nullptr, *gx = nullptr;
+    const int64_t *gidx_bare = nullptr, *gidx_bold = nullptr;
+    int32_t *nuse_bare = nullptr, *nuse_bold = nullptr;  // sequences the bookkeeping used (N_seqs >= 3)
+    std::vector<int32_t> use_bare, use_bold;
+    std::vector<int> rows_bare, rows_bold;
+    if (rq.given) {
+        if (c->no_alloc) return fail(c, QIW_E_ARG, "caller-drawn sequences need qiw_inchworm_step");
+        const auto none = [](const QiwGroup &) { return false; };  // bare and bold simplices have volume
+        std::vector<int64_t> idx;
+        int64_t n = 0;
+        const size_t ngb = bare_b.groups.size(), ngo = bold_b.groups.size();
+        if (do_bare && ngb) {
+            idx.resize(ngb);
+            use_bare.resize(bare_b.r_first.size());
+            n += given_row(bare_b, rq.n_seqs, n, none, idx.data(), use_bare.data());
+            rows_bare.push_back(1);
+        }
+        for (int jw = first_w; jw <= last_w && ngo; ++jw) {
+            const size_t o = idx.size(), ou = use_bold.size();
+            idx.resize(o + ngo);
+            use_bold.resize(ou + bold_b.r_first.size());
+            n += given_row(bold_b, rq.n_seqs, n, none, idx.data() + o, use_bold.data() + ou);
+            rows_bold.push_back(jw + 1);
+        }
+        if ((rc = given_buffers(c, std::max(bare_b.max_order, bold_b.max_order), n, &gm, &gx))) return rc;
+        int64_t *didx = (int64_t *)dev(c, "given.idx", std::max<size_t>(idx.size(), 1) * 8, &rc);
+        if (rq.n_seqs >= 3) {
+            nuse_bare = (int32_t *)dev(c, "given.nuse_bare", (size_t)m.ntau * std::max<size_t>(bare_b.r_first.size(), 1) * 4, &rc);
+            nuse_bold = (int32_t *)dev(c, "given.nuse_bold", (size_t)m.ntau * std::max<size_t>(bold_b.r_first.size(), 1) * 4, &rc);
+        }
+        if (rc) return rc;
+        if (!idx.empty()) HIPCHK(c, hipMemcpy(didx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice));
+        gidx_bare = didx;
+        gidx_bold = didx + (rows_bare.empty() ? 0 : ngb);
+    }
     const bool warped = want_warp && !bold_b.groups.empty();
     if (warped) {
         std::vector<WarpGroup> wg;
@@ -1583,8 +1728,11 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         return rqmc ? qiw_launch_finalize_step_rqmc(c->stream, fa) : qiw_launch_finalize_step(c->stream, fa);
     };
     // per-step scrambled sequences: coordinates of every group's sequences for this step
-    auto coords = [&](const Batch &b, const DevBatch &d, int domain, int key) -> hipError_t {
+    auto coords = [&](const Batch &b, const DevBatch &d, int domain, int key, const int64_t *gidx) -> hipError_t {
         if (!b.scrambled || run->N_samples <= 0) return hipSuccess;  // N = 0: order 0 only
+        if (b.given)
+            return qiw_launch_given_coords(c->stream, d.groups, (int)b.groups.size(), run->N_samples, gm, gx,
+                                           c->given_stride, gidx, 1, 0, 0, d.ut);
         return qiw_launch_rqmc_coords(c->stream, d.groups, (int)b.groups.size(), 2 * b.max_order, run->N_samples,
                                       sobm, rq.seed, domain, key, d.ut);
     };
@@ -1606,7 +1754,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     int n_known = c->n_known;
     if (!bare_b.groups.empty()) {
         EvalArgs a = make_eval_args(c, bare_b, bare_d, ptab, eshift, dcoef, dkind, partial);
-        HIPCHK(c, coords(bare_b, bare_d, QIW_RQ_BARE, 0));
+        HIPCHK(c, coords(bare_b, bare_d, QIW_RQ_BARE, 0, gidx_bare));
         HIPCHK(c, mark());
         if ((rc = launch_eval(c, a, (int)bare_b.tasks.size(), bare_b.fmt, bare_b.precision))) return rc;
         HIPCHK(c, mark());
@@ -1621,6 +1769,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         fa.bare = 1;
         fa.normalize = 0;
         fa.n_known = 0;
+        fa.nuse_out = nuse_bare;
         HIPCHK(c, finalize(bare_b, bare_d));
         ++launches;
         if (m.ppgf_kind == QIW_GF_INC_SPLINE) n_known = 2;
@@ -1637,6 +1786,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         fa.group_order_slot = bold_d.g_slot;
         fa.n_groups = (int)bold_b.groups.size();
         fa.bare = 0;
+        fa.nuse_out = nuse_bold;
         // arguments of every step, as the eager loop below would launch them
         auto step_args = [&](int jw) {
             a.step_w = jw;
@@ -1650,7 +1800,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         // flat scalar walk without the warp: the whole loop is one hipGraph, replayed while
         // the launch arguments are unchanged (the other walks stage data from the host per
         // launch and stay eager)
-        bool graph = c->graphs && bold_b.fmt == FMT_SCALAR && !warped && !c->no_alloc;
+        bool graph = c->graphs && bold_b.fmt == FMT_SCALAR && !warped && !c->no_alloc && !rq.given;
         const int n_known_start = n_known;
         if (graph) {
             if (!c->graph) c->graph = new StepGraph;
@@ -1689,7 +1839,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
                 hipError_t err = hipSuccess;
                 for (int jw = first_w; jw <= last_w && err == hipSuccess; ++jw) {
                     step_args(jw);
-                    if ((err = coords(bold_b, bold_d, QIW_RQ_BOLD, jw)) != hipSuccess) break;
+                    if ((err = coords(bold_b, bold_d, QIW_RQ_BOLD, jw, nullptr)) != hipSuccess) break;  // never given
                     if ((err = qiw_launch_eval_flat(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab)) != hipSuccess) break;
                     err = finalize(bold_b, bold_d);
                 }
@@ -1729,7 +1879,8 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         for (int jw = first_w; jw <= last_w && !graph; ++jw) {
             a.step_w = jw;
             if (warped && (rc = warp_points(c, jw * m.h / m.beta, bold_d.ut))) return rc;
-            HIPCHK(c, coords(bold_b, bold_d, QIW_RQ_BOLD, jw));
+            HIPCHK(c, coords(bold_b, bold_d, QIW_RQ_BOLD, jw,
+                             gidx_bold ? gidx_bold + (size_t)(jw - first_w) * bold_b.groups.size() : nullptr));
             HIPCHK(c, mark());
             if ((rc = launch_eval(c, a, (int)bold_b.tasks.size(), bold_b.fmt, bold_b.precision))) return rc;
             HIPCHK(c, mark());
@@ -1758,6 +1909,8 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
         c->stats.kernel_ms += ms;
     }
     c->stats.n_kernel_launches += launches;
+    if ((rc = check_given_use(c, bare_b, use_bare, nuse_bare, rows_bare))) return rc;
+    if ((rc = check_given_use(c, bold_b, use_bold, nuse_bold, rows_bold))) return rc;
 
     std::vector<double> ptab_h((size_t)m.ntau * m.bt), por((size_t)n_slots * m.ntau * m.bt), lam_h(m.ntau);
     std::vector<double> pstd_h(pstd ? por.size() : 0);
@@ -1849,6 +2002,7 @@ int build_diff_batch(qiw_ctx *c, const qiw_run *run, Batch &b) {
     const RandPlan rq = rand_plan(run);
     b.n_eval = rq.n_eval();
     b.scrambled = rq.scrambled;
+    b.given = rq.given;
     b.fmt = program_format(m, max_order, run->precision);
     b.max_order = max_order;
     b.spt = samples_per_task(c, m, b.fmt, n_groups * b.n_eval * std::max<int64_t>(run->N_samples, 1));
@@ -1910,6 +2064,30 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
     const DevBatch &d = p.d;
     // the reference sums the orders' self-energies (src/inchworm.jl:606), which needs one
     if (b.groups.empty()) return fail(c, QIW_E_ARG, "diff_inchworm needs a self-energy order >= 1");
+    // caller-drawn sequences: every step's in step order; at tau_f_prev = tau_i (step 0) the
+    // sampled self-energy simplices have zero volume
+    const uint32_t *gm = nullptr, *gx = nullptr;
+    const int64_t *gidx = nullptr;
+    int32_t *nuse = nullptr;
+    std::vector<int32_t> use;
+    std::vector<int> rows;
+    if (rq.given && run->N_samples > 0) {
+        const size_t ng = b.groups.size(), nr = b.r_first.size();
+        std::vector<int64_t> idx((size_t)(m.ntau - 1) * ng);
+        use.resize((size_t)(m.ntau - 1) * nr);
+        int64_t n = 0;
+        for (int st = 0; st + 1 < m.ntau; ++st) {
+            n += given_row(b, rq.n_seqs, n, [&](const QiwGroup &) { return st == 0; }, idx.data() + st * ng,
+                           use.data() + st * nr);
+            rows.push_back(st + 1);
+        }
+        if ((rc = given_buffers(c, b.max_order, n, &gm, &gx))) return rc;
+        int64_t *didx = (int64_t *)dev(c, "given.idx", std::max<size_t>(idx.size(), 1) * 8, &rc);
+        if (rq.n_seqs >= 3) nuse = (int32_t *)dev(c, "given.nuse_diff", (size_t)m.ntau * std::max<size_t>(nr, 1) * 4, &rc);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpy(didx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice));
+        gidx = didx;
+    }
     double *dcoef, *eshift;
     int32_t *dkind;
     if ((rc = ensure_model_buffers(c, &dcoef, &dkind, &eshift))) return rc;
@@ -1965,6 +2143,7 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
     fa.n_seqs = rq.n_seqs;
     fa.target_std = rq.target_std;
     fa.porders_std = pstd;
+    fa.nuse_out = nuse;
     fa.P = ptab;
     fa.hdiag = hdiag;
     fa.dims = ddims;
@@ -1994,7 +2173,10 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
         fa.jt = st + sig_shift;
         fa.tau_w = fa.tau_f = st * m.h;
         a.step_w = st;
-        if (b.scrambled)
+        if (gidx)
+            HIPCHK(c, qiw_launch_given_coords(c->stream, d.groups, (int)b.groups.size(), run->N_samples, gm, gx,
+                                              c->given_stride, gidx + (size_t)st * b.groups.size(), 1, 0, 0, d.ut));
+        else if (b.scrambled && !b.given)
             HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, (int)b.groups.size(), 2 * b.max_order,
                                              run->N_samples > 0 ? run->N_samples : 1, sobm, rq.seed, QIW_RQ_DIFF, st,
                                              d.ut));
@@ -2014,6 +2196,7 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
         c->stats.kernel_ms += ms;
     }
     c->stats.n_kernel_launches += launches;
+    if ((rc = check_given_use(c, b, use, nuse, rows))) return rc;
     std::vector<double> P_h(tab), por((size_t)n_slots * tab), pstd_h(pstd ? por.size() : 0);
     HIPCHK(c, hipMemcpy(P_h.data(), ptab, tab * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(por.data(), porders, por.size() * 8, hipMemcpyDeviceToHost));
@@ -2115,6 +2298,40 @@ static int qiw_inchworm_step_bare_impl(qiw_ctx *c, const qiw_run *run, double *v
 }
 
 namespace {
+// The groups of operator pair ci (src/inchworm.jl:957-981): every n_pts_after of every
+// order with topologies, compiled.
+int build_corr_groups(qiw_ctx *c, const qiw_run *run, int ci, Batch &b, bool *first_is_order0) {
+    const HostModel &m = c->m;
+    const RandPlan rq = rand_plan(run);
+    int max_order = 0;
+    for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
+    b.fmt = program_format(m, max_order, run->precision);
+    b.max_order = max_order;
+    b.precision = run->precision;
+    b.n_eval = rq.n_eval();
+    b.scrambled = rq.scrambled;
+    b.given = rq.given;
+    *first_is_order0 = false;
+    for (int oi = 0; oi < run->n_orders; ++oi) {
+        int order = run->orders[oi];
+        if (order > 0 && run->N_samples <= 0) continue;
+        int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
+        for (int na = lo; na <= hi; ++na) {
+            Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
+            comp.chunk_words = c->tune.chunk_words > 0 ? c->tune.chunk_words : kChunkWords;
+            comp.chunk_cost = c->tune.chunk_cost > 0 ? c->tune.chunk_cost : kChunkCost;
+            Compiled cg;
+            if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
+            if (cg.n_top == 0) continue;
+            if (b.groups.empty() && order == 0) *first_is_order0 = true;
+            b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
+            push_sequences(b, std::move(cg), b.n_eval, [](int) {});
+        }
+    }
+    if (b.r_first.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 (order, n_pts_after) groups");
+    return 0;
+}
+
 // correlator_2p(expansion, grid, orders, N_samples, RequestStdDev(); rand_params)
 // (src/inchworm.jl:918-1052); corr_std_out may be NULL.
 int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *corr_std_out) {
@@ -2158,6 +2375,7 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
     };
     std::vector<PairOut> pending;
     pending.reserve(m.ncorr);
+    int64_t given_next = 0;  // caller-drawn sequences taken by the pairs so far
     for (int ci = 0; ci < m.ncorr; ++ci) {
         // groups (src/inchworm.jl:957-981): every n_pts_after of every order
         const std::string tag = "corr" + std::to_string(ci);
@@ -2166,30 +2384,7 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         if (p.key != key) {
             p = Plan{};
             Batch &b = p.b;
-            int max_order = 0;
-            for (int oi = 0; oi < run->n_orders; ++oi) max_order = std::max(max_order, (int)run->orders[oi]);
-            b.fmt = program_format(m, max_order, run->precision);
-            b.max_order = max_order;
-            b.precision = run->precision;
-            b.n_eval = rq.n_eval();
-            b.scrambled = rq.scrambled;
-            for (int oi = 0; oi < run->n_orders; ++oi) {
-                int order = run->orders[oi];
-                if (order > 0 && run->N_samples <= 0) continue;
-                int lo = order == 0 ? 0 : 1, hi = order == 0 ? 0 : 2 * order - 1;
-                for (int na = lo; na <= hi; ++na) {
-                    Compiler comp(m, QIW_G_CORR, order, na, ci, b.fmt, b.max_order);
-                    comp.chunk_words = c->tune.chunk_words > 0 ? c->tune.chunk_words : kChunkWords;
-                    comp.chunk_cost = c->tune.chunk_cost > 0 ? c->tune.chunk_cost : kChunkCost;
-                    Compiled cg;
-                    if (!comp.compile(cg)) return fail(c, QIW_E_OFFDIAG, comp.err);
-                    if (cg.n_top == 0) continue;
-                    if (b.groups.empty() && order == 0) p.first_is_order0 = true;
-                    b.lds = std::max(b.lds, lds_slots(cg.g, m, b.fmt));
-                    push_sequences(b, std::move(cg), b.n_eval, [](int) {});
-                }
-            }
-            if (b.r_first.size() > 64) return fail(c, QIW_E_LIMIT, "more than 64 (order, n_pts_after) groups");
+            if ((rc = build_corr_groups(c, run, ci, b, &p.first_is_order0))) return rc;
             if (!b.groups.empty()) {
                 int ng = (int)b.groups.size();
                 b.spt = samples_per_task(c, m, b.fmt, (int64_t)m.ntau * ng * std::max<int64_t>(run->N_samples, 1));
@@ -2236,9 +2431,42 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         double *dout = (double *)dev(c, "corr.out", (size_t)m.ntau * 8, &rc);
         double *dout_std = rqmc ? (double *)dev(c, "corr.out_std", (size_t)m.ntau * 8, &rc) : nullptr;
         if (rc) return rc;
-        if (sobm && run->N_samples > 0)  // one set of scrambled sequences for all tau rows of the pair
+        // caller-drawn sequences: tau rows 1 .. n_tau-1 of every pair in turn, each with its own
+        // coordinates (row j at ut + j * ut_total); at tau_A = beta the simplices after tau_A
+        // have zero volume
+        double *ut_rows = nullptr;
+        int32_t *nuse = nullptr;
+        std::vector<int32_t> use;
+        std::vector<int> rows;
+        if (b.given && run->N_samples > 0) {
+            const size_t nr = b.r_first.size();
+            std::vector<int64_t> idx((size_t)(m.ntau - 1) * ng);
+            use.resize((size_t)(m.ntau - 1) * nr);
+            int64_t n = 0;
+            for (int j = 1; j < m.ntau; ++j) {
+                n += given_row(b, rq.n_seqs, given_next + n,
+                               [&](const QiwGroup &g) { return g.d_after > 0 && j == m.ntau - 1; },
+                               idx.data() + (size_t)(j - 1) * ng, use.data() + (size_t)(j - 1) * nr);
+                rows.push_back(j);
+            }
+            const uint32_t *gm = nullptr, *gx = nullptr;
+            if ((rc = given_buffers(c, b.max_order, given_next + n, &gm, &gx))) return rc;
+            given_next += n;
+            if ((double)m.ntau * b.ut_total * 8 > kMaxGivenCoordBytes)
+                return fail(c, QIW_E_LIMIT, "coordinates of every tau row exceed 64 GiB (lower N_samples or N_seqs)");
+            // the buffers are reused by the next pair: queued pairs finish with them first
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            ut_rows = (double *)dev(c, "corr.ut_rows", (size_t)m.ntau * b.ut_total * 8, &rc);
+            int64_t *didx = (int64_t *)dev(c, "given.idx", idx.size() * 8, &rc);
+            if (rq.n_seqs >= 3) nuse = (int32_t *)dev(c, "given.nuse_corr", (size_t)m.ntau * nr * 4, &rc);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpy(didx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice));
+            HIPCHK(c, qiw_launch_given_coords(c->stream, d.groups, ng, run->N_samples, gm, gx, c->given_stride, didx,
+                                              m.ntau - 1, 1, b.ut_total, ut_rows));
+        } else if (sobm && !b.given && run->N_samples > 0) {  // one set of scrambled sequences for all tau rows
             HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, ng, 2 * b.max_order, run->N_samples, sobm, rq.seed,
                                              QIW_RQ_CORR, ci, d.ut));
+        }
         pending.push_back(PairOut{ci, std::vector<double>(m.ntau), std::vector<double>(m.ntau, 0.0), nullptr, nullptr,
                                   false});
         PairOut &po = pending.back();
@@ -2248,6 +2476,10 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         HIPCHK(c, hipEventCreate(&e1));
         HIPCHK(c, hipEventRecord(e0, c->stream));
         EvalArgs a = make_eval_args(c, b, d, ptab, eshift, dcoef, dkind, partial);
+        if (ut_rows) {
+            a.ut = ut_rows;
+            a.ut_row_stride = b.ut_total;
+        }
         if ((rc = launch_eval(c, a, (int)b.tasks.size(), b.fmt, b.precision))) return rc;
         HIPCHK(c, hipEventRecord(e1, c->stream));
         CorrFinalizeArgs fa{};
@@ -2270,7 +2502,9 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
         fa.n_seqs = rq.n_seqs;
         fa.target_std = rq.target_std;
         fa.out_std = dout_std;
+        fa.nuse_out = nuse;
         HIPCHK(c, rqmc ? qiw_launch_finalize_corr_rqmc(c->stream, fa) : qiw_launch_finalize_corr(c->stream, fa));
+        if ((rc = check_given_use(c, b, use, nuse, rows))) return rc;
         HIPCHK(c, hipMemcpyAsync(po.out.data(), dout, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
         if (dout_std)
             HIPCHK(c, hipMemcpyAsync(po.out_std.data(), dout_std, m.ntau * 8, hipMemcpyDeviceToHost, c->stream));
@@ -2592,8 +2826,60 @@ extern "C" int qiw_set_model(qiw_ctx *c, const qiw_model *mi) {
     return guarded(c, [&] { return qiw_set_model_impl(c, mi); });
 }
 
+// The dimensions of the caller-drawn sequences a call consumes, in order (as the runs
+// above assign them).
+static int64_t qiw_scramble_dims_impl(qiw_ctx *c, const qiw_run *run, int32_t call, int64_t cap, int32_t *dims) {
+    if (!c) return QIW_E_ARG;
+    if (cap > 0 && !dims) return fail(c, QIW_E_ARG, "null dims");
+    int rc = check_run(c, run);
+    if (rc) return rc;
+    const HostModel &m = c->m;
+    const RandPlan rq = rand_plan(run);
+    std::vector<int32_t> out;
+    const auto none = [](const QiwGroup &) { return false; };
+    if (run->N_samples > 0) {
+        if (call == QIW_CALL_INCHWORM || call == QIW_CALL_STEP_BARE) {
+            Batch b;
+            if ((rc = build_step_batch(c, run, true, orders_all(run), b))) return rc;
+            given_row(b, rq.n_seqs, 0, none, nullptr, nullptr, &out);
+        }
+        if (call == QIW_CALL_INCHWORM || call == QIW_CALL_STEP) {
+            Batch b;
+            if ((rc = build_step_batch(c, run, false, orders_all(run), b))) return rc;
+            const int steps = call == QIW_CALL_STEP ? 1 : m.ntau - 2;
+            for (int i = 0; i < steps; ++i) given_row(b, rq.n_seqs, 0, none, nullptr, nullptr, &out);
+        } else if (call == QIW_CALL_CORRELATOR) {
+            for (int ci = 0; ci < m.ncorr; ++ci) {
+                Batch b;
+                bool o0 = false;
+                if ((rc = build_corr_groups(c, run, ci, b, &o0))) return rc;
+                for (int j = 1; j < m.ntau; ++j)
+                    given_row(b, rq.n_seqs, 0, [&](const QiwGroup &g) { return g.d_after > 0 && j == m.ntau - 1; },
+                              nullptr, nullptr, &out);
+            }
+        } else if (call == QIW_CALL_DIFF) {
+            Batch b;
+            if ((rc = build_diff_batch(c, run, b))) return rc;
+            for (int st = 0; st + 1 < m.ntau; ++st)
+                given_row(b, rq.n_seqs, 0, [&](const QiwGroup &) { return st == 0; }, nullptr, nullptr, &out);
+        } else if (call != QIW_CALL_STEP_BARE) {
+            return fail(c, QIW_E_ARG, "unknown call kind");
+        }
+    }
+    for (int64_t i = 0; i < (int64_t)out.size() && i < cap; ++i) dims[i] = out[i];
+    return (int64_t)out.size();
+}
+
 extern "C" int qiw_set_sobol(qiw_ctx *c, int32_t dim, const uint32_t *m, const uint32_t *shift) {
     return guarded(c, [&] { return qiw_set_sobol_impl(c, dim, m, shift); });
+}
+
+extern "C" int qiw_set_scrambles(qiw_ctx *c, int64_t n, int32_t stride, const uint32_t *m, const uint32_t *shift) {
+    return guarded(c, [&] { return qiw_set_scrambles_impl(c, n, stride, m, shift); });
+}
+
+extern "C" int64_t qiw_scramble_dims(qiw_ctx *c, const qiw_run *run, int32_t call, int64_t cap, int32_t *dims) {
+    return guarded(c, [&] { return qiw_scramble_dims_impl(c, run, call, cap, dims); });
 }
 
 extern "C" int qiw_set_ppgf(qiw_ctx *c, int32_t n_known, const double *P, const double *der0) {

@@ -390,7 +390,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     }
     const int64_t sample = (int64_t)task.first_sample + lane;
     const bool active = sample < g.n_samples;
-    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples +
+    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (int64_t)(g.d_before + g.d_after) * g.n_samples +
                                         (active ? sample : 0)]
                                  : 1.0;
 
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
     if (g.sampled) {
         const int64_t N = g.n_samples;
-        const double *u = a.ut + g.ut_off + (active ? sample : 0);
+        const double *u = a.ut + g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (active ? sample : 0);
         const double dfw = tau_f - tau_w, dwi = tau_w - tau_i;
 #pragma unroll
         for (int j = 0; j < 2 * KM; ++j) {
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
         for (int p = 0; p <= QIW_NMAX; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
         if (g.sampled) {
             const int64_t N = g.n_samples;
-            const double *u = a.ut + g.ut_off + sample;
+            const double *u = a.ut + g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + sample;
 #pragma unroll
             for (int j = 0; j < 2 * QIW_KMAX; ++j) {
                 if (j < g.d_after) {
@@ -1023,7 +1023,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
     for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
     if (g.sampled) {
         const int64_t N = g.n_samples;
-        const double *u = a.ut + g.ut_off + (active ? sample : 0);
+        const double *u = a.ut + g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (active ? sample : 0);
         const double dfw = tau_f - tau_w, dwi = tau_w - tau_i;
 #pragma unroll
         for (int j = 0; j < 2 * KM; ++j) {
@@ -1076,7 +1076,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
 #pragma unroll
     for (int i = 0; i < MB; ++i) st.acc[i] = 0.0;
 
-    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples +
+    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (int64_t)(g.d_before + g.d_after) * g.n_samples +
                                         (active ? sample : 0)]
                                  : 1.0;
     double absum = 0.0;
@@ -1247,7 +1247,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
         for (int p = 0; p <= NM; ++p) t[p] = (p == 1) ? tau_i : (p == pos_w) ? tau_w : (p == n) ? tau_f : 0.0;
         if (g.sampled) {
             const int64_t N = g.n_samples;
-            const double *u = a.ut + g.ut_off + sample;
+            const double *u = a.ut + g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + sample;
 #pragma unroll
             for (int j = 0; j < 2 * KM; ++j) {
                 if (j < g.d_after) {
@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
                 }
             }
         }
-        const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)(g.d_before + g.d_after) * g.n_samples + sample] : 1.0;
+        const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (int64_t)(g.d_before + g.d_after) * g.n_samples + sample] : 1.0;
         // ---- i*P blocks of every segment into LDS (lanes over BT elements) ----
         for (int L = 2; L <= n; ++L) {
             double dt = pick(t, L) - pick(t, L - 1);
@@ -1651,6 +1651,41 @@ __global__ __launch_bounds__(256) void k_rqmc_coords(const QiwGroup *__restrict_
     for (int k = 0; k < d; ++k) out[(int64_t)k * ng + r] = u[k];
 }
 
+// Simplex coordinates from caller-drawn sequences (the states of the ScrambledSobolSeq
+// objects the reference constructs with the caller's RNG, src/randomization.jl:94 and
+// src/scrambled_sobol.jl:66-143); grid: 256-sample blocks x groups x rows.  Row r of group
+// g maps sequence idx[r * n_groups + g] (gm: [.][stride][32] direction numbers, gx:
+// [.][stride] shifts) as k_rqmc_coords maps a scrambled one, into ut + (row0 + r) *
+// row_stride.
+__global__ __launch_bounds__(256) void k_given_coords(const QiwGroup *__restrict__ groups,
+                                                      const uint32_t *__restrict__ gm, const uint32_t *__restrict__ gx,
+                                                      int stride, int64_t n, const int64_t *__restrict__ idx,
+                                                      int row0, int64_t row_stride, double *__restrict__ ut) {
+    const QiwGroup g = groups[blockIdx.y];
+    const int64_t seq = idx[(int64_t)blockIdx.z * gridDim.y + blockIdx.y];
+    if (!g.sampled || seq < 0) return;
+    const bool sigma = g.kind == QIW_G_SIGMA;
+    const int d = sigma ? 2 * g.order - 2 : g.d_before + g.d_after;  // Sobol dimensions used
+    __shared__ uint32_t ms[2 * QIW_KMAX * 32];
+    __shared__ uint32_t xs[2 * QIW_KMAX];
+    for (int i = threadIdx.x; i < d * 32; i += blockDim.x) ms[i] = gm[(seq * stride + i / 32) * 32 + i % 32];
+    for (int i = threadIdx.x; i < d; i += blockDim.x) xs[i] = gx[seq * stride + i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ng = g.n_samples;  // <= n
+    if (r >= ng) return;
+    double x[2 * QIW_KMAX], u[2 * QIW_KMAX];
+    for (int k = 0; k < d; ++k) x[k] = ldexp((double)sobol_word(ms, xs[k], k, (uint64_t)r), -32);
+    double *out = ut + (int64_t)(row0 + blockIdx.z) * row_stride + g.ut_off;
+    if (sigma) {
+        sigma_row_values(x, g.order, u);
+        for (int k = 0; k < 2 * g.order; ++k) out[(int64_t)k * ng + r] = u[k];
+        return;
+    }
+    root_map(x, g.d_after, g.d_before, u);
+    for (int k = 0; k < d; ++k) out[(int64_t)k * ng + r] = u[k];
+}
+
 // Step bookkeeping with randomized qMC (inchworm_step, src/inchworm.jl:172-195), one
 // block of 256 threads after all per-sequence sums are in gsum.  For each (order,
 // n_pts_after) group, the prefactor-scaled sums of its sequences are the samples; the
@@ -1695,6 +1730,7 @@ __device__ void finalize_step_rqmc_body(const FinalizeArgs &a) {
                 break;
             }
         }
+        if (a.nuse_out && tid == 0) a.nuse_out[(int64_t)jf * a.n_real + r] = nuse;
         for (int e = tid; e < bt; e += blockDim.x) {
             auto x = [&](int s) { return __dmul_rn(fac, gsum[(int64_t)(base + (cnt > 1 ? s : 0)) * bt + e]); };
             const double mean = __dmul_rn(seq_sum(x, nuse), 1.0 / nuse);
@@ -1817,6 +1853,7 @@ __global__ __launch_bounds__(256) void k_finalize_corr_rqmc(CorrFinalizeArgs a) 
                     nuse = n;
                     break;
                 }
+            if (a.nuse_out) a.nuse_out[(int64_t)row * a.n_real + r] = nuse;
             tot += seq_sum(x, nuse) / nuse;
             tot_std += qiw_group_exact(g) ? 0.0 : (nuse > 1 ? seq_std(x, nuse) : __builtin_nan(""));
         }
@@ -1974,6 +2011,15 @@ hipError_t qiw_launch_sigma_coords(hipStream_t st, const uint32_t *m, const uint
 hipError_t qiw_launch_finalize_corr_rqmc(hipStream_t st, const CorrFinalizeArgs &a) {
     if (a.n_seqs < 1 || a.n_seqs > QIW_MAX_SEQS) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_finalize_corr_rqmc, dim3(a.n_rows), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t qiw_launch_given_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int64_t n, const uint32_t *gm,
+                                   const uint32_t *gx, int stride, const int64_t *idx, int n_rows, int row0,
+                                   int64_t row_stride, double *ut) {
+    if (n_groups < 1 || n < 1 || n_rows < 1 || stride < 1 || stride > 2 * QIW_KMAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_given_coords, dim3((unsigned)((n + 255) / 256), (unsigned)n_groups, (unsigned)n_rows), dim3(256),
+                       0, st, groups, gm, gx, stride, n, idx, row0, row_stride, ut);
     return hipGetLastError();
 }
 

@@ -52,8 +52,11 @@ struct QiwModel
     corr_B::Ptr{QiwSbm}
 end
 
-# Randomized qMC (qiw_rand): the scramble bits of every sequence come from libqiw's
-# counter-based generator keyed by `seed`; one seed is drawn from rand_params.rng per call.
+# Randomized qMC (qiw_rand).  With rand_params.rng given and target_std = 0 the sequences
+# are the reference's own (scramble = 2): stage_scrambles! constructs the
+# ScrambledSobolSeq objects the reference's loops would, in their order, from rng.  With a
+# stopping target the scramble bits come from libqiw's counter-based generator keyed by a
+# seed drawn from rng once per call (scramble = 1).
 struct QiwRand
     N_seqs::Int32
     scramble::Int32
@@ -61,9 +64,15 @@ struct QiwRand
     target_std::Float64
 end
 
+caller_drawn(rp::RandomizationParams) = rp.rng !== nothing && rp.target_std == 0.0
+
 QiwRand(rp::RandomizationParams) =
-    QiwRand(Int32(rp.N_seqs), rp.rng === nothing ? Int32(0) : Int32(1),
-            rp.rng === nothing ? UInt64(0) : rand(rp.rng, UInt64), rp.target_std)
+    QiwRand(Int32(rp.N_seqs), rp.rng === nothing ? Int32(0) : caller_drawn(rp) ? Int32(2) : Int32(1),
+            (rp.rng === nothing || caller_drawn(rp)) ? UInt64(0) : rand(rp.rng, UInt64), rp.target_std)
+
+const QIW_CALL_INCHWORM = Int32(0)
+const QIW_CALL_CORRELATOR = Int32(3)
+const QIW_CALL_DIFF = Int32(4)
 
 struct QiwRun
     n_orders::Int32
@@ -74,6 +83,29 @@ struct QiwRun
     n_pts_after_max::Int32
     precision::Int32
     rand_params::Ptr{QiwRand}   # C_NULL: RandomizationParams()
+end
+
+# The sequences of one call under scramble = 2: qiw_scramble_dims lists their dimensions in
+# the order the reference constructs them (src/randomization.jl:94); each is drawn from rng
+# exactly as the reference draws it and staged as (m[i][d][j], x[i][d]).
+function stage_scrambles!(ctx, run::Ref{QiwRun}, call::Int32, rp::RandomizationParams)
+    caller_drawn(rp) || return nothing
+    sig = (Ptr{Cvoid}, Ref{QiwRun}, Int32, Int64, Ptr{Int32})
+    n = ccall((:qiw_scramble_dims, libqiw), Int64, sig, ctx.h, run, call, 0, C_NULL)
+    n < 0 && check(ctx, Cint(n))
+    dims = Vector{Int32}(undef, n)
+    check(ctx, Cint(min(0, ccall((:qiw_scramble_dims, libqiw), Int64, sig, ctx.h, run, call, n, dims))))
+    stride = Int(maximum(dims; init = Int32(2)))
+    m = zeros(UInt32, 32, stride, n)
+    x = zeros(UInt32, stride, n)
+    for (i, D) in enumerate(dims)
+        seq = ScrambledSobolSeq(Int(D); scramble_rng = rp.rng)
+        m[:, 1:D, i] = permutedims(seq.m)
+        x[1:D, i] = seq.x
+    end
+    check(ctx, ccall((:qiw_set_scrambles, libqiw), Cint, (Ptr{Cvoid}, Int64, Int32, Ptr{UInt32}, Ptr{UInt32}),
+                     ctx.h, n, Int32(stride), m, x))
+    return nothing
 end
 
 # Learned warp of the unit cube (qiw_warp_config); no counterpart in the reference
@@ -241,8 +273,8 @@ Drop-in for QInchworm.inchworm! on one MI355X.  Writes expansion.P and returns
 :487-491): Dict(order => kd.zero(expansion.P)) filled by set_ppgf! with the order's
 contribution of the initial step (orders_bare) and of every regular step (orders), in
 step order.  With rand_params.N_seqs > 1 every group is integrated with up to N_seqs
-scrambled sequences (one seed drawn from rand_params.rng per call) and P_orders_std holds
-the standard deviations.
+scrambled sequences (with target_std = 0 the reference's own, drawn from rand_params.rng
+in its order) and P_orders_std holds the standard deviations.
 """
 function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_bare, N_samples::Int64;
                    n_pts_after_max::Int64 = typemax(Int64), rand_params::RandomizationParams = RandomizationParams(),
@@ -263,6 +295,7 @@ function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_ba
         run = Ref(QiwRun(length(orders_v), pointer(orders_v), length(bare_v), pointer(bare_v), N_samples,
                          n_pts_after_max == typemax(Int64) ? 0 : Int32(n_pts_after_max), precision,
                          Base.unsafe_convert(Ptr{QiwRand}, rq)))
+        stage_scrambles!(ctx, run, QIW_CALL_INCHWORM, rand_params)
         check(ctx, ccall((:qiw_inchworm, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                          ctx.h, run, P, Po, Ps))
@@ -307,6 +340,7 @@ function diff_inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_sa
     GC.@preserve keep orders_v rq begin
         run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, precision,
                          Base.unsafe_convert(Ptr{QiwRand}, rq)))
+        stage_scrambles!(ctx, run, QIW_CALL_DIFF, rand_params)
         check(ctx, ccall((:qiw_diff_inchworm, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                          ctx.h, run, P, So, Ss))
@@ -354,6 +388,7 @@ function correlator_2p_impl(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, 
     GC.@preserve keep orders_v rq begin
         run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, 0,
                          Base.unsafe_convert(Ptr{QiwRand}, rq)))
+        stage_scrambles!(ctx, run, QIW_CALL_CORRELATOR, rand_params)
         check(ctx, ccall((:qiw_correlator_2p_std, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}), ctx.h, run, out, out_std))
     end

@@ -18,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .randomization import RandomizationParams, RequestStdDev
+from .randomization import RandomizationParams, RequestStdDev, draw_sequences
 from .sobol import ScrambledSobolSeq
 
 _ctx = None
@@ -75,6 +75,46 @@ def _rand(rand_params):
     return rand_params.struct_args()
 
 
+_CALLS = {"inchworm": 0, "step": 1, "step_bare": 2, "correlator": 3, "diff": 4}
+
+
+def scramble_dims(run, call, ctx=None):
+    """qiw_scramble_dims: the dimensions 2k of the sequences a call (kind "inchworm",
+    "step", "step_bare", "correlator" or "diff") with the run struct consumes, in the
+    reference's construction order."""
+    ctx = ctx or context()
+    n = ctx.lib.qiw_scramble_dims(ctx.h, run.ref(), _CALLS[call], 0, None)
+    if n < 0:
+        ctx.check(n)
+    out = np.zeros(max(n, 1), np.int32)
+    ctx.check(min(0, ctx.lib.qiw_scramble_dims(ctx.h, run.ref(), _CALLS[call], n, out.ctypes.data_as(abi._i32p))))
+    return [int(v) for v in out[:n]]
+
+
+def _stage_given(ctx, rand_params, run, call):
+    """Stage the caller-drawn sequences of one call (qiw_set_scrambles) when rand_params
+    asks for them: ScrambledSobolSeq(D, scramble_rng=rng) for each D qiw_scramble_dims
+    lists."""
+    if rand_params is None or not rand_params.caller_drawn:
+        return
+    dims = scramble_dims(run, call, ctx)
+    stride = max(dims + [2])
+    m, x = draw_sequences(rand_params.rng, dims, stride)
+    set_scrambles(m, x, ctx)
+
+
+def set_scrambles(m, x=None, ctx=None):
+    """qiw_set_scrambles: m [n][stride][32], x [n][stride] (None: zeros)."""
+    ctx = ctx or context()
+    m = np.ascontiguousarray(m, dtype=np.uint32)
+    n, stride = (m.shape[0], m.shape[1]) if m.ndim == 3 else (0, 1)
+    xp = None
+    if x is not None:
+        x = np.ascontiguousarray(x, dtype=np.uint32)
+        xp = x.ctypes.data_as(abi._u32p)
+    ctx.check(ctx.lib.qiw_set_scrambles(ctx.h, n, stride, m.ctypes.data_as(abi._u32p) if n else None, xp))
+
+
 def inchworm(expansion, grid, orders, orders_bare, N_samples, n_pts_after_max=None,
              precision=abi.QIW_PREC_FP64, seq=None, ctx=None, rand_params=None):
     """inchworm! (src/inchworm.jl:332-498): writes the bold propagator into expansion.P and
@@ -85,6 +125,8 @@ def inchworm(expansion, grid, orders, orders_bare, N_samples, n_pts_after_max=No
     orders, orders_bare = list(orders), list(orders_bare)
     _stage(ctx, expansion, max(orders + orders_bare + [0]), seq)
     run = abi.RunStruct(orders, orders_bare, N_samples, n_pts_after_max or 0, precision, rand=_rand(rand_params))
+    if rand_params is not None and rand_params.caller_drawn:
+        _stage_given(ctx, rand_params, run, "inchworm")
     oall = sorted(set(orders) | set(orders_bare))
     nt, bt = grid.n_tau, expansion.bt_size
     Po = np.zeros((len(oall), nt, bt), np.complex128)
@@ -105,6 +147,8 @@ def inchworm_step(expansion, n_w, orders, N_samples, n_pts_after_max=None, seq=N
     _stage(ctx, expansion, max(orders + [0]), seq)
     _set_P(ctx, expansion)
     run = abi.RunStruct(orders, [], N_samples, n_pts_after_max or 0, precision, rand=_rand(rand_params))
+    if rand_params is not None and rand_params.caller_drawn:
+        _stage_given(ctx, rand_params, run, "step")
     val = np.zeros(expansion.bt_size, np.complex128)
     per = np.zeros((len(orders), expansion.bt_size), np.complex128)
     ctx.check(ctx.lib.qiw_inchworm_step(ctx.h, run.ref(), int(n_w), val.ctypes.data_as(abi._f64p),
@@ -118,6 +162,8 @@ def inchworm_step_bare(expansion, orders_bare, N_samples, seq=None, ctx=None, ra
     orders_bare = list(orders_bare)
     _stage(ctx, expansion, max(orders_bare + [0]), seq)
     run = abi.RunStruct([], orders_bare, N_samples, rand=_rand(rand_params))
+    if rand_params is not None and rand_params.caller_drawn:
+        _stage_given(ctx, rand_params, run, "step_bare")
     val = np.zeros(expansion.bt_size, np.complex128)
     per = np.zeros((len(orders_bare), expansion.bt_size), np.complex128)
     ctx.check(ctx.lib.qiw_inchworm_step_bare(ctx.h, run.ref(), val.ctypes.data_as(abi._f64p),
@@ -134,6 +180,8 @@ def diff_inchworm(expansion, grid, orders, N_samples, precision=abi.QIW_PREC_FP6
     orders = list(orders)
     _stage(ctx, expansion, max(orders + [0]), seq)
     run = abi.RunStruct(orders, [], N_samples, 0, precision, rand=_rand(rand_params))
+    if rand_params is not None and rand_params.caller_drawn:
+        _stage_given(ctx, rand_params, run, "diff")
     nt, bt = grid.n_tau, expansion.bt_size
     So = np.zeros((len(orders), nt, bt), np.complex128)
     Ss = np.zeros_like(So)
@@ -156,6 +204,8 @@ def correlator_2p(expansion, grid, orders, N_samples, *flag, seq=None, ctx=None,
     _stage(ctx, expansion, max(orders + [0]), seq)
     _set_P(ctx, expansion)
     run = abi.RunStruct(orders, [], N_samples, 0, precision, rand=_rand(rand_params))
+    if rand_params is not None and rand_params.caller_drawn:
+        _stage_given(ctx, rand_params, run, "correlator")
     out = np.zeros((len(expansion.corr_operators), grid.n_tau), np.complex128)
     if not flag:
         ctx.check(ctx.lib.qiw_correlator_2p(ctx.h, run.ref(), out.ctypes.data_as(abi._f64p)))

@@ -43,6 +43,10 @@ def lib():
         L.orc_diff_inchworm.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p, C.c_int,
                                         f64p, f64p, f64p]
         L.orc_rq_scramble.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u32p, u32p, u32p]
+        L.orc_set_scrambles.argtypes = [C.c_int64, C.c_int, u32p, u32p]
+        L.orc_scrambles_used.restype = C.c_int64
+        L.orc_scrambles_log.argtypes = [C.c_int64, C.POINTER(C.c_int64)]
+        L.orc_scrambles_log.restype = C.c_int64
         L.orc_rq_stream.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int]
         L.orc_rq_stream.restype = C.c_uint64
         L.orc_eval_topologies.argtypes = [C.POINTER(abi.qiw_model), C.c_int, C.c_int, C.c_int, C.c_double,
@@ -113,9 +117,9 @@ def inchworm_step(exp, orders, N, jw, P=None, n_known=None, der0=None, sobol=Non
     return val, per
 
 
-def inchworm_step_bare(exp, orders_bare, N, sobol=None):
+def inchworm_step_bare(exp, orders_bare, N, sobol=None, rand=None):
     ms = exp.to_struct()
-    run = abi.RunStruct([], orders_bare, N)
+    run = abi.RunStruct([], orders_bare, N, rand=rand)
     m, x0 = sobol if sobol is not None else sobol_tables(2 * max(list(orders_bare) + [1]))
     val = np.zeros(exp.bt_size, np.complex128)
     per = np.zeros((len(orders_bare), exp.bt_size), np.complex128)
@@ -148,6 +152,26 @@ def rq_scramble(seed, stream, m):
     xo = np.zeros(D, np.uint32)
     lib().orc_rq_scramble(int(seed), int(stream), D, _p(m, C.c_uint32), _p(mo, C.c_uint32), _p(xo, C.c_uint32))
     return mo, xo
+
+
+def set_scrambles(m, x):
+    """Stage caller-drawn sequences (m [n][stride][32], x [n][stride]) for scramble = 2;
+    the oracle consumes them in the order its loops (the reference's) construct sequences."""
+    m = np.ascontiguousarray(m, dtype=np.uint32)
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    lib().orc_set_scrambles(int(m.shape[0]), int(m.shape[1]), _p(m, C.c_uint32), _p(x, C.c_uint32))
+
+
+def scrambles_used():
+    return int(lib().orc_scrambles_used())
+
+
+def scrambles_log():
+    """(domain, key, order, n_pts_after, s) of every sequence consumed since set_scrambles."""
+    n = int(lib().orc_scrambles_log(0, None))
+    out = np.zeros((max(n, 1), 5), np.int64)
+    lib().orc_scrambles_log(n, _p(out, C.c_int64))
+    return [tuple(int(v) for v in row) for row in out[:n]]
 
 
 def rq_stream(domain, key, order, n_after, seq):

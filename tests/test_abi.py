@@ -12,7 +12,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     hdr = open(os.path.join(REPO, "include", "qiw.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?(qiw_\w+)\s*\(", hdr, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|void|const char \*)\s*\*?(qiw_\w+)\s*\(", hdr, re.M)))
 
 
 def test_header_declarations_are_exported():
@@ -26,7 +26,7 @@ def test_header_declarations_are_exported():
 
 def test_abi_version_and_no_gpu_error():
     lib = abi.load_library()
-    assert lib.qiw_abi_version() == 4
+    assert lib.qiw_abi_version() == 5
     if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
         h = C.c_void_p()
         assert lib.qiw_create(C.byref(h), 0) == abi.QIW_E_NO_GPU
