@@ -1,14 +1,21 @@
 """Benchmark of the qMC diagram-evaluation path on one MI355X.
 
-Workload (one bench step): the single-orbital Anderson impurity of the C1 configuration --
-H = 0.1 (n_up + n_dn) + 1.0 n_up n_dn, 4 states in 1x1 blocks by (N_up, N_dn), semi-elliptic
-bath (D = 2, V = 0.5), beta = 10, n_tau = 200 -- solved by inchworm! at orders 0:4 / 0:4
-with N = 2^14 unscrambled Sobol points per group, followed by correlator_2p for G, d and
-chi at orders 0:3, everything in float64 on the GPU.
+Headline workload: the two-orbital Hubbard-Kanamori impurity (U = 2, J = 0.2, eps = -1
+relative to half filling; 16 states in blocks 1, 4, 6, 4, 1 by particle number; 4
+semi-elliptic baths, D = 2, V = 0.5, 8 pairs; beta = 10, n_tau = 200) solved by inchworm!
+at orders 0:4 with N = 2^16 unscrambled Sobol points per (order, n_pts_after) group, the
+chain products in float32 with float64 (compensated) sums and bookkeeping.  One bench step
+is one bold step (all 16 sampled groups and the exact order-0 group in one evaluator
+launch, then the on-device step bookkeeping); K steps are timed at tau_w = grid[100],
+grid[101], ..., and the time to solution of the whole inchworm! is extrapolated as the
+initial step plus 198 bold steps (every bold step does the same work).
 
 Prints one JSON line: diagram evaluations per second (one evaluation = one (topology, qMC
 sample) integrand evaluation summed over initial sectors and pair assignments), with the
-roofline of the evaluator kernel and the CPU float64 oracle timed on a bounded sample.
+roofline of the evaluator kernel, the float64 oracle timed on a bounded sample of the same
+step on the host cores, the float32 error against float64, and secondary lines for the
+single-orbital solve (C1), the three-orbital step (C3), the randomized-qMC study and the
+differential formulation.
 """
 import argparse
 import contextlib
@@ -22,20 +29,55 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "qinchworm.jl_amd"))
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
-FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_eval_flat_c1_hbm_traffic.md")
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, published (vector with packed FP32, and matrix)
+FP32_VALU_UNPACKED_TFLOPS = 78.6  # FP32 VALU without packed math (disabled in this build)
+C1_TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_eval_flat_c1_hbm_traffic.md")
+C2_TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_eval_vec_c2_hbm_traffic.md")
 
 
-def measured_traffic():
-    """HBM bytes per evaluator launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """One progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(path):
+    """HBM bytes per evaluator launch from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (copy-calibrated), or None."""
     try:
-        for line in open(TRAFFIC_PROFILE):
+        for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def host_cpu():
+    """Host CPU model, logical CPUs and the OpenMP thread setting of this process."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads():
+    """All the host cores this process is meant to use: OMP_NUM_THREADS when set (the GPU
+    host's CPU share), else every logical CPU."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    try:
+        n = int(env) if env else 0
+    except ValueError:
+        n = 0
+    return max(1, min(n, os.cpu_count() or 1) if n > 0 else (os.cpu_count() or 1))
 
 
 def c2_expansion():
@@ -66,7 +108,7 @@ def two_orbital(log2n=14, cpu_n=16, with_cpu=True):
     out["fp32_rel_err_vs_fp64"] = float(np.linalg.norm(vals["fp32"] - vals["fp64"]) / np.linalg.norm(vals["fp64"]))
     if with_cpu:
         import oracle_lib as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         O.set_threads(threads)
         O.reset_counters()
         kc = c2_expansion()
@@ -117,7 +159,7 @@ def cpu_baseline(N, steps=2):
     """The float64 oracle on a bounded sample of the same workload: the bare step and the
     first `steps` bold steps of the C1 inchworm! (same orders and N), on the host cores."""
     import oracle_lib as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     O.set_threads(threads)
     O.reset_counters()
     exp = c1_expansion()
@@ -155,7 +197,7 @@ def three_orbital(log2n=12, cpu_n=16, with_cpu=True):
     out["gpu_leaves_per_s"] = st["leaves"] / dt
     if with_cpu:
         import oracle_lib as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         O.set_threads(threads)
         O.reset_counters()
         kc = c3_expansion()
@@ -252,7 +294,7 @@ def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678, with_cpu=T
     cpu = {}
     if with_cpu:  # the oracle on the bare step and the first bold steps of the same inchworm!
         import oracle_lib as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         O.set_threads(threads)
         O.reset_counters()
         t0 = time.perf_counter()
@@ -293,7 +335,7 @@ def diff_study(log2n=14, with_cpu=True, cpu_steps=4):
            "gpu_s": dt, "gpu_evals_per_s": st["diagram_evals"] / dt, "evaluator_ms": st["kernel_ms"]}
     if with_cpu:
         import oracle_lib as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         O.set_threads(threads)
         O.reset_counters()
         t0 = time.perf_counter()
@@ -317,15 +359,140 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
 
 
+def single_orbital(log2n, steps, warmup, sync, with_cpu):
+    """The C1 workload (one full solve per step): inchworm! on the single-orbital Anderson
+    model (n_tau = 200, orders 0:4/0:4, N = 2^log2n, float64) followed by correlator_2p for
+    G, d and chi at orders 0:3.  Two timed passes over the same steps: the library default
+    (the bold-step loop replayed from a hipGraph) gives the value; the eager pass, with HIP
+    events around every evaluator launch on its stream, gives the launch average of the
+    roofline."""
+    import qiw
+    N = 2 ** log2n
+    exp = c1_expansion()
+
+    def timed(graphs):
+        qiw.set_graphs(graphs)
+        for _ in range(max(warmup, 1 if graphs else 0)):  # a graph is captured on its first run
+            one_step(exp, N)
+        sync()
+        acc = {"evals": 0.0, "flops": 0.0, "kernel_ms": 0.0, "launches": 0}
+        with no_gc():
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s1, s2 = one_step(exp, N)
+                acc["evals"] += s1["diagram_evals"] + s2["diagram_evals"]
+                acc["flops"] += s1["flops"] + s2["flops"]
+                acc["kernel_ms"] += s1["kernel_ms"] + s2["kernel_ms"]
+                acc["launches"] += s1["n_kernel_launches"] + s2["n_kernel_launches"]
+            sync()
+            acc["dt"] = time.perf_counter() - t0
+        return acc
+
+    g = timed(True)
+    e = timed(False)
+    qiw.set_graphs(True)
+    avg_launch_ms = e["kernel_ms"] / max(e["launches"], 1)
+    achieved = e["flops"] / e["launches"] / (avg_launch_ms * 1e-3) / 1e12 if e["launches"] else 0.0
+    out = {"workload": "C1 single-orbital Anderson, beta=10, n_tau=200, inchworm! orders 0:4/0:4 + correlator_2p "
+                       f"(G, d, chi) orders 0:3, N=2^{log2n}, float64; one step = the whole solve",
+           "value": g["evals"] / g["dt"], "unit": "diagram evals/s", "ms_per_step": 1e3 * g["dt"] / steps,
+           "time_to_solution_s": g["dt"] / steps,
+           "eager": {"ms_per_step": 1e3 * e["dt"] / steps, "value": e["evals"] / e["dt"]},
+           "roofline": {"bound": "salu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(C1_TRAFFIC_PROFILE),
+                        "kernel": "k_eval_flat_t<4,4,2>", "avg_launch_ms": avg_launch_ms,
+                        "note": "float64 scalar walk: one multiply per trie edge, bound by scalar-unit issue"}}
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline(N)
+    return out
+
+
+def two_orbital_headline(log2n, steps, warmup, sync, with_cpu, cpu_log2n=5, err_log2n=12):
+    """The headline: K timed bold steps of the two-orbital inchworm! (float32 chain
+    products) at tau_w = grid[100 + i], the initial step timed once, the float32 error
+    against float64 on one step at N = 2^err_log2n, and the float64 oracle on a bounded
+    sample of the same step (N = 2^cpu_log2n points per group) on all host threads."""
+    import numpy as np
+    import qiw
+    from qiw import abi
+    N = 2 ** log2n
+    fp32 = abi.QIW_PREC_FP32
+    k = c2_expansion()
+    orders = range(0, 5)
+    for i in range(max(warmup, 1)):  # plans and buffers at this N
+        qiw.inchworm_step(k, 100, orders, N, precision=fp32)
+        progress(f"two-orbital warmup step {i + 1} (N=2^{log2n})")
+    sync()
+    acc = {"evals": 0.0, "flops": 0.0, "kernel_ms": 0.0, "launches": 0, "leaves": 0.0}
+    with no_gc():
+        t0 = time.perf_counter()
+        for i in range(steps):
+            qiw.inchworm_step(k, 100 + i, orders, N, precision=fp32)
+            progress(f"two-orbital timed step {i + 1} of {steps}")
+            st = qiw.stats()
+            acc["evals"] += st["diagram_evals"]
+            acc["flops"] += st["flops"]
+            acc["kernel_ms"] += st["kernel_ms"]
+            acc["launches"] += st["n_kernel_launches"]
+            acc["leaves"] += st["leaves"]
+        sync()
+        dt = time.perf_counter() - t0
+    qiw.inchworm_step_bare(k, orders, N)  # plans of the initial step
+    with no_gc():
+        t0 = time.perf_counter()
+        qiw.inchworm_step_bare(k, orders, N)
+        sync()
+        t_bare = time.perf_counter() - t0
+    progress("two-orbital initial step")
+    t_step = dt / steps
+    avg_launch_ms = acc["kernel_ms"] / max(acc["launches"], 1)
+    achieved = acc["flops"] / acc["launches"] / (avg_launch_ms * 1e-3) / 1e12 if acc["launches"] else 0.0
+    # float32 against float64 on one bold step
+    ne = 2 ** err_log2n
+    v32, _ = qiw.inchworm_step(k, 100, orders, ne, precision=fp32)
+    v64, _ = qiw.inchworm_step(k, 100, orders, ne, precision=abi.QIW_PREC_FP64)
+    progress("two-orbital float32 against float64")
+    out = {"value": acc["evals"] / dt, "ms_per_step": 1e3 * t_step, "t_bare_s": t_bare,
+           "time_to_solution_s": t_bare + 198 * t_step,
+           "leaves_per_s": acc["leaves"] / dt,
+           "fp32_vs_fp64": {"N": ne, "rel_err_step": float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64)),
+                            "max_abs_err_step": float(np.abs(v32 - v64).max())},
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP32_PEAK_TFLOPS,
+                        "frac_of_unpacked_fp32_valu": achieved / FP32_VALU_UNPACKED_TFLOPS,
+                        "traffic": measured_traffic(C2_TRAFFIC_PROFILE), "kernel": "k_eval_vec_t<float,6,4,2>",
+                        "avg_launch_ms": avg_launch_ms, "flops_per_launch": acc["flops"] / max(acc["launches"], 1),
+                        "note": "float32 chain products on the vector ALUs (blocks <= 6 do not fill MFMA tiles); "
+                                "packed FP32 is disabled in this build, so 78.6 TFLOP/s is the reachable ceiling"}}
+    if with_cpu:
+        import oracle_lib as O
+        threads = cpu_threads()
+        O.set_threads(threads)
+        O.reset_counters()
+        kc = c2_expansion()
+        ncpu = 2 ** cpu_log2n
+        t0 = time.perf_counter()
+        O.inchworm_step(kc, orders, ncpu, 100)
+        dc = time.perf_counter() - t0
+        progress("two-orbital CPU baseline")
+        _, evals, _ = O.counters()
+        out["cpu_baseline"] = {"value": evals / dc, "unit": "diagram evals/s", "cores": threads, "kind": "port",
+                               "sample": f"the same bold step (tau_w = grid[100], orders 0:4) at N=2^{cpu_log2n} "
+                                         f"points per group: {evals} diagram evals in {dc:.1f} s, float64 oracle, "
+                                         f"OpenMP over samples", **host_cpu()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--log2n", type=int, default=14)
+    ap.add_argument("--log2n", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c2", action="store_true", help="skip the two- and three-orbital step measurements")
-    ap.add_argument("--no-rqmc", action="store_true", help="skip the randomized-qMC study line")
+    ap.add_argument("--headline-only", action="store_true", help="only the two-orbital headline (profiling runs)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the three-orbital step line")
+    ap.add_argument("--no-rqmc", action="store_true", help="skip the randomized-qMC and differential lines")
     ap.add_argument("--block-models-only", action="store_true",
                     help="only the two- and three-orbital steps (vector and matrix-core walks), one JSON line")
     ap.add_argument("--warp-study", action="store_true",
@@ -353,74 +520,48 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None  # noqa: E731
+    h = two_orbital_headline(args.log2n, args.steps, args.warmup, sync, not args.no_cpu)
     N = 2 ** args.log2n
-    exp = c1_expansion()
-    # Two timed passes over the same K steps.  The headline is the library default: the
-    # bold-step loop replayed from a hipGraph.  A replayed graph is timed only as a whole, so
-    # the kernel's average launch duration (roofline) comes from a second pass that launches
-    # every kernel from the host with its own pair of HIP events on the launch stream.
-    import qiw
-
-    def timed(graphs):
-        qiw.set_graphs(graphs)
-        for _ in range(max(args.warmup, 1 if graphs else 0)):  # a graph is captured on its first run
-            one_step(exp, N)
-        sync()
-        acc = {"evals": 0.0, "flops": 0.0, "kernel_ms": 0.0, "launches": 0}
-        with no_gc():
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                s1, s2 = one_step(exp, N)
-                acc["evals"] += s1["diagram_evals"] + s2["diagram_evals"]
-                acc["flops"] += s1["flops"] + s2["flops"]
-                acc["kernel_ms"] += s1["kernel_ms"] + s2["kernel_ms"]
-                acc["launches"] += s1["n_kernel_launches"] + s2["n_kernel_launches"]
-            sync()
-            acc["dt"] = time.perf_counter() - t0
-        return acc
-
-    g = timed(True)
-    e = timed(False)
-    qiw.set_graphs(True)
-    evals, dt = g["evals"], g["dt"]
-    ms_per_step = 1e3 * dt / args.steps
-    flops, launches = e["flops"], e["launches"]
-    avg_launch_ms = e["kernel_ms"] / max(launches, 1)
-    eager = {"ms_per_step": 1e3 * e["dt"] / args.steps, "value": e["evals"] / e["dt"],
-             "note": "the same steps with every kernel launched from the host (source of avg_launch_ms)"}
-    achieved = flops / launches / (avg_launch_ms * 1e-3) / 1e12 if launches else 0.0
     line = {
         "metric": "diagram evals/s; inchworm! time-to-error vs FP64 oracle, warped vs plain qMC",
-        "value": evals / dt,
+        "value": h["value"],
         "unit": "diagram evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic (deterministic model, unscrambled Sobol points)",
-        "config": {"workload": "C1 single-orbital Anderson, beta=10, n_tau=200, inchworm! orders 0:4/0:4 "
-                               f"+ correlator_2p (G, d, chi) orders 0:3, N=2^{args.log2n}, float64",
-                   "n_tau": 200, "orders": "0:4", "N_samples": N},
-        "time_to_solution_s": ms_per_step / 1e3,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "k_eval_flat", "avg_launch_ms": avg_launch_ms,
-                     "avg_launch_from": "eager pass: HIP events around each evaluator launch",
-                     "launches_per_step": launches / args.steps},
-        "eager": eager,
+        "config": {"workload": "two-orbital Hubbard-Kanamori (U=2, J=0.2, eps=-1; blocks 1,4,6,4,1; 8 pairs; "
+                               "beta=10, n_tau=200), inchworm! orders 0:4, N=2^%d per group, float32 chain "
+                               "products with float64 compensated sums; one step = one bold step "
+                               "(tau_w = grid[100+i]); time_to_solution_s extrapolated as the initial step "
+                               "+ 198 bold steps" % args.log2n,
+                   "n_tau": 200, "orders": "0:4", "N_samples": N, "precision": "fp32 chains, fp64 accumulation"},
+        "time_to_solution_s": h["time_to_solution_s"],
+        "time_to_solution_note": "extrapolated: t_bare + 198 x (mean bold step)",
+        "t_bare_s": h["t_bare_s"],
+        "leaves_per_s": h["leaves_per_s"],
+        "fp32_vs_fp64": h["fp32_vs_fp64"],
+        "roofline": h["roofline"],
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(N)
-    if not args.no_c2:
-        line["two_orbital_step"] = two_orbital(with_cpu=not args.no_cpu)
-        line["three_orbital_step"] = three_orbital(with_cpu=not args.no_cpu)
-    if not args.no_rqmc:
-        line["rqmc_study"] = rqmc_study(with_cpu=not args.no_cpu)
-        line["diff_inchworm"] = diff_study(with_cpu=not args.no_cpu)
+    if "cpu_baseline" in h:
+        line["cpu_baseline"] = h["cpu_baseline"]
+        line["speedup_vs_cpu_baseline"] = h["value"] / h["cpu_baseline"]["value"]
+    if not args.headline_only:
+        line["single_orbital"] = single_orbital(14, 3, 1, sync, not args.no_cpu)
+        progress("single-orbital solve")
+        if not args.no_c3:
+            line["three_orbital_step"] = three_orbital(with_cpu=not args.no_cpu)
+            progress("three-orbital step")
+        if not args.no_rqmc:
+            line["rqmc_study"] = rqmc_study(with_cpu=not args.no_cpu)
+            progress("randomized qMC study")
+            line["diff_inchworm"] = diff_study(with_cpu=not args.no_cpu)
+            progress("differential inchworm")
     print(json.dumps(line), flush=True)
 
 

@@ -89,6 +89,12 @@ static inline int qiw_scalar_km(int km, int sm, int tm) { return (km <= 4 && sm 
 // self-energy's.
 enum { QIW_G_BOLD = 0, QIW_G_BARE = 1, QIW_G_CORR = 2, QIW_G_SIGMA = 3 };
 
+// A float32 cell of a grid-container table: value, forward and backward difference
+// (12 bytes, loaded with one instruction).
+struct QiwCell32 {
+    float v, f, b;
+};
+
 // One (order, n_pts_after) group of diagrams, compiled.
 struct QiwGroup {
     int32_t kind;          // QIW_G_*
@@ -244,7 +250,7 @@ struct VecArgs {
     EvalArgs e;
     const double2 *pt;         // [n_tau-1][bt] (value, forward difference) of the i*P table (bold groups)
     const double *ptm;         // [n_tau-1][bt] backward differences (row 0: the forward one)
-    const float4 *pt32;        // (value, forward, backward, 0) in float32
+    const QiwCell32 *pt32;     // (value, forward, backward) in float32
     const double *ediag;       // [bt] E + lambda on the diagonal positions (bare groups)
     const double *ops;         // operator blocks, column-major, padded by 64 entries
     const float *ops32;        // the same in float32
@@ -300,7 +306,7 @@ hipError_t qiw_launch_eval_block(hipStream_t st, const BlockArgs &b, int n_tasks
 hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes);
 hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm, int fp32);
 hipError_t qiw_launch_ptab_cells(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, double *outm,
-                                 float4 *out32);
+                                 QiwCell32 *out32);
 hipError_t qiw_launch_finalize_step(hipStream_t st, const FinalizeArgs &a);
 hipError_t qiw_launch_finalize_corr(hipStream_t st, const CorrFinalizeArgs &a);
 hipError_t qiw_launch_finalize_step_rqmc(hipStream_t st, const FinalizeArgs &a);

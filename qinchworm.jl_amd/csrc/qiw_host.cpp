@@ -1222,14 +1222,14 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
         double2 *pt = (double2 *)dev(c, "vec.pt", ((size_t)(m.ntau - 1) * m.bt + pad) * 16, &rc);
         double *ptm = (double *)dev(c, "vec.ptm", ((size_t)(m.ntau - 1) * m.bt + pad) * 8, &rc);
         float *ops32 = (float *)dev(c, "vec.ops32", (m.op_data.size() + pad) * 4, &rc);
-        float4 *pt32 = (float4 *)dev(c, "vec.pt32", ((size_t)(m.ntau - 1) * m.bt + pad) * 16, &rc);
+        QiwCell32 *pt32 = (QiwCell32 *)dev(c, "vec.pt32", ((size_t)(m.ntau - 1) * m.bt + pad) * sizeof(QiwCell32), &rc);
         if (rc) return rc;
         if (!c->vec_staged) {  // once per model
             std::vector<double> o(m.op_data), ed((size_t)m.bt + 2 * pad, 0.0);
             o.resize(m.op_data.size() + pad, 0.0);
             std::vector<float> o32(o.begin(), o.end());
             HIPCHK(c, hipMemcpy(ops32, o32.data(), o32.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemset(pt32, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 16));
+            HIPCHK(c, hipMemset(pt32, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * sizeof(QiwCell32)));
             HIPCHK(c, hipMemset(ptm, 0, ((size_t)(m.ntau - 1) * m.bt + pad) * 8));
             for (int s = 0; s < m.ns; ++s)
                 for (int i = 0; i < m.dims[s]; ++i) ed[m.bt_off[s] + i + i * m.dims[s]] = m.E[s][i] + m.lambda0;

@@ -777,16 +777,17 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_block(BlockArgs b) {
 // For every index j < n_tau - 1 and BT element e: T[j][e], the forward difference
 // T[j+1][e] - T[j][e] and the backward difference T[j][e] - T[j-1][e] (for j = 0 the
 // forward one, the cell value's convention g(-1) = 2 g(0) - g(1)); float64 as (value,
-// forward) pairs plus the backward differences, float32 as (value, forward, backward, 0).
+// forward) pairs plus the backward differences, float32 as packed 12-byte (value, forward,
+// backward) cells.
 __global__ void k_ptab_cells(const double *__restrict__ T, int n_tau, int bt, double2 *__restrict__ out,
-                             double *__restrict__ outm, float4 *__restrict__ out32) {
+                             double *__restrict__ outm, QiwCell32 *__restrict__ out32) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= (int64_t)(n_tau - 1) * bt) return;
     const double v0 = T[r], v1 = T[r + bt];
     const double dm = r >= bt ? v0 - T[r - bt] : v1 - v0;
     out[r] = make_double2(v0, v1 - v0);
     outm[r] = dm;
-    out32[r] = make_float4((float)v0, (float)(v1 - v0), (float)dm, 0.f);
+    out32[r] = QiwCell32{(float)v0, (float)(v1 - v0), (float)dm};
 }
 
 // Cell tables of the two value types: entry i at weights (wp, wm) = row + wp forward -
@@ -804,14 +805,26 @@ struct CellTab<double> {
 };
 template <>
 struct CellTab<float> {
-    const float4 *__restrict__ p;
+    const QiwCell32 *__restrict__ p;  // one 12-byte load per entry
     __device__ __forceinline__ float at(int64_t i, float wp, float wm) const {
-        const float4 e = p[i];
-        return fma(wp, e.y, fma(-wm, e.z, e.x));
+        const QiwCell32 e = p[i];
+        return fma(wp, e.f, fma(-wm, e.b, e.v));
     }
 };
 
-template <typename V, int MB, int KM, int TM>
+// Read-only tables at wavefront-uniform addresses (operator blocks, atomic energies) are
+// read through the constant address space, so the loads are scalar (one SMEM load for the
+// wavefront, through the scalar cache) instead of vector loads that all 64 lanes repeat.
+template <typename T>
+using ConstPtr = const T __attribute__((address_space(4))) *;
+template <typename T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T *p) {
+    return (ConstPtr<T>)p;
+}
+
+// State of one lane (one qMC sample) of the vector walk, which carries NC columns of the
+// initial sector's block at a time: every propagator entry it loads serves NC products.
+template <typename V, int MB, int KM, int TM, int NC>
 struct VecState {
     static constexpr int NM = 2 * KM + 3;
     int row[NM - 1];      // BT offset of the grid index d of segment L's cell (bold)
@@ -819,45 +832,59 @@ struct VecState {
     V wm[NM - 1];
     double dtb[NM - 1];   // time difference (bare)
     V W[KM * TM];         // i*Delta_tab of the arcs, per arc and table
-    double acc[MB];       // column of the current initial sector
+    double acc[NC][MB];   // columns col .. col + NC - 1 of the current initial sector
+    // LDS of the wavefront: the configuration times t[p] at ts[p * 64 + lane], and the
+    // propagated vectors of levels 2 .. 1 + YL at ys[((L - 2) * NC + c) * MB + i][lane],
+    // which are reused by few children, so they need no registers across the recursion
+    double *ts;
+    V (*ys)[QIW_WAVE];
+    int lane;
     ProgStream ps;
     CellTab<V> pt;
     CellTab<double> pt64;
-    const double *__restrict__ ediag;
-    const V *__restrict__ ops;
+    ConstPtr<double> ediag;
+    ConstPtr<V> ops;
     int bt;
 };
 
-// y = iP_L[s] x for the m x m block of sector s at BT offset off.
-template <int L, typename V, int MB, int KM, int TM, bool BARE>
-__device__ __forceinline__ void prop_apply(const VecState<V, MB, KM, TM> &st, const V (&x)[MB], int m, int off,
-                                           V (&y)[MB]) {
+// y = iP_L[s] x for the m x m block of sector s at BT offset off, NC columns.
+template <int L, typename V, int MB, int KM, int TM, int NC, bool BARE>
+__device__ __forceinline__ void prop_apply(const VecState<V, MB, KM, TM, NC> &st, const V (&x)[NC][MB], int m,
+                                           int off, V (&y)[NC][MB]) {
     if constexpr (BARE) {  // exact atomic propagator: diagonal exp(-dt (E + lambda))
         const double dt = st.dtb[L - 2];
 #pragma unroll
-        for (int i = 0; i < MB; ++i) y[i] = (i < m) ? (V)exp(-dt * st.ediag[off + i * (m + 1)]) * x[i] : (V)0;
+        for (int i = 0; i < MB; ++i) {
+            const V e = (i < m) ? (V)exp(-dt * st.ediag[off + i * (m + 1)]) : (V)0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) y[c][i] = e * x[c][i];
+        }
     } else {
         const int64_t base = st.row[L - 2] + off;
         const V wp = st.wp[L - 2], wm = st.wm[L - 2];
 #pragma unroll
-        for (int i = 0; i < MB; ++i) y[i] = (V)0;
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int i = 0; i < MB; ++i) y[c][i] = (V)0;
 #pragma unroll
         for (int q = 0; q < MB; ++q) {
             if (q < m) {
-                const V xq = x[q];
 #pragma unroll
-                for (int i = 0; i < MB; ++i) y[i] = fma(st.pt.at(base + q * m + i, wp, wm), xq, y[i]);
+                for (int i = 0; i < MB; ++i) {
+                    const V pv = st.pt.at(base + q * m + i, wp, wm);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) y[c][i] = fma(pv, x[c][q], y[c][i]);
+                }
             }
         }
     }
 }
 
 // y = iP_n[s0] x for the last segment (runtime n), at the end of a column (float64)
-template <typename V, int MB, int KM, int TM, bool BARE>
-__device__ __forceinline__ void last_prop(const VecState<V, MB, KM, TM> &st, const double (&t)[2 * KM + 4], int n,
-                                          const double (&x)[MB], int m, int off, double inv_h, int n_tau,
-                                          double (&y)[MB]) {
-    const double tn = pick(t, n), tp = pick(t, n - 1);
+template <typename V, int MB, int KM, int TM, int NC, bool BARE>
+__device__ __forceinline__ void last_prop(const VecState<V, MB, KM, TM, NC> &st, int n, const double (&x)[MB], int m,
+                                          int off, double inv_h, int n_tau, double (&y)[MB]) {
+    const double tn = st.ts[n * QIW_WAVE + st.lane], tp = st.ts[(n - 1) * QIW_WAVE + st.lane];
     if constexpr (BARE) {
         double dt = tn - tp;
         dt = dt > 0.0 ? dt : 0.0;
@@ -879,35 +906,61 @@ __device__ __forceinline__ void last_prop(const VecState<V, MB, KM, TM> &st, con
     }
 }
 
-template <int L, typename V, int MB, int KM, int TM, bool BARE>
-__device__ __forceinline__ void vwalk(VecState<V, MB, KM, TM> &st, int n, const V (&xp)[MB], V wp, int mp,
+#ifndef QIW_VEC_YL
+#define QIW_VEC_YL 4
+#endif
+template <int L, typename V, int MB, int KM, int TM, int NC, bool BARE>
+__device__ __forceinline__ void vwalk(VecState<V, MB, KM, TM, NC> &st, int n, const V (&xp)[NC][MB], V wp, int mp,
                                       int offp, int nchild) {
     if constexpr (L <= 2 * KM + 3) {
-        V y[MB];
-        if (L == n) {  // leaves: iP_n[s0] is applied to the column sum
+        constexpr bool in_lds = L - 2 < QIW_VEC_YL;
+        V y[NC][MB];
+        if (L == n) {  // leaves: iP_n[s0] is applied to the column sums
 #pragma unroll
-            for (int i = 0; i < MB; ++i) y[i] = xp[i];
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int i = 0; i < MB; ++i) y[c][i] = xp[c][i];
         } else {
-            prop_apply<L, V, MB, KM, TM, BARE>(st, xp, mp, offp, y);
+            prop_apply<L, V, MB, KM, TM, NC, BARE>(st, xp, mp, offp, y);
         }
-        for (int c = 0; c < nchild; ++c) {
+        if constexpr (in_lds) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int i = 0; i < MB; ++i) st.ys[((L - 2) * NC + c) * MB + i][st.lane] = y[c][i];
+        }
+        for (int ch = 0; ch < nchild; ++ch) {
+            if constexpr (in_lds) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int i = 0; i < MB; ++i) y[c][i] = st.ys[((L - 2) * NC + c) * MB + i][st.lane];
+            }
             const uint64_t w = st.ps.next();
             const V wv = (w & QIW_F_TAIL) ? wp * pickv(st.W, (int)((w >> 16) & 0xFF)) : wp;
             const int mo = (int)(w & 0x3F);
             const int blk = (int)((w >> 32) & 0xFFFF);
-            V x[MB];
+            V x[NC][MB];
             if (blk == 0xFFFF) {
 #pragma unroll
-                for (int i = 0; i < MB; ++i) x[i] = y[i];
-            } else {
-                const V *__restrict__ O = st.ops + blk;  // m_o x m_p, column-major
+                for (int c = 0; c < NC; ++c)
 #pragma unroll
-                for (int i = 0; i < MB; ++i) x[i] = (V)0;
+                    for (int i = 0; i < MB; ++i) x[c][i] = y[c][i];
+            } else {
+                const ConstPtr<V> O = st.ops + blk;  // m_o x m_p, column-major
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int i = 0; i < MB; ++i) x[c][i] = (V)0;
 #pragma unroll
                 for (int q = 0; q < MB; ++q) {
                     if (q < mp) {
 #pragma unroll
-                        for (int i = 0; i < MB; ++i) x[i] = fma(O[q * mo + i], y[q], x[i]);
+                        for (int i = 0; i < MB; ++i) {
+                            const V o = O[q * mo + i];
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) x[c][i] = fma(o, y[c][q], x[c][i]);
+                        }
                     }
                 }
             }
@@ -915,18 +968,20 @@ __device__ __forceinline__ void vwalk(VecState<V, MB, KM, TM> &st, int n, const 
                 const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -(double)wv : (double)wv)
                                                     : __longlong_as_double((long long)st.ps.next()) * (double)wv;
 #pragma unroll
-                for (int i = 0; i < MB; ++i) st.acc[i] = fma(cw, (double)x[i], st.acc[i]);
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int i = 0; i < MB; ++i) st.acc[c][i] = fma(cw, (double)x[c][i], st.acc[c][i]);
             } else if (L < n) {
-                vwalk<L + 1, V, MB, KM, TM, BARE>(st, n, x, wv, mo, (int)(w >> 52), (int)((w >> 24) & 0xFF));
+                vwalk<L + 1, V, MB, KM, TM, NC, BARE>(st, n, x, wv, mo, (int)(w >> 52), (int)((w >> 24) & 0xFF));
             }
         }
     }
 }
 
-template <typename V, int MB, int KM, int TM, bool BARE>
-__device__ __forceinline__ void vec_program(VecState<V, MB, KM, TM> &st, const EvalArgs &a, const QiwGroup &g,
-                                            const double (&t)[2 * KM + 4], double inv_h, bool active, double wt,
-                                            double &absum, double *__restrict__ prow) {
+template <typename V, int MB, int KM, int TM, int NC, bool BARE>
+__device__ __forceinline__ void vec_program(VecState<V, MB, KM, TM, NC> &st, const EvalArgs &a, const QiwGroup &g,
+                                            double inv_h, bool active, double wt, double &absum,
+                                            double *__restrict__ prow) {
     const int n = g.n_nodes, k = g.order;
     const int ntab = a.dtab.n_tables;
     const uint64_t *seg = nullptr;
@@ -935,32 +990,34 @@ __device__ __forceinline__ void vec_program(VecState<V, MB, KM, TM> &st, const E
         const uint64_t w = st.ps.next();
         const int op = (int)((w >> 48) & 0xF);
         if (op == QIW_OP_ROOT) {
-            // position 1: e_col, or column col of the operator B (correlators)
+            // position 1: e_col, or column col of the operator B (correlators), for the
+            // columns col .. col + NC - 1 that exist (zero beyond)
             const int m1 = (int)(w & 0x3F);
             const int blk = (int)((w >> 32) & 0xFFFF);
-            V x[MB];
-            if (blk == 0xFFFF) {
+            V x[NC][MB];
 #pragma unroll
-                for (int i = 0; i < MB; ++i) x[i] = (i == col) ? (V)1 : (V)0;
-            } else {
+            for (int c = 0; c < NC; ++c) {
+                const bool live = col + c < s_m0;
+                if (blk == 0xFFFF) {
 #pragma unroll
-                for (int i = 0; i < MB; ++i) x[i] = st.ops[blk + col * m1 + i];
+                    for (int i = 0; i < MB; ++i) x[c][i] = (live && i == col + c) ? (V)1 : (V)0;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < MB; ++i) x[c][i] = live ? st.ops[blk + (col + c) * m1 + i] : (V)0;
+                }
             }
-            vwalk<2, V, MB, KM, TM, BARE>(st, n, x, (V)1, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF));
+            vwalk<2, V, MB, KM, TM, NC, BARE>(st, n, x, (V)1, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF));
         } else if (op == QIW_OP_TOPO) {
 #pragma unroll
             for (int ai = 0; ai < KM; ++ai) {
                 const int head = (int)((w >> (8 * ai)) & 0xF), tail = (int)((w >> (8 * ai + 4)) & 0xF);
-                double th = 0.0, tt = 0.0;
+                if (ai < k) {
+                    const double th = st.ts[head * QIW_WAVE + st.lane], tt = st.ts[tail * QIW_WAVE + st.lane];
+                    const ArcPos ap = arc_pos(tt, th, inv_h, a.n_tau);
 #pragma unroll
-                for (int p = 1; p <= 2 * KM + 3; ++p) {
-                    th = (p == head) ? t[p] : th;
-                    tt = (p == tail) ? t[p] : tt;
+                    for (int tb = 0; tb < TM; ++tb)
+                        if (tb < ntab) st.W[ai * TM + tb] = (V)delta_value(a.dtab, tb, ap);
                 }
-                const ArcPos ap = arc_pos(tt, th, inv_h, a.n_tau);
-#pragma unroll
-                for (int tb = 0; tb < TM; ++tb)
-                    if (ai < k && tb < ntab) st.W[ai * TM + tb] = (V)delta_value(a.dtab, tb, ap);
             }
         } else if (op == QIW_OP_SEG) {  // initial sector s0: m0 columns, BT offset
             s_m0 = (int)(w & 0x3F);
@@ -968,30 +1025,42 @@ __device__ __forceinline__ void vec_program(VecState<V, MB, KM, TM> &st, const E
             col = 0;
             seg = st.ps.base + st.ps.idx;
 #pragma unroll
-            for (int i = 0; i < MB; ++i) st.acc[i] = 0.0;
-        } else if (op == QIW_OP_FLUSH) {  // column col of the s0 block is complete
-            double y[MB];
-            last_prop<V, MB, KM, TM, BARE>(st, t, n, st.acc, s_m0, s_off0, inv_h, a.n_tau, y);
+            for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int i = 0; i < MB; ++i) {
-                if (i < s_m0) {
-                    absum += fabs(y[i]);
-                    double v = active ? y[i] * wt : 0.0;
+                for (int i = 0; i < MB; ++i) st.acc[c][i] = 0.0;
+        } else if (op == QIW_OP_FLUSH) {  // columns col .. col + NC - 1 of the s0 block are complete
 #pragma unroll
-                    for (int o = QIW_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, QIW_WAVE);
-                    if (threadIdx.x == 0) prow[s_off0 + col * s_m0 + i] = v;
+            for (int c = 0; c < NC; ++c) {
+                if (col + c < s_m0) {
+                    double y[MB];
+                    last_prop<V, MB, KM, TM, NC, BARE>(st, n, st.acc[c], s_m0, s_off0, inv_h, a.n_tau, y);
+#pragma unroll
+                    for (int i = 0; i < MB; ++i) {
+                        if (i < s_m0) {
+                            absum += fabs(y[i]);
+                            double v = active ? y[i] * wt : 0.0;
+#pragma unroll
+                            for (int o = QIW_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, QIW_WAVE);
+                            if (threadIdx.x == 0) prow[s_off0 + (col + c) * s_m0 + i] = v;
+                        }
+                    }
                 }
-                st.acc[i] = 0.0;
+#pragma unroll
+                for (int i = 0; i < MB; ++i) st.acc[c][i] = 0.0;
             }
-            if (++col < s_m0) st.ps.init(seg, threadIdx.x);
+            col += NC;
+            if (col < s_m0) st.ps.init(seg, threadIdx.x);
         } else if (op == QIW_OP_END) {
             break;
         }
     }
 }
 
-template <typename V, int MB, int KM, int TM>
-__global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
+#ifndef QIW_VEC_MINW
+#define QIW_VEC_MINW 2
+#endif
+template <typename V, int MB, int KM, int TM, int NC>
+__global__ __launch_bounds__(QIW_WAVE, QIW_VEC_MINW) void k_eval_vec_t(VecArgs va) {
     constexpr int NM = 2 * KM + 3;
     const EvalArgs &a = va.e;
     const int lane = threadIdx.x;
@@ -1042,18 +1111,26 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
         }
     }
 
-    VecState<V, MB, KM, TM> st;
+    constexpr int YL = QIW_VEC_YL < NM - 1 ? QIW_VEC_YL : NM - 1;
+    __shared__ double s_t[(NM + 1) * QIW_WAVE];
+    __shared__ V s_y[(YL > 0 ? YL : 1) * NC * MB][QIW_WAVE];
+#pragma unroll
+    for (int p = 0; p <= NM; ++p) s_t[p * QIW_WAVE + lane] = t[p];
+    VecState<V, MB, KM, TM, NC> st;
+    st.ts = s_t;
+    st.ys = s_y;
+    st.lane = lane;
     if constexpr (sizeof(V) == 8) {
         st.pt.p = va.pt;
         st.pt.m = va.ptm;
-        st.ops = (const V *)va.ops;
+        st.ops = as_const((const V *)va.ops);
     } else {
         st.pt.p = va.pt32;
-        st.ops = (const V *)va.ops32;
+        st.ops = as_const((const V *)va.ops32);
     }
     st.pt64.p = va.pt;
     st.pt64.m = va.ptm;
-    st.ediag = va.ediag;
+    st.ediag = as_const(va.ediag);
     st.bt = va.bt;
 #pragma unroll
     for (int L = 2; L <= NM; ++L) {
@@ -1074,18 +1151,20 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_vec_t(VecArgs va) {
 #pragma unroll
     for (int q = 0; q < KM * TM; ++q) st.W[q] = (V)0;
 #pragma unroll
-    for (int i = 0; i < MB; ++i) st.acc[i] = 0.0;
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < MB; ++i) st.acc[c][i] = 0.0;
 
-    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (int64_t)(g.d_before + g.d_after) * g.n_samples +
-                                        (active ? sample : 0)]
+    const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)task.tau_index * a.ut_row_stride +
+                                        (int64_t)(g.d_before + g.d_after) * g.n_samples + (active ? sample : 0)]
                                  : 1.0;
     double absum = 0.0;
     double *prow = a.partial + (int64_t)task.out_index * va.bt;
     st.ps.init(a.prog + g.prog_off + task.chunk, lane);
     if (g.kind == QIW_G_BARE)
-        vec_program<V, MB, KM, TM, true>(st, a, g, t, inv_h, active, wt, absum, prow);
+        vec_program<V, MB, KM, TM, NC, true>(st, a, g, inv_h, active, wt, absum, prow);
     else
-        vec_program<V, MB, KM, TM, false>(st, a, g, t, inv_h, active, wt, absum, prow);
+        vec_program<V, MB, KM, TM, NC, false>(st, a, g, inv_h, active, wt, absum, prow);
     if (a.per_sample && active) a.per_sample[g.ps_off + sample] = absum;
 }
 
@@ -1663,9 +1742,11 @@ __global__ __launch_bounds__(256) void k_given_coords(const QiwGroup *__restrict
                                                       int row0, int64_t row_stride, double *__restrict__ ut) {
     const QiwGroup g = groups[blockIdx.y];
     const int64_t seq = idx[(int64_t)blockIdx.z * gridDim.y + blockIdx.y];
-    if (!g.sampled || seq < 0) return;
+    if (!g.sampled) return;
     const bool sigma = g.kind == QIW_G_SIGMA;
     const int d = sigma ? 2 * g.order - 2 : g.d_before + g.d_after;  // Sobol dimensions used
+    // the exact order-1 self-energy group uses no sequence (d = 0) but still has its row
+    if (seq < 0 && d > 0) return;
     __shared__ uint32_t ms[2 * QIW_KMAX * 32];
     __shared__ uint32_t xs[2 * QIW_KMAX];
     for (int i = threadIdx.x; i < d * 32; i += blockDim.x) ms[i] = gm[(seq * stride + i / 32) * 32 + i % 32];
@@ -1934,17 +2015,22 @@ hipError_t qiw_launch_eval_flat(hipStream_t st, const EvalArgs &a, int n_tasks, 
 
 // Register budget of the vector walk: MB = largest block, KM = largest order, TM = tables;
 // fp32 selects float32 chain products.
+#ifndef QIW_VEC_NC32
+#define QIW_VEC_NC32 2
+#endif
+// float32 chains carry QIW_VEC_NC32 columns per pass; float64 chains one (register budget).
 hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, int mb, int km, int tm, int fp32) {
+    constexpr int NC = QIW_VEC_NC32;
     if (mb <= 6 && km <= 4 && tm <= 2) {
         if (fp32)
-            hipLaunchKernelGGL((k_eval_vec_t<float, 6, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+            hipLaunchKernelGGL((k_eval_vec_t<float, 6, 4, 2, NC>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
         else
-            hipLaunchKernelGGL((k_eval_vec_t<double, 6, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+            hipLaunchKernelGGL((k_eval_vec_t<double, 6, 4, 2, 1>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
     } else if (mb <= 6 && km <= 4 && tm <= 4) {
         if (fp32)
-            hipLaunchKernelGGL((k_eval_vec_t<float, 6, 4, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+            hipLaunchKernelGGL((k_eval_vec_t<float, 6, 4, 4, NC>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
         else
-            hipLaunchKernelGGL((k_eval_vec_t<double, 6, 4, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
+            hipLaunchKernelGGL((k_eval_vec_t<double, 6, 4, 4, 1>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, v);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1952,7 +2038,7 @@ hipError_t qiw_launch_eval_vec(hipStream_t st, const VecArgs &v, int n_tasks, in
 }
 
 hipError_t qiw_launch_ptab_cells(hipStream_t st, const double *T, int n_tau, int bt, double2 *out, double *outm,
-                                 float4 *out32) {
+                                 QiwCell32 *out32) {
     int64_t n = (int64_t)(n_tau - 1) * bt;
     hipLaunchKernelGGL(k_ptab_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n_tau, bt, out, outm,
                        out32);
