@@ -407,7 +407,7 @@ def single_orbital(log2n, steps, warmup, sync, with_cpu):
     return out
 
 
-def two_orbital_headline(log2n, steps, warmup, sync, with_cpu, cpu_log2n=5, err_log2n=12):
+def two_orbital_headline(log2n, steps, warmup, sync, with_cpu, cpu_log2n=8, err_log2n=12):
     """The headline: K timed bold steps of the two-orbital inchworm! (float32 chain
     products) at tau_w = grid[100 + i], the initial step timed once, the float32 error
     against float64 on one step at N = 2^err_log2n, and the float64 oracle on a bounded
@@ -486,7 +486,7 @@ def two_orbital_headline(log2n, steps, warmup, sync, with_cpu, cpu_log2n=5, err_
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log2n", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
