@@ -460,10 +460,12 @@ def two_orbital_headline(log2n, steps, warmup, sync, with_cpu, cpu_log2n=8, err_
            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / FP32_PEAK_TFLOPS,
                         "frac_of_unpacked_fp32_valu": achieved / FP32_VALU_UNPACKED_TFLOPS,
-                        "traffic": measured_traffic(C2_TRAFFIC_PROFILE), "kernel": "k_eval_vec_t<float,6,4,2>",
+                        "traffic": measured_traffic(C2_TRAFFIC_PROFILE), "kernel": "k_eval_vec_t<float,6,4,2,2>",
                         "avg_launch_ms": avg_launch_ms, "flops_per_launch": acc["flops"] / max(acc["launches"], 1),
                         "note": "float32 chain products on the vector ALUs (blocks <= 6 do not fill MFMA tiles); "
-                                "packed FP32 is disabled in this build, so 78.6 TFLOP/s is the reachable ceiling"}}
+                                "packed FP32 is disabled in this build, so 78.6 TFLOP/s is the reachable ceiling; "
+                                "the busiest unit is the vector-memory address path (per-lane propagator-cell "
+                                "gathers, 79 % busy, profiles/k_eval_vec_c2_counters.md), VALU issue 39 %"}}
     if with_cpu:
         import oracle_lib as O
         threads = cpu_threads()
