@@ -53,8 +53,11 @@ enum {
                                 in t1 - t2 inside a diagonal cell                        */
     QIW_GF_CUBIC_SPLINE = 1, /* SplineInterpolatedGF: natural cubic spline through the
                                 grid values (src/spline_gf.jl:194-242)                   */
-    QIW_GF_INC_SPLINE = 2    /* IncSplineImaginaryTimeGF: incremental quadratic spline
+    QIW_GF_INC_SPLINE = 2,   /* IncSplineImaginaryTimeGF: incremental quadratic spline
                                 (src/spline_gf.jl:273-425, src/utility.jl:106-157)      */
+    QIW_GF_DLR = 3           /* keldysh_dlr.DLRImaginaryTimeGF (src/keldysh_dlr.jl:156-
+                                190): an exact pole sum over its DLR frequencies (pair
+                                propagators only; qiw_model.pair_dlr_*)                  */
 };
 
 /* ---- arithmetic of the chain products ------------------------------------------------ */
@@ -92,12 +95,23 @@ typedef struct {
     const qiw_sbm *pair_op_f;      /* [n_pairs] operator_f (later, "tail" end)            */
     const double *pair_prop;       /* [n_pairs][n_tau] complex128: propagator values on
                                       the grid, g_p(tau_j)                                */
-    const int32_t *pair_prop_kind; /* [n_pairs] QIW_GF_GRID_LINEAR or QIW_GF_CUBIC_SPLINE */
+    const int32_t *pair_prop_kind; /* [n_pairs] QIW_GF_GRID_LINEAR, QIW_GF_CUBIC_SPLINE or
+                                      QIW_GF_DLR                                           */
     const int32_t *pair_prop_xi;   /* [n_pairs] statistics sign (-1 fermionic, +1 bosonic),
                                       used when the time difference is negative           */
     int32_t n_corr;                /* correlator operator pairs (A, B), add_corr_operators! */
     const qiw_sbm *corr_A;         /* [n_corr] A, placed at tau                            */
     const qiw_sbm *corr_B;         /* [n_corr] B, placed at 0                              */
+    /* Pair propagators of kind QIW_GF_DLR: g_p(tau) = sum_k c_pk K(tau, w_pk) for
+     * 0 <= tau <= beta (xi_p g_p(beta + tau) for tau < 0, src/keldysh_dlr.jl:185-200),
+     * K(tau, w) = e^{-w tau} / (1 + e^{-w beta}) the fermionic kernel of the DLR
+     * coefficients that Lehmann.jl's dlr2tau sums (its Spectral.kernelFermiT); c_pk are
+     * the GF's coefficients (G.mat.data), w_pk its grid's DLR frequencies (G.grid.dlr.ω).
+     * The grid values pair_prop are not read for these pairs. */
+    int32_t dlr_rank;              /* frequencies per DLR propagator, padded with zero
+                                      coefficients (0: no DLR pair)                       */
+    const double *pair_dlr_omega;  /* [n_pairs][dlr_rank] w_pk (NULL when dlr_rank = 0)    */
+    const double *pair_dlr_coef;   /* [n_pairs][dlr_rank] complex128 c_pk                  */
 } qiw_model;
 
 /* Randomized qMC, RandomizationParams (src/randomization.jl:46-57) consumed by

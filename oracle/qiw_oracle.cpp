@@ -522,12 +522,34 @@ static inline cd grid_bilinear(const GridCell &c, G vals) {
 }
 
 // A scalar propagator of an InteractionPair: g(t_f, t_i)
+// The fermionic DLR kernel e^{-w tau} / (1 + e^{-w beta}), 0 <= tau <= beta: the basis in
+// which Lehmann.jl's dlr2tau sums a DLRImaginaryTimeGF's coefficients (called at
+// src/keldysh_dlr.jl:174; Lehmann.jl Spectral.kernelFermiT).
+static double dlr_kernel(double tau, double w, double beta) {
+    return w >= 0.0 ? std::exp(-w * tau) / (1.0 + std::exp(-w * beta))
+                    : std::exp(w * (beta - tau)) / (1.0 + std::exp(w * beta));
+}
+
 struct ScalarGF {
     int kind = QIW_GF_GRID_LINEAR;
     int xi = -1;
     double beta = 1, h = 1;
     std::vector<cd> vals;
     CubicSpline spl;
+    std::vector<double> dlr_w;  // QIW_GF_DLR: frequencies and coefficients
+    std::vector<cd> dlr_c;
+    // DLRImaginaryTimeGF (src/keldysh_dlr.jl:172-200): the pole sum at the time difference,
+    // or xi times it at beta + ref1 - ref2 when ref1 < ref2.
+    cd dlr_eval(double ref1, double ref2) const {
+        double d = ref1 - ref2, sign = 1.0;
+        if (!(ref1 >= ref2)) {
+            d += beta;
+            sign = xi;
+        }
+        cd s = 0;
+        for (size_t k = 0; k < dlr_w.size(); ++k) s += dlr_c[k] * dlr_kernel(d, dlr_w[k], beta);
+        return sign * s;
+    }
     void build(const double *v, int n, int kind_, int xi_, double beta_) {
         kind = kind_;
         xi = xi_;
@@ -542,6 +564,7 @@ struct ScalarGF {
     // interpolation above (the evaluator calls it with ref1 >= ref2 only, after its ordering
     // clamp, src/topology_eval.jl:408-411).
     cd operator()(double ref1, double ref2) const {
+        if (kind == QIW_GF_DLR) return dlr_eval(ref1, ref2);
         if (kind == QIW_GF_CUBIC_SPLINE)
             return ref1 >= ref2 ? spl.eval(ref1 - ref2) : double(xi) * spl.eval(beta + ref1 - ref2);
         if (!(ref1 >= ref2)) return cd(0);
@@ -594,6 +617,14 @@ static bool load_model(const qiw_model *mi, Model &m) {
         if (!load_sbm(mi->pair_op_f[p], m.dims, m.pair_ops[p].second)) return false;
         m.props[p].build(mi->pair_prop + (size_t)2 * p * m.ntau, m.ntau, mi->pair_prop_kind[p],
                          mi->pair_prop_xi[p], m.beta);
+        if (mi->pair_prop_kind[p] == QIW_GF_DLR) {
+            if (mi->dlr_rank < 1 || !mi->pair_dlr_omega || !mi->pair_dlr_coef) return false;
+            for (int k = 0; k < mi->dlr_rank; ++k) {
+                const size_t i = (size_t)p * mi->dlr_rank + k;
+                m.props[p].dlr_w.push_back(mi->pair_dlr_omega[i]);
+                m.props[p].dlr_c.push_back(cd(mi->pair_dlr_coef[2 * i], mi->pair_dlr_coef[2 * i + 1]));
+            }
+        }
     }
     m.ncorr = mi->n_corr;
     m.corr_ops.resize(m.ncorr);
@@ -1749,6 +1780,23 @@ extern "C" int orc_eval_topologies(const qiw_model *mi, int kind, int order, int
         set_err("A block off-diagonal contribution to a pseudo-particle propagator detected");
         return QIW_E_OFFDIAG;
     }
+    return 0;
+}
+
+// A DLR lookup for unit tests: g(t1, t2) of the pole sum (w[k], c[k]) with statistics xi.
+extern "C" int orc_dlr_eval(int rank, const double *w, const double *c, int xi, double beta, double t1, double t2,
+                            double *out) {
+    ScalarGF g;
+    g.kind = QIW_GF_DLR;
+    g.xi = xi;
+    g.beta = beta;
+    for (int k = 0; k < rank; ++k) {
+        g.dlr_w.push_back(w[k]);
+        g.dlr_c.push_back(cd(c[2 * k], c[2 * k + 1]));
+    }
+    const cd v = g(t1, t2);
+    out[0] = v.real();
+    out[1] = v.imag();
     return 0;
 }
 

@@ -43,6 +43,8 @@ struct HostModel {
     int ntab = 0;
     std::vector<double> dcoef;    // [ntab][ntau-1][4]
     std::vector<int32_t> dkind;
+    int dlr_rank = 0;             // poles per DLR table
+    std::vector<double> dlr;      // [ntab][dlr_rank] (w, coefficient of i*Delta)
     std::vector<std::vector<int>> attachable;
     int bt = 0;
     std::vector<int> bt_off;

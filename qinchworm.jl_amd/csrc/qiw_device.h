@@ -136,7 +136,10 @@ struct QiwDeltaTables {
     int32_t n_tau;
     double h, beta;
     int32_t grid_mask;     // bit t set: table t is a grid container (else a natural cubic spline)
-    const int32_t *kind;   // [n_tables] QIW_GF_GRID_LINEAR / QIW_GF_CUBIC_SPLINE
+    int32_t dlr_mask;      // bit t set: table t is a DLR pole sum (dlr below)
+    int32_t dlr_rank;      // poles per DLR table
+    const double *dlr;     // [n_tables][dlr_rank] (frequency w, coefficient of i*Delta)
+    const int32_t *kind;   // [n_tables] QIW_GF_GRID_LINEAR / QIW_GF_CUBIC_SPLINE / QIW_GF_DLR
     // [n_tables][n_tau-1][4]: a spline table holds the cubic (c0..c3) of interval j in the
     // time difference; a grid table holds, for index d, (g(d), g(d+1) - g(d), g(d) - g(d-1), 0)
     // with g(-1) = 2 g(0) - g(1) (the grid cell interpolation of the kernels' grid_cell)

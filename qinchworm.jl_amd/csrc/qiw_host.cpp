@@ -1213,6 +1213,8 @@ int ensure_model_buffers(qiw_ctx *c, double **dcoef, int32_t **dkind, double **e
     const HostModel &m = c->m;
     *dcoef = (double *)dev(c, "delta.coef", std::max<size_t>(m.dcoef.size(), 1) * 8, &rc);
     *dkind = (int32_t *)dev(c, "delta.kind", std::max<size_t>(m.dkind.size(), 1) * 4, &rc);
+    double *ddlr = (double *)dev(c, "delta.dlr", std::max<size_t>(m.dlr.size(), 1) * 8, &rc);
+    if (!rc && !m.dlr.empty()) HIPCHK(c, hipMemcpy(ddlr, m.dlr.data(), m.dlr.size() * 8, hipMemcpyHostToDevice));
     size_t n_eig = 0;
     for (int s = 0; s < m.ns; ++s) n_eig += m.dims[s];
     *eshift = (double *)dev(c, "eshift", n_eig * 8, &rc);
@@ -1355,8 +1357,16 @@ EvalArgs make_eval_args(qiw_ctx *c, const Batch &bt, const DevBatch &d, double *
     a.dtab.beta = m.beta;
     a.dtab.kind = dkind;
     a.dtab.grid_mask = 0;
+    a.dtab.dlr_mask = 0;
+    a.dtab.dlr_rank = m.dlr_rank;
+    {
+        int rc = 0;
+        a.dtab.dlr = (const double *)dev(c, "delta.dlr", std::max<size_t>(m.dlr.size(), 1) * 8, &rc);
+    }
+    for (int t = 0; t < m.ntab; ++t)
+        if (m.dkind[t] == QIW_GF_DLR) a.dtab.dlr_mask |= 1 << t;
     for (int t = 0; t < m.ntab && t < 31; ++t)
-        if (m.dkind[t] != QIW_GF_CUBIC_SPLINE) a.dtab.grid_mask |= 1 << t;
+        if (m.dkind[t] == QIW_GF_GRID_LINEAR) a.dtab.grid_mask |= 1 << t;
     a.dtab.coef = dcoef;
     a.ns = m.ns;
     a.n_tau = m.ntau;
@@ -1515,21 +1525,41 @@ static int qiw_set_model_impl(qiw_ctx *c, const qiw_model *mi) {
     m.pair_table.resize(m.np);
     std::vector<std::vector<double>> tabs;
     std::vector<int> tab_kind;
+    m.dlr_rank = 0;
+    for (int p = 0; p < m.np; ++p)
+        if (mi->pair_prop_kind[p] == QIW_GF_DLR) {
+            if (mi->dlr_rank < 1 || !mi->pair_dlr_omega || !mi->pair_dlr_coef)
+                return fail(c, QIW_E_ARG, "a QIW_GF_DLR pair needs dlr_rank > 0, pair_dlr_omega and pair_dlr_coef");
+            if (mi->dlr_rank > 512) return fail(c, QIW_E_LIMIT, "dlr_rank beyond 512");
+            m.dlr_rank = mi->dlr_rank;
+        }
     for (int p = 0; p < m.np; ++p) {
         int rc = load_scalar_op(c, mi->pair_op_i[p], m.dims, m.op_i[p], &m.scalar, m.op_data, m.op_desc);
         if (rc) return rc;
         if ((rc = load_scalar_op(c, mi->pair_op_f[p], m.dims, m.op_f[p], &m.scalar, m.op_data, m.op_desc))) return rc;
-        // i * Delta_p on the grid must be real
-        std::vector<double> y(m.ntau);
-        for (int j = 0; j < m.ntau; ++j) {
-            double re = mi->pair_prop[2 * ((size_t)p * m.ntau + j)], im = mi->pair_prop[2 * ((size_t)p * m.ntau + j) + 1];
-            if (std::fabs(re) > 1e-13 * (1.0 + std::fabs(im)))
-                return fail(c, QIW_E_ARG, "i*Delta must be real for the GPU kernels");
-            y[j] = -im;
-        }
         int kind = mi->pair_prop_kind[p];
-        if (kind != QIW_GF_GRID_LINEAR && kind != QIW_GF_CUBIC_SPLINE)
-            return fail(c, QIW_E_ARG, "pair propagator kind must be GRID_LINEAR or CUBIC_SPLINE");
+        if (kind != QIW_GF_GRID_LINEAR && kind != QIW_GF_CUBIC_SPLINE && kind != QIW_GF_DLR)
+            return fail(c, QIW_E_ARG, "pair propagator kind must be GRID_LINEAR, CUBIC_SPLINE or DLR");
+        // i * Delta_p must be real: grid values, or (frequency, coefficient) pairs of a DLR
+        std::vector<double> y;
+        if (kind == QIW_GF_DLR) {
+            for (int k = 0; k < m.dlr_rank; ++k) {
+                const size_t i = (size_t)p * m.dlr_rank + k;
+                const double re = mi->pair_dlr_coef[2 * i], im = mi->pair_dlr_coef[2 * i + 1];
+                if (std::fabs(re) > 1e-13 * (1.0 + std::fabs(im)))
+                    return fail(c, QIW_E_ARG, "i*Delta must be real for the GPU kernels (DLR coefficients)");
+                y.push_back(mi->pair_dlr_omega[i]);
+                y.push_back(-im);
+            }
+        } else {
+            y.resize(m.ntau);
+            for (int j = 0; j < m.ntau; ++j) {
+                double re = mi->pair_prop[2 * ((size_t)p * m.ntau + j)], im = mi->pair_prop[2 * ((size_t)p * m.ntau + j) + 1];
+                if (std::fabs(re) > 1e-13 * (1.0 + std::fabs(im)))
+                    return fail(c, QIW_E_ARG, "i*Delta must be real for the GPU kernels");
+                y[j] = -im;
+            }
+        }
         int found = -1;
         for (size_t t = 0; t < tabs.size(); ++t)
             if (tab_kind[t] == kind && tabs[t] == y) found = (int)t;
@@ -1543,10 +1573,13 @@ static int qiw_set_model_impl(qiw_ctx *c, const qiw_model *mi) {
     m.ntab = (int)tabs.size();
     if (m.ntab > 31) return fail(c, QIW_E_LIMIT, "more than 31 distinct hybridization tables");
     m.dcoef.assign((size_t)std::max(m.ntab, 1) * (m.ntau - 1) * 4, 0.0);
+    m.dlr.assign((size_t)std::max(m.ntab, 1) * std::max(m.dlr_rank, 1) * 2, 0.0);
     m.dkind = tab_kind;
     for (int t = 0; t < m.ntab; ++t) {
         double *cc = m.dcoef.data() + (size_t)t * (m.ntau - 1) * 4;
-        if (tab_kind[t] == QIW_GF_CUBIC_SPLINE) {
+        if (tab_kind[t] == QIW_GF_DLR) {
+            std::copy(tabs[t].begin(), tabs[t].end(), m.dlr.begin() + (size_t)t * m.dlr_rank * 2);
+        } else if (tab_kind[t] == QIW_GF_CUBIC_SPLINE) {
             natural_spline_coef(tabs[t], m.h, cc);
         } else {  // grid container: value, forward and backward difference per index
             for (int j = 0; j < m.ntau - 1; ++j) {

@@ -164,6 +164,7 @@ __device__ __forceinline__ double cell_value(const GridCell &c, double gm, doubl
 struct ArcPos {
     int j;
     double u;
+    double z;  // the time difference itself (DLR tables)
     GridCell c;
 };
 __device__ __forceinline__ ArcPos arc_pos(double t_tail, double t_head, double inv_h, int n_tau) {
@@ -175,13 +176,27 @@ __device__ __forceinline__ ArcPos arc_pos(double t_tail, double t_head, double i
     j = j < 0 ? 0 : (j > n_tau - 2 ? n_tau - 2 : j);
     p.j = j;
     p.u = x - (double)j;
+    p.z = z;
     p.c = grid_cell(t_tail, t_head, inv_h, n_tau);
     return p;
 }
+
+// The fermionic kernel of the DLR, e^{-w tau} / (1 + e^{-w beta}) for 0 <= tau <= beta, in
+// the form whose exponents are never positive.
+__device__ __forceinline__ double dlr_kernel(double tau, double w, double beta) {
+    return w >= 0.0 ? exp(-w * tau) / (1.0 + exp(-w * beta)) : exp(w * (beta - tau)) / (1.0 + exp(w * beta));
+}
 // i*Delta of table tab at arc position p: a natural cubic spline in the time difference
-// (SplineInterpolatedGF, src/spline_gf.jl:194-219), coefficients (c0..c3) per interval, or
-// a grid container, (g(d), g(d+1) - g(d), g(d) - g(d-1), 0) per index d.
+// (SplineInterpolatedGF, src/spline_gf.jl:194-219), coefficients (c0..c3) per interval, a
+// grid container, (g(d), g(d+1) - g(d), g(d) - g(d-1), 0) per index d, or a DLR pole sum
+// (keldysh_dlr.DLRImaginaryTimeGF, src/keldysh_dlr.jl:172-176).
 __device__ __forceinline__ double delta_value(const QiwDeltaTables &dt, int tab, const ArcPos &p) {
+    if ((dt.dlr_mask >> tab) & 1) {
+        const double *c = dt.dlr + (int64_t)tab * dt.dlr_rank * 2;
+        double s = 0.0;
+        for (int k = 0; k < dt.dlr_rank; ++k) s = fma(c[2 * k + 1], dlr_kernel(p.z, c[2 * k], dt.beta), s);
+        return s;
+    }
     if ((dt.grid_mask >> tab) & 1) {
         const double *c = dt.coef + ((int64_t)tab * (dt.n_tau - 1) + p.c.d) * 4;
         return c[0] + p.c.wp * c[1] - p.c.wm * c[2];

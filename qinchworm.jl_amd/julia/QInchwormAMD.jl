@@ -18,6 +18,7 @@ using QInchworm.ppgf: set_ppgf!
 using QInchworm.sector_block_matrix: SectorBlockMatrix
 using QInchworm.scrambled_sobol: ScrambledSobolSeq
 using QInchworm.randomization: RandomizationParams, RequestStdDev
+using QInchworm.keldysh_dlr: DLRImaginaryTimeGF
 
 const libqiw = get(ENV, "QIW_LIBRARY", joinpath(@__DIR__, "..", "lib", "libqiw.so"))
 
@@ -26,6 +27,7 @@ const QIW_PREC_FP32 = Int32(1)
 const QIW_GF_GRID_LINEAR = Int32(0)
 const QIW_GF_CUBIC_SPLINE = Int32(1)
 const QIW_GF_INC_SPLINE = Int32(2)
+const QIW_GF_DLR = Int32(3)
 
 struct QiwSbm
     n_blocks::Int32
@@ -50,6 +52,9 @@ struct QiwModel
     n_corr::Int32
     corr_A::Ptr{QiwSbm}
     corr_B::Ptr{QiwSbm}
+    dlr_rank::Int32                 # QIW_GF_DLR pairs: frequencies per propagator
+    pair_dlr_omega::Ptr{Float64}    # [n_pairs][dlr_rank]
+    pair_dlr_coef::Ptr{ComplexF64}  # [n_pairs][dlr_rank]
 end
 
 # Randomized qMC (qiw_rand).  With rand_params.rng given and target_std = 0 the sequences
@@ -173,7 +178,29 @@ function prop_table(g, n_tau)
     if g isa SplineInterpolatedGF
         return vec(g.GF.mat.data)[1:n_tau], QIW_GF_CUBIC_SPLINE, Int32(Int(g.GF.ξ))
     end
+    if g isa DLRImaginaryTimeGF  # pole sum; the grid values are not read (zeros)
+        return zeros(ComplexF64, n_tau), QIW_GF_DLR, Int32(Int(g.ξ))
+    end
     return vec(g.mat.data)[1:n_tau], QIW_GF_GRID_LINEAR, Int32(Int(g.ξ))
+end
+
+# (frequencies, coefficients) of the DLR pair propagators, [n_pairs][rank] row-major
+# (zero rows for the other pairs): the DLR grid's frequencies G.grid.dlr.ω and the
+# coefficients G.mat.data that Lehmann.jl's dlr2tau sums (src/keldysh_dlr.jl:172-176).
+function dlr_tables(pairs)
+    dl = [p.propagator for p in pairs if p.propagator isa DLRImaginaryTimeGF]
+    isempty(dl) && return 0, Float64[], ComplexF64[]
+    rank = maximum(length(g.grid.dlr.ω) for g in dl)
+    w = zeros(Float64, rank, length(pairs))
+    c = zeros(ComplexF64, rank, length(pairs))
+    for (i, p) in enumerate(pairs)
+        g = p.propagator
+        g isa DLRImaginaryTimeGF || continue
+        k = length(g.grid.dlr.ω)
+        w[1:k, i] = g.grid.dlr.ω
+        c[1:k, i] = vec(g.mat.data)
+    end
+    return rank, vec(w), vec(c)
 end
 
 function stage!(ctx::Context, exp::Expansion, grid::kd.ImaginaryTimeGrid, max_order::Int, keep)
@@ -189,11 +216,13 @@ function stage!(ctx::Context, exp::Expansion, grid::kd.ImaginaryTimeGrid, max_or
     xis = Int32[t[3] for t in tabs]
     cA = [sbm_arrays(p[1], keep) for p in exp.corr_operators_mat]
     cB = [sbm_arrays(p[2], keep) for p in exp.corr_operators_mat]
-    push!(keep, dims, energies, op_i, op_f, props, kinds, xis, cA, cB)
+    rank, dw, dc = dlr_tables(exp.pairs)
+    push!(keep, dims, energies, op_i, op_f, props, kinds, xis, cA, cB, dw, dc)
     model = Ref(QiwModel(grid.contour.β, n_tau, length(dims), pointer(dims), pointer(energies), ppgf_kind,
                          length(exp.pairs), pointer(op_i), pointer(op_f), pointer(props), pointer(kinds),
                          pointer(xis), length(cA), isempty(cA) ? C_NULL : pointer(cA),
-                         isempty(cB) ? C_NULL : pointer(cB)))
+                         isempty(cB) ? C_NULL : pointer(cB), Int32(rank),
+                         rank == 0 ? C_NULL : pointer(dw), rank == 0 ? C_NULL : pointer(dc)))
     check(ctx, ccall((:qiw_set_model, libqiw), Cint, (Ptr{Cvoid}, Ref{QiwModel}), ctx.h, model))
     # Sobol state of the reference's own generator (unscrambled by default)
     D = max(2 * max_order, 1)

@@ -16,7 +16,7 @@ QIW_OK = 0
 QIW_E_ARG, QIW_E_NOT_POW2, QIW_E_OFFDIAG, QIW_E_NOT_SBM = -1, -2, -3, -4
 QIW_E_DIM, QIW_E_STATE, QIW_E_DEVICE, QIW_E_LIMIT, QIW_E_NO_GPU = -5, -6, -7, -8, -9
 
-QIW_GF_GRID_LINEAR, QIW_GF_CUBIC_SPLINE, QIW_GF_INC_SPLINE = 0, 1, 2
+QIW_GF_GRID_LINEAR, QIW_GF_CUBIC_SPLINE, QIW_GF_INC_SPLINE, QIW_GF_DLR = 0, 1, 2, 3
 QIW_PREC_FP64, QIW_PREC_FP32 = 0, 1
 
 _i32p = C.POINTER(C.c_int32)
@@ -35,7 +35,8 @@ class qiw_model(C.Structure):
                 ("n_pairs", C.c_int32), ("pair_op_i", C.POINTER(qiw_sbm)),
                 ("pair_op_f", C.POINTER(qiw_sbm)), ("pair_prop", _f64p),
                 ("pair_prop_kind", _i32p), ("pair_prop_xi", _i32p), ("n_corr", C.c_int32),
-                ("corr_A", C.POINTER(qiw_sbm)), ("corr_B", C.POINTER(qiw_sbm))]
+                ("corr_A", C.POINTER(qiw_sbm)), ("corr_B", C.POINTER(qiw_sbm)),
+                ("dlr_rank", C.c_int32), ("pair_dlr_omega", _f64p), ("pair_dlr_coef", _f64p)]
 
 
 class qiw_rand(C.Structure):
@@ -76,7 +77,9 @@ class ModelStruct:
     """A qiw_model plus the numpy buffers it points into (kept alive together)."""
 
     def __init__(self, beta, n_tau, dims, energies, ppgf_kind, pairs, props, prop_kinds,
-                 prop_xis, corr_ops):
+                 prop_xis, corr_ops, dlr=None):
+        """dlr: None, or (omega [n_pairs][rank], coef [n_pairs][rank] complex) for the
+        QIW_GF_DLR pairs (rows of other pairs are ignored)."""
         self._keep = []
         self.dims = np.ascontiguousarray(dims, dtype=np.int32)
         self.energies = np.ascontiguousarray(np.concatenate([np.asarray(e, float) for e in energies]))
@@ -101,6 +104,12 @@ class ModelStruct:
             pair_prop=self.props.ctypes.data_as(_f64p),
             pair_prop_kind=ptr(self.prop_kinds, C.c_int32),
             pair_prop_xi=ptr(self.prop_xis, C.c_int32), n_corr=len(corr_ops), corr_A=cA, corr_B=cB)
+        if dlr is not None:
+            self.dlr_w = np.ascontiguousarray(dlr[0], dtype=np.float64)
+            self.dlr_c = np.ascontiguousarray(dlr[1], dtype=np.complex128)
+            self.struct.dlr_rank = int(self.dlr_w.shape[1])
+            self.struct.pair_dlr_omega = self.dlr_w.ctypes.data_as(_f64p)
+            self.struct.pair_dlr_coef = self.dlr_c.ctypes.data_as(_f64p)
 
     def _sbm(self, blocks):
         """blocks: dict s_i -> (s_f, complex matrix)."""

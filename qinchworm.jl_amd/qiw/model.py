@@ -189,8 +189,99 @@ def spline_interpolated(g):
 
 
 def ph_conj(g):
-    """g(beta - tau) (src/utility.jl:210-217, src/spline_gf.jl:250-255)."""
+    """g(beta - tau) (src/utility.jl:210-217, src/spline_gf.jl:250-255).  For a DLR
+    function the pole sum's kernel obeys K(beta - tau, w) = K(tau, -w), so the conjugate
+    is the same coefficients at the negated frequencies (exact; the reference refits on its
+    DLR nodes, src/keldysh_dlr.jl:262-270, which agrees to the DLR tolerance)."""
+    if isinstance(g, DLRGF):
+        return DLRGF(-g.omega, g.coef, g.beta, g.tau, g.xi)
     return ScalarGF(g.values[::-1].copy(), g.kind, g.xi)
+
+
+# ------------------------------------------------------------------------------------
+# Discrete Lehmann representation (keldysh_dlr.DLRImaginaryTimeGF, src/keldysh_dlr.jl)
+# ------------------------------------------------------------------------------------
+def dlr_kernel(tau, w, beta):
+    """The fermionic kernel e^{-w tau} / (1 + e^{-w beta}), 0 <= tau <= beta, in the form
+    whose exponents are never positive; tau and w broadcast."""
+    tau, w = np.broadcast_arrays(np.asarray(tau, float), np.asarray(w, float))
+    pos = w >= 0
+    return np.where(pos, np.exp(-np.abs(w) * tau) / (1.0 + np.exp(-np.abs(w) * beta)),
+                    np.exp(-np.abs(w) * (beta - tau)) / (1.0 + np.exp(-np.abs(w) * beta)))
+
+
+class DLRGF:
+    """g(tau) = sum_k coef_k K(tau, omega_k) (scalar DLRImaginaryTimeGF; pair propagators of
+    kind QIW_GF_DLR).  `values` holds g on the uniform grid (the qiw_model's grid values,
+    not read for this kind)."""
+    kind = abi.QIW_GF_DLR
+
+    def __init__(self, omega, coef, beta, tau, xi=-1):
+        self.omega = np.asarray(omega, float)
+        self.coef = np.asarray(coef, np.complex128)
+        self.beta = float(beta)
+        self.tau = np.asarray(tau, float)
+        self.xi = xi
+        self.values = self(self.tau)
+
+    def __call__(self, tau):
+        return dlr_kernel(np.asarray(tau, float)[:, None], self.omega[None, :], self.beta) @ self.coef
+
+    def __mul__(self, a):
+        return DLRGF(self.omega, self.coef * a, self.beta, self.tau, self.xi)
+
+    __rmul__ = __mul__
+
+
+def _cheb_panels(edges, n):
+    """Chebyshev nodes of the first kind, n per panel [edges[i], edges[i+1]]."""
+    x = np.cos((2 * np.arange(n) + 1) * np.pi / (2 * n))[::-1]
+    return np.concatenate([0.5 * (a + b) + 0.5 * (b - a) * x for a, b in zip(edges[:-1], edges[1:])])
+
+
+def dlr_frequencies(Euv, beta, rtol, n_cheb=24):
+    """Real frequencies of a fermionic DLR of cutoff Euv at tolerance rtol, and the time
+    nodes its coefficients are fitted on, after the construction of Lehmann.jl's DLRGrid
+    (Euv, beta, rtol, isFermi) (Kaye, Chen and Parcollet, Phys. Rev. B 105, 235115
+    (2022)): the kernel K(tau, w) is sampled on composite Chebyshev grids refined
+    dyadically towards w = 0 and towards tau = 0 and beta, and a column-pivoted QR picks the
+    frequencies up to relative size rtol.  The frequencies are picked in pairs +-w (a QR
+    of K(tau, w) stacked on K(beta - tau, w) = K(tau, -w), w >= 0), so that the particle-
+    hole conjugate of a DLR is again one on the same frequencies and a particle-hole
+    symmetric function has a symmetric DLR; the fit nodes are the symmetric fine time grid."""
+    from scipy.linalg import qr
+    lam = Euv * beta
+    m = max(int(np.ceil(np.log2(max(lam, 2.0)))), 1)
+    wedges = np.concatenate([[0.0], lam * 2.0 ** -np.arange(m, -1, -1.0)])
+    xp = _cheb_panels(wedges, n_cheb)                         # w * beta >= 0
+    tedges = np.concatenate([[0.0], 0.5 * 2.0 ** -np.arange(m + 4, -1, -1.0)])
+    th = _cheb_panels(tedges, n_cheb)
+    t = np.concatenate([th, 1.0 - th[::-1]])                  # tau / beta, symmetric
+    A = np.vstack([dlr_kernel(t[:, None], xp[None, :], 1.0), dlr_kernel(1.0 - t[:, None], xp[None, :], 1.0)])
+    _, R, piv = qr(A, mode="economic", pivoting=True)
+    d = np.abs(np.diag(R))
+    xs = np.sort(xp[piv[:int(np.sum(d > rtol * d[0]))]])
+    return np.concatenate([-xs[::-1], xs]) / beta, t * beta
+
+
+def dlr_fit(f, Euv, beta, grid, rtol=1e-6, xi=-1):
+    """The DLR of the function f(tau) (complex, 0 <= tau <= beta): coefficients by least
+    squares on the symmetric fine time grid (Lehmann.jl's tau2dlr fits on its DLR nodes;
+    DLRImaginaryTimeGF(f, grid), src/keldysh_dlr.jl:212-236)."""
+    omega, tau = dlr_frequencies(Euv, beta, rtol)
+    g = np.asarray(f(tau), np.complex128)
+    coef = np.linalg.lstsq(dlr_kernel(tau[:, None], omega[None, :], beta), g, rcond=None)[0]
+    return DLRGF(omega, coef, beta, grid.tau, xi)
+
+
+def bethe_values(t, beta, tau, n_quad=400, eps=0.0):
+    """G(tau) of the semi-elliptic DOS of half-bandwidth 2t centred at eps, at any taus
+    (the quadrature of bethe_gf)."""
+    x, w = np.polynomial.legendre.leggauss(n_quad)
+    theta = 0.5 * np.pi * (x + 1.0)
+    wt = 0.5 * np.pi * w * (2.0 / np.pi) * np.sin(theta) ** 2
+    e = eps + 2.0 * t * np.cos(theta)
+    return -1j * (wt[:, None] * _fermi_weighted_exp(e, beta, np.asarray(tau, float))).sum(axis=0)
 
 
 def _fermi_weighted_exp(eps, beta, tau):
@@ -340,10 +431,21 @@ class Expansion:
     def to_struct(self):
         """The qiw_model this expansion is handed over as."""
         props = np.array([p.propagator.values for p in self.pairs]).reshape(len(self.pairs), self.grid.n_tau)
+        dlr = None
+        ranks = [len(p.propagator.omega) for p in self.pairs if p.propagator.kind == abi.QIW_GF_DLR]
+        if ranks:
+            r = max(ranks)
+            w = np.zeros((len(self.pairs), r))
+            c = np.zeros((len(self.pairs), r), np.complex128)
+            for i, p in enumerate(self.pairs):
+                if p.propagator.kind == abi.QIW_GF_DLR:
+                    k = len(p.propagator.omega)
+                    w[i, :k], c[i, :k] = p.propagator.omega, p.propagator.coef
+            dlr = (w, c)
         return abi.ModelStruct(
             self.grid.beta, self.grid.n_tau, self.ed.dims, self.ed.energies, self.ppgf_kind,
             self.pair_operator_mat, props, [p.propagator.kind for p in self.pairs],
-            [p.propagator.xi for p in self.pairs], self.corr_operators_mat)
+            [p.propagator.xi for p in self.pairs], self.corr_operators_mat, dlr=dlr)
 
 
 # ------------------------------------------------------------------------------------
@@ -387,6 +489,25 @@ def bethe_two_orbitals(n_tau, mu_bethe=0.25, beta=10.0, V=0.5, t_bethe=0.5):
     return Expansion.from_hamiltonian(H, fock, grid, hyb)
 
 
+def bethe_two_orbitals_dlr(n_tau, mu_bethe=0.25, beta=10.0, V=0.5, t_bethe=0.5, Euv=1.25, rtol=1e-6):
+    """The model of test/bethe_dlr.jl:55-97: two non-interacting orbitals, each with the
+    semi-elliptic hybridization V^2 G_bethe(t = t_bethe, eps = mu_bethe) as a DLR
+    (DLRGrid(Euv = 1.25, beta, rtol = 1e-6, isFermi)), pairs c^dag_i/c_i with Delta and
+    c_i/c^dag_i with ph_conj(Delta) built by the low-level Expansion constructor."""
+    fock = FockSpace([1, 2])
+    H = 0.0 * (fock.n(1) + fock.n(2))
+    ed = EDCore(H, fock, lambda b: (fock.occupation(b, 1), fock.occupation(b, 2)))
+    grid = ImaginaryTimeGrid(beta, n_tau)
+    delta = V ** 2 * dlr_fit(lambda tau: bethe_values(t_bethe, beta, tau, eps=mu_bethe), Euv, beta, grid, rtol)
+    pairs = []
+    for i in (1, 2):
+        pairs.append(InteractionPair(fock.c_dag(i), fock.c(i), delta))
+        pairs.append(InteractionPair(fock.c(i), fock.c_dag(i), ph_conj(delta)))
+    exp = Expansion(ed, grid, pairs)
+    exp.fock = fock
+    return exp
+
+
 def fock_density_matrix(expansion, P):
     """The density matrix in the Fock basis (KeldyshED full_hs_matrix(tofockbasis(...)))
     from i P(beta) normalised to unit trace; P is the (n_tau, sum m^2) table."""
@@ -402,7 +523,7 @@ def fock_density_matrix(expansion, P):
     return rho
 
 
-def kanamori(n_orb, U, J, eps_shift, beta, n_tau, V, D=2.0, interpolate=False):
+def kanamori(n_orb, U, J, eps_shift, beta, n_tau, V, D=2.0, interpolate=False, dlr=False):
     """n_orb-orbital Hubbard-Kanamori impurity (spin-rotation invariant):
     H = mu sum n + U sum_a n_a,up n_a,dn + (U-2J) sum_{a<b,s} n_a,s n_b,-s
         + (U-3J) sum_{a<b,s} n_a,s n_b,s
@@ -432,7 +553,10 @@ def kanamori(n_orb, U, J, eps_shift, beta, n_tau, V, D=2.0, interpolate=False):
             H = H + J * cd[(a, "up")] @ cd[(a, "dn")] @ c[(b, "dn")] @ c[(b, "up")]
     ed = EDCore(H, fock, lambda b: bin(b).count("1"))
     grid = ImaginaryTimeGrid(beta, n_tau)
-    delta = V ** 2 * bethe_gf(D / 2.0, beta, grid)
+    if dlr:  # the same bath as a DLR (cutoff the band edge D, tolerance 1e-6)
+        delta = V ** 2 * dlr_fit(lambda tau: bethe_values(D / 2.0, beta, tau), D, beta, grid, rtol=1e-6)
+    else:
+        delta = V ** 2 * bethe_gf(D / 2.0, beta, grid)
     if interpolate:
         delta = spline_interpolated(delta)
     pairs = []

@@ -43,6 +43,7 @@ def lib():
         L.orc_diff_inchworm.argtypes = [C.POINTER(abi.qiw_model), C.POINTER(abi.qiw_run), u32p, u32p, C.c_int,
                                         f64p, f64p, f64p]
         L.orc_rq_scramble.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u32p, u32p, u32p]
+        L.orc_dlr_eval.argtypes = [C.c_int, f64p, f64p, C.c_int, C.c_double, C.c_double, C.c_double, f64p]
         L.orc_set_scrambles.argtypes = [C.c_int64, C.c_int, u32p, u32p]
         L.orc_scrambles_used.restype = C.c_int64
         L.orc_scrambles_log.argtypes = [C.c_int64, C.POINTER(C.c_int64)]
@@ -152,6 +153,16 @@ def rq_scramble(seed, stream, m):
     xo = np.zeros(D, np.uint32)
     lib().orc_rq_scramble(int(seed), int(stream), D, _p(m, C.c_uint32), _p(mo, C.c_uint32), _p(xo, C.c_uint32))
     return mo, xo
+
+
+def dlr_eval(g, t1, t2):
+    """The oracle's lookup of a DLR function g (model.DLRGF) at (t1, t2)."""
+    w = np.ascontiguousarray(g.omega, dtype=np.float64)
+    c = np.ascontiguousarray(g.coef, dtype=np.complex128)
+    out = np.zeros(2)
+    lib().orc_dlr_eval(len(w), _p(w, C.c_double), c.ctypes.data_as(C.POINTER(C.c_double)), int(g.xi), float(g.beta),
+                       float(t1), float(t2), _p(out, C.c_double))
+    return complex(out[0], out[1])
 
 
 def set_scrambles(m, x):
