@@ -265,6 +265,71 @@ def warp_study(n_tau=50, orders=(0, 1, 2), ref_log2n=20, log2ns=tuple(range(10, 
             "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
 
 
+def warp_time_to_error(log2ns=tuple(range(10, 17)), judge_log2n=20, jw=100, iters=4000, n_train=2 ** 14,
+                       p_log2n=8):
+    """The metric's time-to-error leg on the learned-warp workload: the two-orbital model of
+    the headline (n_tau = 200, orders 0:4), one bold step at tau_w = grid[jw] in float32
+    chain products, plain and warped qMC at N = 2^10 .. 2^16, each judged against the plain
+    step at N = 2^judge_log2n.  The propagator is that of a plain float32 solve at
+    N = 2^p_log2n (the same P for every estimate); the flow (4 rational-quadratic layers,
+    width 64) is trained on |integrand| samples of 8 bold steps, 2^14 per group.  The judge
+    runs the float32 chain products (1.8e-8 from float64 on one step, far below the errors
+    it judges); a float64 judge at 2^20 would take about 17 minutes.  Time per estimate is
+    the evaluator time of the step (qiw_stats.kernel_ms); time to error is interpolated in
+    log-log between the measured N."""
+    import numpy as np
+    import qiw
+    from qiw import abi, model
+    fp32 = abi.QIW_PREC_FP32
+    k = model.kanamori(2, 2.0, 0.2, -1.0, 10.0, 200, 0.5)
+    orders = list(range(0, 5))
+    qiw.inchworm(k, k.grid, orders, orders, 2 ** p_log2n, precision=fp32)
+    progress(f"warp leg: propagator (plain solve at 2^{p_log2n})")
+    P = k.P.copy()
+    qiw.warp_init(iters=iters, batch=8192, n_train_steps=8, n_train=n_train, lr=2e-3, seed=1)
+    t0 = time.perf_counter()
+    hist = qiw.warp_train(k, orders)
+    train_s = time.perf_counter() - t0
+    progress(f"warp leg: trained ({train_s:.1f} s)")
+    k.P = P.copy()
+    qiw.warp_enable(False)
+    ref, _ = qiw.inchworm_step(k, jw, orders, 2 ** judge_log2n, precision=fp32)
+    progress(f"warp leg: judge at 2^{judge_log2n}")
+    res = {"plain": [], "warped": []}
+    for ln in log2ns:
+        for name, on in (("plain", False), ("warped", True)):
+            qiw.warp_enable(on)
+            k.P = P.copy()
+            v, _ = qiw.inchworm_step(k, jw, orders, 2 ** ln, precision=fp32)
+            st = qiw.stats()
+            res[name].append((2 ** ln, float(np.linalg.norm(v - ref) / np.linalg.norm(ref)), st["kernel_ms"] / 1e3))
+        progress(f"warp leg: N=2^{ln}")
+    qiw.warp_enable(False)
+
+    def time_to(err, rows):  # log-log interpolation of time against error
+        e = np.log([r[1] for r in rows])
+        t = np.log([r[2] for r in rows])
+        x = np.log(err)
+        if x > e.max() or x < e.min():
+            return None
+        order = np.argsort(e)
+        return float(np.exp(np.interp(x, e[order], t[order])))
+
+    targets = [r[1] for r in res["warped"]]
+    speedups = []
+    for err in targets:
+        tp, tw = time_to(err, res["plain"]), time_to(err, res["warped"])
+        speedups.append(None if (tp is None or tw is None) else tp / tw)
+    return {"workload": f"two-orbital Hubbard-Kanamori (the headline model), n_tau=200, orders 0:4, bold step "
+                        f"tau_w=grid[{jw}], float32 chains; judge: plain at 2^{judge_log2n}",
+            "N": [r[0] for r in res["plain"]], "rel_err_plain": [r[1] for r in res["plain"]],
+            "rel_err_warped": [r[1] for r in res["warped"]], "time_s_plain": [r[2] for r in res["plain"]],
+            "time_s_warped": [r[2] for r in res["warped"]],
+            "time_to_error_speedup_at_warped_errors": speedups, "train_s": train_s,
+            "train_iters": iters, "train_samples_per_step_group": n_train,
+            "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
+
+
 def rqmc_study(order=4, n_tau=200, log2n=14, n_seqs=8, seed=12345678, with_cpu=True, cpu_steps=2):
     """The reference's randomized-qMC study (bench/rqmc/rqmc.jl:37-117) on one GPU: the
     Bethe single-level model, inchworm! orders 0:order (bare and bold) and then
@@ -499,6 +564,9 @@ def main():
                     help="only the two- and three-orbital steps (vector and matrix-core walks), one JSON line")
     ap.add_argument("--warp-study", action="store_true",
                     help="only run the learned-warp error-versus-samples study and print its JSON line")
+    ap.add_argument("--warp-time-to-error", action="store_true",
+                    help="only run the time-to-error leg of the learned warp on the headline model "
+                         "(orders 0:4, n_tau=200; about 10 minutes) and print its JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE hardware counters) and exit")
@@ -508,6 +576,9 @@ def main():
         return
     if args.warp_study:
         print(json.dumps(warp_study()), flush=True)
+        return
+    if args.warp_time_to_error:
+        print(json.dumps(warp_time_to_error()), flush=True)
         return
     if args.block_models_only:
         print(json.dumps({"two_orbital_step": two_orbital(with_cpu=not args.no_cpu),

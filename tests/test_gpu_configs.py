@@ -94,3 +94,33 @@ def test_c2_bold_step_order4(precision):
         assert err < RTOL and err4 < 1e-9
     else:
         assert 0 < err < 2e-6 and err4 < 2e-5
+
+
+def wide_blocks_one_bath(n_tau=12):
+    """Three-orbital Kanamori impurity (blocks 1, 6, 15, 20, 15, 6, 1) with a bath on one
+    spin-orbital only (2 pairs): order-4 diagrams stay cheap for the oracle while the
+    blocks go to the matrix-core walk (float32) and the LDS block walk (float64)."""
+    k3 = model.kanamori(3, 2.0, 0.2, -1.0, 10.0, n_tau, 0.5)
+    return model.Expansion(k3.ed, k3.grid, k3.pairs[:2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_wide_blocks_bold_step_order4(precision):
+    """k_eval_block (float64) and k_eval_mfma_t<4, 2, 2> (float32) at order 4."""
+    prec = abi.QIW_PREC_FP64 if precision == "fp64" else abi.QIW_PREC_FP32
+    O.set_threads(16)
+    ec = wide_blocks_one_bath()
+    P, _, _ = O.inchworm(ec, range(0, 3), range(0, 3), 2 ** 3, n_steps_max=4)
+    eg = wide_blocks_one_bath()
+    for e in (eg, ec):
+        e.P, e.P_n_known = P.copy(), e.grid.n_tau
+    print(f"wide blocks: bold step n_w=5, orders 0:4, N=2^3, {precision}", flush=True)
+    v_gpu, per_gpu = qiw.inchworm_step(eg, 5, range(0, 5), 2 ** 3, precision=prec)
+    v_cpu, per_cpu = O.inchworm_step(ec, range(0, 5), 2 ** 3, 5)
+    err, err4 = rel(v_gpu, v_cpu), np.abs(per_gpu[4] - per_cpu[4]).max() / np.abs(per_cpu[4]).max()
+    print(f"wide blocks order 4 ({precision}): rel err step {err:.2e}, order-4 part {err4:.2e}")
+    assert np.abs(per_cpu[4]).max() > 0
+    if precision == "fp64":
+        assert err < RTOL and err4 < 1e-9
+    else:
+        assert 0 < err < 2e-6 and err4 < 2e-5
