@@ -265,6 +265,38 @@ def warp_study(n_tau=50, orders=(0, 1, 2), ref_log2n=20, log2ns=tuple(range(10, 
             "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
 
 
+def judge_step(k, jw, orders, log2n, block_log2n, precision):
+    """The plain qMC step at N = 2^log2n as the mean of its 2^(log2n - block_log2n) blocks of
+    2^block_log2n consecutive Sobol points, one launch each.  Block b of a Sobol sequence is
+    its first 2^m points (m = block_log2n) digitally shifted: the point index b 2^m + r has
+    the Gray code gray(r) xor (gray(b) << m) xor ((b & 1) << (m - 1)), so the block's shift
+    is the xor of the direction numbers of the last two terms."""
+    import numpy as np
+    import qiw
+    from qiw.sobol import ScrambledSobolSeq
+    D = 2 * max(orders)
+    base = ScrambledSobolSeq(D)
+    m = block_log2n
+
+    class Shifted:
+        pass
+
+    acc = None
+    nb = 2 ** (log2n - m)
+    for b in range(nb):
+        bits = ((b ^ (b >> 1)) << m) ^ ((b & 1) << (m - 1))
+        shift = base.x0.copy()
+        for j in range(32):
+            if (bits >> j) & 1:
+                shift ^= base.m[:, j]
+        seq = Shifted()
+        seq.D, seq.m, seq.x0 = D, base.m, shift.astype(np.uint32)
+        v, _ = qiw.inchworm_step(k, jw, orders, 2 ** m, precision=precision, seq=seq)
+        acc = v if acc is None else acc + v
+        progress(f"judge block {b + 1} of {nb} (2^{m} points each)")
+    return acc / nb
+
+
 def warp_time_to_error(log2ns=tuple(range(10, 17)), judge_log2n=20, jw=100, iters=4000, n_train=2 ** 14,
                        p_log2n=8):
     """The metric's time-to-error leg on the learned-warp workload: the two-orbital model of
@@ -293,8 +325,7 @@ def warp_time_to_error(log2ns=tuple(range(10, 17)), judge_log2n=20, jw=100, iter
     progress(f"warp leg: trained ({train_s:.1f} s)")
     k.P = P.copy()
     qiw.warp_enable(False)
-    ref, _ = qiw.inchworm_step(k, jw, orders, 2 ** judge_log2n, precision=fp32)
-    progress(f"warp leg: judge at 2^{judge_log2n}")
+    ref = judge_step(k, jw, orders, judge_log2n, 16, fp32)
     res = {"plain": [], "warped": []}
     for ln in log2ns:
         for name, on in (("plain", False), ("warped", True)):
