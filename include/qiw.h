@@ -337,6 +337,7 @@ typedef struct {
     int64_t leaves;             /* (topology, sample, initial sector, pair assignment)
                                    leaves evaluated                                       */
     double flops;               /* algorithmic real FLOPs of the chain products           */
+    int64_t graph_replays;      /* bold-step loops replayed from a hipGraph               */
 } qiw_stats;
 int qiw_get_stats(qiw_ctx *ctx, qiw_stats *out);
 

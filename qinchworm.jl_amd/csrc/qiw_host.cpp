@@ -1228,7 +1228,10 @@ int ensure_model_buffers(qiw_ctx *c, double **dcoef, int32_t **dkind, double **e
     return 0;
 }
 
-// Launches the evaluator matching the model's sector structure.
+// Launches the evaluator matching the model's sector structure.  The vector and matrix-core
+// walks stage their operator tables once per model on their first launch (synchronous
+// copies), so a launch can be captured into a hipGraph only after that
+// (c->vec_staged / c->mfma_staged).
 int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precision) {
     const HostModel &m = c->m;
     const size_t lds_bytes = (size_t)a.lds_slots * sizeof(double);
@@ -1888,10 +1891,13 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
             fa.n_known = (m.ppgf_kind == QIW_GF_INC_SPLINE) ? n_known : m.ntau;
             fa.normalize = normalize ? 1 : 0;
         };
-        // flat scalar walk without the warp: the whole loop is one hipGraph, replayed while
-        // the launch arguments are unchanged (the other walks stage data from the host per
-        // launch and stay eager)
-        bool graph = c->graphs && bold_b.fmt == FMT_SCALAR && !warped && !c->no_alloc && !rq.given;
+        // without the warp the whole loop is one hipGraph, replayed while the launch arguments
+        // are unchanged: the flat scalar walk, and the vector and matrix-core walks once their
+        // tables are staged (from the second call on); the LDS block walk copies host data per
+        // launch and stays eager
+        const bool staged = bold_b.fmt == FMT_SCALAR ||
+                            (bold_b.fmt == FMT_VEC && (m.mmax > 6 ? c->mfma_staged : c->vec_staged));
+        bool graph = c->graphs && staged && !warped && !c->no_alloc && !rq.given;
         const int n_known_start = n_known;
         if (graph) {
             if (!c->graph) c->graph = new StepGraph;
@@ -1910,6 +1916,8 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
             put(&run->N_samples, 8);
             put(&m.ns, 4);
             put(&m.ntab, 4);
+            put(&bold_b.fmt, 4);
+            put(&bold_b.precision, 4);
             fa.n_real = (int)bold_b.r_first.size();  // as finalize() sets them for this batch
             fa.real_first = bold_d.r_first;
             fa.real_count = bold_d.r_count;
@@ -1931,7 +1939,10 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
                 for (int jw = first_w; jw <= last_w && err == hipSuccess; ++jw) {
                     step_args(jw);
                     if ((err = coords(bold_b, bold_d, QIW_RQ_BOLD, jw, nullptr)) != hipSuccess) break;  // never given
-                    if ((err = qiw_launch_eval_flat(c->stream, a, n_tasks, a.max_order, m.ns, m.ntab)) != hipSuccess) break;
+                    if (launch_eval(c, a, n_tasks, bold_b.fmt, bold_b.precision) != 0) {
+                        err = hipErrorUnknown;
+                        break;
+                    }
                     err = finalize(bold_b, bold_d);
                 }
                 hipGraph_t gr = nullptr;
@@ -1956,6 +1967,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
             StepGraph &G = *c->graph;
             HIPCHK(c, hipEventRecord(G.ev[0], c->stream));
             HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
+            c->stats.graph_replays += 1;
             HIPCHK(c, hipEventRecord(G.ev[1], c->stream));
             graph_ev.push_back({G.ev[0], G.ev[1]});
             n_known = n_known_start;

@@ -63,7 +63,7 @@ class qiw_tuning(C.Structure):
 class qiw_stats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("kernel_ms", C.c_double),
                 ("n_kernel_launches", C.c_int64), ("diagram_evals", C.c_int64),
-                ("leaves", C.c_int64), ("flops", C.c_double)]
+                ("leaves", C.c_int64), ("flops", C.c_double), ("graph_replays", C.c_int64)]
 
 
 def ptr(a, ctype):

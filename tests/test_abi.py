@@ -39,7 +39,7 @@ def test_struct_layout_matches_header():
     assert abi.qiw_run.N_samples.offset == 32 and C.sizeof(abi.qiw_run) == 56
     assert abi.qiw_run.rand_params.offset == 48
     assert abi.qiw_rand.seed.offset == 8 and C.sizeof(abi.qiw_rand) == 24
-    assert C.sizeof(abi.qiw_stats) == 48
+    assert C.sizeof(abi.qiw_stats) == 56 and abi.qiw_model.dlr_rank.offset == 104
     assert abi.qiw_warp_config.n_train.offset == 24 and C.sizeof(abi.qiw_warp_config) == 48
     assert abi.qiw_tuning.chunk_cost.offset == 8 and abi.qiw_tuning.best_ms.offset == 24
     assert C.sizeof(abi.qiw_tuning) == 32

@@ -5,7 +5,7 @@ import pytest
 
 import oracle_lib as O
 import qiw
-from qiw import model
+from qiw import abi, model
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-10
@@ -214,3 +214,30 @@ def test_real_nn_interaction_is_rejected():
     with pytest.raises(qiw.abi.QiwError) as err:
         qiw.inchworm(e, e.grid, range(0, 2), range(0, 2), 2 ** 4)
     assert err.value.code == qiw.abi.QIW_E_ARG
+
+
+@pytest.mark.parametrize("walk", ["vector", "matrix-core"])
+def test_graph_replay_of_block_walks(walk):
+    """The bold-step loop of the vector and matrix-core walks is replayed from a hipGraph
+    (from the second call on, once the walk's tables are staged); the replay gives the eager
+    loop's result bit for bit."""
+    if walk == "vector":
+        make = lambda: model.kanamori(2, 2.0, 0.2, -1.0, 10.0, 10, 0.5)  # noqa: E731
+        prec = abi.QIW_PREC_FP32
+    else:
+        def make():
+            k3 = model.kanamori(3, 2.0, 0.2, -1.0, 10.0, 8, 0.5)
+            return model.Expansion(k3.ed, k3.grid, k3.pairs[:2])
+        prec = abi.QIW_PREC_FP32
+    print(f"graphs: {walk} walk, whole inchworm!, orders 0:2", flush=True)
+    runs, replays = [], []
+    for graphs in (True, True, False):
+        qiw.set_graphs(graphs)
+        e = make()
+        qiw.inchworm(e, e.grid, range(0, 3), range(0, 3), 2 ** 5, precision=prec)
+        runs.append(e.P.copy())
+        replays.append(qiw.stats()["graph_replays"])
+    qiw.set_graphs(True)
+    assert replays[1] == 1 and replays[2] == 0
+    np.testing.assert_array_equal(runs[1], runs[2])
+    np.testing.assert_array_equal(runs[0], runs[2])
