@@ -267,6 +267,7 @@ struct MfmaArgs {
     const double *ediag;       // [bt] E + lambda on the diagonal positions (bare groups)
     const uint8_t *emask;      // [bt] 1 on the diagonal positions
     int bt;
+    int ops_zero;              // position of a 0.0 in ops (the first element past the blocks)
 };
 
 // ---- randomized qMC scrambles -------------------------------------------------------------

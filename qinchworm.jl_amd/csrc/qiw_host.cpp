@@ -1264,7 +1264,8 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
             c->mfma_staged = true;
         }
         const int KM = a.max_order <= 3 ? 3 : 4, NM = 2 * KM + 3;
-        const size_t lds = ((((size_t)(NM - 1) * m.bt + 3) & ~(size_t)3) + 32 * 34) * 4;  // + scratch, kScrLd 34
+        // i*P blocks, the 0.0 read by lanes outside a block, and the Y stack of the walk
+        const size_t lds = ((((size_t)(NM - 1) * m.bt + 4) & ~(size_t)3) + (size_t)(NM - 2) * m.mmax * 16) * 4;
         if (lds > 160 * 1024) return fail(c, QIW_E_LIMIT, "propagator blocks exceed the 160 KiB of LDS of a CU");
         MfmaArgs mf{};
         mf.e = a;
@@ -1272,6 +1273,7 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
         mf.ediag = ediag;
         mf.emask = emask;
         mf.bt = m.bt;
+        mf.ops_zero = (int)m.op_data.size();
         HIPCHK(c, qiw_launch_eval_mfma(c->stream, mf, n_tasks, a.max_order, m.ntab, lds));
         return 0;
     }

@@ -1188,130 +1188,209 @@ __global__ __launch_bounds__(QIW_WAVE, QIW_VEC_MINW) void k_eval_vec_t(VecArgs v
 //
 // One wavefront evaluates one sample at a time (all lanes share the sample's times, so
 // propagators and arc values are uniform).  Partial products X (m_L x m_s0, all columns at
-// once) are kept as 16x16 tiles in the accumulator layout of v_mfma_f32_16x16x4f32 (lane l,
-// element r of tile (ti, tj): row 16 ti + 4 (l / 16) + r, column 16 tj + l % 16); a product
-// C = A X streams A (an i*P block from LDS or an operator block from global memory) as the
-// MFMA A operand, one column-major 16x4 slice per instruction, and passes X through a
-// 32x32 LDS scratch that turns the accumulator layout into B-operand slices.  The walk
-// follows the vector programs (one segment per initial sector) with all columns of s0 at
-// once; products of arc values travel as a float64 weight, leaves accumulate into float64
-// accumulators, and the last propagator iP_n[s0] is applied once per segment.
-// float32 products (the QIW_PREC_FP32 mode); sums over samples in float64.
+// once) stay in registers as 16x16 tiles of v_mfma_f32_16x16x4f32 accumulators.  The
+// instruction's D operand puts hardware row 4 (l / 16) + r of column l % 16 in element r of
+// lane l, and its B operand takes row l / 16 of a 4-row slice from lane l.  The rows of
+// every tile are stored transposed within each 4x4 group: element r of lane l holds logical
+// row 16 ti + 4 r + l / 16.  Then element r of every lane is, as it stands, the B-operand
+// slice of logical rows 16 tk + 4 r .. + 3, so a product C = A X feeds X to the next
+// product without moving it; A (an i*P block from LDS or an operator block from global
+// memory, column-major) is read in the matching order: lane l loads row rho(l % 16) of the
+// tile, rho(4 a + b) = 4 b + a, at column 16 tk + 4 r + l / 16, which puts logical row
+// 16 ti + 4 r + q into hardware row 4 q + r of C, the same stored order again.  A product
+// over K rows takes ceil(K / 4) slices per tile pair.  The walk follows the vector
+// programs (one segment per initial sector) with all columns of s0 at once; products of arc
+// values travel as a float64 weight, leaves accumulate into float64 accumulators, and the
+// last propagator iP_n[s0] is applied once per segment.  float32 products (the
+// QIW_PREC_FP32 mode); sums over samples in float64.
 // ---------------------------------------------------------------------------------------
 typedef float mf4 __attribute__((ext_vector_type(4)));
-typedef float mf2 __attribute__((ext_vector_type(2)));
 
-// Leading dimension of the 32-column scratch.  34 = 2 (mod 32) spreads both accesses over
-// the 32 LDS banks: a B-operand read (lane (q, c) at column 16 tj + c, row 4 kc + q) hits
-// bank 2c + q, distinct within each half wave, and a tile store, two 8-byte writes per lane,
-// covers banks 2c, 2c + 1 in each 16-lane group.  (A stride of 32 put all 16 columns of a
-// read in one bank: 8 conflict cycles per access, 45 % of the walk's time.)
-constexpr int kScrLd = 34;
-
-template <int TB>
+// A wavefront walks the columns of s0 in passes of 16 TC (one pass for blocks up to 16
+// wide); rows take TR tiles.
+template <int TR, int TC>
 struct MTile {
-    mf4 t[TB][TB];
+    mf4 t[TR][TC];
 };
 
-template <int KM, int TM, int TB>
+template <int KM, int TM, int TR, int TC>
 struct MState {
-    typename RegVec<double, regvec_size(KM * TM)>::type W;  // arc values of the sample
-    double acc[TB][TB][4];                                  // leaf sums, accumulator layout
+    double acc[TR][TC][4];          // leaf sums, in the tiles' order
     ProgStream ps;
     const float *__restrict__ pl;   // i*P blocks of the sample in LDS: [(L - 2) * bt + BT offset]
-    float *__restrict__ scr;        // 32-column column-major scratch, leading dimension kScrLd
     const float *__restrict__ ops;  // operator blocks, column-major
+    const double *__restrict__ sw;  // arc values of the sample's topology, in LDS
     int bt;
+    int pl_zero, ops_zero;          // positions of a 0.0 in pl and in ops
 };
 
-// C = A X for A (m_a x K, column-major, leading dimension lda) and X (K x n).
-template <int TB>
-__device__ __forceinline__ MTile<TB> mmul(const float *__restrict__ A, int lda, int m_a, int K, const MTile<TB> &X,
-                                          int n, float *__restrict__ scr) {
+// Logical row (within its tile) of element r of lane l.
+__device__ __forceinline__ int mrow(int l, int r) { return 4 * r + (l >> 4); }
+
+// C = A X for A (m_a x K, column-major, leading dimension lda, at base + off) and X (K x n),
+// both tiles in the stored row order.  All A elements of the product are loaded before the
+// first MFMA, so their latencies overlap (one wait per product instead of one per slice);
+// lanes outside A read the 0.0 at base + zpos.
+template <int TR, int TC>
+__device__ __forceinline__ MTile<TR, TC> mmul(const float *__restrict__ base, int off, int lda, int m_a, int K,
+                                              const MTile<TR, TC> &X, int n, int zpos) {
     const int l = threadIdx.x, q = l >> 4, c = l & 15;
+    const int hr = 4 * (c & 3) + (c >> 2);  // rho(c): the logical row this lane feeds
+    const int mt = (m_a + 15) >> 4, nt = (n + 15) >> 4;
+    float av[TR][TR][4];  // [tk][ti][r]: A(16 ti + hr, 16 tk + 4 r + q)
 #pragma unroll
-    for (int ti = 0; ti < TB; ++ti)
+    for (int tk = 0; tk < TR; ++tk)
 #pragma unroll
-        for (int tj = 0; tj < TB; ++tj) {
-            float *p = &scr[(16 * tj + c) * kScrLd + 16 * ti + 4 * q];
-            *(mf2 *)p = mf2{X.t[ti][tj][0], X.t[ti][tj][1]};
-            *(mf2 *)(p + 2) = mf2{X.t[ti][tj][2], X.t[ti][tj][3]};
-        }
-    MTile<TB> C;
+        for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-    for (int ti = 0; ti < TB; ++ti)
+            for (int r = 0; r < 4; ++r) av[tk][ti][r] = 0.f;
 #pragma unroll
-        for (int tj = 0; tj < TB; ++tj) C.t[ti][tj] = mf4{0.f, 0.f, 0.f, 0.f};
-    const int mt = (m_a + 15) >> 4, nt = (n + 15) >> 4, nk = (K + 3) >> 2;
+    for (int tk = 0; tk < TR; ++tk) {
+        if (16 * tk < K) {
 #pragma unroll
-    for (int kc = 0; kc < 4 * TB; ++kc) {
-        if (kc < nk) {
-            const int k = 4 * kc + q;
-#pragma unroll
-            for (int ti = 0; ti < TB; ++ti) {
+            for (int ti = 0; ti < TR; ++ti) {
                 if (ti < mt) {
-                    const int row = 16 * ti + c;
-                    const float a = (k < K && row < m_a) ? A[k * lda + row] : 0.f;
 #pragma unroll
-                    for (int tj = 0; tj < TB; ++tj) {
-                        if (tj < nt) {
-                            const float b = scr[(16 * tj + c) * kScrLd + k];
-                            C.t[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, C.t[ti][tj], 0, 0, 0);
-                        }
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = 16 * tk + 4 * r + q, row = 16 * ti + hr;
+                        av[tk][ti][r] = base[(unsigned)(k < K && row < m_a ? off + k * lda + row : zpos)];
                     }
                 }
             }
         }
     }
+    MTile<TR, TC> C;
+#pragma unroll
+    for (int ti = 0; ti < TR; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TC; ++tj) C.t[ti][tj] = mf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tk = 0; tk < TR; ++tk)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (16 * tk + 4 * r < K) {
+#pragma unroll
+                for (int ti = 0; ti < TR; ++ti) {
+                    if (ti < mt) {
+#pragma unroll
+                        for (int tj = 0; tj < TC; ++tj)
+                            if (TC == 1 || tj < nt)
+                                C.t[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tk][ti][r], X.t[tk][tj][r],
+                                                                                   C.t[ti][tj], 0, 0, 0);
+                    }
+                }
+            }
+        }
     return C;
 }
 
-template <int L, int KM, int TM, int TB>
-__device__ __forceinline__ void mwalk(MState<KM, TM, TB> &st, int n, const MTile<TB> &Xp, double wp, int mp, int offp,
-                                      int nchild, int m0) {
-    if constexpr (L <= 2 * KM + 3) {
-        const MTile<TB> Y = (L == n) ? Xp : mmul<TB>(st.pl + (L - 2) * st.bt + offp, mp, mp, mp, Xp, m0, st.scr);
-        for (int c = 0; c < nchild; ++c) {
-            const uint64_t w = st.ps.next();
-            const double wv = (w & QIW_F_TAIL) ? wp * st.W[(int)((w >> 16) & 0xFF)] : wp;
-            const int mo = (int)(w & 0x3F);
-            const int blk = (int)((w >> 32) & 0xFFFF);
-            const MTile<TB> X = (blk == 0xFFFF) ? Y : mmul<TB>(st.ops + blk, mo, mo, mp, Y, m0, st.scr);
-            if (w & QIW_F_LEAF) {
-                const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -wv : wv)
-                                                    : __longlong_as_double((long long)st.ps.next()) * wv;
+// One root of the trie (a ROOT word with X0 = O_1 or the identity on s0), walked
+// depth-first without recursion.  Y (= iP_L X of the parent, shared by a node's children)
+// of the current level stays in registers; when the walk descends from a level that has
+// children left, that level's Y goes to an LDS stack frame (row-major, mp rows of 16
+// columns) and its uniform state (children left, rows, weight, frame offset) to small LDS
+// arrays, and both come back when the walk returns to it.  A level whose last child is
+// being walked saves nothing, and the walk returns past it.  The loop carries only the
+// current level in registers.
+template <int KM, int TM, int TR>
+__device__ __forceinline__ void mwalk(MState<KM, TM, TR, 1> &st, int n, const MTile<TR, 1> &X0, int m1, int off1,
+                                      int nch1, int nc, int *__restrict__ s_rm, double *__restrict__ s_wp,
+                                      int *__restrict__ s_yo, float *__restrict__ ystk) {
+    const int l = threadIdx.x, q = l >> 4, c = l & 15;
+    int L = 2, mp = m1, rem = nch1, yoff = 0;
+    double wp = 1.0;
+    MTile<TR, 1> Y = (n == 2) ? X0 : mmul<TR, 1>(st.pl, off1, m1, m1, m1, X0, nc, st.pl_zero);
+    for (;;) {
+        if (rem == 0) {  // back to the nearest level above with children left
+            int v = 0;
+            do {
+                if (L == 2) return;
+                --L;
+                v = s_rm[L];
+            } while ((v & 0xFFFF) == 0);
+            rem = v & 0xFFFF;
+            mp = v >> 16;
+            wp = s_wp[L];
+            yoff = s_yo[L];
+            const float *fr = ystk + yoff;
 #pragma unroll
-                for (int ti = 0; ti < TB; ++ti)
+            for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TB; ++tj)
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * ti + 4 * r + q;
+                    const float y = fr[(row < mp ? row : 0) * 16 + c];
+                    Y.t[ti][0][r] = row < mp ? y : 0.f;
+                }
+            continue;
+        }
+        --rem;
+        const uint64_t w = st.ps.next();
+        const double wv = (w & QIW_F_TAIL) ? wp * st.sw[(int)((w >> 16) & 0xFF)] : wp;
+        const int mo = (int)(w & 0x3F);
+        const int blk = (int)((w >> 32) & 0xFFFF);
+        const MTile<TR, 1> X = (blk == 0xFFFF) ? Y : mmul<TR, 1>(st.ops, blk, mo, mo, mp, Y, nc, st.ops_zero);
+        if (w & QIW_F_LEAF) {
+            const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -wv : wv)
+                                                : __longlong_as_double((long long)st.ps.next()) * wv;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) st.acc[ti][tj][r] = fma(cw, (double)X.t[ti][tj][r], st.acc[ti][tj][r]);
-            } else if (L < n) {
-                mwalk<L + 1, KM, TM, TB>(st, n, X, wv, mo, (int)(w >> 52), (int)((w >> 24) & 0xFF), m0);
+            for (int ti = 0; ti < TR; ++ti)
+                if (ti == 0 || 16 * ti < mo)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) st.acc[ti][0][r] = fma(cw, (double)X.t[ti][0][r], st.acc[ti][0][r]);
+        } else if (L < n) {  // an inner node: its children follow
+            if (rem > 0) {
+                float *fr = ystk + yoff;
+#pragma unroll
+                for (int ti = 0; ti < TR; ++ti)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * ti + 4 * r + q;
+                        if (row < mp) fr[row * 16 + c] = Y.t[ti][0][r];
+                    }
+                s_rm[L] = rem | (mp << 16);
+                s_wp[L] = wp;
+                s_yo[L] = yoff;
+                yoff += mp * 16;
+            } else {
+                s_rm[L] = 0;
             }
+            ++L;
+            wp = wv;
+            mp = mo;
+            rem = (int)((w >> 24) & 0xFF);
+            Y = (L == n) ? X : mmul<TR, 1>(st.pl, (L - 2) * st.bt + (int)(w >> 52), mo, mo, mo, X, nc, st.pl_zero);
         }
     }
 }
 
-template <int KM, int TM, int TB>
+template <int KM, int TM, int TR, int TC>
 __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
     constexpr int NM = 2 * KM + 3;
-    extern __shared__ float mlds[];  // [(NM - 1) * bt] i*P blocks, then the 32 x kScrLd scratch
+    static_assert(TC == 1, "the Y stack frames hold 16 columns");
+    extern __shared__ float mlds[];  // [(NM - 1) * bt] i*P blocks
+    __shared__ int s_rm[16];        // per level of the walk: children left | rows << 16
+    __shared__ double s_wp[16];     // per level: the weight of the arc values above
+    __shared__ int s_yo[16];        // per level: offset of its Y frame in the LDS stack
+    __shared__ double s_t[16];      // the times of the configuration positions 0..NM
+    __shared__ double s_w[16];      // the arc values of the current topology
     const EvalArgs &a = ma.e;
-    const int lane = threadIdx.x, q = lane >> 4, cl = lane & 15;
+    const int lane = threadIdx.x, cl = lane & 15;
     const QiwTask task = a.tasks[blockIdx.x];
     const QiwGroup g = a.groups[task.group];
     const int n = g.n_nodes, k = g.order, bt = ma.bt;
     const int ntab = a.dtab.n_tables;
     const double inv_h = 1.0 / a.h;
     float *pl = mlds;
-    MState<KM, TM, TB> st;
+    MState<KM, TM, TR, TC> st;
     st.pl = pl;
-    st.scr = mlds + (((NM - 1) * bt + 3) & ~3);
     st.ops = ma.ops;
+    st.sw = s_w;
     st.bt = bt;
-#pragma unroll
-    for (int e = 0; e < regvec_size(KM * TM); ++e) st.W[e] = 0.0;
+    st.pl_zero = (NM - 1) * bt;
+    st.ops_zero = ma.ops_zero;
+    if (lane == 0) pl[st.pl_zero] = 0.f;
+    float *ystk = pl + ((st.pl_zero + 4) & ~3);  // the Y stack: (NM - 2) frames of up to 32 x 16
+    for (int e = lane; e < 16; e += QIW_WAVE) s_w[e] = 0.0;
 
     double tau_i = 0.0, tau_w, tau_f;
     if (g.kind == QIW_G_BARE) {
@@ -1359,6 +1438,8 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
             }
         }
         const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (int64_t)(g.d_before + g.d_after) * g.n_samples + sample] : 1.0;
+#pragma unroll
+        for (int p = 0; p <= NM; ++p) s_t[p] = t[p];
         // ---- i*P blocks of every segment into LDS (lanes over BT elements) ----
         for (int L = 2; L <= n; ++L) {
             double dt = pick(t, L) - pick(t, L - 1);
@@ -1375,22 +1456,24 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
         __syncthreads();
         // ---- walk the program: one segment per initial sector ----
         double mag = 0.0;
-        int m0 = 1, off0 = 0;
+        int m0 = 1, off0 = 0, col0 = 0;
         st.ps.init(a.prog + g.prog_off + task.chunk, lane);
+        const uint64_t *seg_base = st.ps.base;  // the stream just after the segment's SEG word
+        int seg_idx = 0;
         for (;;) {
             const uint64_t w = st.ps.next();
             const int op = (int)((w >> 48) & 0xF);
             if (op == QIW_OP_ROOT) {
                 const int m1 = (int)(w & 0x3F);
                 const int blk = (int)((w >> 32) & 0xFFFF);
-                MTile<TB> X;
+                MTile<TR, TC> X;
 #pragma unroll
-                for (int ti = 0; ti < TB; ++ti)
+                for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TB; ++tj)
+                    for (int tj = 0; tj < TC; ++tj)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int row = 16 * ti + 4 * q + r, col = 16 * tj + cl;
+                            const int row = 16 * ti + mrow(lane, r), col = col0 + 16 * tj + cl;
                             float v = 0.f;
                             if (blk == 0xFFFF)
                                 v = (row == col && row < m0) ? 1.f : 0.f;
@@ -1398,53 +1481,63 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
                                 v = st.ops[blk + col * m1 + row];
                             X.t[ti][tj][r] = v;
                         }
-                mwalk<2, KM, TM, TB>(st, n, X, 1.0, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF), m0);
+                mwalk<KM, TM, TR>(st, n, X, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF), m0 - col0, s_rm, s_wp, s_yo,
+                                  ystk);
             } else if (op == QIW_OP_TOPO) {
 #pragma unroll
                 for (int ai = 0; ai < KM; ++ai) {
                     const int head = (int)((w >> (8 * ai)) & 0xF), tail = (int)((w >> (8 * ai + 4)) & 0xF);
-                    double th = 0.0, tt = 0.0;
-#pragma unroll
-                    for (int p = 1; p <= NM; ++p) {
-                        th = (p == head) ? t[p] : th;
-                        tt = (p == tail) ? t[p] : tt;
-                    }
-                    const ArcPos ap = arc_pos(tt, th, inv_h, a.n_tau);
+                    const ArcPos ap = arc_pos(s_t[tail], s_t[head], inv_h, a.n_tau);
 #pragma unroll
                     for (int tb = 0; tb < TM; ++tb)
-                        if (ai < k && tb < ntab) st.W[ai * TM + tb] = delta_value(a.dtab, tb, ap);
+                        if (ai < k && tb < ntab) s_w[ai * TM + tb] = delta_value(a.dtab, tb, ap);
                 }
             } else if (op == QIW_OP_SEG) {
                 m0 = (int)(w & 0x3F);
                 off0 = (int)(w >> 52);
+                col0 = 0;
+                seg_base = st.ps.base;
+                seg_idx = st.ps.idx;
 #pragma unroll
-                for (int ti = 0; ti < TB; ++ti)
+                for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TB; ++tj)
+                    for (int tj = 0; tj < TC; ++tj)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) st.acc[ti][tj][r] = 0.0;
             } else if (op == QIW_OP_FLUSH) {  // the s0 block of this sample: iP_n[s0] times the leaf sum
-                MTile<TB> S;
+                MTile<TR, TC> S;
 #pragma unroll
-                for (int ti = 0; ti < TB; ++ti)
+                for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TB; ++tj)
+                    for (int tj = 0; tj < TC; ++tj)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) S.t[ti][tj][r] = (float)st.acc[ti][tj][r];
-                const MTile<TB> R = mmul<TB>(pl + (n - 2) * bt + off0, m0, m0, m0, S, m0, st.scr);
+                const MTile<TR, TC> R = mmul<TR, TC>(pl, (n - 2) * bt + off0, m0, m0, m0, S, m0 - col0, st.pl_zero);
 #pragma unroll
-                for (int ti = 0; ti < TB; ++ti)
+                for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TB; ++tj)
+                    for (int tj = 0; tj < TC; ++tj)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int row = 16 * ti + 4 * q + r, col = 16 * tj + cl;
+                            const int row = 16 * ti + mrow(lane, r), col = col0 + 16 * tj + cl;
                             if (row < m0 && col < m0) {
                                 const double v = (double)R.t[ti][tj][r];
                                 prow[off0 + row + col * m0] += wt * v;
                                 mag += fabs(v);
                             }
                         }
+                if (col0 + 16 * TC < m0) {  // the next columns of s0: walk the segment again
+                    col0 += 16 * TC;
+                    st.ps.base = seg_base;
+                    st.ps.blk = seg_base[lane];
+                    st.ps.idx = seg_idx;
+#pragma unroll
+                    for (int ti = 0; ti < TR; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < TC; ++tj)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) st.acc[ti][tj][r] = 0.0;
+                }
             } else if (op == QIW_OP_END) {
                 break;
             }
@@ -2064,9 +2157,9 @@ hipError_t qiw_launch_ptab_cells(hipStream_t st, const double *T, int n_tau, int
 hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes) {
     if (tm > 2) return hipErrorInvalidValue;
     if (km <= 3)
-        hipLaunchKernelGGL((k_eval_mfma_t<3, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+        hipLaunchKernelGGL((k_eval_mfma_t<3, 2, 2, 1>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
     else if (km <= 4)
-        hipLaunchKernelGGL((k_eval_mfma_t<4, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+        hipLaunchKernelGGL((k_eval_mfma_t<4, 2, 2, 1>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
