@@ -55,6 +55,14 @@ struct HostModel {
     int mmax = 1;
     std::vector<double> op_data;    // all operator blocks, column-major, concatenated
     std::vector<int32_t> op_desc;   // [n_blocks][3]: offset, m_f, m_i
+    // Layout of the matrix-core walk (blocks wider than 6): operator blocks row-major with
+    // rows padded to a multiple of 4 floats (offsets 16-byte aligned, four zeros past the
+    // last block), and the i*P blocks of one segment in LDS the same way.
+    std::vector<float> ops_rm;
+    std::vector<int32_t> op_rm_off;  // [n_blocks]
+    std::vector<int32_t> lds_off;    // [ns] offset of each sector's block within a segment
+    std::vector<int32_t> pinv;       // [btp] bt element at each LDS position, -1 for padding
+    int btp = 0;
 };
 
 

@@ -263,11 +263,14 @@ struct VecArgs {
 // Arguments of the matrix-core evaluator (k_eval_mfma_t)
 struct MfmaArgs {
     EvalArgs e;
-    const float *ops;          // operator blocks, column-major, float32
+    const float *ops;          // operator blocks, row-major with rows padded to 4 floats, float32
     const double *ediag;       // [bt] E + lambda on the diagonal positions (bare groups)
     const uint8_t *emask;      // [bt] 1 on the diagonal positions
-    int bt;
-    int ops_zero;              // position of a 0.0 in ops (the first element past the blocks)
+    const int32_t *pinv;       // [btp] element of the bt layout at each LDS position of a
+                               // segment's i*P blocks (row-major, padded rows), -1 for padding
+    const int32_t *bto;        // [ns] offset of each sector's block in the bt layout
+    int bt, btp;
+    int ops_zero;              // position of four 0.0 in ops (16-byte aligned, past the blocks)
 };
 
 // ---- randomized qMC scrambles -------------------------------------------------------------

@@ -350,8 +350,7 @@ struct Compiler {
         uint64_t w;
         if (fmt == FMT_VEC) {
             w = (uint64_t)m.dims[t.s_out] | ((uint64_t)t.wslot << 16) |
-                ((uint64_t)t.children.size() << 24) | (op_offset(t.blk) << 32) |
-                ((uint64_t)m.bt_off[t.s_out] << 52);
+                ((uint64_t)t.children.size() << 24) | (op_offset(t.blk) << 32) | (blk_off(t.s_out) << 52);
         } else {
             w = (uint64_t)t.level | ((uint64_t)t.s_in << 8) | ((uint64_t)t.wslot << 16) |
                 ((uint64_t)(t.blk & 0xFFFF) << 24) | ((uint64_t)t.s_out << 40);
@@ -427,7 +426,14 @@ struct Compiler {
     }
 
     // offset of operator block blk in the operator data (0xFFFF: identity)
-    uint64_t op_offset(int blk) const { return blk == 0xFFFF ? 0xFFFF : (uint64_t)m.op_desc[3 * blk]; }
+    uint64_t op_offset(int blk) const {
+        if (blk == 0xFFFF) return 0xFFFF;
+        return wide() ? (uint64_t)m.op_rm_off[blk] : (uint64_t)m.op_desc[3 * blk];
+    }
+    // vector programs of models with blocks wider than 6 run on the matrix-core walk, which
+    // reads the row-major operator blocks and its own LDS layout of the i*P blocks
+    bool wide() const { return fmt == FMT_VEC && m.mmax > 6; }
+    uint64_t blk_off(int s) const { return wide() ? (uint64_t)m.lds_off[s] : (uint64_t)m.bt_off[s]; }
 
     bool compile(Compiled &out) {
         n = (kind == QIW_G_BARE) ? 2 * order + 2 : 2 * order + 3;
@@ -517,7 +523,7 @@ struct Compiler {
                     if (order > 0) out.prog.push_back(topo_word);
                     out.prog.push_back(((uint64_t)QIW_OP_ROOT << 48) | (uint64_t)m.dims[s] |
                                        ((uint64_t)trie[0].children.size() << 24) | (op_offset(root_blk) << 32) |
-                                       ((uint64_t)m.bt_off[s] << 52));
+                                       (blk_off(s) << 52));
                     // the root's i*P (level 2) for its children
                     if (n > 2) out.flops_per_sample += 2.0 * m.dims[s] * m.dims[s] * m.dims[s0];
                     for (size_t i = 0; i < trie[0].children.size(); ++i) emit(trie[0].children[i], out);
@@ -539,7 +545,7 @@ struct Compiler {
             auto put_chunk = [&](size_t t0, size_t t1) {
                 for (int s = 0; s < m.ns; ++s) {
                     out.prog.push_back(((uint64_t)QIW_OP_SEG << 48) | (uint64_t)m.dims[s] | ((uint64_t)s << 8) |
-                                       ((uint64_t)m.bt_off[s] << 52));
+                                       (blk_off(s) << 52));
                     for (size_t t = t0; t < t1; ++t) out.prog.insert(out.prog.end(), vtop[t][s].begin(), vtop[t][s].end());
                     out.prog.push_back((uint64_t)QIW_OP_FLUSH << 48);
                 }
@@ -978,6 +984,66 @@ void order_tasks(Batch &b) {
         xcd_interleave(b.tasks);
 }
 
+// The matrix-core walk's layout of the operator blocks and of the i*P blocks of a segment:
+// row-major, rows padded to a multiple of 4 floats, every block 16-byte aligned, so a lane
+// reads four consecutive elements of a row with one load.
+static void wide_layout(HostModel &m) {
+    const size_t nb = m.op_desc.size() / 3;
+    m.op_rm_off.assign(nb, 0);
+    m.ops_rm.clear();
+    for (size_t b = 0; b < nb; ++b) {
+        const int off = m.op_desc[3 * b], mf = m.op_desc[3 * b + 1], mi = m.op_desc[3 * b + 2];
+        const int ldr = (mi + 3) & ~3;
+        m.op_rm_off[b] = (int32_t)m.ops_rm.size();
+        m.ops_rm.resize(m.ops_rm.size() + (size_t)mf * ldr, 0.f);
+        for (int i = 0; i < mf; ++i)
+            for (int j = 0; j < mi; ++j) m.ops_rm[m.op_rm_off[b] + (size_t)i * ldr + j] = (float)m.op_data[off + i + j * mf];
+    }
+    m.ops_rm.resize(m.ops_rm.size() + 4, 0.f);  // the four zeros read by lanes outside a block
+    m.lds_off.assign(m.ns, 0);
+    m.btp = 0;
+    for (int s = 0; s < m.ns; ++s) {
+        m.lds_off[s] = m.btp;
+        m.btp += m.dims[s] * ((m.dims[s] + 3) & ~3);
+    }
+    m.pinv.assign(m.btp, -1);
+    for (int s = 0; s < m.ns; ++s) {
+        const int d = m.dims[s], ldr = (d + 3) & ~3;
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) m.pinv[m.lds_off[s] + i * ldr + j] = m.bt_off[s] + i + j * d;
+    }
+}
+
+// Rows the Y stack of the matrix-core walk can hold at once: the largest sum of block sizes
+// over `levels` consecutive levels of a path.  Consecutive levels are joined by an operator
+// (of a pair, or of a correlator) or, at most once per path, by the identity node at tau_w.
+static int ystack_rows(const HostModel &m, int levels) {
+    if (levels <= 0) return 0;
+    std::vector<std::vector<int>> nb(m.ns);  // nb[s]: sectors an operator takes to s
+    for (const std::vector<ScalarOp> *ops : {&m.op_i, &m.op_f, &m.corr_A, &m.corr_B})
+        for (const ScalarOp &o : *ops)
+            for (int s = 0; s < m.ns && s < (int)o.sf.size(); ++s)
+                if (o.sf[s] >= 0) nb[o.sf[s]].push_back(s);
+    // best[u][s]: largest sum over paths ending in sector s, u = identity node used
+    std::vector<int> b0(m.dims.begin(), m.dims.end()), b1(m.ns, -1), n0(m.ns), n1(m.ns);
+    for (int l = 1; l < levels; ++l) {
+        for (int s = 0; s < m.ns; ++s) {
+            int x0 = -1, x1 = b0[s];
+            for (int p : nb[s]) {
+                x0 = std::max(x0, b0[p]);
+                x1 = std::max(x1, b1[p]);
+            }
+            n0[s] = x0 < 0 ? -1 : m.dims[s] + x0;
+            n1[s] = x1 < 0 ? -1 : m.dims[s] + x1;
+        }
+        b0.swap(n0);
+        b1.swap(n1);
+    }
+    int r = 0;
+    for (int s = 0; s < m.ns; ++s) r = std::max(r, std::max(b0[s], b1[s]));
+    return r;
+}
+
 // Samples per task.  The scalar and vector walks map one sample to a lane (64 per task);
 // the block and matrix-core walks evaluate one sample at a time per wavefront, so their
 // tasks are cut smaller until a launch holds at least 4 tasks per SIMD of the device.
@@ -1243,13 +1309,15 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
     if (fmt == FMT_VEC && m.mmax > 6) {  // wide blocks: the matrix-core walk
         int rc = 0;
         const size_t pad = 64;
-        float *ops32 = (float *)dev(c, "mfma.ops32", (m.op_data.size() + pad) * 4, &rc);
+        float *ops32 = (float *)dev(c, "mfma.ops32", (m.ops_rm.size() + pad) * 4, &rc);
         double *ediag = (double *)dev(c, "mfma.ediag", ((size_t)m.bt + pad) * 8, &rc);
         uint8_t *emask = (uint8_t *)dev(c, "mfma.emask", (size_t)m.bt + pad, &rc);
+        int32_t *pinv = (int32_t *)dev(c, "mfma.pinv", ((size_t)m.btp + pad) * 4, &rc);
+        int32_t *bto = (int32_t *)dev(c, "mfma.bto", ((size_t)m.ns + pad) * 4, &rc);
         if (rc) return rc;
         if (!c->mfma_staged) {  // once per model
-            std::vector<float> o32(m.op_data.begin(), m.op_data.end());
-            o32.resize(m.op_data.size() + pad, 0.0f);
+            std::vector<float> o32(m.ops_rm.begin(), m.ops_rm.end());
+            o32.resize(m.ops_rm.size() + pad, 0.0f);
             std::vector<double> ed((size_t)m.bt + pad, 0.0);
             std::vector<uint8_t> mk((size_t)m.bt + pad, 0);
             for (int s = 0; s < m.ns; ++s)
@@ -1261,19 +1329,25 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
             HIPCHK(c, hipMemcpy(ops32, o32.data(), o32.size() * 4, hipMemcpyHostToDevice));
             HIPCHK(c, hipMemcpy(ediag, ed.data(), ed.size() * 8, hipMemcpyHostToDevice));
             HIPCHK(c, hipMemcpy(emask, mk.data(), mk.size(), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(pinv, m.pinv.data(), m.pinv.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(bto, m.bt_off.data(), m.bt_off.size() * 4, hipMemcpyHostToDevice));
             c->mfma_staged = true;
         }
         const int KM = a.max_order <= 3 ? 3 : 4, NM = 2 * KM + 3;
-        // i*P blocks, the 0.0 read by lanes outside a block, and the Y stack of the walk
-        const size_t lds = ((((size_t)(NM - 1) * m.bt + 4) & ~(size_t)3) + (size_t)(NM - 2) * m.mmax * 16) * 4;
+        // i*P blocks, the four 0.0 read by lanes outside a block, and the Y stack of the walk
+        const size_t lds =
+            ((((size_t)(NM - 1) * m.btp + 3) & ~(size_t)3) + 4 + (size_t)ystack_rows(m, NM - 2) * 16) * 4;
         if (lds > 160 * 1024) return fail(c, QIW_E_LIMIT, "propagator blocks exceed the 160 KiB of LDS of a CU");
         MfmaArgs mf{};
         mf.e = a;
         mf.ops = ops32;
         mf.ediag = ediag;
         mf.emask = emask;
+        mf.pinv = pinv;
+        mf.bto = bto;
         mf.bt = m.bt;
-        mf.ops_zero = (int)m.op_data.size();
+        mf.btp = m.btp;
+        mf.ops_zero = (int)m.ops_rm.size() - 4;
         HIPCHK(c, qiw_launch_eval_mfma(c->stream, mf, n_tasks, a.max_order, m.ntab, lds));
         return 0;
     }
@@ -1618,8 +1692,9 @@ static int qiw_set_model_impl(qiw_ctx *c, const qiw_model *mi) {
     if (m.op_data.empty()) m.op_data.push_back(0.0);
     m.vec_ok = !m.scalar_kernel && m.mmax <= 6 && m.ntab <= 4 && m.ns <= 255 && m.bt < 4096 &&
                m.op_data.size() < 0xFFFF - 64;
+    if (m.mmax > 6) wide_layout(m);
     m.mfma_ok = !m.scalar_kernel && m.mmax <= 32 && m.ntab <= 2 && m.ns <= 255 && m.bt < 4096 &&
-                m.op_data.size() < 0xFFFF - 64;
+                m.op_data.size() < 0xFFFF - 64 && m.ops_rm.size() < 0xFFFF && m.btp < 4096;
     c->m = m;
     c->has_model = true;
     std::string fp = model_fingerprint(m);

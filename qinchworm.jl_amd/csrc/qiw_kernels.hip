@@ -1187,98 +1187,77 @@ __global__ __launch_bounds__(QIW_WAVE, QIW_VEC_MINW) void k_eval_vec_t(VecArgs v
 // The diagram evaluator for wide sector blocks (up to 32) on the matrix cores.
 //
 // One wavefront evaluates one sample at a time (all lanes share the sample's times, so
-// propagators and arc values are uniform).  Partial products X (m_L x m_s0, all columns at
-// once) stay in registers as 16x16 tiles of v_mfma_f32_16x16x4f32 accumulators.  The
-// instruction's D operand puts hardware row 4 (l / 16) + r of column l % 16 in element r of
-// lane l, and its B operand takes row l / 16 of a 4-row slice from lane l.  The rows of
-// every tile are stored transposed within each 4x4 group: element r of lane l holds logical
-// row 16 ti + 4 r + l / 16.  Then element r of every lane is, as it stands, the B-operand
-// slice of logical rows 16 tk + 4 r .. + 3, so a product C = A X feeds X to the next
-// product without moving it; A (an i*P block from LDS or an operator block from global
-// memory, column-major) is read in the matching order: lane l loads row rho(l % 16) of the
-// tile, rho(4 a + b) = 4 b + a, at column 16 tk + 4 r + l / 16, which puts logical row
-// 16 ti + 4 r + q into hardware row 4 q + r of C, the same stored order again.  A product
-// over K rows takes ceil(K / 4) slices per tile pair.  The walk follows the vector
-// programs (one segment per initial sector) with all columns of s0 at once; products of arc
-// values travel as a float64 weight, leaves accumulate into float64 accumulators, and the
-// last propagator iP_n[s0] is applied once per segment.  float32 products (the
-// QIW_PREC_FP32 mode); sums over samples in float64.
+// propagators and arc values are uniform).  Partial products X (m_L x m_s0, 16 columns of
+// s0 per pass) stay in registers as 16x16 tiles of v_mfma_f32_16x16x4f32 accumulators: lane
+// l = 16 q + c holds rows 16 ti + 4 q + r (r = 0..3) of column c.  The instruction takes
+// row l / 16 of its 4-row B slice from lane l, so element s of every lane is, as it stands,
+// the B slice of rows {16 tk + 4 q + s}; the A operand of that slice is A(16 ti + c,
+// 16 tk + 4 q + s) in lane l, and the 16x16 result lands in the same layout.  A product
+// C = A X therefore feeds the next product without moving X, and each lane reads its A
+// values of a tile pair as four consecutive floats of a row-major A (one 16-byte load): the
+// operator blocks (global memory) and the i*P blocks of the sample (LDS) are stored
+// row-major with rows padded to a multiple of 4 floats.  Lanes past the block's rows or
+// columns read four zeros.  The walk follows the vector programs (one segment per initial
+// sector); products of arc values travel as a float64 weight, leaves accumulate into
+// float64 accumulators, and the last propagator iP_n[s0] is applied once per segment.
+// float32 products (the QIW_PREC_FP32 mode); sums over samples in float64.
 // ---------------------------------------------------------------------------------------
 typedef float mf4 __attribute__((ext_vector_type(4)));
 
-// A wavefront walks the columns of s0 in passes of 16 TC (one pass for blocks up to 16
-// wide); rows take TR tiles.
-template <int TR, int TC>
+template <int TR>
 struct MTile {
-    mf4 t[TR][TC];
+    mf4 t[TR];  // row tiles of one 16-column pass
 };
 
-template <int KM, int TM, int TR, int TC>
+template <int KM, int TM, int TR>
 struct MState {
-    double acc[TR][TC][4];          // leaf sums, in the tiles' order
+    double acc[TR][4];              // leaf sums, in the tile layout
     ProgStream ps;
-    const float *__restrict__ pl;   // i*P blocks of the sample in LDS: [(L - 2) * bt + BT offset]
-    const float *__restrict__ ops;  // operator blocks, column-major
+    const float *__restrict__ pl;   // i*P blocks of the sample in LDS: [(L - 2) * btp + LDS offset]
+    const float *__restrict__ ops;  // operator blocks, row-major, rows padded to 4 floats
     const double *__restrict__ sw;  // arc values of the sample's topology, in LDS
-    int bt;
-    int pl_zero, ops_zero;          // positions of a 0.0 in pl and in ops
+    int btp;                        // floats per segment of i*P blocks in LDS
+    int pl_zero, ops_zero;          // positions of four 0.0 in pl and in ops
 };
 
-// Logical row (within its tile) of element r of lane l.
-__device__ __forceinline__ int mrow(int l, int r) { return 4 * r + (l >> 4); }
-
-// C = A X for A (m_a x K, column-major, leading dimension lda, at base + off) and X (K x n),
-// both tiles in the stored row order.  All A elements of the product are loaded before the
-// first MFMA, so their latencies overlap (one wait per product instead of one per slice);
-// lanes outside A read the 0.0 at base + zpos.
-template <int TR, int TC>
-__device__ __forceinline__ MTile<TR, TC> mmul(const float *__restrict__ base, int off, int lda, int m_a, int K,
-                                              const MTile<TR, TC> &X, int n, int zpos) {
+// C = A X for A (m_a x K, row-major with rows of round4(K) floats, at base + off) and X
+// (K x 16 columns), both in the tile layout.  All A values are loaded (one 16-byte load per
+// tile pair) before the first MFMA.
+template <int TR>
+__device__ __forceinline__ MTile<TR> mmul(const float *__restrict__ base, int off, int m_a, int K,
+                                          const MTile<TR> &X, int zpos) {
     const int l = threadIdx.x, q = l >> 4, c = l & 15;
-    const int hr = 4 * (c & 3) + (c >> 2);  // rho(c): the logical row this lane feeds
-    const int mt = (m_a + 15) >> 4, nt = (n + 15) >> 4;
-    float av[TR][TR][4];  // [tk][ti][r]: A(16 ti + hr, 16 tk + 4 r + q)
+    const int ldr = (K + 3) & ~3, mt = (m_a + 15) >> 4;
+    mf4 av[TR][TR];  // [tk][ti]: A(16 ti + c, 16 tk + 4 q + s), s = 0..3
 #pragma unroll
     for (int tk = 0; tk < TR; ++tk)
 #pragma unroll
-        for (int ti = 0; ti < TR; ++ti)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) av[tk][ti][r] = 0.f;
+        for (int ti = 0; ti < TR; ++ti) av[tk][ti] = mf4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int tk = 0; tk < TR; ++tk) {
         if (16 * tk < K) {
+            const bool kin = 16 * tk + 4 * q < K;
 #pragma unroll
             for (int ti = 0; ti < TR; ++ti) {
                 if (ti < mt) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int k = 16 * tk + 4 * r + q, row = 16 * ti + hr;
-                        av[tk][ti][r] = base[(unsigned)(k < K && row < m_a ? off + k * lda + row : zpos)];
-                    }
+                    const int row = 16 * ti + c;
+                    const int idx = (kin && row < m_a) ? off + row * ldr + 16 * tk + 4 * q : zpos;
+                    av[tk][ti] = *(const mf4 *)(base + (unsigned)idx);
                 }
             }
         }
     }
-    MTile<TR, TC> C;
+    MTile<TR> C;
 #pragma unroll
-    for (int ti = 0; ti < TR; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < TC; ++tj) C.t[ti][tj] = mf4{0.f, 0.f, 0.f, 0.f};
+    for (int ti = 0; ti < TR; ++ti) C.t[ti] = mf4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int tk = 0; tk < TR; ++tk)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            if (16 * tk + 4 * r < K) {
+        for (int s = 0; s < 4; ++s) {
+            if (16 * tk + s < K) {
 #pragma unroll
-                for (int ti = 0; ti < TR; ++ti) {
-                    if (ti < mt) {
-#pragma unroll
-                        for (int tj = 0; tj < TC; ++tj)
-                            if (TC == 1 || tj < nt)
-                                C.t[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tk][ti][r], X.t[tk][tj][r],
-                                                                                   C.t[ti][tj], 0, 0, 0);
-                    }
-                }
+                for (int ti = 0; ti < TR; ++ti)
+                    if (ti < mt) C.t[ti] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tk][ti][s], X.t[tk][s], C.t[ti], 0, 0, 0);
             }
         }
     return C;
@@ -1293,13 +1272,13 @@ __device__ __forceinline__ MTile<TR, TC> mmul(const float *__restrict__ base, in
 // being walked saves nothing, and the walk returns past it.  The loop carries only the
 // current level in registers.
 template <int KM, int TM, int TR>
-__device__ __forceinline__ void mwalk(MState<KM, TM, TR, 1> &st, int n, const MTile<TR, 1> &X0, int m1, int off1,
-                                      int nch1, int nc, int *__restrict__ s_rm, double *__restrict__ s_wp,
-                                      int *__restrict__ s_yo, float *__restrict__ ystk) {
+__device__ __forceinline__ void mwalk(MState<KM, TM, TR> &st, int n, const MTile<TR> &X0, int m1, int off1, int nch1,
+                                      int *__restrict__ s_rm, double *__restrict__ s_wp, int *__restrict__ s_yo,
+                                      float *__restrict__ ystk) {
     const int l = threadIdx.x, q = l >> 4, c = l & 15;
     int L = 2, mp = m1, rem = nch1, yoff = 0;
     double wp = 1.0;
-    MTile<TR, 1> Y = (n == 2) ? X0 : mmul<TR, 1>(st.pl, off1, m1, m1, m1, X0, nc, st.pl_zero);
+    MTile<TR> Y = (n == 2) ? X0 : mmul<TR>(st.pl, off1, m1, m1, X0, st.pl_zero);
     for (;;) {
         if (rem == 0) {  // back to the nearest level above with children left
             int v = 0;
@@ -1317,9 +1296,9 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, TR, 1> &st, int n, const MT
             for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * ti + 4 * r + q;
+                    const int row = 16 * ti + 4 * q + r;
                     const float y = fr[(row < mp ? row : 0) * 16 + c];
-                    Y.t[ti][0][r] = row < mp ? y : 0.f;
+                    Y.t[ti][r] = row < mp ? y : 0.f;
                 }
             continue;
         }
@@ -1328,7 +1307,7 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, TR, 1> &st, int n, const MT
         const double wv = (w & QIW_F_TAIL) ? wp * st.sw[(int)((w >> 16) & 0xFF)] : wp;
         const int mo = (int)(w & 0x3F);
         const int blk = (int)((w >> 32) & 0xFFFF);
-        const MTile<TR, 1> X = (blk == 0xFFFF) ? Y : mmul<TR, 1>(st.ops, blk, mo, mo, mp, Y, nc, st.ops_zero);
+        const MTile<TR> X = (blk == 0xFFFF) ? Y : mmul<TR>(st.ops, blk, mo, mp, Y, st.ops_zero);
         if (w & QIW_F_LEAF) {
             const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -wv : wv)
                                                 : __longlong_as_double((long long)st.ps.next()) * wv;
@@ -1336,7 +1315,7 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, TR, 1> &st, int n, const MT
             for (int ti = 0; ti < TR; ++ti)
                 if (ti == 0 || 16 * ti < mo)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) st.acc[ti][0][r] = fma(cw, (double)X.t[ti][0][r], st.acc[ti][0][r]);
+                    for (int r = 0; r < 4; ++r) st.acc[ti][r] = fma(cw, (double)X.t[ti][r], st.acc[ti][r]);
         } else if (L < n) {  // an inner node: its children follow
             if (rem > 0) {
                 float *fr = ystk + yoff;
@@ -1344,8 +1323,8 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, TR, 1> &st, int n, const MT
                 for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * ti + 4 * r + q;
-                        if (row < mp) fr[row * 16 + c] = Y.t[ti][0][r];
+                        const int row = 16 * ti + 4 * q + r;
+                        if (row < mp) fr[row * 16 + c] = Y.t[ti][r];
                     }
                 s_rm[L] = rem | (mp << 16);
                 s_wp[L] = wp;
@@ -1358,38 +1337,38 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, TR, 1> &st, int n, const MT
             wp = wv;
             mp = mo;
             rem = (int)((w >> 24) & 0xFF);
-            Y = (L == n) ? X : mmul<TR, 1>(st.pl, (L - 2) * st.bt + (int)(w >> 52), mo, mo, mo, X, nc, st.pl_zero);
+            Y = (L == n) ? X : mmul<TR>(st.pl, (L - 2) * st.btp + (int)(w >> 52), mo, mo, X, st.pl_zero);
         }
     }
 }
 
-template <int KM, int TM, int TR, int TC>
+template <int KM, int TM, int TR>
 __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
     constexpr int NM = 2 * KM + 3;
-    static_assert(TC == 1, "the Y stack frames hold 16 columns");
-    extern __shared__ float mlds[];  // [(NM - 1) * bt] i*P blocks
+    // [(NM - 1) * btp] i*P blocks, four 0.0, then the Y stack of the walk
+    extern __shared__ __attribute__((aligned(16))) float mlds[];
     __shared__ int s_rm[16];        // per level of the walk: children left | rows << 16
     __shared__ double s_wp[16];     // per level: the weight of the arc values above
     __shared__ int s_yo[16];        // per level: offset of its Y frame in the LDS stack
     __shared__ double s_t[16];      // the times of the configuration positions 0..NM
     __shared__ double s_w[16];      // the arc values of the current topology
     const EvalArgs &a = ma.e;
-    const int lane = threadIdx.x, cl = lane & 15;
+    const int lane = threadIdx.x, q = lane >> 4, cl = lane & 15;
     const QiwTask task = a.tasks[blockIdx.x];
     const QiwGroup g = a.groups[task.group];
-    const int n = g.n_nodes, k = g.order, bt = ma.bt;
+    const int n = g.n_nodes, k = g.order, bt = ma.bt, btp = ma.btp;
     const int ntab = a.dtab.n_tables;
     const double inv_h = 1.0 / a.h;
     float *pl = mlds;
-    MState<KM, TM, TR, TC> st;
+    MState<KM, TM, TR> st;
     st.pl = pl;
     st.ops = ma.ops;
     st.sw = s_w;
-    st.bt = bt;
-    st.pl_zero = (NM - 1) * bt;
+    st.btp = btp;
+    st.pl_zero = ((NM - 1) * btp + 3) & ~3;
     st.ops_zero = ma.ops_zero;
-    if (lane == 0) pl[st.pl_zero] = 0.f;
-    float *ystk = pl + ((st.pl_zero + 4) & ~3);  // the Y stack: (NM - 2) frames of up to 32 x 16
+    if (lane < 4) pl[st.pl_zero + lane] = 0.f;
+    float *ystk = pl + st.pl_zero + 4;
     for (int e = lane; e < 16; e += QIW_WAVE) s_w[e] = 0.0;
 
     double tau_i = 0.0, tau_w, tau_f;
@@ -1440,23 +1419,30 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
         const double wt = g.weighted ? a.ut[g.ut_off + (int64_t)task.tau_index * a.ut_row_stride + (int64_t)(g.d_before + g.d_after) * g.n_samples + sample] : 1.0;
 #pragma unroll
         for (int p = 0; p <= NM; ++p) s_t[p] = t[p];
-        // ---- i*P blocks of every segment into LDS (lanes over BT elements) ----
+        // ---- i*P blocks of every segment into LDS, row-major with padded rows (lanes over
+        // the LDS positions; padding positions get 0.0) ----
         for (int L = 2; L <= n; ++L) {
             double dt = pick(t, L) - pick(t, L - 1);
             dt = dt > 0.0 ? dt : 0.0;
-            float *dst = pl + (L - 2) * bt;
+            float *dst = pl + (L - 2) * btp;
             if (g.kind == QIW_G_BARE) {
-                for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = ma.emask[e] ? (float)exp(-dt * ma.ediag[e]) : 0.f;
+                for (int p = lane; p < btp; p += QIW_WAVE) {
+                    const int e = ma.pinv[p];
+                    dst[p] = (e >= 0 && ma.emask[e]) ? (float)exp(-dt * ma.ediag[e]) : 0.f;
+                }
             } else {
                 const GridCell gc = grid_cell(pick(t, L), pick(t, L - 1), inv_h, a.n_tau);
                 const double *r0 = a.ptab + (int64_t)gc.d * bt, *r1 = r0 + bt, *rm = gc.d > 0 ? r0 - bt : r0;
-                for (int e = lane; e < bt; e += QIW_WAVE) dst[e] = (float)cell_value(gc, rm[e], r0[e], r1[e]);
+                for (int p = lane; p < btp; p += QIW_WAVE) {
+                    const int e = ma.pinv[p];
+                    dst[p] = e >= 0 ? (float)cell_value(gc, rm[e], r0[e], r1[e]) : 0.f;
+                }
             }
         }
         __syncthreads();
         // ---- walk the program: one segment per initial sector ----
         double mag = 0.0;
-        int m0 = 1, off0 = 0, col0 = 0;
+        int m0 = 1, off0 = 0, bto0 = 0, col0 = 0;
         st.ps.init(a.prog + g.prog_off + task.chunk, lane);
         const uint64_t *seg_base = st.ps.base;  // the stream just after the segment's SEG word
         int seg_idx = 0;
@@ -1466,23 +1452,21 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
             if (op == QIW_OP_ROOT) {
                 const int m1 = (int)(w & 0x3F);
                 const int blk = (int)((w >> 32) & 0xFFFF);
-                MTile<TR, TC> X;
+                const int ldr0 = (m0 + 3) & ~3;
+                MTile<TR> X;
 #pragma unroll
                 for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TC; ++tj)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = 16 * ti + mrow(lane, r), col = col0 + 16 * tj + cl;
-                            float v = 0.f;
-                            if (blk == 0xFFFF)
-                                v = (row == col && row < m0) ? 1.f : 0.f;
-                            else if (row < m1 && col < m0)
-                                v = st.ops[blk + col * m1 + row];
-                            X.t[ti][tj][r] = v;
-                        }
-                mwalk<KM, TM, TR>(st, n, X, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF), m0 - col0, s_rm, s_wp, s_yo,
-                                  ystk);
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * ti + 4 * q + r, col = col0 + cl;
+                        float v = 0.f;
+                        if (blk == 0xFFFF)
+                            v = (row == col && row < m0) ? 1.f : 0.f;
+                        else if (row < m1 && col < m0)
+                            v = st.ops[blk + row * ldr0 + col];
+                        X.t[ti][r] = v;
+                    }
+                mwalk<KM, TM, TR>(st, n, X, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF), s_rm, s_wp, s_yo, ystk);
             } else if (op == QIW_OP_TOPO) {
 #pragma unroll
                 for (int ai = 0; ai < KM; ++ai) {
@@ -1495,48 +1479,41 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
             } else if (op == QIW_OP_SEG) {
                 m0 = (int)(w & 0x3F);
                 off0 = (int)(w >> 52);
+                bto0 = ma.bto[(int)((w >> 8) & 0xFF)];
                 col0 = 0;
                 seg_base = st.ps.base;
                 seg_idx = st.ps.idx;
 #pragma unroll
                 for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TC; ++tj)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) st.acc[ti][tj][r] = 0.0;
+                    for (int r = 0; r < 4; ++r) st.acc[ti][r] = 0.0;
             } else if (op == QIW_OP_FLUSH) {  // the s0 block of this sample: iP_n[s0] times the leaf sum
-                MTile<TR, TC> S;
+                MTile<TR> S;
 #pragma unroll
                 for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TC; ++tj)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) S.t[ti][tj][r] = (float)st.acc[ti][tj][r];
-                const MTile<TR, TC> R = mmul<TR, TC>(pl, (n - 2) * bt + off0, m0, m0, m0, S, m0 - col0, st.pl_zero);
+                    for (int r = 0; r < 4; ++r) S.t[ti][r] = (float)st.acc[ti][r];
+                const MTile<TR> R = mmul<TR>(pl, (n - 2) * btp + off0, m0, m0, S, st.pl_zero);
 #pragma unroll
                 for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < TC; ++tj)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = 16 * ti + mrow(lane, r), col = col0 + 16 * tj + cl;
-                            if (row < m0 && col < m0) {
-                                const double v = (double)R.t[ti][tj][r];
-                                prow[off0 + row + col * m0] += wt * v;
-                                mag += fabs(v);
-                            }
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * ti + 4 * q + r, col = col0 + cl;
+                        if (row < m0 && col < m0) {
+                            const double v = (double)R.t[ti][r];
+                            prow[bto0 + row + col * m0] += wt * v;
+                            mag += fabs(v);
                         }
-                if (col0 + 16 * TC < m0) {  // the next columns of s0: walk the segment again
-                    col0 += 16 * TC;
+                    }
+                if (col0 + 16 < m0) {  // the next 16 columns of s0: walk the segment again
+                    col0 += 16;
                     st.ps.base = seg_base;
                     st.ps.blk = seg_base[lane];
                     st.ps.idx = seg_idx;
 #pragma unroll
                     for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
-                        for (int tj = 0; tj < TC; ++tj)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) st.acc[ti][tj][r] = 0.0;
+                        for (int r = 0; r < 4; ++r) st.acc[ti][r] = 0.0;
                 }
             } else if (op == QIW_OP_END) {
                 break;
@@ -2157,9 +2134,9 @@ hipError_t qiw_launch_ptab_cells(hipStream_t st, const double *T, int n_tau, int
 hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes) {
     if (tm > 2) return hipErrorInvalidValue;
     if (km <= 3)
-        hipLaunchKernelGGL((k_eval_mfma_t<3, 2, 2, 1>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+        hipLaunchKernelGGL((k_eval_mfma_t<3, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
     else if (km <= 4)
-        hipLaunchKernelGGL((k_eval_mfma_t<4, 2, 2, 1>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+        hipLaunchKernelGGL((k_eval_mfma_t<4, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
