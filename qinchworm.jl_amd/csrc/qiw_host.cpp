@@ -1334,9 +1334,10 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
             c->mfma_staged = true;
         }
         const int KM = a.max_order <= 3 ? 3 : 4, NM = 2 * KM + 3;
-        // i*P blocks, the four 0.0 read by lanes outside a block, and the Y stack of the walk
-        const size_t lds =
-            ((((size_t)(NM - 1) * m.btp + 3) & ~(size_t)3) + 4 + (size_t)ystack_rows(m, NM - 2) * 16) * 4;
+        // i*P blocks, the four 0.0 read by lanes outside a block, 64 floats for the writes of
+        // absent rows, and the stack of the walk (a 4-float header and the rows of each frame)
+        const size_t lds = ((((size_t)(NM - 1) * m.btp + 3) & ~(size_t)3) + 4 + 64 + (size_t)(NM - 2) * 4 +
+                            (size_t)ystack_rows(m, NM - 2) * 16) * 4;
         if (lds > 160 * 1024) return fail(c, QIW_E_LIMIT, "propagator blocks exceed the 160 KiB of LDS of a CU");
         MfmaArgs mf{};
         mf.e = a;

@@ -1211,133 +1211,138 @@ struct MTile {
 
 template <int KM, int TM, int TR>
 struct MState {
+    typename RegVec<double, regvec_size(KM * TM)>::type W;  // arc values of the topology
     double acc[TR][4];              // leaf sums, in the tile layout
     ProgStream ps;
     const float *__restrict__ pl;   // i*P blocks of the sample in LDS: [(L - 2) * btp + LDS offset]
     const float *__restrict__ ops;  // operator blocks, row-major, rows padded to 4 floats
-    const double *__restrict__ sw;  // arc values of the sample's topology, in LDS
+    float *__restrict__ ystk;       // the walk's stack of saved levels (LDS)
     int btp;                        // floats per segment of i*P blocks in LDS
     int pl_zero, ops_zero;          // positions of four 0.0 in pl and in ops
+    int dummy;                      // 64 floats of LDS that take the writes of absent rows
 };
 
 // C = A X for A (m_a x K, row-major with rows of round4(K) floats, at base + off) and X
-// (K x 16 columns), both in the tile layout.  All A values are loaded (one 16-byte load per
-// tile pair) before the first MFMA.
-template <int TR>
-__device__ __forceinline__ MTile<TR> mmul(const float *__restrict__ base, int off, int m_a, int K,
-                                          const MTile<TR> &X, int zpos) {
+// (K x 16 columns), both in the tile layout, for blocks up to 32.  One 16-byte load per
+// tile pair, all issued before the first MFMA; a row tile of X takes all four of its slices
+// (slices past K meet zeros in A).
+__device__ __forceinline__ MTile<2> mmul(const float *__restrict__ base, int off, int m_a, int K, const MTile<2> &X,
+                                         int zpos) {
     const int l = threadIdx.x, q = l >> 4, c = l & 15;
-    const int ldr = (K + 3) & ~3, mt = (m_a + 15) >> 4;
-    mf4 av[TR][TR];  // [tk][ti]: A(16 ti + c, 16 tk + 4 q + s), s = 0..3
+    const int ldr = (K + 3) & ~3;
+    const bool two_m = m_a > 16, two_k = K > 16;
+    const mf4 z = mf4{0.f, 0.f, 0.f, 0.f};
+    auto ld = [&](int tk, int ti) {
+        const int row = 16 * ti + c, kk = 16 * tk + 4 * q;
+        const int idx = (kk < K && row < m_a) ? off + row * ldr + kk : zpos;
+        return *(const mf4 *)(base + (unsigned)idx);
+    };
+    const mf4 a00 = ld(0, 0);
+    const mf4 a01 = two_m ? ld(0, 1) : z;
+    const mf4 a10 = two_k ? ld(1, 0) : z;
+    const mf4 a11 = (two_k && two_m) ? ld(1, 1) : z;
+    MTile<2> C;
+    C.t[0] = z;
+    C.t[1] = z;
+    if (!two_m) {
 #pragma unroll
-    for (int tk = 0; tk < TR; ++tk)
+        for (int s = 0; s < 4; ++s) C.t[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a00[s], X.t[0][s], C.t[0], 0, 0, 0);
+        if (two_k) {
 #pragma unroll
-        for (int ti = 0; ti < TR; ++ti) av[tk][ti] = mf4{0.f, 0.f, 0.f, 0.f};
+            for (int s = 0; s < 4; ++s) C.t[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a10[s], X.t[1][s], C.t[0], 0, 0, 0);
+        }
+    } else {
 #pragma unroll
-    for (int tk = 0; tk < TR; ++tk) {
-        if (16 * tk < K) {
-            const bool kin = 16 * tk + 4 * q < K;
+        for (int s = 0; s < 4; ++s) {
+            C.t[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a00[s], X.t[0][s], C.t[0], 0, 0, 0);
+            C.t[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a01[s], X.t[0][s], C.t[1], 0, 0, 0);
+        }
+        if (two_k) {
 #pragma unroll
-            for (int ti = 0; ti < TR; ++ti) {
-                if (ti < mt) {
-                    const int row = 16 * ti + c;
-                    const int idx = (kin && row < m_a) ? off + row * ldr + 16 * tk + 4 * q : zpos;
-                    av[tk][ti] = *(const mf4 *)(base + (unsigned)idx);
-                }
+            for (int s = 0; s < 4; ++s) {
+                C.t[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a10[s], X.t[1][s], C.t[0], 0, 0, 0);
+                C.t[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a11[s], X.t[1][s], C.t[1], 0, 0, 0);
             }
         }
     }
-    MTile<TR> C;
-#pragma unroll
-    for (int ti = 0; ti < TR; ++ti) C.t[ti] = mf4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int tk = 0; tk < TR; ++tk)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (16 * tk + s < K) {
-#pragma unroll
-                for (int ti = 0; ti < TR; ++ti)
-                    if (ti < mt) C.t[ti] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tk][ti][s], X.t[tk][s], C.t[ti], 0, 0, 0);
-            }
-        }
     return C;
 }
 
 // One root of the trie (a ROOT word with X0 = O_1 or the identity on s0), walked
 // depth-first without recursion.  Y (= iP_L X of the parent, shared by a node's children)
-// of the current level stays in registers; when the walk descends from a level that has
-// children left, that level's Y goes to an LDS stack frame (row-major, mp rows of 16
-// columns) and its uniform state (children left, rows, weight, frame offset) to small LDS
-// arrays, and both come back when the walk returns to it.  A level whose last child is
-// being walked saves nothing, and the walk returns past it.  The loop carries only the
-// current level in registers.
-template <int KM, int TM, int TR>
-__device__ __forceinline__ void mwalk(MState<KM, TM, TR> &st, int n, const MTile<TR> &X0, int m1, int off1, int nch1,
-                                      int *__restrict__ s_rm, double *__restrict__ s_wp, int *__restrict__ s_yo,
-                                      float *__restrict__ ystk) {
+// of the current level stays in registers.  When the walk descends from a level that has
+// children left, it pushes a frame on the LDS stack: a 16-byte header (children left,
+// rows, level, the previous frame, the weight of the arc values above) and Y (row-major,
+// mp rows of 16 columns); a level whose last child is being walked pushes nothing.  Back
+// from a subtree, the walk pops the top frame, header and Y in one round trip, and resumes
+// that level.  The loop carries only the current level in registers.
+template <int KM, int TM>
+__device__ __forceinline__ void mwalk(MState<KM, TM, 2> &st, int n, const MTile<2> &X0, int m1, int off1, int nch1) {
     const int l = threadIdx.x, q = l >> 4, c = l & 15;
-    int L = 2, mp = m1, rem = nch1, yoff = 0;
+    int L = 2, mp = m1, rem = nch1, top = -1, next = 0;
     double wp = 1.0;
-    MTile<TR> Y = (n == 2) ? X0 : mmul<TR>(st.pl, off1, m1, m1, X0, st.pl_zero);
+    MTile<2> Y = (n == 2) ? X0 : mmul(st.pl, off1, m1, m1, X0, st.pl_zero);
     for (;;) {
-        if (rem == 0) {  // back to the nearest level above with children left
-            int v = 0;
-            do {
-                if (L == 2) return;
-                --L;
-                v = s_rm[L];
-            } while ((v & 0xFFFF) == 0);
-            rem = v & 0xFFFF;
-            mp = v >> 16;
-            wp = s_wp[L];
-            yoff = s_yo[L];
-            const float *fr = ystk + yoff;
+        if (rem == 0) {  // back to the level of the top frame
+            if (top < 0) return;
+            const int4 h = *(const int4 *)(st.ystk + top);
+            const float *fr = st.ystk + top + 4;
+            const int rows = (next - top - 4) >> 4;  // rows of the top frame (it ends at next)
+            float y[2][4];
 #pragma unroll
-            for (int ti = 0; ti < TR; ++ti)
+            for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * ti + 4 * q + r;
-                    const float y = fr[(row < mp ? row : 0) * 16 + c];
-                    Y.t[ti][r] = row < mp ? y : 0.f;
+                    y[ti][r] = fr[(row < rows ? row : 0) * 16 + c];
                 }
+            next = top;
+            rem = h.x & 0xFF;
+            mp = (h.x >> 8) & 0xFF;
+            L = h.x >> 16;
+            top = h.y;
+            wp = __hiloint2double(h.w, h.z);
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Y.t[ti][r] = (16 * ti + 4 * q + r < mp) ? y[ti][r] : 0.f;
             continue;
         }
         --rem;
         const uint64_t w = st.ps.next();
-        const double wv = (w & QIW_F_TAIL) ? wp * st.sw[(int)((w >> 16) & 0xFF)] : wp;
         const int mo = (int)(w & 0x3F);
         const int blk = (int)((w >> 32) & 0xFFFF);
-        const MTile<TR> X = (blk == 0xFFFF) ? Y : mmul<TR>(st.ops, blk, mo, mp, Y, st.ops_zero);
+        const MTile<2> X = (blk == 0xFFFF) ? Y : mmul(st.ops, blk, mo, mp, Y, st.ops_zero);
+        const double wv = (w & QIW_F_TAIL) ? wp * st.W[(int)((w >> 16) & 0xFF)] : wp;
         if (w & QIW_F_LEAF) {
             const double cw = (w & QIW_FV_UNIT) ? ((w & QIW_FV_NEG) ? -wv : wv)
                                                 : __longlong_as_double((long long)st.ps.next()) * wv;
 #pragma unroll
-            for (int ti = 0; ti < TR; ++ti)
-                if (ti == 0 || 16 * ti < mo)
+            for (int r = 0; r < 4; ++r) st.acc[0][r] = fma(cw, (double)X.t[0][r], st.acc[0][r]);
+            if (mo > 16) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) st.acc[ti][r] = fma(cw, (double)X.t[ti][r], st.acc[ti][r]);
+                for (int r = 0; r < 4; ++r) st.acc[1][r] = fma(cw, (double)X.t[1][r], st.acc[1][r]);
+            }
         } else if (L < n) {  // an inner node: its children follow
             if (rem > 0) {
-                float *fr = ystk + yoff;
+                float *fr = st.ystk + next;
+                if (l == 0)
+                    *(int4 *)fr = make_int4(rem | (mp << 8) | (L << 16), top, __double2loint(wp), __double2hiint(wp));
 #pragma unroll
-                for (int ti = 0; ti < TR; ++ti)
+                for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = 16 * ti + 4 * q + r;
-                        if (row < mp) fr[row * 16 + c] = Y.t[ti][r];
+                        st.ystk[row < mp ? next + 4 + row * 16 + c : st.dummy + l] = Y.t[ti][r];
                     }
-                s_rm[L] = rem | (mp << 16);
-                s_wp[L] = wp;
-                s_yo[L] = yoff;
-                yoff += mp * 16;
-            } else {
-                s_rm[L] = 0;
+                top = next;
+                next += 4 + mp * 16;
             }
             ++L;
             wp = wv;
             mp = mo;
             rem = (int)((w >> 24) & 0xFF);
-            Y = (L == n) ? X : mmul<TR>(st.pl, (L - 2) * st.btp + (int)(w >> 52), mo, mo, X, st.pl_zero);
+            Y = (L == n) ? X : mmul(st.pl, (L - 2) * st.btp + (int)(w >> 52), mo, mo, X, st.pl_zero);
         }
     }
 }
@@ -1345,13 +1350,11 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, TR> &st, int n, const MTile
 template <int KM, int TM, int TR>
 __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
     constexpr int NM = 2 * KM + 3;
-    // [(NM - 1) * btp] i*P blocks, four 0.0, then the Y stack of the walk
+    static_assert(TR == 2, "tiles for blocks up to 32");
+    // [(NM - 1) * btp] i*P blocks, four 0.0, 64 floats for the writes of absent rows, then the
+    // stack of the walk
     extern __shared__ __attribute__((aligned(16))) float mlds[];
-    __shared__ int s_rm[16];        // per level of the walk: children left | rows << 16
-    __shared__ double s_wp[16];     // per level: the weight of the arc values above
-    __shared__ int s_yo[16];        // per level: offset of its Y frame in the LDS stack
     __shared__ double s_t[16];      // the times of the configuration positions 0..NM
-    __shared__ double s_w[16];      // the arc values of the current topology
     const EvalArgs &a = ma.e;
     const int lane = threadIdx.x, q = lane >> 4, cl = lane & 15;
     const QiwTask task = a.tasks[blockIdx.x];
@@ -1363,13 +1366,14 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
     MState<KM, TM, TR> st;
     st.pl = pl;
     st.ops = ma.ops;
-    st.sw = s_w;
     st.btp = btp;
     st.pl_zero = ((NM - 1) * btp + 3) & ~3;
     st.ops_zero = ma.ops_zero;
+    st.ystk = pl + st.pl_zero + 68;  // past the four zeros and the 64 floats for absent rows
+    st.dummy = -64;                   // those 64 floats, relative to ystk
     if (lane < 4) pl[st.pl_zero + lane] = 0.f;
-    float *ystk = pl + st.pl_zero + 4;
-    for (int e = lane; e < 16; e += QIW_WAVE) s_w[e] = 0.0;
+#pragma unroll
+    for (int e = 0; e < regvec_size(KM * TM); ++e) st.W[e] = 0.0;
 
     double tau_i = 0.0, tau_w, tau_f;
     if (g.kind == QIW_G_BARE) {
@@ -1466,7 +1470,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
                             v = st.ops[blk + row * ldr0 + col];
                         X.t[ti][r] = v;
                     }
-                mwalk<KM, TM, TR>(st, n, X, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF), s_rm, s_wp, s_yo, ystk);
+                mwalk<KM, TM>(st, n, X, m1, (int)(w >> 52), (int)((w >> 24) & 0xFF));
             } else if (op == QIW_OP_TOPO) {
 #pragma unroll
                 for (int ai = 0; ai < KM; ++ai) {
@@ -1474,7 +1478,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
                     const ArcPos ap = arc_pos(s_t[tail], s_t[head], inv_h, a.n_tau);
 #pragma unroll
                     for (int tb = 0; tb < TM; ++tb)
-                        if (ai < k && tb < ntab) s_w[ai * TM + tb] = delta_value(a.dtab, tb, ap);
+                        if (ai < k && tb < ntab) st.W[ai * TM + tb] = delta_value(a.dtab, tb, ap);
                 }
             } else if (op == QIW_OP_SEG) {
                 m0 = (int)(w & 0x3F);
@@ -1493,7 +1497,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
                 for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) S.t[ti][r] = (float)st.acc[ti][r];
-                const MTile<TR> R = mmul<TR>(pl, (n - 2) * btp + off0, m0, m0, S, st.pl_zero);
+                const MTile<TR> R = mmul(pl, (n - 2) * btp + off0, m0, m0, S, st.pl_zero);
 #pragma unroll
                 for (int ti = 0; ti < TR; ++ti)
 #pragma unroll
