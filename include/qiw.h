@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define QIW_ABI_VERSION 5
+#define QIW_ABI_VERSION 6
 
 /* ---- return codes (every reference @assert on this path maps to one) ---------------- */
 enum {
@@ -157,6 +157,12 @@ typedef struct {
  *                 the reference's stopping rule ends (qiw_scramble_dims lists the exact
  *                 dimensions).  A call with too few staged sequences, or with fewer
  *                 dimensions than 2k, returns QIW_E_ARG before it evaluates anything.
+ *   scramble = 3  rng given, target_std = 0, any sequence type: the sequence plug point
+ *                 seq_type of mean_std_from_randomization (src/randomization.jl:86-94,
+ *                 src/inchworm.jl:129,233,339; e.g. RandomSeq, src/utility.jl:336-414).
+ *                 The caller constructs seq_type(D; scramble_rng = rng) in the order of
+ *                 scramble = 2, draws N_samples points of each with next!, and stages them
+ *                 with qiw_set_points; they are consumed as under scramble = 2.
  * The GPU evaluates all N_seqs sequences and applies the stopping rule afterwards; the
  * result is that of the reference's sequential loop over the same sequences. */
 typedef struct {
@@ -210,8 +216,17 @@ int qiw_set_sobol(qiw_ctx *ctx, int32_t dim, const uint32_t *m, const uint32_t *
  * its sequences).  n = 0 clears them.  They stay staged for later calls. */
 int qiw_set_scrambles(qiw_ctx *ctx, int64_t n, int32_t stride, const uint32_t *m, const uint32_t *shift);
 
+/* qiw_set_points: the caller-supplied sequences of qiw_rand.scramble = 3, replacing the
+ * seq_type(D; scramble_rng = rng) constructions of mean_std_from_randomization
+ * (src/randomization.jl:94) and the next! calls of contour_integral
+ * (src/qmc_integrate.jl:597-612): point r of sequence i is x[(i * n_points + r) * stride
+ * + d], d = 0 .. stride-1 (1 <= stride <= 12; a group of order k reads dimensions
+ * 0 .. 2k-1, a call with N_samples reads points 0 .. N_samples-1, n_points >= N_samples),
+ * every coordinate in [0, 1].  n = 0 clears them.  They stay staged for later calls. */
+int qiw_set_points(qiw_ctx *ctx, int64_t n, int64_t n_points, int32_t stride, const double *x);
+
 /* qiw_scramble_dims: the dimensions D = 2k of the sequences a call with `run` consumes
- * under scramble = 2, in the order it consumes them -- the order in which the reference
+ * under scramble = 2 or 3, in the order it consumes them -- the order in which the reference
  * constructs ScrambledSobolSeq(D; scramble_rng = rng) objects, so a caller draws exactly
  * these from its RNG and stages them with qiw_set_scrambles.  N_seqs per sampled group,
  * except min(2, N_seqs) for a group whose integral vanishes identically (no contributing

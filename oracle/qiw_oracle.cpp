@@ -149,7 +149,10 @@ struct SobolSeq {  // ScrambledSobolSeq{D}
     std::vector<uint32_t> x;  // [D]
     uint32_t n = 0;
     SobolSeq(int D_, const uint32_t *m_all, int m_dims, const uint32_t *shift) : D(D_) {
-        m.assign(m_all, m_all + (size_t)D * 32);
+        if (m_all)
+            m.assign(m_all, m_all + (size_t)D * 32);
+        else
+            m.assign((size_t)D * 32, 0u);
         (void)m_dims;
         x.assign((size_t)D, 0u);
         if (shift)
@@ -1009,6 +1012,9 @@ struct Evaluator {
 struct SobolSource {
     const uint32_t *m;
     const uint32_t *shift;
+    const double *pts = nullptr;  // caller-supplied points instead (scramble = 3): [npts][pstride]
+    int pstride = 0;
+    int64_t npts = 0;
 };
 
 static int g_threads = 1;
@@ -1017,7 +1023,8 @@ static int g_threads = 1;
 struct RandParams {
     int n_seqs = 1;
     bool scramble = false;
-    bool given = false;  // caller-drawn sequences
+    bool given = false;   // the caller's sequences (scramble = 2 or 3)
+    bool points = false;  // ... as points (scramble = 3)
     uint64_t seed = 0;
     double target_std = 0.0;
 };
@@ -1026,7 +1033,8 @@ static RandParams rand_params(const qiw_run *run) {
     if (run->rand_params) {
         r.n_seqs = run->rand_params->N_seqs;
         r.scramble = run->rand_params->scramble != 0;
-        r.given = run->rand_params->scramble == 2;
+        r.given = run->rand_params->scramble == 2 || run->rand_params->scramble == 3;
+        r.points = run->rand_params->scramble == 3;
         r.seed = run->rand_params->seed;
         r.target_std = run->rand_params->target_std;
     }
@@ -1056,6 +1064,25 @@ extern "C" int orc_set_scrambles(int64_t n, int stride, const uint32_t *m, const
     g_given.x.assign(x, x + (size_t)n * stride);
     return 0;
 }
+// Caller-supplied point sequences (qiw_rand.scramble = 3): the points that the sequences
+// seq_type(D; scramble_rng = rng) of mean_std_from_randomization (src/randomization.jl:86-94)
+// yield through next! (src/qmc_integrate.jl:597-612), consumed as under scramble = 2;
+// point r of sequence i is x[(i * npts + r) * stride + d].
+static struct {
+    std::vector<double> x;
+    int stride = 0;
+    int64_t n = 0, npts = 0;
+} g_points;
+
+extern "C" int orc_set_points(int64_t n, int64_t npts, int stride, const double *x) {
+    g_points.n = n;
+    g_points.npts = npts;
+    g_points.stride = stride;
+    g_points.x.assign(x, x + (size_t)n * npts * stride);
+    g_given.next = 0;
+    g_given.log.clear();
+    return 0;
+}
 extern "C" int64_t orc_scrambles_used(void) { return g_given.next; }
 extern "C" int64_t orc_scrambles_log(int64_t cap, int64_t *out) {
     for (size_t i = 0; i < g_given.log.size() && (int64_t)i < 5 * cap; ++i) out[i] = g_given.log[i];
@@ -1071,10 +1098,14 @@ struct SeqStore {
 static SobolSource sequence_source(const RandParams &rq, SobolSource staged, int dim, int domain, int64_t key,
                                    int order, int n_after, int seq, SeqStore &store) {
     if (rq.given) {
-        if (g_given.next >= g_given.n || dim > g_given.stride)
-            throw std::runtime_error("not enough caller-drawn sequences");
+        const int64_t have = rq.points ? g_points.n : g_given.n;
+        const int stride = rq.points ? g_points.stride : g_given.stride;
+        if (g_given.next >= have || dim > stride) throw std::runtime_error("not enough caller-drawn sequences");
         const int64_t i = g_given.next++;
         for (int64_t v : {(int64_t)domain, key, (int64_t)order, (int64_t)n_after, (int64_t)seq}) g_given.log.push_back(v);
+        if (rq.points)
+            return SobolSource{nullptr, nullptr, g_points.x.data() + (size_t)i * g_points.npts * stride, stride,
+                               g_points.npts};
         store.m.assign(g_given.m.begin() + i * g_given.stride * 32, g_given.m.begin() + i * g_given.stride * 32 + (size_t)dim * 32);
         store.x.assign(g_given.x.begin() + i * g_given.stride, g_given.x.begin() + i * g_given.stride + std::max(dim, 1));
         return SobolSource{store.m.data(), store.x.data()};
@@ -1134,6 +1165,7 @@ static std::vector<CMat> integrate_group(const Model &m, const PPGF *P, bool bol
                                          int64_t sample_first = 0, int64_t sample_count = -1, int n_var = -1) {
     int d = n_var >= 0 ? n_var : 2 * order;  // variable times (self-energy diagrams: 2k - 2)
     if (sample_count < 0) sample_count = N;
+    if (sob.pts && sample_first + sample_count > sob.npts) throw std::runtime_error("too few caller-supplied points");
     double J = tr.jacobian();
     cd dir = 1.0;
     for (int i = 0; i < d; ++i) dir *= -I;  // prod(branch_direction) over the imaginary branch
@@ -1152,12 +1184,16 @@ static std::vector<CMat> integrate_group(const Model &m, const PPGF *P, bool bol
         for (int64_t c = 0; c < nch; ++c) {
             int64_t i0 = sample_first + c * CH;
             int64_t i1 = std::min(sample_first + sample_count, i0 + CH);
-            SobolSeq seq(d, sob.m, d, sob.shift);
-            seq.skip(i0, true);
+            SobolSeq seq(d, sob.pts ? nullptr : sob.m, d, sob.pts ? nullptr : sob.shift);
+            if (!sob.pts) seq.skip(i0, true);
             std::vector<CMat> acc(m.ns);
             for (int s = 0; s < m.ns; ++s) acc[s] = CMat(m.dims[s], m.dims[s]);
             for (int64_t i = i0; i < i1; ++i) {
-                seq.next(x.data());
+                if (sob.pts) {
+                    for (int k = 0; k < d; ++k) x[k] = sob.pts[i * sob.pstride + k];
+                } else {
+                    seq.next(x.data());
+                }
                 tr.apply(x.data(), u.data());
                 bool ok = true;
                 for (int k = 0; k < d; ++k)

@@ -109,6 +109,8 @@ struct qiw_ctx {
     // caller-drawn sequences (qiw_set_scrambles), on the device as buffers given.m / given.x
     int64_t given_n = 0;
     int given_stride = 0;
+    int64_t points_n = 0, points_npts = 0;  // caller-supplied point sequences (qiw_set_points)
+    int points_stride = 0;
     // propagator state (host mirror of the device table), i*P in BT layout
     std::vector<double> ip;  // [ntau][bt]
     int n_known = 1;

@@ -326,10 +326,18 @@ hipError_t qiw_launch_finalize_corr_rqmc(hipStream_t st, const CorrFinalizeArgs 
 // Simplex coordinates of every sampled group's sequence for one step: the staged
 // direction numbers m (dimensions 0..dim_max-1) LMS + shift scrambled with the bits of the
 // group's stream (domain, key, order, n_after, seq).
-// coordinates from caller-drawn sequences: row r = 0..n_rows-1 of group g (at ut + (row0 +
+// The caller's sequences of a call: the states of caller-drawn scrambled Sobol sequences
+// (gm: [.][stride][32] direction numbers, gx: [.][stride] shifts), or, with pts set, the
+// points of caller-supplied sequences (pts: [.][npts][stride]).
+struct GivenSrc {
+    const uint32_t *gm = nullptr, *gx = nullptr;
+    const double *pts = nullptr;
+    int stride = 0;
+    int64_t npts = 0;
+};
+// coordinates from the caller's sequences: row r = 0..n_rows-1 of group g (at ut + (row0 +
 // r) * row_stride) maps sequence idx[r * n_groups + g] (< 0: none)
-hipError_t qiw_launch_given_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int64_t n, const uint32_t *gm,
-                                   const uint32_t *gx, int stride, const int64_t *idx, int n_rows, int row0,
-                                   int64_t row_stride, double *ut);
+hipError_t qiw_launch_given_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int64_t n, const GivenSrc &src,
+                                   const int64_t *idx, int n_rows, int row0, int64_t row_stride, double *ut);
 hipError_t qiw_launch_rqmc_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int max_dim, int64_t n,
                                   const uint32_t *m, uint64_t seed, int domain, int64_t key, double *ut);

@@ -744,7 +744,8 @@ int upload(qiw_ctx *c, const std::string &tag, Batch &b, int64_t N, DevBatch &d,
 struct RandPlan {
     int n_seqs = 1;
     bool scrambled = false;
-    bool given = false;  // the sequences are the caller's (qiw_set_scrambles)
+    bool given = false;   // the sequences are the caller's (qiw_set_scrambles or qiw_set_points)
+    bool points = false;  // ... given as points (qiw_set_points)
     uint64_t seed = 0;
     double target_std = 0.0;
     int n_eval() const { return scrambled ? n_seqs : 1; }  // sequences evaluated per group
@@ -754,7 +755,8 @@ RandPlan rand_plan(const qiw_run *run) {
     if (run && run->rand_params) {
         r.n_seqs = run->rand_params->N_seqs;
         r.scrambled = run->rand_params->scramble != 0;
-        r.given = run->rand_params->scramble == 2;
+        r.given = run->rand_params->scramble == 2 || run->rand_params->scramble == 3;
+        r.points = run->rand_params->scramble == 3;
         r.seed = run->rand_params->seed;
         r.target_std = run->rand_params->target_std;
     }
@@ -816,18 +818,30 @@ int check_given_use(qiw_ctx *c, const Batch &b, const std::vector<int32_t> &expe
     return 0;
 }
 
-// Caller-drawn sequences for a call that consumes `need` of them, for groups up to order
-// max_order: checks the staged ones and returns their device copies.
-int given_buffers(qiw_ctx *c, int max_order, int64_t need, const uint32_t **gm, const uint32_t **gx) {
-    if (need > c->given_n)
-        return fail(c, QIW_E_ARG, "the call consumes " + std::to_string(need) + " caller-drawn sequences, " +
-                                      std::to_string(c->given_n) + " are staged (qiw_set_scrambles)");
-    if (need > 0 && c->given_stride < 2 * max_order)
-        return fail(c, QIW_E_ARG, "caller-drawn sequences have fewer dimensions than 2 * order");
+// The caller's sequences for a call that consumes `need` of them, for groups up to order
+// max_order and N_samples n: checks the staged ones and returns their device copies
+// (scrambled Sobol states, or points with `points`).
+int given_buffers(qiw_ctx *c, bool points, int max_order, int64_t need, int64_t n, GivenSrc *src) {
+    const int64_t have = points ? c->points_n : c->given_n;
+    const int stride = points ? c->points_stride : c->given_stride;
+    const char *what = points ? " caller-supplied point sequences, " : " caller-drawn sequences, ";
+    const char *how = points ? " are staged (qiw_set_points)" : " are staged (qiw_set_scrambles)";
+    if (need > have) return fail(c, QIW_E_ARG, "the call consumes " + std::to_string(need) + what + std::to_string(have) + how);
+    if (need > 0 && stride < 2 * max_order)
+        return fail(c, QIW_E_ARG, "the caller's sequences have fewer dimensions than 2 * order");
+    if (need > 0 && points && c->points_npts < n)
+        return fail(c, QIW_E_ARG, "the caller's point sequences hold fewer points than N_samples");
     int rc = 0;
-    const size_t n = (size_t)std::max<int64_t>(c->given_n, 1) * std::max(c->given_stride, 1);
-    *gm = (const uint32_t *)dev(c, "given.m", n * 32 * 4, &rc);
-    *gx = (const uint32_t *)dev(c, "given.x", n * 4, &rc);
+    *src = GivenSrc{};
+    src->stride = stride;
+    if (points) {
+        src->npts = c->points_npts;
+        src->pts = (const double *)dev(c, "given.pts", (size_t)std::max<int64_t>(have, 1) * std::max<int64_t>(c->points_npts, 1) * std::max(stride, 1) * 8, &rc);
+    } else {
+        const size_t rows = (size_t)std::max<int64_t>(have, 1) * std::max(stride, 1);
+        src->gm = (const uint32_t *)dev(c, "given.m", rows * 32 * 4, &rc);
+        src->gx = (const uint32_t *)dev(c, "given.x", rows * 4, &rc);
+    }
     return rc;
 }
 
@@ -875,10 +889,10 @@ int check_run(qiw_ctx *c, const qiw_run *run) {
         const qiw_rand &r = *run->rand_params;
         if (r.N_seqs < 1) return fail(c, QIW_E_ARG, "rand_params.N_seqs must be > 0");
         if (r.N_seqs > QIW_MAX_SEQS) return fail(c, QIW_E_LIMIT, "rand_params.N_seqs beyond 64");
-        if (r.scramble < 0 || r.scramble > 2) return fail(c, QIW_E_ARG, "rand_params.scramble must be 0, 1 or 2");
+        if (r.scramble < 0 || r.scramble > 3) return fail(c, QIW_E_ARG, "rand_params.scramble must be 0, 1, 2 or 3");
         // the reference draws sequence s + 1 of a group only if it does not stop after s
-        if (r.scramble == 2 && r.target_std != 0.0)
-            return fail(c, QIW_E_ARG, "caller-drawn sequences (scramble = 2) need target_std = 0");
+        if (r.scramble >= 2 && r.target_std != 0.0)
+            return fail(c, QIW_E_ARG, "the caller's sequences (scramble = 2, 3) need target_std = 0");
         if (!(r.target_std >= 0.0)) return fail(c, QIW_E_ARG, "rand_params.target_std must be >= 0");
     }
     return 0;
@@ -1749,6 +1763,31 @@ static int qiw_set_scrambles_impl(qiw_ctx *c, int64_t n, int32_t stride, const u
     return QIW_OK;
 }
 
+static int qiw_set_points_impl(qiw_ctx *c, int64_t n, int64_t n_points, int32_t stride, const double *x) {
+    if (!c) return QIW_E_ARG;
+    if (n < 0 || (n > 0 && (!x || n_points < 1 || stride < 1 || stride > 2 * QIW_KMAX)))
+        return fail(c, QIW_E_ARG, "qiw_set_points: n >= 0 sequences of n_points >= 1 points of 1 .. 12 dimensions");
+    if ((double)n * (double)n_points * stride * 8 > kMaxGivenCoordBytes)
+        return fail(c, QIW_E_LIMIT, "qiw_set_points: the points exceed 64 GiB");
+    (void)hipSetDevice(c->device);
+    c->points_n = 0;
+    c->points_npts = 0;
+    c->points_stride = 0;
+    if (n == 0) return QIW_OK;
+    const size_t count = (size_t)n * n_points * stride;
+    for (size_t i = 0; i < count; ++i)
+        if (!(x[i] >= 0.0 && x[i] <= 1.0)) return fail(c, QIW_E_ARG, "qiw_set_points: points must lie in [0, 1]");
+    int rc = 0;
+    double *p = (double *)dev(c, "given.pts", count * 8, &rc);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // earlier calls may still read the buffer
+    HIPCHK(c, hipMemcpy(p, x, count * 8, hipMemcpyHostToDevice));
+    c->points_n = n;
+    c->points_npts = n_points;
+    c->points_stride = stride;
+    return QIW_OK;
+}
+
 static int qiw_set_ppgf_impl(qiw_ctx *c, int32_t n_known, const double *P, const double *der0) {
     if (!c || !P) return QIW_E_ARG;
     if (!c->has_model) return fail(c, QIW_E_STATE, "no model staged");
@@ -1806,7 +1845,7 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     Batch &bare_b = bp->b, &bold_b = sp->b;
     DevBatch &bare_d = bp->d, &bold_d = sp->d;
     // caller-drawn sequences: the initial step's, then every bold step's in step order
-    const uint32_t *gm = nullptr, *gx = nullptr;
+    GivenSrc gsrc;
     const int64_t *gidx_bare = nullptr, *gidx_bold = nullptr;
     int32_t *nuse_bare = nullptr, *nuse_bold = nullptr;  // sequences the bookkeeping used (N_seqs >= 3)
     std::vector<int32_t> use_bare, use_bold;
@@ -1830,7 +1869,8 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
             n += given_row(bold_b, rq.n_seqs, n, none, idx.data() + o, use_bold.data() + ou);
             rows_bold.push_back(jw + 1);
         }
-        if ((rc = given_buffers(c, std::max(bare_b.max_order, bold_b.max_order), n, &gm, &gx))) return rc;
+        if ((rc = given_buffers(c, rq.points, std::max(bare_b.max_order, bold_b.max_order), n, run->N_samples, &gsrc)))
+            return rc;
         int64_t *didx = (int64_t *)dev(c, "given.idx", std::max<size_t>(idx.size(), 1) * 8, &rc);
         if (rq.n_seqs >= 3) {
             nuse_bare = (int32_t *)dev(c, "given.nuse_bare", (size_t)m.ntau * std::max<size_t>(bare_b.r_first.size(), 1) * 4, &rc);
@@ -1903,8 +1943,8 @@ int run_inchworm(qiw_ctx *c, const qiw_run *run, bool do_bare, int first_w, int 
     auto coords = [&](const Batch &b, const DevBatch &d, int domain, int key, const int64_t *gidx) -> hipError_t {
         if (!b.scrambled || run->N_samples <= 0) return hipSuccess;  // N = 0: order 0 only
         if (b.given)
-            return qiw_launch_given_coords(c->stream, d.groups, (int)b.groups.size(), run->N_samples, gm, gx,
-                                           c->given_stride, gidx, 1, 0, 0, d.ut);
+            return qiw_launch_given_coords(c->stream, d.groups, (int)b.groups.size(), run->N_samples, gsrc, gidx, 1, 0, 0,
+                                           d.ut);
         return qiw_launch_rqmc_coords(c->stream, d.groups, (int)b.groups.size(), 2 * b.max_order, run->N_samples,
                                       sobm, rq.seed, domain, key, d.ut);
     };
@@ -2247,7 +2287,7 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
     if (b.groups.empty()) return fail(c, QIW_E_ARG, "diff_inchworm needs a self-energy order >= 1");
     // caller-drawn sequences: every step's in step order; at tau_f_prev = tau_i (step 0) the
     // sampled self-energy simplices have zero volume
-    const uint32_t *gm = nullptr, *gx = nullptr;
+    GivenSrc gsrc;
     const int64_t *gidx = nullptr;
     int32_t *nuse = nullptr;
     std::vector<int32_t> use;
@@ -2262,7 +2302,7 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
                            use.data() + st * nr);
             rows.push_back(st + 1);
         }
-        if ((rc = given_buffers(c, b.max_order, n, &gm, &gx))) return rc;
+        if ((rc = given_buffers(c, rq.points, b.max_order, n, run->N_samples, &gsrc))) return rc;
         int64_t *didx = (int64_t *)dev(c, "given.idx", std::max<size_t>(idx.size(), 1) * 8, &rc);
         if (rq.n_seqs >= 3) nuse = (int32_t *)dev(c, "given.nuse_diff", (size_t)m.ntau * std::max<size_t>(nr, 1) * 4, &rc);
         if (rc) return rc;
@@ -2355,8 +2395,8 @@ int run_diff(qiw_ctx *c, const qiw_run *run, double *S_out, double *S_std_out) {
         fa.tau_w = fa.tau_f = st * m.h;
         a.step_w = st;
         if (gidx)
-            HIPCHK(c, qiw_launch_given_coords(c->stream, d.groups, (int)b.groups.size(), run->N_samples, gm, gx,
-                                              c->given_stride, gidx + (size_t)st * b.groups.size(), 1, 0, 0, d.ut));
+            HIPCHK(c, qiw_launch_given_coords(c->stream, d.groups, (int)b.groups.size(), run->N_samples, gsrc,
+                                              gidx + (size_t)st * b.groups.size(), 1, 0, 0, d.ut));
         else if (b.scrambled && !b.given)
             HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, (int)b.groups.size(), 2 * b.max_order,
                                              run->N_samples > 0 ? run->N_samples : 1, sobm, rq.seed, QIW_RQ_DIFF, st,
@@ -2630,8 +2670,8 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
                                idx.data() + (size_t)(j - 1) * ng, use.data() + (size_t)(j - 1) * nr);
                 rows.push_back(j);
             }
-            const uint32_t *gm = nullptr, *gx = nullptr;
-            if ((rc = given_buffers(c, b.max_order, given_next + n, &gm, &gx))) return rc;
+            GivenSrc gsrc;
+            if ((rc = given_buffers(c, rq.points, b.max_order, given_next + n, run->N_samples, &gsrc))) return rc;
             given_next += n;
             if ((double)m.ntau * b.ut_total * 8 > kMaxGivenCoordBytes)
                 return fail(c, QIW_E_LIMIT, "coordinates of every tau row exceed 64 GiB (lower N_samples or N_seqs)");
@@ -2642,8 +2682,8 @@ int run_correlator(qiw_ctx *c, const qiw_run *run, double *corr_out, double *cor
             if (rq.n_seqs >= 3) nuse = (int32_t *)dev(c, "given.nuse_corr", (size_t)m.ntau * nr * 4, &rc);
             if (rc) return rc;
             HIPCHK(c, hipMemcpy(didx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice));
-            HIPCHK(c, qiw_launch_given_coords(c->stream, d.groups, ng, run->N_samples, gm, gx, c->given_stride, didx,
-                                              m.ntau - 1, 1, b.ut_total, ut_rows));
+            HIPCHK(c, qiw_launch_given_coords(c->stream, d.groups, ng, run->N_samples, gsrc, didx, m.ntau - 1, 1,
+                                              b.ut_total, ut_rows));
         } else if (sobm && !b.given && run->N_samples > 0) {  // one set of scrambled sequences for all tau rows
             HIPCHK(c, qiw_launch_rqmc_coords(c->stream, d.groups, ng, 2 * b.max_order, run->N_samples, sobm, rq.seed,
                                              QIW_RQ_CORR, ci, d.ut));
@@ -3057,6 +3097,10 @@ extern "C" int qiw_set_sobol(qiw_ctx *c, int32_t dim, const uint32_t *m, const u
 
 extern "C" int qiw_set_scrambles(qiw_ctx *c, int64_t n, int32_t stride, const uint32_t *m, const uint32_t *shift) {
     return guarded(c, [&] { return qiw_set_scrambles_impl(c, n, stride, m, shift); });
+}
+
+extern "C" int qiw_set_points(qiw_ctx *c, int64_t n, int64_t n_points, int32_t stride, const double *x) {
+    return guarded(c, [&] { return qiw_set_points_impl(c, n, n_points, stride, x); });
 }
 
 extern "C" int64_t qiw_scramble_dims(qiw_ctx *c, const qiw_run *run, int32_t call, int64_t cap, int32_t *dims) {

@@ -1856,6 +1856,37 @@ __global__ __launch_bounds__(256) void k_given_coords(const QiwGroup *__restrict
     for (int k = 0; k < d; ++k) out[(int64_t)k * ng + r] = u[k];
 }
 
+// Simplex coordinates from caller-supplied points (the sequence plug point seq_type of
+// mean_std_from_randomization, src/randomization.jl:86-99: any sequence the caller
+// constructs as seq_type(D; scramble_rng) and draws with next!, e.g. RandomSeq,
+// src/utility.jl:336-385).  Row r of group g maps the first n_samples points of sequence
+// idx[r * n_groups + g] (pts: [.][npts][stride]) as k_given_coords maps Sobol points.
+__global__ __launch_bounds__(256) void k_points_coords(const QiwGroup *__restrict__ groups,
+                                                       const double *__restrict__ pts, int stride, int64_t npts,
+                                                       const int64_t *__restrict__ idx, int row0, int64_t row_stride,
+                                                       double *__restrict__ ut) {
+    const QiwGroup g = groups[blockIdx.y];
+    const int64_t seq = idx[(int64_t)blockIdx.z * gridDim.y + blockIdx.y];
+    if (!g.sampled) return;
+    const bool sigma = g.kind == QIW_G_SIGMA;
+    const int d = sigma ? 2 * g.order - 2 : g.d_before + g.d_after;
+    if (seq < 0 && d > 0) return;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ng = g.n_samples;  // <= npts
+    if (r >= ng) return;
+    double x[2 * QIW_KMAX], u[2 * QIW_KMAX];
+    const double *p = pts + ((seq < 0 ? 0 : seq) * npts + r) * stride;
+    for (int k = 0; k < d; ++k) x[k] = p[k];
+    double *out = ut + (int64_t)(row0 + blockIdx.z) * row_stride + g.ut_off;
+    if (sigma) {
+        sigma_row_values(x, g.order, u);
+        for (int k = 0; k < 2 * g.order; ++k) out[(int64_t)k * ng + r] = u[k];
+        return;
+    }
+    root_map(x, g.d_after, g.d_before, u);
+    for (int k = 0; k < d; ++k) out[(int64_t)k * ng + r] = u[k];
+}
+
 // Step bookkeeping with randomized qMC (inchworm_step, src/inchworm.jl:172-195), one
 // block of 256 threads after all per-sequence sums are in gsum.  For each (order,
 // n_pts_after) group, the prefactor-scaled sums of its sequences are the samples; the
@@ -2189,12 +2220,18 @@ hipError_t qiw_launch_finalize_corr_rqmc(hipStream_t st, const CorrFinalizeArgs 
     return hipGetLastError();
 }
 
-hipError_t qiw_launch_given_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int64_t n, const uint32_t *gm,
-                                   const uint32_t *gx, int stride, const int64_t *idx, int n_rows, int row0,
-                                   int64_t row_stride, double *ut) {
-    if (n_groups < 1 || n < 1 || n_rows < 1 || stride < 1 || stride > 2 * QIW_KMAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_given_coords, dim3((unsigned)((n + 255) / 256), (unsigned)n_groups, (unsigned)n_rows), dim3(256),
-                       0, st, groups, gm, gx, stride, n, idx, row0, row_stride, ut);
+hipError_t qiw_launch_given_coords(hipStream_t st, const QiwGroup *groups, int n_groups, int64_t n, const GivenSrc &src,
+                                   const int64_t *idx, int n_rows, int row0, int64_t row_stride, double *ut) {
+    if (n_groups < 1 || n < 1 || n_rows < 1 || src.stride < 1 || src.stride > 2 * QIW_KMAX) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)n_groups, (unsigned)n_rows);
+    if (src.pts) {
+        if (src.npts < n) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_points_coords, grid, dim3(256), 0, st, groups, src.pts, src.stride, src.npts, idx, row0,
+                           row_stride, ut);
+    } else {
+        hipLaunchKernelGGL(k_given_coords, grid, dim3(256), 0, st, groups, src.gm, src.gx, src.stride, n, idx, row0,
+                           row_stride, ut);
+    }
     return hipGetLastError();
 }
 

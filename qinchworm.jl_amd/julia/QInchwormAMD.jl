@@ -16,7 +16,7 @@ using QInchworm.spline_gf: SplineInterpolatedGF, IncSplineImaginaryTimeGF, make_
 using QInchworm.utility: extend!
 using QInchworm.ppgf: set_ppgf!
 using QInchworm.sector_block_matrix: SectorBlockMatrix
-using QInchworm.scrambled_sobol: ScrambledSobolSeq
+using QInchworm.scrambled_sobol: ScrambledSobolSeq, next!
 using QInchworm.randomization: RandomizationParams, RequestStdDev
 using QInchworm.keldysh_dlr: DLRImaginaryTimeGF
 
@@ -61,7 +61,9 @@ end
 # are the reference's own (scramble = 2): stage_scrambles! constructs the
 # ScrambledSobolSeq objects the reference's loops would, in their order, from rng.  With a
 # stopping target the scramble bits come from libqiw's counter-based generator keyed by a
-# seed drawn from rng once per call (scramble = 1).
+# seed drawn from rng once per call (scramble = 1).  Any other seq_type (the reference's
+# sequence plug point, src/randomization.jl:82-94; e.g. RandomSeq, src/utility.jl:336-414)
+# is constructed the same way and its points are staged (scramble = 3).
 struct QiwRand
     N_seqs::Int32
     scramble::Int32
@@ -71,9 +73,14 @@ end
 
 caller_drawn(rp::RandomizationParams) = rp.rng !== nothing && rp.target_std == 0.0
 
-QiwRand(rp::RandomizationParams) =
-    QiwRand(Int32(rp.N_seqs), rp.rng === nothing ? Int32(0) : caller_drawn(rp) ? Int32(2) : Int32(1),
-            (rp.rng === nothing || caller_drawn(rp)) ? UInt64(0) : rand(rp.rng, UInt64), rp.target_std)
+function QiwRand(rp::RandomizationParams, seq_type::Type = ScrambledSobolSeq)
+    if seq_type !== ScrambledSobolSeq
+        rp.target_std == 0.0 || throw(ArgumentError("a seq_type other than ScrambledSobolSeq needs target_std = 0"))
+        return QiwRand(Int32(rp.N_seqs), Int32(3), UInt64(0), 0.0)
+    end
+    return QiwRand(Int32(rp.N_seqs), rp.rng === nothing ? Int32(0) : caller_drawn(rp) ? Int32(2) : Int32(1),
+                   (rp.rng === nothing || caller_drawn(rp)) ? UInt64(0) : rand(rp.rng, UInt64), rp.target_std)
+end
 
 const QIW_CALL_INCHWORM = Int32(0)
 const QIW_CALL_CORRELATOR = Int32(3)
@@ -93,14 +100,32 @@ end
 # The sequences of one call under scramble = 2: qiw_scramble_dims lists their dimensions in
 # the order the reference constructs them (src/randomization.jl:94); each is drawn from rng
 # exactly as the reference draws it and staged as (m[i][d][j], x[i][d]).
-function stage_scrambles!(ctx, run::Ref{QiwRun}, call::Int32, rp::RandomizationParams)
-    caller_drawn(rp) || return nothing
+# Under scramble = 3 the sequences are seq_type(D; scramble_rng = rng), constructed in the
+# same order, and their first N_samples points (next!, src/qmc_integrate.jl:597-612) are
+# staged as x[d, r, i].
+function stage_scrambles!(ctx, run::Ref{QiwRun}, call::Int32, rp::RandomizationParams,
+                          seq_type::Type = ScrambledSobolSeq)
+    points = seq_type !== ScrambledSobolSeq
+    (points || caller_drawn(rp)) || return nothing
     sig = (Ptr{Cvoid}, Ref{QiwRun}, Int32, Int64, Ptr{Int32})
     n = ccall((:qiw_scramble_dims, libqiw), Int64, sig, ctx.h, run, call, 0, C_NULL)
     n < 0 && check(ctx, Cint(n))
     dims = Vector{Int32}(undef, n)
     check(ctx, Cint(min(0, ccall((:qiw_scramble_dims, libqiw), Int64, sig, ctx.h, run, call, n, dims))))
     stride = Int(maximum(dims; init = Int32(2)))
+    if points
+        N = max(Int(run[].N_samples), 1)
+        x = zeros(Float64, stride, N, n)
+        for (i, D) in enumerate(dims)
+            seq = seq_type(Int(D); scramble_rng = rp.rng)
+            for r in 1:N
+                x[1:D, r, i] = next!(seq)
+            end
+        end
+        check(ctx, ccall((:qiw_set_points, libqiw), Cint, (Ptr{Cvoid}, Int64, Int64, Int32, Ptr{Float64}),
+                         ctx.h, n, N, Int32(stride), x))
+        return nothing
+    end
     m = zeros(UInt32, 32, stride, n)
     x = zeros(UInt32, stride, n)
     for (i, D) in enumerate(dims)
@@ -307,10 +332,10 @@ in its order) and P_orders_std holds the standard deviations.
 """
 function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_bare, N_samples::Int64;
                    n_pts_after_max::Int64 = typemax(Int64), rand_params::RandomizationParams = RandomizationParams(),
-                   precision::Int32 = QIW_PREC_FP64, ctx::Context = context())
+                   seq_type::Type = ScrambledSobolSeq, precision::Int32 = QIW_PREC_FP64, ctx::Context = context())
     @assert N_samples == 0 || ispow2(N_samples)
     @assert rand_params.N_seqs > 0
-    rq = Ref(QiwRand(rand_params))
+    rq = Ref(QiwRand(rand_params, seq_type))
     keep = Any[]
     orders_v = Int32.(collect(orders)); bare_v = Int32.(collect(orders_bare))
     stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, bare_v, Int32[0]))), keep)
@@ -324,7 +349,7 @@ function inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, orders_ba
         run = Ref(QiwRun(length(orders_v), pointer(orders_v), length(bare_v), pointer(bare_v), N_samples,
                          n_pts_after_max == typemax(Int64) ? 0 : Int32(n_pts_after_max), precision,
                          Base.unsafe_convert(Ptr{QiwRand}, rq)))
-        stage_scrambles!(ctx, run, QIW_CALL_INCHWORM, rand_params)
+        stage_scrambles!(ctx, run, QIW_CALL_INCHWORM, rand_params, seq_type)
         check(ctx, ccall((:qiw_inchworm, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                          ctx.h, run, P, Po, Ps))
@@ -354,10 +379,11 @@ nothing, so a caller of the reference's signature ignores them.
 """
 function diff_inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64;
                         rand_params::RandomizationParams = RandomizationParams(),
-                        precision::Int32 = QIW_PREC_FP64, ctx::Context = context())
+                        seq_type::Type = ScrambledSobolSeq, precision::Int32 = QIW_PREC_FP64,
+                        ctx::Context = context())
     @assert N_samples == 0 || ispow2(N_samples)
     @assert rand_params.N_seqs > 0
-    rq = Ref(QiwRand(rand_params))
+    rq = Ref(QiwRand(rand_params, seq_type))
     keep = Any[]
     orders_v = Int32.(collect(orders))
     stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, Int32[0]))), keep)
@@ -369,7 +395,7 @@ function diff_inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_sa
     GC.@preserve keep orders_v rq begin
         run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, precision,
                          Base.unsafe_convert(Ptr{QiwRand}, rq)))
-        stage_scrambles!(ctx, run, QIW_CALL_DIFF, rand_params)
+        stage_scrambles!(ctx, run, QIW_CALL_DIFF, rand_params, seq_type)
         check(ctx, ccall((:qiw_diff_inchworm, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                          ctx.h, run, P, So, Ss))
@@ -390,20 +416,22 @@ RequestStdDev method returns the tuple (values, standard deviations).
 """
 function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64;
                        rand_params::RandomizationParams = RandomizationParams(),
+                       seq_type::Type = ScrambledSobolSeq,
                        ctx::Context = context())::Vector{kd.ImaginaryTimeGF{ComplexF64, true}}
-    return correlator_2p_impl(exp, grid, orders, N_samples, rand_params, false, ctx)[1]
+    return correlator_2p_impl(exp, grid, orders, N_samples, rand_params, seq_type, false, ctx)[1]
 end
 
 function correlator_2p(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64, ::RequestStdDev;
-                       rand_params::RandomizationParams = RandomizationParams(), ctx::Context = context())
-    return correlator_2p_impl(exp, grid, orders, N_samples, rand_params, true, ctx)
+                       rand_params::RandomizationParams = RandomizationParams(),
+                       seq_type::Type = ScrambledSobolSeq, ctx::Context = context())
+    return correlator_2p_impl(exp, grid, orders, N_samples, rand_params, seq_type, true, ctx)
 end
 
 function correlator_2p_impl(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_samples::Int64,
-                            rand_params::RandomizationParams, with_std::Bool, ctx::Context)
+                            rand_params::RandomizationParams, seq_type::Type, with_std::Bool, ctx::Context)
     @assert N_samples == 0 || ispow2(N_samples)
     @assert rand_params.N_seqs > 0
-    rq = Ref(QiwRand(rand_params))
+    rq = Ref(QiwRand(rand_params, seq_type))
     keep = Any[]
     orders_v = Int32.(collect(orders))
     stage!(ctx, exp, grid, Int(maximum(vcat(orders_v, Int32[0]))), keep)
@@ -417,7 +445,7 @@ function correlator_2p_impl(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, 
     GC.@preserve keep orders_v rq begin
         run = Ref(QiwRun(length(orders_v), pointer(orders_v), 0, C_NULL, N_samples, 0, 0,
                          Base.unsafe_convert(Ptr{QiwRand}, rq)))
-        stage_scrambles!(ctx, run, QIW_CALL_CORRELATOR, rand_params)
+        stage_scrambles!(ctx, run, QIW_CALL_CORRELATOR, rand_params, seq_type)
         check(ctx, ccall((:qiw_correlator_2p_std, libqiw), Cint,
                          (Ptr{Cvoid}, Ref{QiwRun}, Ptr{ComplexF64}, Ptr{ComplexF64}), ctx.h, run, out, out_std))
     end

@@ -5,9 +5,9 @@ libqiw.so (include/qiw.h).  Setup (model construction, topologies, Sobol directi
 happens here; all sampling and diagram evaluation runs in the native library on the GPU.
 """
 from . import abi, diagrammatics, model, randomization, sobol  # noqa: F401
-from .randomization import RandomizationParams, RequestStdDev  # noqa: F401
+from .randomization import RandomizationParams, RandomSeq, RequestStdDev  # noqa: F401
 from .inchworm import (autotune, context, correlator_2p, density_matrix, diff_inchworm, get_tuning, inchworm,  # noqa: F401
                        inchworm_step, inchworm_step_bare, normalize, partition_function, scramble_dims,
-                       set_graphs, set_scrambles, set_tuning, stats)
+                       set_graphs, set_points, set_scrambles, set_tuning, stats)
 from .warp import (group_index, warp_apply, warp_enable, warp_init, warp_loss_grad,  # noqa: F401
                    warp_params, warp_train)

@@ -150,7 +150,7 @@ class RunStruct:
 
 # Exported symbols of libqiw (every declaration of include/qiw.h)
 EXPORTS = ["qiw_create", "qiw_destroy", "qiw_last_error", "qiw_abi_version", "qiw_set_model",
-           "qiw_set_sobol", "qiw_set_scrambles", "qiw_scramble_dims", "qiw_set_ppgf", "qiw_get_ppgf", "qiw_inchworm", "qiw_inchworm_step",
+           "qiw_set_sobol", "qiw_set_scrambles", "qiw_set_points", "qiw_scramble_dims", "qiw_set_ppgf", "qiw_get_ppgf", "qiw_inchworm", "qiw_inchworm_step",
            "qiw_inchworm_step_bare", "qiw_inchworm_step_noalloc", "qiw_correlator_2p", "qiw_correlator_2p_std", "qiw_diff_inchworm",
            "qiw_partition_function",
            "qiw_density_matrix", "qiw_sobol_points", "qiw_simplex_points", "qiw_get_stats",
@@ -179,6 +179,7 @@ def load_library(path=None):
     lib.qiw_set_model.argtypes = [vp, C.POINTER(qiw_model)]
     lib.qiw_set_sobol.argtypes = [vp, C.c_int32, _u32p, _u32p]
     lib.qiw_set_scrambles.argtypes = [vp, C.c_int64, C.c_int32, _u32p, _u32p]
+    lib.qiw_set_points.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32, _f64p]
     lib.qiw_scramble_dims.argtypes = [vp, C.POINTER(qiw_run), C.c_int32, C.c_int64, _i32p]
     lib.qiw_scramble_dims.restype = C.c_int64
     lib.qiw_set_ppgf.argtypes = [vp, C.c_int32, _f64p, _f64p]

@@ -18,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .randomization import RandomizationParams, RequestStdDev, draw_sequences
+from .randomization import RandomizationParams, RequestStdDev, draw_points, draw_sequences
 from .sobol import ScrambledSobolSeq
 
 _ctx = None
@@ -99,6 +99,10 @@ def _stage_given(ctx, rand_params, run, call):
         return
     dims = scramble_dims(run, call, ctx)
     stride = max(dims + [2])
+    if rand_params.seq_type is not None:
+        n = max(int(run.struct.N_samples), 1)
+        set_points(draw_points(rand_params.rng, rand_params.seq_type, dims, stride, n), ctx)
+        return
     m, x = draw_sequences(rand_params.rng, dims, stride)
     set_scrambles(m, x, ctx)
 
@@ -113,6 +117,15 @@ def set_scrambles(m, x=None, ctx=None):
         x = np.ascontiguousarray(x, dtype=np.uint32)
         xp = x.ctypes.data_as(abi._u32p)
     ctx.check(ctx.lib.qiw_set_scrambles(ctx.h, n, stride, m.ctypes.data_as(abi._u32p) if n else None, xp))
+
+
+def set_points(x, ctx=None):
+    """qiw_set_points: x [n][n_points][stride] in [0, 1] (the caller's point sequences of
+    qiw_rand.scramble = 3)."""
+    ctx = ctx or context()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n, npts, stride = x.shape if x.ndim == 3 and x.shape[0] > 0 else (0, 1, 1)
+    ctx.check(ctx.lib.qiw_set_points(ctx.h, n, npts, stride, x.ctypes.data_as(abi._f64p) if n else None))
 
 
 def inchworm(expansion, grid, orders, orders_bare, N_samples, n_pts_after_max=None,

@@ -45,6 +45,7 @@ def lib():
         L.orc_rq_scramble.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u32p, u32p, u32p]
         L.orc_dlr_eval.argtypes = [C.c_int, f64p, f64p, C.c_int, C.c_double, C.c_double, C.c_double, f64p]
         L.orc_set_scrambles.argtypes = [C.c_int64, C.c_int, u32p, u32p]
+        L.orc_set_points.argtypes = [C.c_int64, C.c_int64, C.c_int, f64p]
         L.orc_scrambles_used.restype = C.c_int64
         L.orc_scrambles_log.argtypes = [C.c_int64, C.POINTER(C.c_int64)]
         L.orc_scrambles_log.restype = C.c_int64
@@ -171,6 +172,13 @@ def set_scrambles(m, x):
     m = np.ascontiguousarray(m, dtype=np.uint32)
     x = np.ascontiguousarray(x, dtype=np.uint32)
     lib().orc_set_scrambles(int(m.shape[0]), int(m.shape[1]), _p(m, C.c_uint32), _p(x, C.c_uint32))
+
+
+def set_points(x):
+    """Stage caller-supplied point sequences (x [n][n_points][stride]) for scramble = 3,
+    consumed as under scramble = 2."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_set_points(int(x.shape[0]), int(x.shape[1]), int(x.shape[2]), _p(x, C.c_double))
 
 
 def scrambles_used():

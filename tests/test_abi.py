@@ -26,7 +26,7 @@ def test_header_declarations_are_exported():
 
 def test_abi_version_and_no_gpu_error():
     lib = abi.load_library()
-    assert lib.qiw_abi_version() == 5
+    assert lib.qiw_abi_version() == 6
     if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not os.path.exists("/dev/kfd"):
         h = C.c_void_p()
         assert lib.qiw_create(C.byref(h), 0) == abi.QIW_E_NO_GPU
