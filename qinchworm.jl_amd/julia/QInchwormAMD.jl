@@ -401,8 +401,18 @@ function diff_inchworm!(exp::Expansion, grid::kd.ImaginaryTimeGrid, orders, N_sa
                          ctx.h, run, P, So, Ss))
     end
     load_ppgf!(ctx, exp, grid)
-    return (Dict(Int(o) => So[:, :, i] for (i, o) in enumerate(orders_v)),
-            Dict(Int(o) => Ss[:, :, i] for (i, o) in enumerate(orders_v)))
+    # Σ_orders as the reference builds them (src/inchworm.jl:698-699, :770-771): containers
+    # kd.zero(expansion.P) filled with set_ppgf! in step order.  The rows are where the
+    # reference's set_ppgf! leaves each step's value: every row of a grid container, rows
+    # 2..n_tau (appended one per step) of an incremental one.
+    Σ_orders = Dict(Int(o) => kd.zero(exp.P) for o in orders_v)
+    Σ_orders_std = Dict(Int(o) => kd.zero(exp.P) for o in orders_v)
+    first_row = first(exp.P) isa IncSplineImaginaryTimeGF ? 2 : 1
+    for n in first_row:n_tau, (i, o) in enumerate(orders_v)
+        set_ppgf!(Σ_orders[Int(o)], grid[1], grid[n], sbm_row(exp, view(So, :, :, i), n))
+        set_ppgf!(Σ_orders_std[Int(o)], grid[1], grid[n], sbm_row(exp, view(Ss, :, :, i), n))
+    end
+    return Σ_orders, Σ_orders_std
 end
 
 """
