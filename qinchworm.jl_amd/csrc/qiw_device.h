@@ -271,7 +271,10 @@ struct MfmaArgs {
     const int32_t *bto;        // [ns] offset of each sector's block in the bt layout
     int bt, btp;
     int ops_zero;              // position of four 0.0 in ops (16-byte aligned, past the blocks)
+    int wave_lds;              // floats of LDS per wavefront (absent-row writes and walk stack)
 };
+// Wavefronts per workgroup of k_eval_mfma_t (tasks come in units of this many)
+#define QIW_MFMA_WAVES 2
 
 // ---- randomized qMC scrambles -------------------------------------------------------------
 #define QIW_MAX_SEQS 64                 // largest N_seqs

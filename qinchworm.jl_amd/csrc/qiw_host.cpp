@@ -622,7 +622,13 @@ struct Batch {
     double flops_per_launch = 0;  // algorithmic FLOPs of one launch
     int64_t leaves_per_launch = 0, diagram_evals_per_launch = 0;
     bool overflow = false;  // the launch would exceed kMaxTasks workgroups
+    int wg_waves = 1;       // wavefronts per workgroup: consecutive tasks of a workgroup share
+                            // its sample block (matrix-core walk: kMfmaWaves)
 };
+
+// Wavefronts per workgroup of the matrix-core walk: the two walk different topology chunks
+// of the same samples and share the samples' i*P blocks in LDS.
+constexpr int kMfmaWaves = QIW_MFMA_WAVES;
 
 // Workgroups of one launch: task rows and grid sizes are 32-bit and every task is a
 // 24-byte record on the host and the device, so a launch holds at most 2^28 tasks
@@ -934,11 +940,14 @@ void xcd_interleave(std::vector<QiwTask> &tasks) {
 // the least-loaded XCD, each XCD's list is laid out at positions x, x + 8, ...  (On the
 // two-orbital step at N = 2^14 this takes the L2 fetches of a launch from 41 GiB to
 // 0.17 GiB at the same run time.)
-void xcd_by_chunk(Batch &b) {
+// With units of w tasks (the tasks of one workgroup, consecutive), a unit is placed as one,
+// by the chunk of its first task.
+void xcd_by_chunk(Batch &b, int w) {
     std::map<std::pair<int, int>, size_t> key_of;  // (program, chunk) -> bucket
     std::vector<std::vector<QiwTask>> buckets;
     std::vector<double> cost;
-    for (const QiwTask &t : b.tasks) {
+    for (size_t u = 0; u < b.tasks.size(); u += w) {
+        const QiwTask &t = b.tasks[u];
         const Compiled &cg = b.groups[t.group];
         const int prog = cg.replica_of >= 0 ? cg.replica_of : t.group;
         auto it = key_of.emplace(std::make_pair(prog, t.chunk), buckets.size()).first;
@@ -946,8 +955,8 @@ void xcd_by_chunk(Batch &b) {
             buckets.emplace_back();
             cost.push_back(0.0);
         }
-        buckets[it->second].push_back(t);
-        cost[it->second] += (double)(cg.edges_per_sample + cg.leaves_per_sample) / std::max<size_t>(cg.chunks.size(), 1);
+        buckets[it->second].insert(buckets[it->second].end(), b.tasks.begin() + u, b.tasks.begin() + u + w);
+        cost[it->second] += (double)w * (cg.edges_per_sample + cg.leaves_per_sample) / std::max<size_t>(cg.chunks.size(), 1);
     }
     std::vector<size_t> order(buckets.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
@@ -966,9 +975,31 @@ void xcd_by_chunk(Batch &b) {
         more = false;
         for (int x = 0; x < kXcds; ++x)
             if (pos[x] < lists[x].size()) {
-                out.push_back(lists[x][pos[x]++]);
+                out.insert(out.end(), lists[x].begin() + pos[x], lists[x].begin() + pos[x] + w);
+                pos[x] += w;
                 more = true;
             }
+    }
+    b.tasks.swap(out);
+}
+
+// Workgroups of b.wg_waves tasks: the tasks of one sample block of a group (its topology
+// chunks, consecutive after order_tasks' bucketing) are cut into units of wg_waves, the
+// last one filled up with empty tasks (chunk -1, no output row).
+void unit_tasks(Batch &b) {
+    const int w = b.wg_waves;
+    std::vector<QiwTask> out;
+    out.reserve(b.tasks.size() + b.tasks.size() / 2 + w);
+    size_t i = 0;
+    while (i < b.tasks.size()) {
+        size_t j = i + 1;
+        const QiwTask &t0 = b.tasks[i];
+        while (j < b.tasks.size() && b.tasks[j].group == t0.group && b.tasks[j].first_sample == t0.first_sample &&
+               b.tasks[j].tau_index == t0.tau_index)
+            ++j;
+        out.insert(out.end(), b.tasks.begin() + i, b.tasks.begin() + j);
+        for (size_t k = j - i; k % w; ++k) out.push_back(QiwTask{t0.group, t0.first_sample, t0.tau_index, -1, -1, 0});
+        i = j;
     }
     b.tasks.swap(out);
 }
@@ -992,10 +1023,14 @@ void order_tasks(Batch &b) {
     std::vector<QiwTask> out(b.tasks.size());
     for (const QiwTask &t : b.tasks) out[start[t.group]++] = t;
     b.tasks.swap(out);
-    if (b.fmt == FMT_VEC)
-        xcd_by_chunk(b);
-    else
+    if (b.wg_waves > 1) {
+        unit_tasks(b);
+        xcd_by_chunk(b, b.wg_waves);
+    } else if (b.fmt == FMT_VEC) {
+        xcd_by_chunk(b, 1);
+    } else {
         xcd_interleave(b.tasks);
+    }
 }
 
 // The matrix-core walk's layout of the operator blocks and of the i*P blocks of a segment:
@@ -1117,6 +1152,7 @@ int build_step_batch(qiw_ctx *c, const qiw_run *run, bool bare, const std::vecto
     b.max_order = max_order;
     b.spt = samples_per_task(c, m, b.fmt, n_groups * b.n_eval * std::max<int64_t>(run->N_samples, 1));
     b.precision = run->precision;
+    b.wg_waves = (b.fmt == FMT_VEC && m.mmax > 6) ? kMfmaWaves : 1;
     for (int oi = 0; oi < n_o; ++oi) {
         int order = ord[oi];
         if (order > 0 && run->N_samples <= 0) continue;
@@ -1348,10 +1384,11 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
             c->mfma_staged = true;
         }
         const int KM = a.max_order <= 3 ? 3 : 4, NM = 2 * KM + 3;
-        // i*P blocks, the four 0.0 read by lanes outside a block, 64 floats for the writes of
-        // absent rows, and the stack of the walk (a 4-float header and the rows of each frame)
-        const size_t lds = ((((size_t)(NM - 1) * m.btp + 3) & ~(size_t)3) + 4 + 64 + (size_t)(NM - 2) * 4 +
-                            (size_t)ystack_rows(m, NM - 2) * 16) * 4;
+        // i*P blocks and four 0.0 read by lanes outside a block, shared by the workgroup; per
+        // wavefront 64 floats for the writes of absent rows and the stack of its walk (a
+        // 4-float header and the rows of each frame)
+        const size_t wave_lds = 64 + (size_t)(NM - 2) * 4 + (size_t)ystack_rows(m, NM - 2) * 16;
+        const size_t lds = ((((size_t)(NM - 1) * m.btp + 3) & ~(size_t)3) + 4 + QIW_MFMA_WAVES * wave_lds) * 4;
         if (lds > 160 * 1024) return fail(c, QIW_E_LIMIT, "propagator blocks exceed the 160 KiB of LDS of a CU");
         MfmaArgs mf{};
         mf.e = a;
@@ -1363,6 +1400,7 @@ int launch_eval(qiw_ctx *c, const EvalArgs &a, int n_tasks, int fmt, int precisi
         mf.bt = m.bt;
         mf.btp = m.btp;
         mf.ops_zero = (int)m.ops_rm.size() - 4;
+        mf.wave_lds = (int)wave_lds;
         HIPCHK(c, qiw_launch_eval_mfma(c->stream, mf, n_tasks, a.max_order, m.ntab, lds));
         return 0;
     }
@@ -2228,6 +2266,7 @@ int build_diff_batch(qiw_ctx *c, const qiw_run *run, Batch &b) {
     b.max_order = max_order;
     b.spt = samples_per_task(c, m, b.fmt, n_groups * b.n_eval * std::max<int64_t>(run->N_samples, 1));
     b.precision = run->precision;
+    b.wg_waves = (b.fmt == FMT_VEC && m.mmax > 6) ? kMfmaWaves : 1;
     for (int oi = 0; oi < run->n_orders; ++oi) {
         const int order = run->orders[oi];
         if (order == 0 || (order > 1 && run->N_samples <= 0)) continue;  // :557, :567
@@ -2529,6 +2568,7 @@ int build_corr_groups(qiw_ctx *c, const qiw_run *run, int ci, Batch &b, bool *fi
     b.fmt = program_format(m, max_order, run->precision);
     b.max_order = max_order;
     b.precision = run->precision;
+    b.wg_waves = (b.fmt == FMT_VEC && m.mmax > 6) ? kMfmaWaves : 1;
     b.n_eval = rq.n_eval();
     b.scrambled = rq.scrambled;
     b.given = rq.given;

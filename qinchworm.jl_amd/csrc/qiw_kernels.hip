@@ -1228,7 +1228,7 @@ struct MState {
 // (slices past K meet zeros in A).
 __device__ __forceinline__ MTile<2> mmul(const float *__restrict__ base, int off, int m_a, int K, const MTile<2> &X,
                                          int zpos) {
-    const int l = threadIdx.x, q = l >> 4, c = l & 15;
+    const int l = threadIdx.x & (QIW_WAVE - 1), q = l >> 4, c = l & 15;
     const int ldr = (K + 3) & ~3;
     const bool two_m = m_a > 16, two_k = K > 16;
     const mf4 z = mf4{0.f, 0.f, 0.f, 0.f};
@@ -1278,7 +1278,7 @@ __device__ __forceinline__ MTile<2> mmul(const float *__restrict__ base, int off
 // that level.  The loop carries only the current level in registers.
 template <int KM, int TM>
 __device__ __forceinline__ void mwalk(MState<KM, TM, 2> &st, int n, const MTile<2> &X0, int m1, int off1, int nch1) {
-    const int l = threadIdx.x, q = l >> 4, c = l & 15;
+    const int l = threadIdx.x & (QIW_WAVE - 1), q = l >> 4, c = l & 15;
     int L = 2, mp = m1, rem = nch1, top = -1, next = 0;
     double wp = 1.0;
     MTile<2> Y = (n == 2) ? X0 : mmul(st.pl, off1, m1, m1, X0, st.pl_zero);
@@ -1347,17 +1347,21 @@ __device__ __forceinline__ void mwalk(MState<KM, TM, 2> &st, int n, const MTile<
     }
 }
 
-template <int KM, int TM, int TR>
-__global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
+// A workgroup of W wavefronts: tasks W b .. W b + W - 1, one per wavefront, are topology
+// chunks of the same sample block (a task with chunk -1 only helps stage); they share the
+// samples' i*P blocks, staged by all W wavefronts, and each walks its chunk.
+template <int KM, int TM, int TR, int W>
+__global__ __launch_bounds__(QIW_WAVE * W, 2) void k_eval_mfma_t(MfmaArgs ma) {
     constexpr int NM = 2 * KM + 3;
     static_assert(TR == 2, "tiles for blocks up to 32");
-    // [(NM - 1) * btp] i*P blocks, four 0.0, 64 floats for the writes of absent rows, then the
-    // stack of the walk
+    // [(NM - 1) * btp] i*P blocks, four 0.0, then per wavefront 64 floats for the writes of
+    // absent rows and the stack of its walk (wave_lds floats each)
     extern __shared__ __attribute__((aligned(16))) float mlds[];
     __shared__ double s_t[16];      // the times of the configuration positions 0..NM
     const EvalArgs &a = ma.e;
-    const int lane = threadIdx.x, q = lane >> 4, cl = lane & 15;
-    const QiwTask task = a.tasks[blockIdx.x];
+    const int wv = threadIdx.x / QIW_WAVE, lane = threadIdx.x & (QIW_WAVE - 1), q = lane >> 4, cl = lane & 15;
+    const QiwTask task = a.tasks[blockIdx.x * W + wv];
+    const bool walks = task.chunk >= 0;
     const QiwGroup g = a.groups[task.group];
     const int n = g.n_nodes, k = g.order, bt = ma.bt, btp = ma.btp;
     const int ntab = a.dtab.n_tables;
@@ -1369,9 +1373,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
     st.btp = btp;
     st.pl_zero = ((NM - 1) * btp + 3) & ~3;
     st.ops_zero = ma.ops_zero;
-    st.ystk = pl + st.pl_zero + 68;  // past the four zeros and the 64 floats for absent rows
-    st.dummy = -64;                   // those 64 floats, relative to ystk
-    if (lane < 4) pl[st.pl_zero + lane] = 0.f;
+    st.ystk = pl + st.pl_zero + 4 + wv * ma.wave_lds + 64;  // past the 64 floats for absent rows
+    st.dummy = -64;                                         // those 64 floats, relative to ystk
+    if (threadIdx.x < 4) pl[st.pl_zero + threadIdx.x] = 0.f;
 #pragma unroll
     for (int e = 0; e < regvec_size(KM * TM); ++e) st.W[e] = 0.0;
 
@@ -1389,8 +1393,9 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
         tau_w = task.tau_index * a.h;
         tau_f = a.beta;
     }
-    double *prow = a.partial + (int64_t)task.out_index * bt;
-    for (int e = lane; e < bt; e += QIW_WAVE) prow[e] = 0.0;
+    double *prow = a.partial + (int64_t)(walks ? task.out_index : 0) * bt;
+    if (walks)
+        for (int e = lane; e < bt; e += QIW_WAVE) prow[e] = 0.0;
     __threadfence_block();
     __syncthreads();
     const int pos_w = (g.kind == QIW_G_BARE) ? -1 : g.pos_w;
@@ -1430,14 +1435,14 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
             dt = dt > 0.0 ? dt : 0.0;
             float *dst = pl + (L - 2) * btp;
             if (g.kind == QIW_G_BARE) {
-                for (int p = lane; p < btp; p += QIW_WAVE) {
+                for (int p = threadIdx.x; p < btp; p += QIW_WAVE * W) {
                     const int e = ma.pinv[p];
                     dst[p] = (e >= 0 && ma.emask[e]) ? (float)exp(-dt * ma.ediag[e]) : 0.f;
                 }
             } else {
                 const GridCell gc = grid_cell(pick(t, L), pick(t, L - 1), inv_h, a.n_tau);
                 const double *r0 = a.ptab + (int64_t)gc.d * bt, *r1 = r0 + bt, *rm = gc.d > 0 ? r0 - bt : r0;
-                for (int p = lane; p < btp; p += QIW_WAVE) {
+                for (int p = threadIdx.x; p < btp; p += QIW_WAVE * W) {
                     const int e = ma.pinv[p];
                     dst[p] = e >= 0 ? (float)cell_value(gc, rm[e], r0[e], r1[e]) : 0.f;
                 }
@@ -1447,10 +1452,10 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
         // ---- walk the program: one segment per initial sector ----
         double mag = 0.0;
         int m0 = 1, off0 = 0, bto0 = 0, col0 = 0;
-        st.ps.init(a.prog + g.prog_off + task.chunk, lane);
+        st.ps.init(a.prog + g.prog_off + (walks ? task.chunk : 0), lane);
         const uint64_t *seg_base = st.ps.base;  // the stream just after the segment's SEG word
         int seg_idx = 0;
-        for (;;) {
+        while (walks) {  // until the chunk's END word
             const uint64_t w = st.ps.next();
             const int op = (int)((w >> 48) & 0xF);
             if (op == QIW_OP_ROOT) {
@@ -1523,7 +1528,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_mfma_t(MfmaArgs ma) {
                 break;
             }
         }
-        if (a.per_sample) {
+        if (a.per_sample && walks) {
 #pragma unroll
             for (int o = QIW_WAVE / 2; o > 0; o >>= 1) mag += __shfl_xor(mag, o, QIW_WAVE);
             if (lane == 0) a.per_sample[g.ps_off + sample] = mag;
@@ -2167,11 +2172,13 @@ hipError_t qiw_launch_ptab_cells(hipStream_t st, const double *T, int n_tau, int
 
 // Wide blocks on the matrix cores: orders up to 3 or 4, blocks up to 32, <= 2 tables.
 hipError_t qiw_launch_eval_mfma(hipStream_t st, const MfmaArgs &m, int n_tasks, int km, int tm, size_t lds_bytes) {
-    if (tm > 2) return hipErrorInvalidValue;
+    if (tm > 2 || n_tasks % QIW_MFMA_WAVES) return hipErrorInvalidValue;
     if (km <= 3)
-        hipLaunchKernelGGL((k_eval_mfma_t<3, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+        hipLaunchKernelGGL((k_eval_mfma_t<3, 2, 2, QIW_MFMA_WAVES>), dim3(n_tasks / QIW_MFMA_WAVES),
+                           dim3(QIW_WAVE * QIW_MFMA_WAVES), lds_bytes, st, m);
     else if (km <= 4)
-        hipLaunchKernelGGL((k_eval_mfma_t<4, 2, 2>), dim3(n_tasks), dim3(QIW_WAVE), lds_bytes, st, m);
+        hipLaunchKernelGGL((k_eval_mfma_t<4, 2, 2, QIW_MFMA_WAVES>), dim3(n_tasks / QIW_MFMA_WAVES),
+                           dim3(QIW_WAVE * QIW_MFMA_WAVES), lds_bytes, st, m);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
