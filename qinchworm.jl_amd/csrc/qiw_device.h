@@ -274,7 +274,7 @@ struct MfmaArgs {
     int wave_lds;              // floats of LDS per wavefront (absent-row writes and walk stack)
 };
 // Wavefronts per workgroup of k_eval_mfma_t (tasks come in units of this many)
-#define QIW_MFMA_WAVES 2
+#define QIW_MFMA_WAVES 4
 
 // ---- randomized qMC scrambles -------------------------------------------------------------
 #define QIW_MAX_SEQS 64                 // largest N_seqs

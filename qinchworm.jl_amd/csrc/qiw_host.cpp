@@ -626,8 +626,9 @@ struct Batch {
                             // its sample block (matrix-core walk: kMfmaWaves)
 };
 
-// Wavefronts per workgroup of the matrix-core walk: the two walk different topology chunks
-// of the same samples and share the samples' i*P blocks in LDS.
+// Wavefronts per workgroup of the matrix-core walk: they walk different topology chunks of
+// the same samples and share the samples' i*P blocks in LDS (four: two workgroups of 65 KB
+// per CU, two wavefronts per SIMD at the walk's 256 registers, on the three-orbital model).
 constexpr int kMfmaWaves = QIW_MFMA_WAVES;
 
 // Workgroups of one launch: task rows and grid sizes are 32-bit and every task is a
