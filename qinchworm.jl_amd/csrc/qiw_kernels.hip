@@ -190,8 +190,10 @@ __device__ __forceinline__ double dlr_kernel(double tau, double w, double beta) 
 // (SplineInterpolatedGF, src/spline_gf.jl:194-219), coefficients (c0..c3) per interval, a
 // grid container, (g(d), g(d+1) - g(d), g(d) - g(d-1), 0) per index d, or a DLR pole sum
 // (keldysh_dlr.DLRImaginaryTimeGF, src/keldysh_dlr.jl:172-176).
+// DLR = false leaves out the pole sums (a launch with no DLR table: fewer registers).
+template <bool DLR = true>
 __device__ __forceinline__ double delta_value(const QiwDeltaTables &dt, int tab, const ArcPos &p) {
-    if ((dt.dlr_mask >> tab) & 1) {
+    if (DLR && ((dt.dlr_mask >> tab) & 1)) {
         const double *c = dt.dlr + (int64_t)tab * dt.dlr_rank * 2;
         double s = 0.0;
         for (int k = 0; k < dt.dlr_rank; ++k) s = fma(c[2 * k + 1], dlr_kernel(p.z, c[2 * k], dt.beta), s);
@@ -370,7 +372,7 @@ struct FlatShape {
         else out = P5[i_];                                  \
     } while (0)
 
-template <int KM, int SM, int TM>
+template <int KM, int SM, int TM, bool DLR>
 __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
     using F = FlatShape<KM, SM>;
     constexpr int NM = F::NM, NC = F::NC;
@@ -548,7 +550,7 @@ __global__ __launch_bounds__(QIW_WAVE) void k_eval_flat_t(EvalArgs a) {
 #pragma unroll
                 for (int tb = 0; tb < TM; ++tb) {
                     const int tt = tb < ntab ? tb : (ntab > 0 ? ntab - 1 : 0);
-                    const double v = delta_value(a.dtab, tt, ap);
+                    const double v = delta_value<DLR>(a.dtab, tt, ap);
                     if (ai * TM + tb < 16)
                         W[(ai * TM + tb) & (NW - 1)] = v;
                     else
@@ -2129,12 +2131,18 @@ hipError_t qiw_launch_simplex_points(hipStream_t st, const uint32_t *m, const ui
 hipError_t qiw_launch_eval_flat(hipStream_t st, const EvalArgs &a, int n_tasks, int km, int sm, int tm) {
     if (km > QIW_KMAX || sm > 8 || tm > 4) return hipErrorInvalidValue;
     const int KM = qiw_scalar_km(km, sm, tm), SM = qiw_scalar_sm(km, sm, tm), TM = qiw_scalar_tm(km, sm, tm);
-    if (KM == 4 && SM == 4 && TM == 2)
-        hipLaunchKernelGGL((k_eval_flat_t<4, 4, 2>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    // the pole sums of DLR tables cost registers: instantiations without them for the rest
+    const bool dlr = a.dtab.dlr_mask != 0;
+    if (KM == 4 && SM == 4 && TM == 2 && !dlr)
+        hipLaunchKernelGGL((k_eval_flat_t<4, 4, 2, false>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    else if (KM == 4 && SM == 4 && TM == 2)
+        hipLaunchKernelGGL((k_eval_flat_t<4, 4, 2, true>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+    else if (KM == 4 && !dlr)
+        hipLaunchKernelGGL((k_eval_flat_t<4, 8, 4, false>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
     else if (KM == 4)
-        hipLaunchKernelGGL((k_eval_flat_t<4, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+        hipLaunchKernelGGL((k_eval_flat_t<4, 8, 4, true>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
     else
-        hipLaunchKernelGGL((k_eval_flat_t<QIW_KMAX, 8, 4>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
+        hipLaunchKernelGGL((k_eval_flat_t<QIW_KMAX, 8, 4, true>), dim3(n_tasks), dim3(QIW_WAVE), 0, st, a);
     return hipGetLastError();
 }
 
