@@ -446,6 +446,17 @@ def diff_study(log2n=14, with_cpu=True, cpu_steps=4):
     return out
 
 
+def device_sync():
+    """torch.cuda.synchronize on a GPU host, a no-op where torch sees no GPU."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.synchronize
+    except Exception:
+        pass
+    return lambda: None
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: 2 x mib MiB of HBM traffic per launch."""
     import torch
@@ -498,7 +509,7 @@ def single_orbital(log2n, steps, warmup, sync, with_cpu):
            "eager": {"ms_per_step": 1e3 * e["dt"] / steps, "value": e["evals"] / e["dt"]},
            "roofline": {"bound": "salu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(C1_TRAFFIC_PROFILE),
-                        "kernel": "k_eval_flat_t<4,4,2>", "avg_launch_ms": avg_launch_ms,
+                        "kernel": "k_eval_flat_t<4,4,2,false>", "avg_launch_ms": avg_launch_ms,
                         "note": "float64 scalar walk: one multiply per trie edge, bound by scalar-unit issue"}}
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline(N)
@@ -595,6 +606,8 @@ def main():
     ap.add_argument("--no-rqmc", action="store_true", help="skip the randomized-qMC and differential lines")
     ap.add_argument("--block-models-only", action="store_true",
                     help="only the two- and three-orbital steps (vector and matrix-core walks), one JSON line")
+    ap.add_argument("--c1-only", action="store_true",
+                    help="only the single-orbital solve (flat scalar walk), no CPU leg (profiling runs)")
     ap.add_argument("--c3-only", type=int, default=0, metavar="LOG2N",
                     help="only the three-orbital bold step (matrix-core walk) at N=2^LOG2N, no CPU leg "
                          "(profiling runs)")
@@ -616,6 +629,10 @@ def main():
     if args.warp_time_to_error:
         print(json.dumps(warp_time_to_error()), flush=True)
         return
+    if args.c1_only:
+        print(json.dumps({"single_orbital": single_orbital(14, args.steps, args.warmup, device_sync(), False)}),
+              flush=True)
+        return
     if args.c3_only > 0:
         print(json.dumps({"three_orbital_step": three_orbital(log2n=args.c3_only, with_cpu=False)}), flush=True)
         return
@@ -627,11 +644,7 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
               "designed but not built", file=sys.stderr)
         sys.exit(2)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
+    sync = device_sync()
     h = two_orbital_headline(args.log2n, args.steps, args.warmup, sync, not args.no_cpu)
     N = 2 ** args.log2n
     line = {
