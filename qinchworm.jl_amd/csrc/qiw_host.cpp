@@ -550,10 +550,50 @@ struct Compiler {
                     out.prog.push_back((uint64_t)QIW_OP_FLUSH << 48);
                 }
             };
+            std::vector<int64_t> tcost(vtop.size(), 0);
+            int64_t total = 0;
+            for (size_t t = 0; t < vtop.size(); ++t) {
+                for (int s = 0; s < m.ns; ++s) tcost[t] += (int64_t)m.dims[s] * (int64_t)vtop[t][s].size();
+                total += tcost[t];
+            }
+            // Chunks close at a topology once they reach chunk_cost.  The matrix-core walk
+            // runs the chunks of one sample block QIW_MFMA_WAVES to a workgroup, which lasts
+            // as long as its longest chunk: there the count is rounded up to a multiple of
+            // QIW_MFMA_WAVES (as far as the topologies go) and the chunks are cut under the
+            // smallest cap on a chunk's cost that packs the topologies, in order, into that
+            // many chunks (never a larger chunk than the rule above gives).
+            auto n_chunks = [&](int64_t cap) {  // cap > 0: close before exceeding cap
+                int64_t n = 1, cur = 0;
+                for (size_t t = 0, t0 = 0; t < vtop.size(); ++t) {
+                    if (t > t0 && (cap > 0 ? cur + tcost[t] > cap : cur >= chunk_cost)) {
+                        ++n;
+                        cur = 0;
+                        t0 = t;
+                    }
+                    cur += tcost[t];
+                }
+                return n;
+            };
+            int64_t cap = 0;
+            if (chunk_cost > 0 && wide() && vtop.size() > 1) {
+                int64_t target = n_chunks(0);
+                target = (target + QIW_MFMA_WAVES - 1) / QIW_MFMA_WAVES * QIW_MFMA_WAVES;
+                target = std::min<int64_t>(target, (int64_t)vtop.size());
+                int64_t lo = *std::max_element(tcost.begin(), tcost.end()), hi = std::max<int64_t>(total, 1);
+                while (lo < hi) {
+                    const int64_t mid = lo + (hi - lo) / 2;
+                    if (n_chunks(mid) <= target)
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                cap = std::max<int64_t>(lo, 1);
+            }
             size_t t0 = 0;
             int64_t cost = 0;
             for (size_t t = 0; t < vtop.size(); ++t) {
-                if (chunk_cost > 0 && t > t0 && cost >= chunk_cost) {
+                const bool cut = chunk_cost > 0 && t > t0 && (cap > 0 ? cost + tcost[t] > cap : cost >= chunk_cost);
+                if (cut) {
                     put_chunk(t0, t);
                     out.prog.push_back((uint64_t)QIW_OP_END << 48);
                     while (out.prog.size() % QIW_WAVE) out.prog.push_back((uint64_t)QIW_OP_END << 48);
@@ -561,7 +601,7 @@ struct Compiler {
                     t0 = t;
                     cost = 0;
                 }
-                for (int s = 0; s < m.ns; ++s) cost += (int64_t)m.dims[s] * (int64_t)vtop[t][s].size();
+                cost += tcost[t];
             }
             put_chunk(t0, vtop.size());
         }
