@@ -236,6 +236,7 @@ struct Compiled {
     std::vector<uint32_t> topo;   // per topology: head/tail nibbles of arcs 0-3, of arcs 4-5
     std::vector<double> coefs;    // scalar programs: leaf coefficients other than +-1
     std::vector<int32_t> chunks{0};  // program offsets (64-bit words) of the topology chunks
+    std::vector<double> chunk_share;  // vector programs: each chunk's share of the group's cost
     int64_t leaves_per_sample = 0;
     int64_t edges_per_sample = 0;
     // block programs: real FLOPs of the chain products one sample costs (per column of the
@@ -591,9 +592,11 @@ struct Compiler {
             }
             size_t t0 = 0;
             int64_t cost = 0;
+            std::vector<size_t> chunk_first{0};  // first topology of each chunk
             for (size_t t = 0; t < vtop.size(); ++t) {
                 const bool cut = chunk_cost > 0 && t > t0 && (cap > 0 ? cost + tcost[t] > cap : cost >= chunk_cost);
                 if (cut) {
+                    chunk_first.push_back(t);
                     put_chunk(t0, t);
                     out.prog.push_back((uint64_t)QIW_OP_END << 48);
                     while (out.prog.size() % QIW_WAVE) out.prog.push_back((uint64_t)QIW_OP_END << 48);
@@ -604,6 +607,11 @@ struct Compiler {
                 cost += tcost[t];
             }
             put_chunk(t0, vtop.size());
+            out.chunk_share.assign(out.chunks.size(), 0.0);  // the chunks' shares of the cost
+            for (size_t t = 0, ci = 0; t < vtop.size(); ++t) {
+                while (ci + 1 < chunk_first.size() && t == chunk_first[ci + 1]) ++ci;
+                out.chunk_share[ci] += total > 0 ? (double)tcost[t] / (double)total : 1.0 / (double)vtop.size();
+            }
         }
         if (fmt == FMT_SCALAR) {  // 32-bit words, two to a 64-bit program element
             put32((uint32_t)QIW_FL_OP_END << 28);
@@ -981,8 +989,12 @@ void xcd_interleave(std::vector<QiwTask> &tasks) {
 // the least-loaded XCD, each XCD's list is laid out at positions x, x + 8, ...  (On the
 // two-orbital step at N = 2^14 this takes the L2 fetches of a launch from 41 GiB to
 // 0.17 GiB at the same run time.)
-// With units of w tasks (the tasks of one workgroup, consecutive), a unit is placed as one,
-// by the chunk of its first task.
+// A one-wavefront task is priced at its chunk's share of its group's cost (the compiler's
+// words x columns), so that a group's short last chunk does not count as a full one
+// (two-orbital step at N = 2^14: 5.08 s -> 4.90 s against equal shares).  With units of w
+// tasks (the tasks of one workgroup, consecutive), a unit is placed as one, by the chunk of
+// its first task, at w equal shares: pricing units by their chunks' shares (summed, or the
+// longest) made the three-orbital step 0.4-0.6 % slower.
 void xcd_by_chunk(Batch &b, int w) {
     std::map<std::pair<int, int>, size_t> key_of;  // (program, chunk) -> bucket
     std::vector<std::vector<QiwTask>> buckets;
@@ -997,7 +1009,15 @@ void xcd_by_chunk(Batch &b, int w) {
             cost.push_back(0.0);
         }
         buckets[it->second].insert(buckets[it->second].end(), b.tasks.begin() + u, b.tasks.begin() + u + w);
-        cost[it->second] += (double)w * (cg.edges_per_sample + cg.leaves_per_sample) / std::max<size_t>(cg.chunks.size(), 1);
+        const Compiled &pc = b.groups[prog];
+        const double work = (double)(cg.edges_per_sample + cg.leaves_per_sample);
+        const double even = 1.0 / (double)std::max<size_t>(pc.chunks.size(), 1);
+        if (w == 1) {
+            const size_t ci = std::lower_bound(pc.chunks.begin(), pc.chunks.end(), t.chunk) - pc.chunks.begin();
+            cost[it->second] += work * (ci < pc.chunk_share.size() ? pc.chunk_share[ci] : even);
+        } else {
+            cost[it->second] += (double)w * work * even;
+        }
     }
     std::vector<size_t> order(buckets.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
