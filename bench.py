@@ -267,7 +267,7 @@ def warp_study(n_tau=50, orders=(0, 1, 2), ref_log2n=20, log2ns=tuple(range(10, 
             "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:]))}
 
 
-def judge_step(k, jw, orders, log2n, block_log2n, precision):
+def reference_step(k, jw, orders, log2n, block_log2n, precision):
     """The plain qMC step at N = 2^log2n as the mean of its 2^(log2n - block_log2n) blocks of
     2^block_log2n consecutive Sobol points, one launch each.  Block b of a Sobol sequence is
     its first 2^m points (m = block_log2n) digitally shifted: the point index b 2^m + r has
@@ -295,20 +295,20 @@ def judge_step(k, jw, orders, log2n, block_log2n, precision):
         seq.D, seq.m, seq.x0 = D, base.m, shift.astype(np.uint32)
         v, _ = qiw.inchworm_step(k, jw, orders, 2 ** m, precision=precision, seq=seq)
         acc = v if acc is None else acc + v
-        progress(f"judge block {b + 1} of {nb} (2^{m} points each)")
+        progress(f"reference block {b + 1} of {nb} (2^{m} points each)")
     return acc / nb
 
 
-def warp_time_to_error(log2ns=tuple(range(10, 17)), judge_log2n=20, jw=100, iters=4000, n_train=2 ** 14,
+def warp_time_to_error(log2ns=tuple(range(10, 17)), ref_log2n=20, jw=100, iters=4000, n_train=2 ** 14,
                        p_log2n=8):
     """The metric's time-to-error leg on the learned-warp workload: the two-orbital model of
     the headline (n_tau = 200, orders 0:4), one bold step at tau_w = grid[jw] in float32
-    chain products, plain and warped qMC at N = 2^10 .. 2^16, each judged against the plain
-    step at N = 2^judge_log2n.  The propagator is that of a plain float32 solve at
+    chain products, plain and warped qMC at N = 2^10 .. 2^16, each compared with the plain
+    step at N = 2^ref_log2n (the reference estimate).  The propagator is that of a plain float32 solve at
     N = 2^p_log2n (the same P for every estimate); the flow (4 rational-quadratic layers,
-    width 64) is trained on |integrand| samples of 8 bold steps, 2^14 per group.  The judge
-    runs the float32 chain products (1.8e-8 from float64 on one step, far below the errors
-    it judges); a float64 judge at 2^20 would take about 17 minutes.  Time per estimate is
+    width 64) is trained on |integrand| samples of 8 bold steps, 2^14 per group.  The
+    reference estimate runs the float32 chain products (1.8e-8 from float64 on one step, far
+    below the errors it measures); a float64 one at 2^20 would take about 17 minutes.  Time per estimate is
     the evaluator time of the step (qiw_stats.kernel_ms); time to error is interpolated in
     log-log between the measured N."""
     import numpy as np
@@ -327,7 +327,7 @@ def warp_time_to_error(log2ns=tuple(range(10, 17)), judge_log2n=20, jw=100, iter
     progress(f"warp leg: trained ({train_s:.1f} s)")
     k.P = P.copy()
     qiw.warp_enable(False)
-    ref = judge_step(k, jw, orders, judge_log2n, 16, fp32)
+    ref = reference_step(k, jw, orders, ref_log2n, 16, fp32)
     res = {"plain": [], "warped": []}
     for ln in log2ns:
         for name, on in (("plain", False), ("warped", True)):
@@ -354,7 +354,7 @@ def warp_time_to_error(log2ns=tuple(range(10, 17)), judge_log2n=20, jw=100, iter
         tp, tw = time_to(err, res["plain"]), time_to(err, res["warped"])
         speedups.append(None if (tp is None or tw is None) else tp / tw)
     return {"workload": f"two-orbital Hubbard-Kanamori (the headline model), n_tau=200, orders 0:4, bold step "
-                        f"tau_w=grid[{jw}], float32 chains; judge: plain at 2^{judge_log2n}",
+                        f"tau_w=grid[{jw}], float32 chains; reference: plain at 2^{ref_log2n}",
             "N": [r[0] for r in res["plain"]], "rel_err_plain": [r[1] for r in res["plain"]],
             "rel_err_warped": [r[1] for r in res["warped"]], "time_s_plain": [r[2] for r in res["plain"]],
             "time_s_warped": [r[2] for r in res["warped"]],
