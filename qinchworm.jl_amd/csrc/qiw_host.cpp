@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/qiw.h"
+#include "qiw_chunks.h"
 #include "qiw_ctx.h"
 #include "qiw_device.h"
 
@@ -541,8 +542,8 @@ struct Compiler {
         if (vec_like) {
             // Chunks of topologies, each a program of its own: every initial sector gets a
             // segment in every chunk, so every output element of a task row is written.
-            // A chunk closes once its words, weighted by the columns each segment is
-            // replayed for, reach chunk_cost.
+            // Where the chunks start: qiw_chunk_starts, on the topologies' words weighted by
+            // the columns each segment is replayed for.
             auto put_chunk = [&](size_t t0, size_t t1) {
                 for (int s = 0; s < m.ns; ++s) {
                     out.prog.push_back(((uint64_t)QIW_OP_SEG << 48) | (uint64_t)m.dims[s] | ((uint64_t)s << 8) |
@@ -557,54 +558,15 @@ struct Compiler {
                 for (int s = 0; s < m.ns; ++s) tcost[t] += (int64_t)m.dims[s] * (int64_t)vtop[t][s].size();
                 total += tcost[t];
             }
-            // Chunks close at a topology once they reach chunk_cost.  The matrix-core walk
-            // runs the chunks of one sample block QIW_MFMA_WAVES to a workgroup, which lasts
-            // as long as its longest chunk: there the count is rounded up to a multiple of
-            // QIW_MFMA_WAVES (as far as the topologies go) and the chunks are cut under the
-            // smallest cap on a chunk's cost that packs the topologies, in order, into that
-            // many chunks (never a larger chunk than the rule above gives).
-            auto n_chunks = [&](int64_t cap) {  // cap > 0: close before exceeding cap
-                int64_t n = 1, cur = 0;
-                for (size_t t = 0, t0 = 0; t < vtop.size(); ++t) {
-                    if (t > t0 && (cap > 0 ? cur + tcost[t] > cap : cur >= chunk_cost)) {
-                        ++n;
-                        cur = 0;
-                        t0 = t;
-                    }
-                    cur += tcost[t];
-                }
-                return n;
-            };
-            int64_t cap = 0;
-            if (chunk_cost > 0 && wide() && vtop.size() > 1) {
-                int64_t target = n_chunks(0);
-                target = (target + QIW_MFMA_WAVES - 1) / QIW_MFMA_WAVES * QIW_MFMA_WAVES;
-                target = std::min<int64_t>(target, (int64_t)vtop.size());
-                int64_t lo = *std::max_element(tcost.begin(), tcost.end()), hi = std::max<int64_t>(total, 1);
-                while (lo < hi) {
-                    const int64_t mid = lo + (hi - lo) / 2;
-                    if (n_chunks(mid) <= target)
-                        hi = mid;
-                    else
-                        lo = mid + 1;
-                }
-                cap = std::max<int64_t>(lo, 1);
-            }
+            const std::vector<size_t> chunk_first = qiw_chunk_starts(tcost, chunk_cost, wide() ? QIW_MFMA_WAVES : 1);
             size_t t0 = 0;
-            int64_t cost = 0;
-            std::vector<size_t> chunk_first{0};  // first topology of each chunk
-            for (size_t t = 0; t < vtop.size(); ++t) {
-                const bool cut = chunk_cost > 0 && t > t0 && (cap > 0 ? cost + tcost[t] > cap : cost >= chunk_cost);
-                if (cut) {
-                    chunk_first.push_back(t);
-                    put_chunk(t0, t);
-                    out.prog.push_back((uint64_t)QIW_OP_END << 48);
-                    while (out.prog.size() % QIW_WAVE) out.prog.push_back((uint64_t)QIW_OP_END << 48);
-                    out.chunks.push_back((int32_t)out.prog.size());
-                    t0 = t;
-                    cost = 0;
-                }
-                cost += tcost[t];
+            for (size_t ci = 1; ci < chunk_first.size(); ++ci) {
+                const size_t t = chunk_first[ci];
+                put_chunk(t0, t);
+                out.prog.push_back((uint64_t)QIW_OP_END << 48);
+                while (out.prog.size() % QIW_WAVE) out.prog.push_back((uint64_t)QIW_OP_END << 48);
+                out.chunks.push_back((int32_t)out.prog.size());
+                t0 = t;
             }
             put_chunk(t0, vtop.size());
             out.chunk_share.assign(out.chunks.size(), 0.0);  // the chunks' shares of the cost
