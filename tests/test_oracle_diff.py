@@ -61,3 +61,24 @@ def test_order_one_self_energy_closed_form():
         r2 = s_emp / (d_bwd[a] * P[a, 0]) if abs(d_bwd[a] * P[a, 0]) > 0 else None
         assert r1 is not None and r2 is not None
         assert abs(r1 - r2) < 1e-12 and abs(abs(r1) - 1.0) < 1e-12
+
+
+def test_order_one_self_energy_closed_form_grid_container():
+    """The same closed form with the bold P on the grid (interpolate_ppgf = false, so Σ =
+    zero(expansion.P) is a grid container too): Σ_orders holds the self-energy of step a
+    (tau_f_prev = grid[a]) at tau_f = grid[a + 1], as set_ppgf!(Σ_orders[order], τ_i, τ_f, ...)
+    writes it (src/inchworm.jl:766-769), for either container; a row shift pairs Delta and P
+    of other times and fails the unit ratio."""
+    e0 = golden_model()
+    exp = model.Expansion(e0.ed, e0.grid, e0.pairs, interpolate_ppgf=False)
+    P, So, _ = O.diff_inchworm(exp, [1], 2 ** 4, n_steps_max=5)
+    d_fwd = exp.pairs[0].propagator.values
+    d_bwd = exp.pairs[1].propagator.values
+    assert np.all(So[1][0] == 0) and np.all(So[1][6:] == 0)  # rows of steps not taken
+    for a in range(1, 5):
+        s_occ, s_emp = So[1][a + 1]
+        r1 = s_occ / (d_fwd[a] * P[a, 1])
+        r2 = s_emp / (d_bwd[a] * P[a, 0])
+        assert abs(r1 - r2) < 1e-12 and abs(abs(r1) - 1.0) < 1e-12
+        # and not the values of the neighbouring rows
+        assert abs(s_occ / (d_fwd[a + 1] * P[a + 1, 1]) - s_emp / (d_bwd[a + 1] * P[a + 1, 0])) > 1e-6
