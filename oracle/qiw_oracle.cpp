@@ -24,10 +24,15 @@
 //   src/inchworm.jl          diff_inchworm_step!, diff_inchworm!      (diff_step, orc_diff_inchworm)
 //
 // Parity of each part is pinned by the tests in tests/test_oracle_*.py (golden vectors of
-// the reference's test suite, SciPy, closed forms).  Grid-stored Green's functions are
-// evaluated by linear interpolation in the time difference; for the bold propagator this is
-// pinned by the reference's golden first bold step (test/inchworm.h5), for a grid-stored
-// hybridization function (QIW_GF_GRID_LINEAR pairs) no reference test pins it.
+// the reference's test suite, SciPy, closed forms).  Grid-stored Green's functions (the
+// bold P, QIW_GF_GRID_LINEAR pairs) are read with Keldysh's cell rule (grid_cell below),
+// pinned by test/inchworm.h5 and test/topology_eval.h5.  The Sobol direction numbers come
+// from the caller, so the oracle holds no table of its own.
+//
+// Parity unpinned: the rows of a grid-stored self-energy (interpolate_ppgf = false) read
+// by the Σ * P convolution of diff_step (SigmaTable).  They follow set_ppgf! on an
+// ImaginaryTimeGF (src/inchworm.jl:612-625), and no reference test covers them; the rows
+// of the Σ_orders output of the same run are pinned by the order-1 closed form.
 
 #include <algorithm>
 #include <cmath>
