@@ -620,6 +620,10 @@ def main():
                     help="only run 3 device-to-device copies of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE hardware counters) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
+              "designed but not built", file=sys.stderr)
+        sys.exit(2)
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
         return
@@ -640,10 +644,6 @@ def main():
         print(json.dumps({"two_orbital_step": two_orbital(with_cpu=not args.no_cpu),
                           "three_orbital_step": three_orbital(with_cpu=not args.no_cpu)}), flush=True)
         return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
-              "designed but not built", file=sys.stderr)
-        sys.exit(2)
     sync = device_sync()
     h = two_orbital_headline(args.log2n, args.steps, args.warmup, sync, not args.no_cpu)
     N = 2 ** args.log2n
